@@ -1,0 +1,338 @@
+// Device scene: triangle soup in BVH leaf order for traversal, the same
+// triangles in scene order for re-intersection and shading, materials, area
+// lights and the pinhole camera. Ray queries replace the reference's Embree
+// calls (scene.cpp:80-111); re-intersection and area sampling follow
+// trianglemesh.cpp:30-79 and 303-363; the camera follows camera.cpp:38-74.
+#pragma once
+#include "dev_math.h"
+
+namespace drmlt {
+
+constexpr int kBvhStack = 48;  // the builder caps tree depth below this
+
+struct DevCamera {
+    float s2c[16], c2s[16], c2w[16], w2c[16];
+    float dist, nearClip, farClip;
+    int W, H;
+};
+
+struct DevScene {
+    // traversal data (leaf order): p0 | e1 | e2, .w of tA holds the scene-order index
+    const float4 *tA, *tB, *tC;
+    // BVH: 4 float4 per node: (l.lo xyz, l.hi x) (l.hi yz, r.lo xy) (r.lo z, r.hi xyz) (int left, right, lcount, rcount)
+    const float4 *nodes;
+    // scene order
+    const float4 *sp0, *se1, *se2;  // p0, e1 = p1 - p0, e2 = p2 - p0 (float, as the reference computes them)
+    const float *sn;                // 9 floats per triangle: n0, n1, n2
+    const int *tmat, *tlight;
+    const float *mats;              // 16 floats per material
+    int nt, nlights;
+    const int *lightTris;           // first, count
+    const float *lightEmit;         // rgb, weight
+    const float *lightArea;         // total area per light
+    const float *lightCdf;          // per light: count+1 cdf entries, at lightCdfOff[l]
+    const int *lightCdfOff;
+    const float *lightFunc;         // per light: triangle areas (func), at lightCdfOff[l] - l
+    const float *pickCdf;           // nlights + 1
+    float lightWeightSum;
+    DevCamera cam;
+};
+
+// ---------------------------------------------------------------- traversal
+// Moller-Trumbore with the same operation order as the CPU restatement; no FMA
+// contraction so that the hit search is bit-identical to a float CPU search.
+DEV bool tri_hit(const float4 &A, const float4 &B, const float4 &Cc, const float o[3], const float d[3], float tmin, float tmax,
+                 float &t) {
+#pragma clang fp contract(off)
+    const float e1x = B.x, e1y = B.y, e1z = B.z, e2x = Cc.x, e2y = Cc.y, e2z = Cc.z;
+    float s1x = d[1] * e2z - d[2] * e2y, s1y = d[2] * e2x - d[0] * e2z, s1z = d[0] * e2y - d[1] * e2x;
+    float div = s1x * e1x + s1y * e1y + s1z * e1z;
+    if (div == 0.0f) return false;
+    float inv = 1.0f / div;
+    float sx = o[0] - A.x, sy = o[1] - A.y, sz = o[2] - A.z;
+    float u = (sx * s1x + sy * s1y + sz * s1z) * inv;
+    if (!(u >= 0.0f)) return false;
+    float s2x = sy * e1z - sz * e1y, s2y = sz * e1x - sx * e1z, s2z = sx * e1y - sy * e1x;
+    float v = (d[0] * s2x + d[1] * s2y + d[2] * s2z) * inv;
+    if (!(v >= 0.0f) || !(u + v <= 1.0f)) return false;
+    t = (e2x * s2x + e2y * s2y + e2z * s2z) * inv;
+    return t > tmin && t < tmax;
+}
+
+DEV bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz, const float o[3], const float inv[3], float tmin,
+              float tmax, float &tnear) {
+    float t0 = tmin, t1 = tmax;
+    float a, b;
+#define DRMLT_SLAB(lo, hi, k)                          \
+    a = (lo - o[k]) * inv[k];                          \
+    b = (hi - o[k]) * inv[k];                          \
+    if (!(a != a) && !(b != b)) {                      \
+        float lo_ = fminf(a, b), hi_ = fmaxf(a, b);    \
+        t0 = fmaxf(t0, lo_);                           \
+        t1 = fminf(t1, hi_);                           \
+    }
+    DRMLT_SLAB(lox, hix, 0)
+    DRMLT_SLAB(loy, hiy, 1)
+    DRMLT_SLAB(loz, hiz, 2)
+#undef DRMLT_SLAB
+    tnear = t0;
+    return t0 <= t1;
+}
+
+// next representable float above a positive finite x
+DEV float next_up(float x) { return __int_as_float(__float_as_int(x) + 1); }
+
+// Closest hit; ties in t go to the smaller scene-order index. Returns the
+// scene-order triangle index or -1.
+DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], float tmin, float tmax, float &tbest) {
+    float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    int stack[kBvhStack];
+    int sp = 0;
+    int best = -1;
+    tbest = tmax;
+    int node = 0;
+    for (;;) {
+        const float4 n0 = S.nodes[4 * node + 0], n1 = S.nodes[4 * node + 1], n2 = S.nodes[4 * node + 2];
+        const int4 ch = *reinterpret_cast<const int4 *>(&S.nodes[4 * node + 3]);
+        float tl, tr;
+        bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, o, inv, tmin, tbest, tl);
+        bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, inv, tmin, tbest, tr);
+        // leaves are tested in place
+        if (hl && ch.z > 0) {
+            for (int k = ch.x; k < ch.x + ch.z; k++) {
+                float t;
+                const float4 A = S.tA[k];
+                if (tri_hit(A, S.tB[k], S.tC[k], o, d, tmin, best < 0 ? tmax : next_up(tbest), t)) {
+                    int id = __float_as_int(A.w);
+                    if (t < tbest || (t == tbest && (best < 0 || id < best))) { tbest = t; best = id; }
+                }
+            }
+            hl = false;
+        }
+        if (hr && ch.w > 0) {
+            for (int k = ch.y; k < ch.y + ch.w; k++) {
+                float t;
+                const float4 A = S.tA[k];
+                if (tri_hit(A, S.tB[k], S.tC[k], o, d, tmin, best < 0 ? tmax : next_up(tbest), t)) {
+                    int id = __float_as_int(A.w);
+                    if (t < tbest || (t == tbest && (best < 0 || id < best))) { tbest = t; best = id; }
+                }
+            }
+            hr = false;
+        }
+        if (hl && hr) {
+            int nearC = tl <= tr ? ch.x : ch.y, farC = tl <= tr ? ch.y : ch.x;
+            stack[sp++] = farC;
+            node = nearC;
+        } else if (hl) {
+            node = ch.x;
+        } else if (hr) {
+            node = ch.y;
+        } else {
+            // pop, skipping entries whose box now lies beyond the best hit is
+            // not possible without their distance; pop unconditionally
+            if (sp == 0) break;
+            node = stack[--sp];
+        }
+    }
+    return best;
+}
+
+DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float tmin, float tmax) {
+    float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    int stack[kBvhStack];
+    int sp = 0;
+    int node = 0;
+    for (;;) {
+        const float4 n0 = S.nodes[4 * node + 0], n1 = S.nodes[4 * node + 1], n2 = S.nodes[4 * node + 2];
+        const int4 ch = *reinterpret_cast<const int4 *>(&S.nodes[4 * node + 3]);
+        float tl, tr;
+        bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, o, inv, tmin, tmax, tl);
+        bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, inv, tmin, tmax, tr);
+        if (hl && ch.z > 0) {
+            for (int k = ch.x; k < ch.x + ch.z; k++) {
+                float t;
+                if (tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t)) return true;
+            }
+            hl = false;
+        }
+        if (hr && ch.w > 0) {
+            for (int k = ch.y; k < ch.y + ch.w; k++) {
+                float t;
+                if (tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t)) return true;
+            }
+            hr = false;
+        }
+        if (hl && hr) {
+            stack[sp++] = ch.y;
+            node = ch.x;
+        } else if (hl) {
+            node = ch.x;
+        } else if (hr) {
+            node = ch.y;
+        } else {
+            if (sp == 0) break;
+            node = stack[--sp];
+        }
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- geometry in T
+template <class T> struct Isect {
+    V3<T> pos, gn, sn;
+    int prim;
+};
+template <class T> struct RayT {
+    V3<T> org, dir;
+    float minT, maxT;
+};
+
+DEV V3<float> f3(const float4 &a) { return V3<float>(a.x, a.y, a.z); }
+
+// trianglemesh.cpp:30-79 (re-intersect the named triangle; no u >= 0 or t check)
+template <class T> DEV bool re_intersect(const DevScene &S, int prim, const RayT<T> &r, Isect<T> &is) {
+    const V3<T> p0 = lift<T>(f3(S.sp0[prim])), e1 = lift<T>(f3(S.se1[prim])), e2 = lift<T>(f3(S.se2[prim]));
+    is.gn = normalize(cross(e1, e2));
+    V3<T> s1 = cross(r.dir, e2);
+    T div = dot(s1, e1);
+    if (val(div) == 0) return false;
+    T inv = T(1) / div;
+    V3<T> sv = r.org - p0;
+    T u = dot(sv, s1) * inv;
+    V3<T> s2 = cross(sv, e1);
+    T v = dot(r.dir, s2) * inv;
+    if (!(val(v) >= 0 && val(u + v) <= 1)) return false;
+    T t = dot(e2, s2) * inv;
+    T w = T(1) - u - v;
+    is.pos = r.org + t * r.dir;
+    const float *n = S.sn + 9 * prim;
+    is.sn = normalize(w * vconst<T>(n[0], n[1], n[2]) + u * vconst<T>(n[3], n[4], n[5]) + v * vconst<T>(n[6], n[7], n[8]));
+    if (val(dot(is.gn, is.sn)) < 0) is.gn = -is.gn;
+    is.prim = prim;
+    return true;
+}
+
+// path.cpp:95-104: closest hit on the float ray, then re-intersect in T
+template <class T> DEV bool intersect(const DevScene &S, const RayT<T> &r, Isect<T> &is) {
+    V3<float> o = lower(r.org), d = lower(r.dir);
+    float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    float tb;
+    int prim = trace_closest(S, of, df, r.minT, r.maxT, tb);
+    if (prim < 0) return false;
+    return re_intersect(S, prim, r, is);
+}
+// scene.cpp:95-111
+template <class T> DEV bool occluded(const DevScene &S, const V3<T> &org, const V3<T> &dir, T dist) {
+    V3<float> o = lower(org), d = lower(dir);
+    float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    return trace_any(S, of, df, kIsectEps, (1.0f - kShadowEps) * float(val(dist)));
+}
+
+// ---------------------------------------------------------------- camera
+template <class T> DEV V3<T> xform_point(const float *m, const V3<T> &p) {
+    T x = T(m[0]) * p.x + T(m[1]) * p.y + T(m[2]) * p.z + T(m[3]);
+    T y = T(m[4]) * p.x + T(m[5]) * p.y + T(m[6]) * p.z + T(m[7]);
+    T z = T(m[8]) * p.x + T(m[9]) * p.y + T(m[10]) * p.z + T(m[11]);
+    T w = T(m[12]) * p.x + T(m[13]) * p.y + T(m[14]) * p.z + T(m[15]);
+    T iw = T(1) / w;
+    return V3<T>(x * iw, y * iw, z * iw);
+}
+template <class T> DEV V3<T> xform_vec(const float *m, const V3<T> &v) {
+    return V3<T>(T(m[0]) * v.x + T(m[1]) * v.y + T(m[2]) * v.z, T(m[4]) * v.x + T(m[5]) * v.y + T(m[6]) * v.z,
+                 T(m[8]) * v.x + T(m[9]) * v.y + T(m[10]) * v.z);
+}
+// camera.cpp:38-48
+template <class T> DEV void sample_primary(const DevCamera &c, const T &sx, const T &sy, RayT<T> &r) {
+    V3<T> o = xform_point(c.s2c, V3<T>(sx, sy, T(0)));
+    V3<T> d = normalize(o);
+    float invZ = 1.0f / float(val(d.z));
+    r.org = xform_point(c.c2w, V3<T>(T(0), T(0), T(0)));
+    r.dir = xform_vec(c.c2w, d);
+    r.minT = c.nearClip * invZ;
+    r.maxT = c.farClip * invZ;
+}
+// camera.cpp:58-74
+template <class T> DEV bool project_point(const DevCamera &c, const V3<T> &p, T &sx, T &sy) {
+    V3<T> cp = xform_point(c.w2c, p);
+    if (val(cp.z) < c.nearClip || val(cp.z) > c.farClip) return false;
+    V3<T> rp = xform_point(c.c2s, cp);
+    if (val(rp.x) < 0 || val(rp.x) > 1 || val(rp.y) < 0 || val(rp.y) > 1) return false;
+    sx = rp.x;
+    sy = rp.y;
+    return true;
+}
+
+// ---------------------------------------------------------------- lights
+// distribution.h:46-52 (SampleDiscrete: upper_bound over the normalised cdf)
+DEV int sample_discrete(const float *cdf, int count, float u) {
+    int lo = 0, hi = count + 1;  // first index with cdf[idx] > u
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] > u) hi = mid; else lo = mid + 1;
+    }
+    int off = lo - 1;
+    return off < 0 ? 0 : (off > count - 1 ? count - 1 : off);
+}
+DEV float pick_light_prob(const DevScene &S, int li) { return S.lightEmit[4 * li + 3] / S.lightWeightSum; }
+DEV int pick_light(const DevScene &S, float u) { return sample_discrete(S.pickCdf, S.nlights, u); }
+DEV int pick_light_tri(const DevScene &S, int li, float u) {
+    return S.lightTris[2 * li] + sample_discrete(S.lightCdf + S.lightCdfOff[li], S.lightTris[2 * li + 1], u);
+}
+
+// trianglemesh.cpp:303-363
+template <class T> DEV void sample_tri(const DevScene &S, int li, int prim, const T &u0, const T &u1, V3<T> &pos, V3<T> &nrm, float &pdf) {
+    const T a = s_sqrt(T(1) - u0);
+    const T b1 = T(1) - a, b2 = a * u1;
+    pos = lift<T>(f3(S.sp0[prim])) + lift<T>(f3(S.se1[prim])) * b1 + lift<T>(f3(S.se2[prim])) * b2;
+    const float *n = S.sn + 9 * prim;
+    nrm = normalize(vconst<T>(n[0], n[1], n[2]) * (T(1) - b1 - b2) + vconst<T>(n[3], n[4], n[5]) * b1 + vconst<T>(n[6], n[7], n[8]) * b2);
+    pdf = 1.0f / S.lightArea[li];
+}
+// trianglemesh.cpp:236-272 (float only: used when a generated path is converted)
+DEV void get_sample_param(const DevScene &S, int prim, const V3<float> &pos, float &u0, float &u1) {
+    const V3<float> e0 = pos - f3(S.sp0[prim]), e1 = f3(S.se1[prim]), e2 = f3(S.se2[prim]);
+    const float d11 = dot(e1, e1), d12 = dot(e1, e2), d22 = dot(e2, e2), d01 = dot(e0, e1), d02 = dot(e0, e2);
+    const float invDen = 1.0f / (d11 * d22 - d12 * d12);
+    const float b1 = (d22 * d01 - d12 * d02) * invDen, b2 = (d11 * d02 - d12 * d01) * invDen;
+    const float a = 1.0f - b1;
+    u0 = 1.0f - a * a;
+    u1 = b2 / a;
+}
+// arealight.cpp:28-54
+template <class T> DEV bool light_sample_direct(const DevScene &S, int li, int prim, const V3<T> &pos, const T &u0, const T &u1,
+                                                V3<T> &toL, T &dist, V3<T> &contrib, T &cosL, T &dPdf, T &emPdf) {
+    V3<T> pl, nl;
+    float shapePdf;
+    sample_tri(S, li, prim, u0, u1, pl, nl, shapePdf);
+    toL = pl - pos;
+    T d2 = length2(toL);
+    dist = s_sqrt(d2);
+    toL = toL * (T(1) / dist);
+    cosL = -dot(toL, nl);
+    if (val(cosL) > kCosEps) {
+        const float *e = S.lightEmit + 4 * li;
+        T k = cosL / (d2 * T(shapePdf));
+        contrib = V3<T>(k * T(e[0]), k * T(e[1]), k * T(e[2]));
+        dPdf = T(shapePdf) * d2 / cosL;
+        emPdf = T(shapePdf) * cosL * T(kINVPI);
+        return true;
+    }
+    return false;
+}
+// arealight.cpp:56-75
+template <class T> DEV void light_emission(const DevScene &S, int li, const V3<T> &dir, const V3<T> &nl, V3<T> &em, T &dPdf, T &emPdf) {
+    T c = -dot(nl, dir);
+    if (val(c) > 0) {
+        const float *e = S.lightEmit + 4 * li;
+        em = vconst<T>(e[0], e[1], e[2]);
+        dPdf = T(1.0f / S.lightArea[li]);
+        emPdf = c * dPdf * T(kINVPI);
+    } else {
+        em = vconst<T>(0, 0, 0);
+        dPdf = T(0);
+        emPdf = T(0);
+    }
+}
+
+}  // namespace drmlt

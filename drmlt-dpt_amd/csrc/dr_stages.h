@@ -1,0 +1,390 @@
+// gfx950 kernels of the DRMLT mutation loop (DRIntegrator::Render,
+// delayed.cpp:334-673) and its parity surface.
+//
+// Plain form: one thread runs one whole DR iteration of one chain
+// (k_dr_plain), one launch per step. Batched form: the iteration is split at
+// the expensive boundaries -- stage 1, derivatives of the current state,
+// Gaussian, stage-2 proposal, derivatives of the proposal, Gaussian, stage-2
+// acceptance -- into persistent kernels over work lists binned by path
+// length. Both forms draw from the same Philox streams (key = seed, counter =
+// iteration, chain, stage tag), so for a given seed they produce the same
+// chains; only the order of the floating-point film additions differs.
+#pragma once
+#include "drmlt_kernels.h"
+
+namespace drmlt {
+
+enum : uint32_t { TAG_S1 = 1, TAG_S2 = 2, TAG_S3 = 3, TAG_PRE = 16, TAG_DIRECT = 17 };
+
+// ------------------------------------------------------------------ helpers
+struct GenArray {  // uniform stream from an array (parity tests)
+    const float *u;
+    int n, i;
+    DEV float uni() { return i < n ? u[i++] : 0.5f; }
+};
+struct DiscNormal {  // N(0, sigma) draws for the discrete dimensions
+    Philox *g;
+    float sigma;
+    DEV float operator()(int) const { return float(sigma * g->normal()); }
+};
+struct OffNormal {   // isotropic proposal offsets N(0, sigma)
+    Philox *g;
+    float sigma;
+    DEV float operator()(int) const { return float(sigma * g->normal()); }
+};
+
+DEV bool is_invalid(double x) { return isnan(x) || isinf(x) || x < 0; }         // delayed.cpp:252
+DEV bool flip_coin(double x, double d) { return (x == 1.0) || (d < x); }        // delayed.cpp:250
+DEV float clamp_ref(float v) { float m = (v < 0.0f) ? 0.0f : v; return (1.0f < m) ? 1.0f : m; }  // utils.h:245-248 (NaN passes)
+
+DEV void store_contrib(const Chains &ch, int p, const POut<float> &o) {
+    ch.rgb[p] = o.contrib.x;
+    ch.rgb[p + ch.NS] = o.contrib.y;
+    ch.rgb[p + 2 * ch.NS] = o.contrib.z;
+    ch.ls[p] = o.lsScore;
+    ch.ss[p] = o.ssScore;
+    ch.screen[p] = o.screen[0];
+    ch.screen[p + ch.NS] = o.screen[1];
+}
+
+DEV void splat(const Chains &ch, const DrParams &P, int W, int H, float *film, int p, float w) {
+    if (P.acceptanceMap || !ch.hasSplat[p]) return;
+    const float k = w * (P.b / ch.ls[p]);
+    const float r = ch.rgb[p] * k, g = ch.rgb[p + ch.NS] * k, b = ch.rgb[p + 2 * ch.NS] * k;
+    if (r == 0.0f && g == 0.0f && b == 0.0f) return;
+    if (!(isfinite(r) && isfinite(g) && isfinite(b))) return;
+    const int ix = min(max(int(ch.screen[p] * W), 0), W - 1), iy = min(max(int(ch.screen[p + ch.NS] * H), 0), H - 1);
+    float *px = film + 3 * (size_t(iy) * W + ix);
+    atomicAdd(px, r);
+    atomicAdd(px + 1, g);
+    atomicAdd(px + 2, b);
+}
+DEV void splat_const(const Chains &ch, int W, int H, float *film, int p, float r, float g, float b) {
+    if (!ch.hasSplat[p]) return;
+    const int ix = min(max(int(ch.screen[p] * W), 0), W - 1), iy = min(max(int(ch.screen[p + ch.NS] * H), 0), H - 1);
+    float *px = film + 3 * (size_t(iy) * W + ix);
+    atomicAdd(px, r);
+    atomicAdd(px + 1, g);
+    atomicAdd(px + 2, b);
+}
+
+// wave-aggregated counter increment
+DEV void count_if(unsigned long long *ctr, bool pred) {
+    const unsigned long long m = __ballot(pred);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
+enum { ST_LARGE = 0, ST_LARGE_ACC, ST_S1, ST_S1_ACC, ST_ISO, ST_ISO_ACC, ST_H2MC, ST_H2MC_ACC };
+
+DEV int perm_cur(int perm) { return perm & 3; }
+DEV int perm_first(int perm) { return (perm >> 2) & 3; }
+DEV int perm_second(int perm) { return (perm >> 4) & 3; }
+DEV int make_perm(int c, int f, int s) { return c | (f << 2) | (s << 4); }
+
+// ------------------------------------------------------------ derivatives
+DEV int jet_passes(int D) {
+    const int nch = (D + kJetK - 1) / kJetK;
+    int n = 0;
+    for (int i = 0; i < D; i++) n += nch - i / kJetK;
+    return n;
+}
+DEV void jet_pass_index(int D, int p, int &row, int &col0) {
+    const int nch = (D + kJetK - 1) / kJetK;
+    for (int i = 0; i < D; i++) {
+        const int k = nch - i / kJetK;
+        if (p < k) { row = i; col0 = (i / kJetK + p) * kJetK; return; }
+        p -= k;
+    }
+    row = 0; col0 = 0;
+}
+// One forward-over-forward pass: gradient block and Hessian row `row`,
+// columns [col0, col0 + K), of log f at the state behind ref.
+// Writes G(j) and Hrow(j) through the callbacks. Returns false if the
+// fixed-topology evaluation produced no contribution.
+template <class S, class GW, class HW>
+DEV bool jet_pass(const DevScene &Sc, const Opts &o, const SRef &ref, int D, int row, int col0, GW gw, HW hw) {
+    JetU<S, kJetK> up{ref.u, ref.s, 0, row, col0};
+    POut<Jet<S, kJetK>> out;
+    perturb_eval<Jet<S, kJetK>, true>(Sc, o, ref, up, DiscNone{}, out);
+    if (!out.valid || !(out.ssScore.v > S(0))) return false;
+    const Jet<S, kJetK> lf = s_log(out.ssScore);
+    for (int j = 0; j < kJetK; j++) {
+        const int c = col0 + j;
+        if (c < D) {
+            gw(c, lf.g[j]);
+            hw(c, lf.h[j]);
+        }
+    }
+    return true;
+}
+
+// Derivative function availability (path.cpp:3186-3206 with maxDervDepth)
+DEV bool has_derivative(int c, int l, int maxD) { return c >= 1 && l >= 0 && c + l >= 3 && c + l - 1 <= maxD; }
+
+// initGaussian (drsmallstep.cpp:129-166) for state slot p, all in one thread.
+DEV void init_gaussian_thread(const DevScene &Sc, const Chains &ch, const DrParams &P, int p, float *gbuf, float *hbuf) {
+    const int c = ch.c[p], l = ch.l[p], D = dim_of(c, l);
+    const GRef G = g_ref(ch, p);
+    if (!has_derivative(c, l, P.maxDervDepth) || D > kGMax) {
+        iso_gaussian(D, P.h2.sigma, G);
+        ch.gInit[p] = 1;
+        return;
+    }
+    for (int i = 0; i < D; i++) gbuf[i] = 0.0f;
+    for (int i = 0; i < D * D; i++) hbuf[i] = 0.0f;
+    const double ssv = ch.ss[p];
+    if (ssv > 1e-15) {
+        const SRef ref = slot_ref(ch, p);
+        bool ok = true;
+        const int np = jet_passes(D);
+        for (int q = 0; q < np && ok; q++) {
+            int row, col0;
+            jet_pass_index(D, q, row, col0);
+            ok = jet_pass<float>(Sc, P.oDeriv, ref, D, row, col0,
+                                 [&](int cc, float gv) { if (row == 0) gbuf[cc] = gv; },
+                                 [&](int cc, float hv) { hbuf[row * D + cc] = hv; hbuf[cc * D + row] = hv; });
+        }
+        bool fin = ok;
+        for (int i = 0; i < D && fin; i++) fin = isfinite(gbuf[i]);
+        for (int i = 0; i < D * D && fin; i++) fin = isfinite(hbuf[i]);
+        if (!fin) {
+            for (int i = 0; i < D; i++) gbuf[i] = 0.0f;
+            for (int i = 0; i < D * D; i++) hbuf[i] = 0.0f;
+        }
+    }
+    compute_gaussian(P.h2, ssv, D, [&](int i) { return double(gbuf[i]); }, [&](int i, int j) { return double(hbuf[i * D + j]); }, G);
+    ch.gInit[p] = 1;
+}
+
+// ------------------------------------------------------------ DR stages
+// Stage 1 (delayed.cpp:404-445). Returns the chain's flags; finishes the
+// iteration (splats, swap) when no second stage follows.
+DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter, float *film,
+                  unsigned long long *stats) {
+    const int C = ch.C;
+    const int perm = ch.perm[chain];
+    const int pc = chain + C * perm_cur(perm), pf = chain + C * perm_first(perm);
+    Philox g(P.seed, iter, chain, TAG_S1);
+    const bool isLarge = flip_coin(P.largeStepProb, g.uni());
+    const bool curValid = ch.valid[pc] != 0;
+    float a1 = 0.0f;
+    bool acc1 = false;
+    const SRef rf = slot_ref(ch, pf);
+    if (!curValid || isLarge) {
+        // LargeStep::Mutate, mutation.cpp:7-124 (multiplexed)
+        const int len = sample_discrete(P.lenCdf, P.nLen, g.uni());
+        const int lgt = min(max(int(g.uni() * float(len + 1)), 0), len);
+        const int cam = len - lgt + 1;
+        POut<float> out;
+        generate_subpath(Sc, P.o, cam, lgt, rf, g, out);
+        ch.gInit[pf] = 0;
+        if (out.valid) {
+            (void)g.uni();  // contribution pick (one candidate when multiplexed)
+            store_contrib(ch, pf, out);
+            ch.hasSplat[pf] = 1;
+            if (curValid) {
+                const int lc = ch.c[pc] + ch.l[pc] - 1, lp = cam + lgt - 1;
+                const float pmfP = P.lenFunc[lp] / (P.lenFuncInt * float(P.nLen));
+                const float pmfC = P.lenFunc[lc] / (P.lenFuncInt * float(P.nLen));
+                a1 = clamp_ref((float(lp + 1) * out.lsScore / pmfP) / (float(lc + 1) * ch.ls[pc] / pmfC));
+            }
+        }
+    } else {
+        // SmallStep::Mutate, mutation.cpp:126-165
+        copy_state(slot_ref(ch, pc), rf);
+        PerturbU<OffNormal> up{rf.u, rf.s, 0, OffNormal{&g, P.perturbStdDev}};
+        POut<float> out;
+        perturb_eval<float, false>(Sc, P.o, rf, up, DiscNormal{&g, P.discreteStdDev}, out);
+        ch.gInit[pf] = 0;
+        if (out.valid) {
+            store_contrib(ch, pf, out);
+            ch.hasSplat[pf] = 1;
+            a1 = clamp_ref(out.ssScore / ch.ss[pc]);
+        }
+    }
+    if (is_invalid(a1)) { a1 = 0.0f; acc1 = false; }
+    else acc1 = flip_coin(a1, g.uni());
+    const bool large = !curValid || isLarge;
+    count_if(stats + ST_LARGE, large);
+    count_if(stats + ST_LARGE_ACC, large && acc1);
+    count_if(stats + ST_S1, !large);
+    count_if(stats + ST_S1_ACC, !large && acc1);
+    const bool doSecond = !acc1 && !isLarge && curValid;
+    bool isAniso = false;
+    if (doSecond) isAniso = flip_coin(P.anisoPerturbProb, g.uni());
+    ch.a1[chain] = a1;
+    const int flags = (acc1 ? 1 : 0) | (doSecond ? 2 : 0) | (isAniso ? 4 : 0) | (isLarge ? 8 : 0);
+    ch.flags[chain] = flags;
+    if (!doSecond) {
+        const int W = Sc.cam.W, H = Sc.cam.H;
+        const float w1 = a1, wx = 1.0f - w1;
+        if (curValid && wx > 0.0f) splat(ch, P, W, H, film, pc, wx);
+        if (w1 > 0.0f) splat(ch, P, W, H, film, pf, w1);
+        if (acc1) {
+            ch.perm[chain] = make_perm(perm_first(perm), perm_cur(perm), perm_second(perm));
+            ch.gInit[pf] = 0;
+            if (!isLarge && P.acceptanceMap) splat_const(ch, W, H, film, pc, 1.0f, 0.0f, 0.0f);
+            ch.valid[pf] = 1;
+        }
+    }
+    return flags;
+}
+
+// Stage-2 proposal (delayed.cpp:448-463): H2MC (drsmallstep.cpp:113-212) or
+// isotropic with the second-stage sigma (mutation.cpp:167-208). For H2MC the
+// current state's Gaussian must be initialised. Stores a2 (double) for aniso
+// before the proposal's Gaussian is known: the kernel ratio is finished in
+// dr_stage2_ratio.
+DEV void dr_stage2_propose(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter) {
+    const int C = ch.C;
+    const int perm = ch.perm[chain];
+    const int pc = chain + C * perm_cur(perm), ps = chain + C * perm_second(perm);
+    const int flags = ch.flags[chain];
+    const bool isAniso = (flags & 4) != 0;
+    Philox g(P.seed, iter, chain, TAG_S2);
+    const SRef rs = slot_ref(ch, ps);
+    copy_state(slot_ref(ch, pc), rs);
+    POut<float> out;
+    if (isAniso) {
+        const int D = dim_of(ch.c[pc], ch.l[pc]);
+        const GRef G = g_ref(ch, pc);
+        double n[kGMax];
+        for (int i = 0; i < D; i++) n[i] = g.normal();
+        for (int i = 0; i < D; i++) {
+            double x = G.m(i);
+            for (int j = 0; j < D; j++) x += G.L(i, j) * n[j];
+            ch.off[i * C + chain] = float(x);
+        }
+        PerturbU<OffArray> up{rs.u, rs.s, 0, OffArray{ch.off + chain, C}};
+        perturb_eval<float, false>(Sc, P.o, rs, up, DiscNormal{&g, P.discreteStdDev}, out);
+    } else {
+        PerturbU<OffNormal> up{rs.u, rs.s, 0, OffNormal{&g, P.anisoStdDev}};
+        perturb_eval<float, false>(Sc, P.o, rs, up, DiscNormal{&g, P.discreteStdDev}, out);
+        ch.gInit[ps] = 0;
+    }
+    double a2 = 0.0;
+    if (out.valid) {
+        store_contrib(ch, ps, out);
+        if (isAniso) {
+            ch.hasSplat[ps] = (out.lsScore > 0.0f && isfinite(out.lsScore)) ? 1 : 0;
+        } else {
+            ch.hasSplat[ps] = 1;
+            a2 = double(clamp_ref(out.ssScore / ch.ss[pc]));
+        }
+    }
+    ch.a2d[chain] = a2;
+    ch.flags[chain] = flags | (out.valid ? 16 : 0);
+}
+
+// H2MC kernel ratio once both Gaussians exist (drsmallstep.cpp:185-196)
+DEV void dr_stage2_ratio(const Chains &ch, int chain) {
+    const int C = ch.C;
+    const int perm = ch.perm[chain];
+    const int pc = chain + C * perm_cur(perm), ps = chain + C * perm_second(perm);
+    const int D = dim_of(ch.c[pc], ch.l[pc]);
+    auto off = [&](int i) { return double(ch.off[i * C + chain]); };
+    const double py = gaussian_logpdf(D, off, 1.0, g_ref(ch, pc));
+    const double px = gaussian_logpdf(D, off, -1.0, g_ref(ch, ps));
+    ch.a2d[chain] = exp(px - py) * double(ch.ss[ps]) / double(ch.ss[pc]);
+}
+
+// Second-stage acceptance (delayed.cpp:466-541), splats (548-561) and state
+// swap (565-606).
+DEV void dr_stage2_accept(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter, float *film,
+                          unsigned long long *stats) {
+    const int C = ch.C;
+    const int perm = ch.perm[chain];
+    const int pc = chain + C * perm_cur(perm), pf = chain + C * perm_first(perm), ps = chain + C * perm_second(perm);
+    const int pstar = chain + 3 * C;
+    const int flags = ch.flags[chain];
+    const bool isAniso = (flags & 4) != 0, zValid = (flags & 16) != 0;
+    const float a1 = ch.a1[chain];
+    double a2 = zValid ? ch.a2d[chain] : 0.0;
+    bool acc2 = false;
+    Philox g(P.seed, iter, chain, TAG_S3);
+    if (is_invalid(a2)) {
+        a2 = 0.0;
+    } else if (P.type == 0) {
+        // Green & Mira: y* = z + (x - y) (mutation.cpp:210-237)
+        float aStar = 0.0f;
+        if (a2 != 0.0 && ch.valid[ps]) {
+            const SRef rz = slot_ref(ch, ps), rt = slot_ref(ch, pstar);
+            copy_state(rz, rt);
+            PerturbU<OffDiff> up{rt.u, rt.s, 0, OffDiff{ch.u + pc, ch.u + pf, ch.NS, ch.NS}};
+            POut<float> out;
+            perturb_eval<float, false>(Sc, P.o, rt, up, DiscNormal{&g, P.discreteStdDev}, out);
+            if (out.valid) aStar = clamp_ref(out.ssScore / ch.ss[ps]);
+        }
+        if (is_invalid(aStar)) aStar = 0.0f;
+        if (aStar == 1.0f) {
+            a2 = 0.0;
+        } else {
+            a2 = fmin(1.0, a2 * double(1.0f - aStar) / double(1.0f - a1));
+            acc2 = flip_coin(a2, g.uni());
+        }
+    } else {
+        // Tierney & Mira (delayed.cpp:502-541)
+        float aStar = fminf(1.0f, ch.ss[pf] / ch.ss[ps]);
+        if (is_invalid(aStar)) aStar = 1.0f;
+        if (aStar == 1.0f) {
+            a2 = 0.0;
+        } else {
+            const int D = dim_of(ch.c[pc], ch.l[pc]);
+            double qzy = 0, qxy = 0;
+            for (int i = 0; i < D; i++) {
+                const double y = ch.u[i * ch.NS + pf];
+                const double dz = double(ch.u[i * ch.NS + ps]) - y, dx = double(ch.u[i * ch.NS + pc]) - y;
+                qzy += dz * dz;
+                qxy += dx * dx;
+            }
+            const double s2 = double(P.perturbStdDev) * double(P.perturbStdDev);
+            const double tr = exp(-0.5 * qzy / s2 + 0.5 * qxy / s2);
+            if (is_invalid(tr)) {
+                a2 = 0.0;
+            } else {
+                a2 = fmin(1.0, a2 * tr * double(1.0f - aStar) / double(1.0f - a1));
+                acc2 = flip_coin(a2, g.uni());
+            }
+        }
+    }
+    count_if(stats + ST_ISO, !isAniso);
+    count_if(stats + ST_ISO_ACC, !isAniso && acc2);
+    count_if(stats + ST_H2MC, isAniso);
+    count_if(stats + ST_H2MC_ACC, isAniso && acc2);
+    const int W = Sc.cam.W, H = Sc.cam.H;
+    const float w1 = a1, w2 = (1.0f - w1) * float(a2), wx = 1.0f - w1 - w2;
+    if (ch.valid[pc] && wx > 0.0f) splat(ch, P, W, H, film, pc, wx);
+    if (w1 > 0.0f) splat(ch, P, W, H, film, pf, w1);
+    if (w2 > 0.0f) splat(ch, P, W, H, film, ps, w2);
+    if (acc2) {
+        ch.perm[chain] = make_perm(perm_second(perm), perm_first(perm), perm_cur(perm));
+        if (!isAniso) ch.gInit[ps] = 0;
+        if (P.acceptanceMap) splat_const(ch, W, H, film, pc, 0.0f, 1.0f, 0.0f);
+        ch.valid[ps] = 1;
+    }
+}
+
+constexpr int kPreMaxL = 16;  // light-subpath vertices of a pre-pass sample
+
+// records: the drmlt_path_rec layout (134 x 4-byte words)
+DEV SRef rec_ref(char *base) {
+    SRef r;
+    int *ip = (int *)base;
+    float *fp = (float *)base;
+    r.c = ip; r.l = ip + 1; r.light = ip + 2; r.lprim = ip + 3;
+    r.screen = fp + 4;
+    r.u = fp + 6;
+    r.cprim = ip + 38; r.lgprim = ip + 54;
+    r.cdisc = fp + 70; r.ldisc = fp + 86; r.crr = fp + 102; r.lrr = fp + 118;
+    r.s = 1;
+    return r;
+}
+constexpr int kRecBytes = 536;
+
+DEV void contrib8(float *c8, const POut<float> &o) {
+    c8[0] = o.valid ? 1.0f : 0.0f;
+    if (!o.valid) { for (int k = 1; k < 8; k++) c8[k] = 0.0f; return; }
+    c8[1] = o.contrib.x; c8[2] = o.contrib.y; c8[3] = o.contrib.z;
+    c8[4] = o.lsScore; c8[5] = o.ssScore; c8[6] = o.screen[0]; c8[7] = o.screen[1];
+}
+
+}  // namespace drmlt

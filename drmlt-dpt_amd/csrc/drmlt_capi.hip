@@ -1,0 +1,907 @@
+// C ABI of libdrmlt_hip.so (declared in include/drmlt_amd.h): context and
+// scene upload, pre-pass / direct pass / DR loop orchestration on one HIP
+// stream, hipGraph capture of the batched step, and the batched
+// function-level parity surface.
+#include "../../include/drmlt_amd.h"
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "kernels_decl.h"
+
+using namespace drmlt;
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(x)                                                                                          \
+    do {                                                                                                   \
+        hipError_t e_ = (x);                                                                               \
+        if (e_ != hipSuccess) return fail(DRMLT_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> static hipError_t dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, n * sizeof(T));
+}
+template <class T> static hipError_t upload(T **p, const T *src, size_t n, hipStream_t s) {
+    hipError_t e = dalloc(p, n);
+    if (e != hipSuccess) return e;
+    if (n) e = hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, s);
+    return e;
+}
+
+// PiecewiseConstant1D cdf (distribution.h:8-33) in float
+static void build_cdf(const float *f, int n, std::vector<float> &cdf, float &funcInt) {
+    cdf.assign(n + 1, 0.0f);
+    for (int i = 1; i < n + 1; i++) cdf[i] = cdf[i - 1] + f[i - 1] / float(n);
+    funcInt = cdf[n];
+    if (funcInt == 0.0f) {
+        for (int i = 1; i < n + 1; i++) cdf[i] = float(i) / float(n);
+    } else {
+        for (int i = 1; i < n + 1; i++) cdf[i] /= funcInt;
+    }
+}
+
+// 4x4 helpers for the camera (camera.cpp:11-28, transform.cpp:4-90)
+static void m4mul(const double a[16], const double b[16], double r[16]) {
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double s = 0;
+            for (int k = 0; k < 4; k++) s += a[4 * i + k] * b[4 * k + j];
+            r[4 * i + j] = s;
+        }
+}
+static void m4inv(const double m[16], double o[16]) {
+    double a[4][8];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 8; j++) a[i][j] = j < 4 ? m[4 * i + j] : (j - 4 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 4; c++) {
+        int p = c;
+        for (int r = c + 1; r < 4; r++)
+            if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+        for (int j = 0; j < 8; j++) std::swap(a[c][j], a[p][j]);
+        const double d = a[c][c];
+        for (int j = 0; j < 8; j++) a[c][j] /= d;
+        for (int r = 0; r < 4; r++)
+            if (r != c) {
+                const double f = a[r][c];
+                for (int j = 0; j < 8; j++) a[r][j] -= f * a[c][j];
+            }
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) o[4 * i + j] = a[i][j + 4];
+}
+static DevCamera make_camera(const float *cam) {
+    const double pos[3] = {cam[0], cam[1], cam[2]}, tgt[3] = {cam[3], cam[4], cam[5]}, up[3] = {cam[6], cam[7], cam[8]};
+    const double fov = cam[9];
+    const int W = int(cam[10]), H = int(cam[11]);
+    const double nearC = cam[12], farC = cam[13];
+    const double aspect = double(W) / double(H);
+    const double S[16] = {-0.5, 0, 0, 0, 0, -0.5 * aspect, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    const double T[16] = {1, 0, 0, -1.0, 0, 1, 0, -1.0 / aspect, 0, 0, 1, 0, 0, 0, 0, 1};
+    const double recip = 1.0 / (farC - nearC);
+    const double cot = 1.0 / std::tan((M_PI / 180.0) * (fov / 2.0));
+    const double Pm[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, farC * recip, -nearC * farC * recip, 0, 0, 1, 0};
+    double ST[16], c2s[16], s2c[16];
+    m4mul(S, T, ST);
+    m4mul(ST, Pm, c2s);
+    m4inv(c2s, s2c);
+    double d[3] = {tgt[0] - pos[0], tgt[1] - pos[1], tgt[2] - pos[2]};
+    const double dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (double &x : d) x /= dl;
+    const double ul = std::sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+    const double un[3] = {up[0] / ul, up[1] / ul, up[2] / ul};
+    double left[3] = {un[1] * d[2] - un[2] * d[1], un[2] * d[0] - un[0] * d[2], un[0] * d[1] - un[1] * d[0]};
+    const double ll = std::sqrt(left[0] * left[0] + left[1] * left[1] + left[2] * left[2]);
+    for (double &x : left) x /= ll;
+    const double nu[3] = {d[1] * left[2] - d[2] * left[1], d[2] * left[0] - d[0] * left[2], d[0] * left[1] - d[1] * left[0]};
+    const double c2w[16] = {left[0], nu[0], d[0], pos[0], left[1], nu[1], d[1], pos[1], left[2], nu[2], d[2], pos[2], 0, 0, 0, 1};
+    double w2c[16];
+    m4inv(c2w, w2c);
+    DevCamera c;
+    for (int i = 0; i < 16; i++) {
+        c.s2c[i] = float(s2c[i]);
+        c.c2s[i] = float(c2s[i]);
+        c.c2w[i] = float(c2w[i]);
+        c.w2c[i] = float(w2c[i]);
+    }
+    c.dist = float(W / (2.0 * std::tan((fov / 2.0) * (M_PI / 180.0))));
+    c.nearClip = float(nearC);
+    c.farClip = float(farC);
+    c.W = W;
+    c.H = H;
+    return c;
+}
+
+struct drmlt_ctx {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    drmlt_options opt;
+    DevScene S;
+    std::vector<void *> sceneAllocs;
+    int W = 0, H = 0, nt = 0;
+    int bvhDepth = 0;
+    // chains
+    bool haveChains = false;
+    Chains ch;
+    std::vector<void *> chainAllocs;
+    float *plainScratch = nullptr;
+    Lists L;
+    uint32_t *iterDev = nullptr;
+    uint32_t iterHost = 0;
+    int64_t itersDone = 0;
+    // length distribution
+    float *lenFunc = nullptr, *lenCdf = nullptr;
+    float lenFuncInt = 0;
+    int nLen = 0;
+    float b = 0;
+    // films and stats
+    float *filmInd = nullptr, *filmDir = nullptr;
+    unsigned long long *stats = nullptr;
+    bool directDone = false;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double timing[8] = {0};
+    hipEvent_t ev[9];
+    bool evInit = false;
+};
+
+static DrParams make_params(const drmlt_ctx *c) {
+    DrParams P;
+    const drmlt_options &o = c->opt;
+    P.largeStepProb = o.large_step_prob;
+    P.perturbStdDev = o.perturb_std_dev;
+    P.anisoPerturbProb = o.aniso_perturb_prob;
+    P.anisoStdDev = o.quirk_ignore_aniso_std_dev ? 0.0025f : o.aniso_perturb_std_dev;
+    P.discreteStdDev = o.discrete_std_dev;
+    P.type = o.type;
+    P.maxDervDepth = o.max_derivatives_depth;
+    P.acceptanceMap = o.acceptance_map;
+    P.b = c->b;
+    P.nLen = c->nLen;
+    P.lenFunc = c->lenFunc;
+    P.lenCdf = c->lenCdf;
+    P.lenFuncInt = c->lenFuncInt;
+    P.seed = o.seed + uint64_t(uint32_t(o.seed_offset)) * 0x9E3779B97F4A7C15ull;
+    P.h2 = make_h2param(o.perturb_std_dev);
+    P.o.roughThr = o.roughness_threshold;
+    P.o.lightCoord = o.use_light_coordinate_sampling;
+    P.o.chadMisQuirk = 0;
+    P.oDeriv = P.o;
+    P.oDeriv.chadMisQuirk = o.quirk_chad_mis;
+    return P;
+}
+
+extern "C" const char *drmlt_last_error(void) { return g_err.c_str(); }
+
+extern "C" int drmlt_options_default(drmlt_options *o) {
+    if (!o) return fail(DRMLT_ERR_ARG, "null options");
+    std::memset(o, 0, sizeof(*o));
+    o->spp = 16;                    // dptoptions.h:12
+    o->num_init_samples = 100000;   // dptoptions.h:13
+    o->min_depth = -1;
+    o->max_depth = -1;
+    o->direct_spp = 16;
+    o->large_step_prob = 0.3f;
+    o->perturb_std_dev = 0.01f;
+    o->roughness_threshold = 0.03f;
+    o->num_chains = 1024;
+    o->seed_offset = 0;
+    o->use_light_coordinate_sampling = 1;
+    o->large_step_multiplexed = 1;
+    o->discrete_std_dev = 0.01f;
+    o->type = 0;
+    o->aniso_perturb_prob = 0.5f;
+    o->aniso_perturb_std_dev = 0.0025f;
+    o->acceptance_map = 0;
+    o->max_derivatives_depth = 9;   // main.cpp:76
+    o->quirk_ignore_aniso_std_dev = 1;
+    o->quirk_chad_mis = 1;
+    o->seed = 1;
+    return DRMLT_OK;
+}
+
+static void free_list(std::vector<void *> &v) {
+    for (void *p : v)
+        if (p) (void)hipFree(p);
+    v.clear();
+}
+
+extern "C" int drmlt_ctx_destroy(drmlt_ctx *c) {
+    if (!c) return DRMLT_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    free_list(c->sceneAllocs);
+    free_list(c->chainAllocs);
+    for (void *p : {(void *)c->lenFunc, (void *)c->lenCdf, (void *)c->filmInd, (void *)c->filmDir, (void *)c->stats, (void *)c->iterDev})
+        if (p) (void)hipFree(p);
+    if (c->evInit)
+        for (auto &e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options *opt, drmlt_ctx **out) {
+    if (!sc || !opt || !out) return fail(DRMLT_ERR_ARG, "null argument");
+    if (sc->num_triangles <= 0 || sc->num_lights <= 0 || sc->num_materials <= 0) return fail(DRMLT_ERR_ARG, "empty scene");
+    if (opt->max_depth > 12 || opt->max_depth < 3) return fail(DRMLT_ERR_ARG, "max_depth must be in [3, 12]");
+    if (opt->max_derivatives_depth > 12) return fail(DRMLT_ERR_ARG, "max_derivatives_depth must be <= 12");
+    if (!opt->large_step_multiplexed) return fail(DRMLT_ERR_ARG, "only multiplexed large steps are implemented");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DRMLT_ERR_HIP, "no HIP device");
+    drmlt_ctx *c = new drmlt_ctx();
+    c->opt = *opt;
+    *out = nullptr;
+    auto bail = [&](int code) { drmlt_ctx_destroy(c); return code; };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(DRMLT_ERR_HIP, "hipGetDevice"));
+    (void)hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(DRMLT_ERR_HIP, "stream"));
+    const int nt = sc->num_triangles;
+    c->nt = nt;
+    // BVH over the triangles (host build, leaf-order copy for traversal)
+    BvhOut bvh;
+    BuildBvh(nt, sc->tri_p, kBvhStack - 2, bvh);
+    c->bvhDepth = bvh.depth;
+    std::vector<float4> tA(nt), tB(nt), tC(nt), s0(nt), s1(nt), s2(nt);
+    for (int i = 0; i < nt; i++) {
+        const float *p = sc->tri_p + 9 * size_t(i);
+        s0[i] = make_float4(p[0], p[1], p[2], 0.0f);
+        s1[i] = make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], 0.0f);
+        s2[i] = make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], 0.0f);
+    }
+    for (int k = 0; k < nt; k++) {
+        const int i = bvh.order[k];
+        int bits;
+        std::memcpy(&bits, &i, 4);
+        float w;
+        std::memcpy(&w, &bits, 4);
+        tA[k] = make_float4(s0[i].x, s0[i].y, s0[i].z, w);
+        tB[k] = s1[i];
+        tC[k] = s2[i];
+    }
+    // lights: area cdf per light (TriangleMesh::SetAreaLight, trianglemesh.cpp:274-290)
+    const int nl = sc->num_lights;
+    std::vector<float> area(nl), cdfAll, funcAll;
+    std::vector<int> cdfOff(nl);
+    std::vector<float> weights(nl);
+    float wsum = 0;
+    for (int li = 0; li < nl; li++) {
+        const int f = sc->light_tris[2 * li], n = sc->light_tris[2 * li + 1];
+        if (f < 0 || n <= 0 || f + n > nt) return bail(fail(DRMLT_ERR_ARG, "light triangle range"));
+        std::vector<float> a(n);
+        float tot = 0;
+        for (int k = 0; k < n; k++) {
+            const float4 e1 = s1[f + k], e2 = s2[f + k];
+            const float cx = e1.y * e2.z - e1.z * e2.y, cy = e1.z * e2.x - e1.x * e2.z, cz = e1.x * e2.y - e1.y * e2.x;
+            a[k] = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+            tot += a[k];
+        }
+        area[li] = tot;
+        std::vector<float> cdf;
+        float fi;
+        build_cdf(a.data(), n, cdf, fi);
+        cdfOff[li] = int(cdfAll.size());
+        cdfAll.insert(cdfAll.end(), cdf.begin(), cdf.end());
+        funcAll.insert(funcAll.end(), a.begin(), a.end());
+        weights[li] = sc->light_emit[4 * li + 3];
+        wsum += weights[li];
+    }
+    std::vector<float> pickCdf;
+    float pfi;
+    build_cdf(weights.data(), nl, pickCdf, pfi);
+    DevScene &S = c->S;
+    hipStream_t s = c->stream;
+    auto up = [&](auto **p, const auto *src, size_t n) {
+        hipError_t e = upload(p, src, n, s);
+        c->sceneAllocs.push_back((void *)*p);
+        return e;
+    };
+    float4 *d_tA, *d_tB, *d_tC, *d_nodes, *d_s0, *d_s1, *d_s2;
+    float *d_sn, *d_mats, *d_le, *d_area, *d_cdf, *d_func, *d_pick;
+    int *d_mat, *d_light, *d_lt, *d_cdfOff;
+    bool ok = up(&d_tA, tA.data(), nt) == hipSuccess && up(&d_tB, tB.data(), nt) == hipSuccess && up(&d_tC, tC.data(), nt) == hipSuccess &&
+              up(&d_nodes, (const float4 *)bvh.nodes.data(), bvh.nodes.size() / 4) == hipSuccess &&
+              up(&d_s0, s0.data(), nt) == hipSuccess && up(&d_s1, s1.data(), nt) == hipSuccess && up(&d_s2, s2.data(), nt) == hipSuccess &&
+              up(&d_sn, sc->tri_n, size_t(nt) * 9) == hipSuccess && up(&d_mat, sc->tri_mat, nt) == hipSuccess &&
+              up(&d_light, sc->tri_light, nt) == hipSuccess && up(&d_mats, sc->materials, size_t(sc->num_materials) * 16) == hipSuccess &&
+              up(&d_lt, sc->light_tris, size_t(nl) * 2) == hipSuccess && up(&d_le, sc->light_emit, size_t(nl) * 4) == hipSuccess &&
+              up(&d_area, area.data(), nl) == hipSuccess && up(&d_cdf, cdfAll.data(), cdfAll.size()) == hipSuccess &&
+              up(&d_func, funcAll.data(), funcAll.size()) == hipSuccess && up(&d_cdfOff, cdfOff.data(), nl) == hipSuccess &&
+              up(&d_pick, pickCdf.data(), pickCdf.size()) == hipSuccess;
+    if (!ok) return bail(fail(DRMLT_ERR_HIP, "scene upload"));
+    for (int i = 0; i < nt; i++)
+        if (sc->tri_mat[i] < 0 || sc->tri_mat[i] >= sc->num_materials || sc->tri_light[i] >= nl)
+            return bail(fail(DRMLT_ERR_ARG, "triangle material / light index out of range"));
+    S.tA = d_tA; S.tB = d_tB; S.tC = d_tC; S.nodes = d_nodes;
+    S.sp0 = d_s0; S.se1 = d_s1; S.se2 = d_s2; S.sn = d_sn;
+    S.tmat = d_mat; S.tlight = d_light; S.mats = d_mats;
+    S.nt = nt; S.nlights = nl;
+    S.lightTris = d_lt; S.lightEmit = d_le; S.lightArea = d_area;
+    S.lightCdf = d_cdf; S.lightCdfOff = d_cdfOff; S.lightFunc = d_func; S.pickCdf = d_pick;
+    S.lightWeightSum = wsum;
+    S.cam = make_camera(sc->camera);
+    c->W = S.cam.W;
+    c->H = S.cam.H;
+    const size_t filmN = size_t(c->W) * c->H * 3;
+    if (dalloc(&c->filmInd, filmN) != hipSuccess || dalloc(&c->filmDir, filmN) != hipSuccess || dalloc(&c->stats, 8) != hipSuccess ||
+        dalloc(&c->iterDev, 1) != hipSuccess)
+        return bail(fail(DRMLT_ERR_HIP, "film allocation"));
+    (void)hipMemsetAsync(c->filmInd, 0, filmN * 4, s);
+    (void)hipMemsetAsync(c->filmDir, 0, filmN * 4, s);
+    (void)hipMemsetAsync(c->stats, 0, 64, s);
+    (void)hipMemsetAsync(c->iterDev, 0, 4, s);
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
+    c->evInit = true;
+    if (hipStreamSynchronize(s) != hipSuccess) return bail(fail(DRMLT_ERR_HIP, "scene upload sync"));
+    *out = c;
+    return DRMLT_OK;
+}
+
+static int alloc_chains(drmlt_ctx *c, int C) {
+    if (c->haveChains) {
+        if (c->ch.C == C) return DRMLT_OK;
+        free_list(c->chainAllocs);
+        c->haveChains = false;
+    }
+    Chains &ch = c->ch;
+    ch.C = C;
+    ch.NS = 4 * C;
+    ch.GS = 3 * C;
+    const size_t NS = ch.NS, GS = ch.GS;
+    auto A = [&](auto **p, size_t n) {
+        hipError_t e = dalloc(p, n);
+        c->chainAllocs.push_back((void *)*p);
+        return e == hipSuccess;
+    };
+    bool ok = A(&ch.c, NS) && A(&ch.l, NS) && A(&ch.light, NS) && A(&ch.lprim, NS) && A(&ch.screen, 2 * NS) && A(&ch.u, 32 * NS) &&
+              A(&ch.cprim, 16 * NS) && A(&ch.lgprim, 16 * NS) && A(&ch.cdisc, 16 * NS) && A(&ch.ldisc, 16 * NS) && A(&ch.crr, 16 * NS) &&
+              A(&ch.lrr, 16 * NS) && A(&ch.valid, NS) && A(&ch.hasSplat, NS) && A(&ch.gInit, NS) && A(&ch.rgb, 3 * NS) &&
+              A(&ch.ls, NS) && A(&ch.ss, NS) && A(&ch.perm, size_t(C)) && A(&ch.gmean, size_t(kGMax) * GS) &&
+              A(&ch.gcovL, size_t(kGMax * kGMax) * GS) && A(&ch.ginv, size_t(kGMax * kGMax) * GS) && A(&ch.gld, GS) &&
+              A(&ch.off, 32 * size_t(C)) && A(&ch.grad, size_t(kGMax) * C) && A(&ch.hess, size_t(kGMax * kGMax) * C) &&
+              A(&ch.a1, size_t(C)) && A(&ch.a2, size_t(C)) && A(&ch.a2d, size_t(C)) && A(&ch.flags, size_t(C)) &&
+              A(&c->plainScratch, size_t(C) * (kGMax + kGMax * kGMax)) && A(&c->L.count, 3 * kMaxBins) &&
+              A(&c->L.items, size_t(3) * kMaxBins * C);
+    if (!ok) return fail(DRMLT_ERR_HIP, "chain state allocation");
+    hipStream_t s = c->stream;
+    (void)hipMemsetAsync(ch.valid, 0, NS * 4, s);
+    (void)hipMemsetAsync(ch.hasSplat, 0, NS * 4, s);
+    (void)hipMemsetAsync(ch.gInit, 0, NS * 4, s);
+    (void)hipMemsetAsync(ch.crr, 0, 16 * NS * 4, s);
+    (void)hipMemsetAsync(ch.lrr, 0, 16 * NS * 4, s);
+    (void)hipMemsetAsync(c->L.count, 0, 3 * kMaxBins * 4, s);
+    c->haveChains = true;
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    return DRMLT_OK;
+}
+
+static bool recs_ok(const drmlt_ctx *c, const drmlt_path_rec *r, int64_t n);
+static bool recs_fixed_ok(const drmlt_path_rec *r, int64_t n);
+
+static void drop_graph(drmlt_ctx *c) {
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+}
+
+static int set_length_dist(drmlt_ctx *c, const float *lf, int n) {
+    drop_graph(c);
+    std::vector<float> cdf;
+    float fi;
+    build_cdf(lf, n, cdf, fi);
+    if (c->lenFunc) (void)hipFree(c->lenFunc);
+    if (c->lenCdf) (void)hipFree(c->lenCdf);
+    c->lenFunc = c->lenCdf = nullptr;
+    HIPCHK(upload(&c->lenFunc, lf, n, c->stream));
+    HIPCHK(upload(&c->lenCdf, cdf.data(), cdf.size(), c->stream));
+    c->lenFuncInt = fi;
+    c->nLen = n;
+    return DRMLT_OK;
+}
+
+// Host-side mirror of the seed records into chain slot 0 (structure of arrays).
+extern "C" int drmlt_seed_chains(drmlt_ctx *c, const drmlt_path_rec *st, const float *c8, int64_t n, float b, const float *lf, int32_t nLen) {
+    if (!c || !st || !c8 || !lf || n <= 0 || nLen <= 0 || nLen > kMaxBins) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!recs_ok(c, st, n) || !recs_fixed_ok(st, n)) return fail(DRMLT_ERR_ARG, "seed record out of range");
+    for (int64_t i = 0; i < n; i++)
+        if (st[i].c + st[i].l - 1 > c->opt.max_depth) return fail(DRMLT_ERR_ARG, "seed path longer than max_depth");
+    int r = alloc_chains(c, int(n));
+    if (r) return r;
+    Chains &ch = c->ch;
+    const size_t C = size_t(n), NS = ch.NS;
+    std::vector<int> ci(NS, 0), li(NS, 0), lig(NS, -1), lpr(NS, -1), valid(NS, 0), perm(C);
+    std::vector<float> scr(2 * NS, 0), u(32 * NS, 0), cd(16 * NS, 0), ld(16 * NS, 0), crr(16 * NS, 1), lrr(16 * NS, 1), rgb(3 * NS, 0),
+        ls(NS, 0), ss(NS, 0);
+    std::vector<int> cp(16 * NS, -1), lp(16 * NS, -1);
+    for (size_t i = 0; i < C; i++) {
+        const drmlt_path_rec &R = st[i];
+        for (int k = 0; k < 4; k++) { ci[i + k * C] = R.c; li[i + k * C] = R.l; }
+        lig[i] = R.light; lpr[i] = R.lprim;
+        scr[i] = c8[8 * i + 6]; scr[i + NS] = c8[8 * i + 7];
+        for (int k = 0; k < 32; k++) u[k * NS + i] = R.u[k];
+        for (int k = 0; k < 16; k++) {
+            cp[k * NS + i] = R.cprim[k]; lp[k * NS + i] = R.lgprim[k];
+            cd[k * NS + i] = R.cdisc[k]; ld[k * NS + i] = R.ldisc[k];
+            crr[k * NS + i] = R.crr[k]; lrr[k * NS + i] = R.lrr[k];
+        }
+        valid[i] = 1;
+        rgb[i] = c8[8 * i + 1]; rgb[i + NS] = c8[8 * i + 2]; rgb[i + 2 * NS] = c8[8 * i + 3];
+        ls[i] = c8[8 * i + 4]; ss[i] = c8[8 * i + 5];
+        perm[i] = 0 | (1 << 2) | (2 << 4);
+    }
+    hipStream_t s = c->stream;
+    auto put = [&](auto *d, const auto &v) { return hipMemcpyAsync(d, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, s); };
+    HIPCHK(put(ch.c, ci)); HIPCHK(put(ch.l, li)); HIPCHK(put(ch.light, lig)); HIPCHK(put(ch.lprim, lpr));
+    HIPCHK(put(ch.screen, scr)); HIPCHK(put(ch.u, u)); HIPCHK(put(ch.cprim, cp)); HIPCHK(put(ch.lgprim, lp));
+    HIPCHK(put(ch.cdisc, cd)); HIPCHK(put(ch.ldisc, ld)); HIPCHK(put(ch.crr, crr)); HIPCHK(put(ch.lrr, lrr));
+    HIPCHK(put(ch.valid, valid)); HIPCHK(put(ch.rgb, rgb)); HIPCHK(put(ch.ls, ls)); HIPCHK(put(ch.ss, ss)); HIPCHK(put(ch.perm, perm));
+    HIPCHK(hipMemsetAsync(ch.hasSplat, 0, NS * 4, s));
+    HIPCHK(hipMemsetAsync(ch.gInit, 0, NS * 4, s));
+    c->b = b;
+    r = set_length_dist(c, lf, nLen);
+    if (r) return r;
+    HIPCHK(hipMemsetAsync(c->filmInd, 0, size_t(c->W) * c->H * 3 * 4, s));
+    HIPCHK(hipMemsetAsync(c->stats, 0, 64, s));
+    c->itersDone = 0;
+    HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_prepass(drmlt_ctx *c, double *bOut, float *lfOut, int32_t nLenOut) {
+    if (!c) return fail(DRMLT_ERR_ARG, "null context");
+    const int N = c->opt.num_init_samples, C = c->opt.num_chains, maxDepth = c->opt.max_depth;
+    if (N <= 0 || C <= 0) return fail(DRMLT_ERR_ARG, "num_init_samples and num_chains must be positive");
+    hipStream_t s = c->stream;
+    int *cnt, *cl;
+    float *ls;
+    double *lenSum, *total;
+    HIPCHK(dalloc(&cnt, N)); HIPCHK(dalloc(&cl, size_t(N) * kPreMaxC)); HIPCHK(dalloc(&ls, size_t(N) * kPreMaxC));
+    HIPCHK(dalloc(&lenSum, kMaxBins)); HIPCHK(dalloc(&total, 1));
+    HIPCHK(hipMemsetAsync(lenSum, 0, kMaxBins * 8, s));
+    HIPCHK(hipMemsetAsync(total, 0, 8, s));
+    DrParams P = make_params(c);
+    hipLaunchKernelGGL(k_prepass_gen, dim3((N + 63) / 64), dim3(64), 0, s, c->S, P.o, N, maxDepth, P.seed ^ 0x5151515151ull, cnt, cl, ls,
+                       lenSum, total);
+    HIPCHK(hipGetLastError());
+    std::vector<int> hcnt(N), hcl(size_t(N) * kPreMaxC);
+    std::vector<float> hls(size_t(N) * kPreMaxC);
+    std::vector<double> hlen(kMaxBins);
+    double htot = 0;
+    HIPCHK(hipMemcpyAsync(hcnt.data(), cnt, N * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hcl.data(), cl, hcl.size() * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hls.data(), ls, hls.size() * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hlen.data(), lenSum, kMaxBins * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&htot, total, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    (void)hipFree(cnt); (void)hipFree(cl); (void)hipFree(ls); (void)hipFree(lenSum); (void)hipFree(total);
+    // candidates in sample order, equal-spaced seeding (Veach's thesis p. 340, delayed.cpp:123-167)
+    std::vector<int> candSample, candCl;
+    std::vector<double> cdf(1, 0.0);
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < hcnt[i]; k++) {
+            candSample.push_back(i);
+            candCl.push_back(hcl[size_t(i) * kPreMaxC + k]);
+            cdf.push_back(cdf.back() + double(hls[size_t(i) * kPreMaxC + k]));
+        }
+    const int ncand = int(candSample.size());
+    if (ncand < 2) return fail(DRMLT_ERR_STATE, "pre-pass found fewer than two contributing paths");
+    const int nLen = maxDepth + 1;
+    std::vector<float> lf(nLen, 0.0f);
+    for (int k = 0; k < nLen && k < kMaxBins; k++) lf[k] = float(hlen[k]);
+    const double interval = cdf.back() / double(C);
+    std::mt19937_64 rng{uint64_t(ncand)};
+    double pos = std::uniform_real_distribution<double>(0.0, interval)(rng);
+    int cdfPos = 0;
+    std::vector<int> selS(C), selCl(C);
+    for (int i = 0; i < C; i++) {
+        while (pos > cdf[cdfPos]) cdfPos = std::min(cdfPos + 1, ncand - 1);
+        const int k = std::max(cdfPos - 1, 0);
+        selS[i] = candSample[k];
+        selCl[i] = candCl[k];
+        pos += interval;
+    }
+    int r = alloc_chains(c, C);
+    if (r) return r;
+    int *dS, *dCl;
+    HIPCHK(upload(&dS, selS.data(), C, s));
+    HIPCHK(upload(&dCl, selCl.data(), C, s));
+    hipLaunchKernelGGL(k_prepass_seed, dim3((C + 63) / 64), dim3(64), 0, s, c->S, P.o, c->ch, maxDepth, P.seed ^ 0x5151515151ull, dS, dCl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    (void)hipFree(dS);
+    (void)hipFree(dCl);
+    c->b = float(htot / double(N));
+    r = set_length_dist(c, lf.data(), nLen);
+    if (r) return r;
+    HIPCHK(hipMemsetAsync(c->filmInd, 0, size_t(c->W) * c->H * 3 * 4, s));
+    HIPCHK(hipMemsetAsync(c->stats, 0, 64, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->itersDone = 0;
+    if (bOut) *bOut = htot / double(N);
+    if (lfOut)
+        for (int k = 0; k < nLenOut; k++) lfOut[k] = k < nLen ? lf[k] : 0.0f;
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_direct_lighting(drmlt_ctx *c) {
+    if (!c) return fail(DRMLT_ERR_ARG, "null context");
+    const int spp = c->opt.direct_spp;
+    hipStream_t s = c->stream;
+    HIPCHK(hipMemsetAsync(c->filmDir, 0, size_t(c->W) * c->H * 3 * 4, s));
+    if (spp > 0 && c->opt.min_depth <= 2) {
+        const long n = long(c->W) * c->H * spp;
+        DrParams P = make_params(c);
+        hipLaunchKernelGGL(k_direct, dim3(unsigned((n + 127) / 128)), dim3(128), 0, s, c->S, P.o, spp, P.seed ^ 0xD1EC7ull, c->filmDir);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    c->directDone = true;
+    return DRMLT_OK;
+}
+
+static int persistent_blocks(const drmlt_ctx *c) { return c->cus * 16; }
+
+static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed) {
+    hipStream_t s = c->stream;
+    const int C = c->ch.C;
+    const int pb = persistent_blocks(c);
+    const int g1 = std::min((C + 63) / 64, pb);
+    if (timed) (void)hipEventRecord(c->ev[0], s);
+    hipLaunchKernelGGL(k_b_stage1, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    if (timed) (void)hipEventRecord(c->ev[1], s);
+    hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
+    if (timed) (void)hipEventRecord(c->ev[2], s);
+    hipLaunchKernelGGL(k_b_gauss, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0);
+    if (timed) (void)hipEventRecord(c->ev[3], s);
+    hipLaunchKernelGGL(k_b_stage2_propose, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
+    if (timed) (void)hipEventRecord(c->ev[4], s);
+    hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
+    if (timed) (void)hipEventRecord(c->ev[5], s);
+    hipLaunchKernelGGL(k_b_gauss, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1);
+    if (timed) (void)hipEventRecord(c->ev[6], s);
+    hipLaunchKernelGGL(k_b_stage2_accept, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    if (timed) (void)hipEventRecord(c->ev[7], s);
+    hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
+    if (timed) (void)hipEventRecord(c->ev[8], s);
+}
+
+extern "C" int drmlt_run(drmlt_ctx *c, int64_t iters, int32_t mode) {
+    if (!c || iters < 0) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!c->haveChains || !c->lenCdf) return fail(DRMLT_ERR_STATE, "chains are not seeded (call drmlt_prepass or drmlt_seed_chains)");
+    if (mode < 0 || mode > 2) return fail(DRMLT_ERR_ARG, "unknown mode");
+    hipStream_t s = c->stream;
+    const DrParams P = make_params(c);
+    const int C = c->ch.C;
+    for (double &t : c->timing) t = 0;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    // the device iteration counter follows the host one (plain form reads the host one)
+    HIPCHK(hipMemcpyAsync(c->iterDev, &c->iterHost, 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(c->L.count, 0, 3 * kMaxBins * 4, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipEventRecord(e0, s));
+    if (mode == DRMLT_PLAIN) {
+        for (int64_t it = 0; it < iters; it++) {
+            hipLaunchKernelGGL(k_dr_plain, dim3((C + 63) / 64), dim3(64), 0, s, c->S, c->ch, P, c->iterHost + uint32_t(it), c->filmInd,
+                               c->stats, c->plainScratch);
+        }
+        HIPCHK(hipGetLastError());
+    } else if (mode == DRMLT_BATCHED) {
+        for (int64_t it = 0; it < iters; it++) {
+            launch_batched_step(c, P, true);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventSynchronize(c->ev[8]));
+            for (int k = 0; k < 7; k++) {
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]);
+                c->timing[1 + k] += ms;
+            }
+        }
+    } else {
+        if (!c->gexec) {
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            launch_batched_step(c, P, false);
+            HIPCHK(hipStreamEndCapture(s, &c->graph));
+            HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        }
+        for (int64_t it = 0; it < iters; it++) HIPCHK(hipGraphLaunch(c->gexec, s));
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    c->timing[0] = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    c->iterHost += uint32_t(iters);
+    c->itersDone += iters;
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_read_timing(drmlt_ctx *c, double *out, int32_t n) {
+    if (!c || !out) return fail(DRMLT_ERR_ARG, "bad argument");
+    for (int k = 0; k < n && k < 8; k++) out[k] = c->timing[k];
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_read_film(drmlt_ctx *c, float *rgb, int32_t w, int32_t h, int32_t which) {
+    if (!c || !rgb || w != c->W || h != c->H) return fail(DRMLT_ERR_ARG, "film size mismatch");
+    const size_t n = size_t(w) * h * 3;
+    std::vector<float> ind(n), dir(n);
+    HIPCHK(hipMemcpyAsync(ind.data(), c->filmInd, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(dir.data(), c->filmDir, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (which == 1) { std::memcpy(rgb, ind.data(), n * 4); return DRMLT_OK; }
+    if (which == 2) { std::memcpy(rgb, dir.data(), n * 4); return DRMLT_OK; }
+    const double sppDone = c->haveChains ? double(c->itersDone) * double(c->ch.C) / (double(w) * h) : 0.0;
+    const float dw = c->opt.direct_spp > 0 ? 1.0f / float(c->opt.direct_spp) : 0.0f;
+    const float iw = sppDone > 0 ? float(1.0 / sppDone) : 0.0f;
+    for (size_t i = 0; i < n; i++) rgb[i] = dw * dir[i] + iw * ind[i];
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_read_stats(drmlt_ctx *c, drmlt_stats *st) {
+    if (!c || !st) return fail(DRMLT_ERR_ARG, "bad argument");
+    unsigned long long v[8];
+    HIPCHK(hipMemcpyAsync(v, c->stats, 64, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    st->large_step = v[0]; st->large_step_accepted = v[1];
+    st->stage1 = v[2]; st->stage1_accepted = v[3];
+    st->stage2_iso = v[4]; st->stage2_iso_accepted = v[5];
+    st->stage2_h2mc = v[6]; st->stage2_h2mc_accepted = v[7];
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_count_stage1_work(drmlt_ctx *, uint64_t *counts3) {
+    if (counts3) counts3[0] = counts3[1] = counts3[2] = 0;
+    return fail(DRMLT_ERR_STATE, "traversal counters are not built into this library");
+}
+
+// ---------------------------------------------------------------- tier 2
+template <class T> struct DBuf {
+    T *p = nullptr;
+    ~DBuf() { if (p) (void)hipFree(p); }
+};
+#define DUP(buf, src, n) HIPCHK(upload(&buf.p, src, size_t(n), s))
+#define DNEW(buf, n) HIPCHK(dalloc(&buf.p, size_t(n)))
+#define DGET(dst, buf, n) HIPCHK(hipMemcpyAsync(dst, buf.p, size_t(n) * sizeof(*buf.p), hipMemcpyDeviceToHost, s))
+
+extern "C" int drmlt_trace(drmlt_ctx *c, int64_t n, const float *org, const float *dir, const float *tmin, const float *tmax, int32_t any,
+                           int32_t *prim, float *t) {
+    if (!c || n < 0 || !org || !dir || !tmin || !tmax || !prim || (!any && !t)) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (n == 0) return DRMLT_OK;
+    hipStream_t s = c->stream;
+    DBuf<float> o, d, a, b, tt;
+    DBuf<int> pr;
+    DUP(o, org, 3 * n); DUP(d, dir, 3 * n); DUP(a, tmin, n); DUP(b, tmax, n); DNEW(pr, n); DNEW(tt, n);
+    hipLaunchKernelGGL(k_trace, dim3(unsigned((n + 127) / 128)), dim3(128), 0, s, c->S, int(n), o.p, d.p, a.p, b.p, any, pr.p, tt.p);
+    HIPCHK(hipGetLastError());
+    DGET(prim, pr, n);
+    if (!any) DGET(t, tt, n);
+    HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}
+
+// Validates path records before any kernel reads them: vertex counts within
+// the fixed slots, light and triangle indices within the scene.
+static bool recs_ok(const drmlt_ctx *c, const drmlt_path_rec *r, int64_t n) {
+    for (int64_t i = 0; i < n; i++) {
+        const drmlt_path_rec &R = r[i];
+        const int cc = R.c, ll = R.l;
+        if (cc < 1 || ll < 0 || cc - 1 > 16 || ll - 1 > 16 || cc + ll - 1 < 2 || 2 * std::max(cc + ll - 1, 2) > 32) return false;
+        if (cc == 1 && ll < 2) return false;
+        if (ll >= 1 && (R.light < 0 || R.light >= c->S.nlights || R.lprim < 0 || R.lprim >= c->nt)) return false;
+        for (int k = 0; k < cc - 1; k++)
+            if (R.cprim[k] < -1 || R.cprim[k] >= c->nt) return false;
+        for (int k = 0; k < ll - 1; k++)
+            if (R.lgprim[k] < -1 || R.lgprim[k] >= c->nt) return false;
+    }
+    return true;
+}
+// For fixed-topology evaluation every vertex must name a triangle.
+static bool recs_fixed_ok(const drmlt_path_rec *r, int64_t n) {
+    for (int64_t i = 0; i < n; i++) {
+        for (int k = 0; k < r[i].c - 1; k++)
+            if (r[i].cprim[k] < 0) return false;
+        for (int k = 0; k < r[i].l - 1; k++)
+            if (r[i].lgprim[k] < 0) return false;
+    }
+    return true;
+}
+
+extern "C" int drmlt_perturb(drmlt_ctx *c, int64_t n, drmlt_path_rec *st, const float *off, const float *disc, float *c8) {
+    if (!c || n < 0 || !st || !off || !disc || !c8) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!recs_ok(c, st, n)) return fail(DRMLT_ERR_ARG, "path record out of range");
+    if (n == 0) return DRMLT_OK;
+    hipStream_t s = c->stream;
+    DBuf<char> r;
+    DBuf<float> o, d, k8;
+    DUP(r, (const char *)st, n * sizeof(drmlt_path_rec)); DUP(o, off, 32 * n); DUP(d, disc, 32 * n); DNEW(k8, 8 * n);
+    const DrParams P = make_params(c);
+    hipLaunchKernelGGL(k_perturb_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.o, int(n), r.p, o.p, d.p, k8.p);
+    HIPCHK(hipGetLastError());
+    DGET((char *)st, r, n * sizeof(drmlt_path_rec));
+    DGET(c8, k8, 8 * n);
+    HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_generate_subpath(drmlt_ctx *c, int64_t n, const int32_t *cl, const float *uni, int32_t nuni, drmlt_path_rec *st,
+                                      float *c8) {
+    if (!c || n < 0 || !cl || !uni || nuni <= 0 || !st || !c8) return fail(DRMLT_ERR_ARG, "bad argument");
+    for (int64_t i = 0; i < n; i++) {
+        const int cc = cl[2 * i], ll = cl[2 * i + 1];
+        if (cc < 1 || ll < 0 || cc + ll - 1 < 2 || cc + ll - 1 > 15 || cc - 1 > 16 || ll - 1 > 16 || (cc == 1 && ll < 2))
+            return fail(DRMLT_ERR_ARG, "strategy out of range");
+    }
+    if (n == 0) return DRMLT_OK;
+    hipStream_t s = c->stream;
+    DBuf<char> r;
+    DBuf<int> k;
+    DBuf<float> u, k8;
+    DNEW(r, n * sizeof(drmlt_path_rec)); DUP(k, cl, 2 * n); DUP(u, uni, size_t(n) * nuni); DNEW(k8, 8 * n);
+    const DrParams P = make_params(c);
+    hipLaunchKernelGGL(k_gensub_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.o, int(n), k.p, u.p, nuni, r.p, k8.p);
+    HIPCHK(hipGetLastError());
+    DGET((char *)st, r, n * sizeof(drmlt_path_rec));
+    DGET(c8, k8, 8 * n);
+    HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_generate_bidir(drmlt_ctx *c, int64_t n, int32_t minDepth, int32_t maxDepth, const float *uni, int32_t nuni, int32_t maxc,
+                                    float *cs, int32_t *nc) {
+    if (!c || n < 0 || !uni || nuni <= 0 || maxc <= 0 || !cs || !nc) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (maxDepth > 15 || maxDepth < 1) return fail(DRMLT_ERR_ARG, "max_depth out of range");
+    if (n == 0) return DRMLT_OK;
+    hipStream_t s = c->stream;
+    DBuf<float> u, o;
+    DBuf<int> k;
+    DUP(u, uni, size_t(n) * nuni); DNEW(o, size_t(n) * maxc * 10); DNEW(k, n);
+    const DrParams P = make_params(c);
+    hipLaunchKernelGGL(k_genbidir_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.o, int(n), minDepth, maxDepth, u.p, nuni, maxc,
+                       o.p, k.p);
+    HIPCHK(hipGetLastError());
+    DGET(cs, o, size_t(n) * maxc * 10);
+    DGET(nc, k, n);
+    HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}
+
+static int jet_passes_host(int D) {
+    const int nch = (D + kJetK - 1) / kJetK;
+    int np = 0;
+    for (int i = 0; i < D; i++) np += nch - i / kJetK;
+    return np;
+}
+
+extern "C" int drmlt_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st, int32_t fp64, double *grad, double *hess, int32_t *okOut) {
+    if (!c || n < 0 || !st || !grad || !hess || !okOut) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!recs_ok(c, st, n) || !recs_fixed_ok(st, n)) return fail(DRMLT_ERR_ARG, "path record out of range");
+    if (n == 0) return DRMLT_OK;
+    std::vector<int> passOff(n + 1, 0);
+    for (int64_t i = 0; i < n; i++) {
+        const int D = 2 * std::max(st[i].c + st[i].l - 1, 2);
+        if (D > kGMax) return fail(DRMLT_ERR_ARG, "derivatives are provided up to D = 24");
+        passOff[i + 1] = passOff[i] + jet_passes_host(D);
+    }
+    const int total = passOff[n];
+    hipStream_t s = c->stream;
+    DBuf<char> r;
+    DBuf<int> po, ok;
+    DUP(r, (const char *)st, n * sizeof(drmlt_path_rec)); DUP(po, passOff.data(), n + 1);
+    std::vector<int> ones(n, 1);
+    DUP(ok, ones.data(), n);
+    const DrParams P = make_params(c);
+    if (fp64) {
+        DBuf<double> g, h;
+        DNEW(g, 32 * n); DNEW(h, 1024 * n);
+        HIPCHK(hipMemsetAsync(g.p, 0, 32 * n * 8, s));
+        HIPCHK(hipMemsetAsync(h.p, 0, 1024 * n * 8, s));
+        hipLaunchKernelGGL(k_gradhess_rec_f64, dim3(unsigned((total + 63) / 64)), dim3(64), 0, s, c->S, P.oDeriv, int(n), r.p, po.p,
+                           total, g.p, h.p, ok.p);
+        HIPCHK(hipGetLastError());
+        DGET(grad, g, 32 * n);
+        DGET(hess, h, 1024 * n);
+    } else {
+        DBuf<float> g, h;
+        DNEW(g, 32 * n); DNEW(h, 1024 * n);
+        HIPCHK(hipMemsetAsync(g.p, 0, 32 * n * 4, s));
+        HIPCHK(hipMemsetAsync(h.p, 0, 1024 * n * 4, s));
+        hipLaunchKernelGGL(k_gradhess_rec_f32, dim3(unsigned((total + 63) / 64)), dim3(64), 0, s, c->S, P.oDeriv, int(n), r.p, po.p,
+                           total, g.p, h.p, ok.p);
+        HIPCHK(hipGetLastError());
+        std::vector<float> hg(32 * n), hh(1024 * n);
+        DGET(hg.data(), g, 32 * n);
+        DGET(hh.data(), h, 1024 * n);
+        HIPCHK(hipStreamSynchronize(s));
+        for (size_t i = 0; i < hg.size(); i++) grad[i] = hg[i];
+        for (size_t i = 0; i < hh.size(); i++) hess[i] = hh[i];
+    }
+    DGET(okOut, ok, n);
+    HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_compute_gaussian_f64(int64_t n, const int32_t *D, const double *sc, double sigma, const double *grad, const double *hess,
+                                          double *mean, double *covL, double *invCov, double *logDet) {
+    if (n < 0 || !D || !sc || !grad || !hess || !mean || !covL || !invCov || !logDet) return fail(DRMLT_ERR_ARG, "bad argument");
+    for (int64_t i = 0; i < n; i++)
+        if (D[i] < 1 || D[i] > kGMax) return fail(DRMLT_ERR_ARG, "D out of range");
+    if (n == 0) return DRMLT_OK;
+    hipStream_t s = 0;
+    DBuf<int> d;
+    DBuf<double> a, g, h, m, L, Q, ld;
+    DUP(d, D, n); DUP(a, sc, n); DUP(g, grad, 32 * n); DUP(h, hess, 1024 * n);
+    DNEW(m, size_t(kGMax) * n); DNEW(L, size_t(kGMax * kGMax) * n); DNEW(Q, size_t(kGMax * kGMax) * n); DNEW(ld, n);
+    hipLaunchKernelGGL(k_gauss_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, int(n), make_h2param(sigma), d.p, a.p, g.p, h.p, m.p, L.p,
+                       Q.p, ld.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hm(size_t(kGMax) * n), hL(size_t(kGMax * kGMax) * n), hQ(size_t(kGMax * kGMax) * n);
+    DGET(hm.data(), m, hm.size()); DGET(hL.data(), L, hL.size()); DGET(hQ.data(), Q, hQ.size()); DGET(logDet, ld, n);
+    HIPCHK(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; i++) {
+        for (int e = 0; e < 32; e++) mean[32 * i + e] = e < kGMax ? hm[size_t(e) * n + i] : 0.0;
+        for (int e = 0; e < kGMax * kGMax; e++) {
+            covL[size_t(kGMax * kGMax) * i + e] = hL[size_t(e) * n + i];
+            invCov[size_t(kGMax * kGMax) * i + e] = hQ[size_t(e) * n + i];
+        }
+    }
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_gaussian_logpdf_f64(int64_t n, const int32_t *D, const double *x, double sign, const double *mean, const double *covL,
+                                         const double *invCov, const double *logDet, double *out) {
+    if (n < 0 || !D || !x || !mean || !covL || !invCov || !logDet || !out) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (n == 0) return DRMLT_OK;
+    for (int64_t i = 0; i < n; i++)
+        if (D[i] < 1 || D[i] > kGMax) return fail(DRMLT_ERR_ARG, "D out of range");
+    // back to structure-of-arrays
+    std::vector<double> m(size_t(kGMax) * n), L(size_t(kGMax * kGMax) * n), Q(size_t(kGMax * kGMax) * n);
+    for (int64_t i = 0; i < n; i++) {
+        for (int e = 0; e < kGMax; e++) m[size_t(e) * n + i] = mean[32 * i + e];
+        for (int e = 0; e < kGMax * kGMax; e++) {
+            L[size_t(e) * n + i] = covL[size_t(kGMax * kGMax) * i + e];
+            Q[size_t(e) * n + i] = invCov[size_t(kGMax * kGMax) * i + e];
+        }
+    }
+    hipStream_t s = 0;
+    DBuf<int> d;
+    DBuf<double> xx, dm, dL, dQ, dl, o;
+    DUP(d, D, n); DUP(xx, x, 32 * n); DUP(dm, m.data(), m.size()); DUP(dL, L.data(), L.size()); DUP(dQ, Q.data(), Q.size());
+    DUP(dl, logDet, n); DNEW(o, n);
+    hipLaunchKernelGGL(k_logpdf_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, int(n), d.p, xx.p, sign, dm.p, dL.p, dQ.p, dl.p, o.p);
+    HIPCHK(hipGetLastError());
+    DGET(out, o, n);
+    HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_dr_alpha2_f64(int64_t n, const double *in6, double *a2) {
+    if (n < 0 || !in6 || !a2) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (n == 0) return DRMLT_OK;
+    hipStream_t s = 0;
+    DBuf<double> i, o;
+    DUP(i, in6, 6 * n); DNEW(o, n);
+    hipLaunchKernelGGL(k_alpha2, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, int(n), i.p, o.p);
+    HIPCHK(hipGetLastError());
+    DGET(a2, o, n);
+    HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}

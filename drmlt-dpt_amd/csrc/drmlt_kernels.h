@@ -1,0 +1,73 @@
+// Kernel-side data shared by drmlt_kernels.hip and the C ABI: chain state in
+// HBM and the per-step work lists of the batched DR pipeline.
+#pragma once
+#include "dev_path.h"
+#include "dev_gauss.h"
+
+namespace drmlt {
+
+constexpr int kJetK = 8;     // Hessian columns per derivative pass
+constexpr int kMaxBins = 16; // path-length bins (path length 0..15)
+
+// Chain state: 4 physical slots per chain (current, first- and second-stage
+// proposal, Green's y*), all fields structure-of-arrays with stride NS = 4C.
+// perm[chain] packs which of the first three physical slots plays current /
+// first / second (2 bits each), so accepting a proposal is an index swap.
+struct Chains {
+    int C, NS, GS;  // chains, state slots (4C), Gaussian slots (3C)
+    int *c, *l, *light, *lprim;
+    float *screen, *u;
+    int *cprim, *lgprim;
+    float *cdisc, *ldisc, *crr, *lrr;
+    int *valid, *hasSplat, *gInit;
+    float *rgb, *ls, *ss;  // rgb [3][NS]
+    int *perm;
+    double *gmean, *gcovL, *ginv, *gld;
+    // per chain scratch for the batched pipeline
+    float *off;            // [32][C] H2MC offset
+    float *grad, *hess;    // [24][C], [24*24][C] derivative output
+    float *a1, *a2;        // stage acceptances
+    double *a2d;
+    int *flags;            // bit0 acc1, bit1 doSecond, bit2 isAniso, bit3 large, bit4 z valid
+};
+
+DEV SRef slot_ref(const Chains &ch, int p) {
+    SRef r;
+    r.c = ch.c + p; r.l = ch.l + p; r.light = ch.light + p; r.lprim = ch.lprim + p;
+    r.screen = ch.screen + p; r.u = ch.u + p;
+    r.cprim = ch.cprim + p; r.lgprim = ch.lgprim + p;
+    r.cdisc = ch.cdisc + p; r.ldisc = ch.ldisc + p; r.crr = ch.crr + p; r.lrr = ch.lrr + p;
+    r.s = ch.NS;
+    return r;
+}
+DEV GRef g_ref(const Chains &ch, int p) {
+    GRef g;
+    g.mean = ch.gmean; g.covL = ch.gcovL; g.invCov = ch.ginv; g.logDet = ch.gld;
+    g.gs = ch.GS;
+    g.k = p;
+    return g;
+}
+
+// DR options the kernels read (src/dptoptions.h:8-37)
+struct DrParams {
+    float largeStepProb, perturbStdDev, anisoPerturbProb, anisoStdDev, discreteStdDev;
+    int type;  // 0 green, 1 mira
+    int maxDervDepth;
+    int acceptanceMap;
+    float b;   // normalisation (average brightness from the pre-pass)
+    int nLen;  // length distribution size
+    const float *lenFunc, *lenCdf;
+    float lenFuncInt;
+    uint64_t seed;
+    H2Param h2;
+    Opts o;
+    Opts oDeriv;  // options for the derivative evaluation (chad MIS quirk)
+};
+
+// Work lists of the batched pipeline, each binned by path length.
+struct Lists {
+    int *count;  // [kMaxBins] per list kind
+    int *items;  // [kMaxBins][C]
+};
+
+}  // namespace drmlt

@@ -1,0 +1,31 @@
+// FP64 derivative kernel of the parity surface (the FP64 check of the FP32 jets).
+#include "dr_stages.h"
+
+namespace drmlt {
+
+// Gradient / Hessian of log f at each record, one thread per (record, pass).
+template <class S>
+DEV void gradhess_rec(int w, const DevScene &Sc, const Opts &o, int n, char *recs, const int *passOff, int totalPasses, S *grad, S *hess, int *okOut) {
+    if (w >= totalPasses) return;
+    int lo = 0, hi = n;  // record with passOff[r] <= w < passOff[r+1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (passOff[mid] <= w) lo = mid; else hi = mid;
+    }
+    const int r = lo, q = w - passOff[r];
+    const SRef ref = rec_ref(recs + size_t(r) * kRecBytes);
+    const int D = dim_of(*ref.c, *ref.l);
+    int row, col0;
+    jet_pass_index(D, q, row, col0);
+    const bool ok = jet_pass<S>(Sc, o, ref, D, row, col0,
+                                [&](int cc, S gv) { if (row == 0) grad[size_t(r) * 32 + cc] = gv; },
+                                [&](int cc, S hv) { hess[(size_t(r) * 32 + row) * 32 + cc] = hv; });
+    if (!ok) okOut[r] = 0;
+}
+
+__global__ void k_gradhess_rec_f64(DevScene Sc, Opts o, int n, char *recs, const int *passOff, int totalPasses, double *grad,
+                                   double *hess, int *okOut) {
+    gradhess_rec<double>(blockIdx.x * blockDim.x + threadIdx.x, Sc, o, n, recs, passOff, totalPasses, grad, hess, okOut);
+}
+
+}  // namespace drmlt

@@ -1,0 +1,26 @@
+// Kernel declarations for the host translation unit (definitions in k_*.hip).
+#pragma once
+#include "drmlt_kernels.h"
+
+namespace drmlt {
+constexpr int kPreMaxC = 64;  // pre-pass contributions kept per sample (k_loop.hip)
+__global__ void k_dr_plain(DevScene, Chains, DrParams, uint32_t, float *, unsigned long long *, float *);
+__global__ void k_b_stage1(DevScene, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
+__global__ void k_b_deriv(DevScene, Chains, DrParams, Lists, int, int);
+__global__ void k_b_gauss(DevScene, Chains, DrParams, Lists, int, int, int);
+__global__ void k_b_stage2_propose(DevScene, Chains, DrParams, const uint32_t *, Lists);
+__global__ void k_b_stage2_accept(DevScene, Chains, DrParams, uint32_t *, float *, unsigned long long *, Lists);
+__global__ void k_b_reset(Lists, uint32_t *, int);
+__global__ void k_prepass_gen(DevScene, Opts, int, int, uint64_t, int *, int *, float *, double *, double *);
+__global__ void k_prepass_seed(DevScene, Opts, Chains, int, uint64_t, const int *, const int *);
+__global__ void k_direct(DevScene, Opts, int, uint64_t, float *);
+__global__ void k_trace(DevScene, int, const float *, const float *, const float *, const float *, int, int *, float *);
+__global__ void k_perturb_rec(DevScene, Opts, int, char *, const float *, const float *, float *);
+__global__ void k_gensub_rec(DevScene, Opts, int, const int *, const float *, int, char *, float *);
+__global__ void k_genbidir_rec(DevScene, Opts, int, int, int, const float *, int, int, float *, int *);
+__global__ void k_gradhess_rec_f32(DevScene, Opts, int, char *, const int *, int, float *, float *, int *);
+__global__ void k_gradhess_rec_f64(DevScene, Opts, int, char *, const int *, int, double *, double *, int *);
+__global__ void k_gauss_rec(int, H2Param, const int *, const double *, const double *, const double *, double *, double *, double *, double *);
+__global__ void k_logpdf_rec(int, const int *, const double *, double, const double *, const double *, const double *, const double *, double *);
+__global__ void k_alpha2(int, const double *, double *);
+}  // namespace drmlt

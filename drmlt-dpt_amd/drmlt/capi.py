@@ -1,0 +1,244 @@
+"""ctypes binding of libdrmlt_hip.so (include/drmlt_amd.h).
+
+The library is the product: every call here goes to the HIP implementation.
+Loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libdrmlt_hip.so")
+
+EXPORTS = [
+    "drmlt_last_error", "drmlt_options_default", "drmlt_ctx_create", "drmlt_ctx_destroy", "drmlt_prepass",
+    "drmlt_seed_chains", "drmlt_direct_lighting", "drmlt_run", "drmlt_read_film", "drmlt_read_stats", "drmlt_read_timing",
+    "drmlt_count_stage1_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
+    "drmlt_grad_hess", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
+]
+
+PLAIN, BATCHED, BATCHED_GRAPH = 0, 1, 2
+
+PATH_REC = np.dtype([
+    ("c", np.int32), ("l", np.int32), ("light", np.int32), ("lprim", np.int32),
+    ("screen", np.float32, 2), ("u", np.float32, 32),
+    ("cprim", np.int32, 16), ("lgprim", np.int32, 16),
+    ("cdisc", np.float32, 16), ("ldisc", np.float32, 16),
+    ("crr", np.float32, 16), ("lrr", np.float32, 16),
+])
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("num_triangles", C.c_int32), ("tri_p", C.c_void_p), ("tri_n", C.c_void_p), ("tri_mat", C.c_void_p),
+                ("tri_light", C.c_void_p), ("num_materials", C.c_int32), ("materials", C.c_void_p),
+                ("num_lights", C.c_int32), ("light_tris", C.c_void_p), ("light_emit", C.c_void_p), ("camera", C.c_void_p)]
+
+
+class Options(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("num_init_samples", C.c_int32), ("min_depth", C.c_int32), ("max_depth", C.c_int32),
+                ("direct_spp", C.c_int32), ("large_step_prob", C.c_float), ("perturb_std_dev", C.c_float),
+                ("roughness_threshold", C.c_float), ("num_chains", C.c_int32), ("seed_offset", C.c_int32),
+                ("use_light_coordinate_sampling", C.c_int32), ("large_step_multiplexed", C.c_int32),
+                ("discrete_std_dev", C.c_float), ("type", C.c_int32), ("aniso_perturb_prob", C.c_float),
+                ("aniso_perturb_std_dev", C.c_float), ("acceptance_map", C.c_int32),
+                ("max_derivatives_depth", C.c_int32), ("quirk_ignore_aniso_std_dev", C.c_int32),
+                ("quirk_chad_mis", C.c_int32), ("seed", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso",
+                                          "stage2_iso_accepted", "stage2_h2mc", "stage2_h2mc_accepted")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.drmlt_last_error.restype = C.c_char_p
+        _lib.drmlt_ctx_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(Options), C.POINTER(C.c_void_p)]
+    return _lib
+
+
+class DrmltError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise DrmltError(f"drmlt error {rc}: {lib().drmlt_last_error().decode()}")
+
+
+def P(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_options():
+    o = Options()
+    check(lib().drmlt_options_default(C.byref(o)))
+    return o
+
+
+# <dpt> keys of the reference (src/parsescene.cpp:505-564) -> Options fields
+_KEYMAP = {
+    "spp": ("spp", int), "numinitsamples": ("num_init_samples", int), "mindepth": ("min_depth", int),
+    "maxdepth": ("max_depth", int), "directspp": ("direct_spp", int), "largestepprob": ("large_step_prob", float),
+    "perturbstddev": ("perturb_std_dev", float), "roughnessthreshold": ("roughness_threshold", float),
+    "numchains": ("num_chains", int), "seedoffset": ("seed_offset", int),
+    "uselightcoordinatesampling": ("use_light_coordinate_sampling", int),
+    "largestepmultiplexed": ("large_step_multiplexed", int), "anisoperturbprob": ("aniso_perturb_prob", float),
+    "anisoperturbstddev": ("aniso_perturb_std_dev", float), "acceptanceMap": ("acceptance_map", int),
+}
+
+
+def options_from_dpt(dpt: dict, **overrides):
+    """Options from a dict of the reference's <dpt> keys (types as the XML
+    parser converts them); 'type' is "green" or "mira"."""
+    o = default_options()
+    for k, v in dpt.items():
+        if k == "type":
+            o.type = 1 if v == "mira" else 0
+        elif k in _KEYMAP:
+            f, cast = _KEYMAP[k]
+            setattr(o, f, cast(bool(v) if isinstance(v, bool) else v))
+    for k, v in overrides.items():
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One drmlt_ctx: the scene on the GPU plus the chain state."""
+
+    def __init__(self, scene: dict, options: Options):
+        self._keep = [np.ascontiguousarray(scene[k]) for k in
+                      ("tri_p", "tri_n", "tri_mat", "tri_light", "mats", "light_tris", "light_emit", "cam")]
+        tp, tn, tm, tl, m, lt, le, cam = self._keep
+        assert tp.dtype == np.float32 and tm.dtype == np.int32 and le.dtype == np.float32
+        d = SceneDesc(len(tp), P(tp), P(tn), P(tm), P(tl), len(m), P(m), len(lt), P(lt), P(le), P(cam))
+        self.opt = options
+        h = C.c_void_p()
+        check(lib().drmlt_ctx_create(C.byref(d), C.byref(options), C.byref(h)))
+        self.h = h
+        self.W, self.H = int(cam[10]), int(cam[11])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().drmlt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- loop
+    def prepass(self, n_len=16):
+        b = C.c_double(0)
+        lf = np.zeros(n_len, np.float32)
+        check(lib().drmlt_prepass(self.h, C.byref(b), P(lf), n_len))
+        return b.value, lf
+
+    def seed_chains(self, states, contrib8, b, length_func):
+        states = np.ascontiguousarray(states, PATH_REC)
+        contrib8 = np.ascontiguousarray(contrib8, np.float32)
+        lf = np.ascontiguousarray(length_func, np.float32)
+        check(lib().drmlt_seed_chains(self.h, P(states), P(contrib8), C.c_int64(len(states)), C.c_float(b), P(lf), len(lf)))
+
+    def direct_lighting(self):
+        check(lib().drmlt_direct_lighting(self.h))
+
+    def run(self, iters, mode=BATCHED_GRAPH):
+        check(lib().drmlt_run(self.h, C.c_int64(iters), mode))
+
+    def film(self, which=0):
+        out = np.zeros((self.H, self.W, 3), np.float32)
+        check(lib().drmlt_read_film(self.h, P(out), self.W, self.H, which))
+        return out
+
+    def stats(self):
+        s = Stats()
+        check(lib().drmlt_read_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def timing(self):
+        out = np.zeros(8)
+        check(lib().drmlt_read_timing(self.h, P(out), 8))
+        return out
+
+    # -- function level
+    def trace(self, org, dirs, tmin, tmax, any_hit=False):
+        n = len(org)
+        org = np.ascontiguousarray(org, np.float32); dirs = np.ascontiguousarray(dirs, np.float32)
+        tmin = np.ascontiguousarray(tmin, np.float32); tmax = np.ascontiguousarray(tmax, np.float32)
+        prim = np.zeros(n, np.int32); t = np.zeros(n, np.float32)
+        check(lib().drmlt_trace(self.h, C.c_int64(n), P(org), P(dirs), P(tmin), P(tmax), int(any_hit), P(prim), P(t)))
+        return prim, t
+
+    def perturb(self, states, off, disc):
+        states = np.array(states, PATH_REC)
+        n = len(states)
+        off = np.ascontiguousarray(off, np.float32).reshape(n, 32)
+        disc = np.ascontiguousarray(disc, np.float32).reshape(n, 32)
+        c8 = np.zeros((n, 8), np.float32)
+        check(lib().drmlt_perturb(self.h, C.c_int64(n), P(states), P(off), P(disc), P(c8)))
+        return states, c8
+
+    def generate_subpath(self, cl, uni):
+        cl = np.ascontiguousarray(cl, np.int32)
+        n = len(cl)
+        uni = np.ascontiguousarray(uni, np.float32).reshape(n, -1)
+        st = np.zeros(n, PATH_REC); c8 = np.zeros((n, 8), np.float32)
+        check(lib().drmlt_generate_subpath(self.h, C.c_int64(n), P(cl), P(uni), uni.shape[1], P(st), P(c8)))
+        return st, c8
+
+    def generate_bidir(self, uni, min_depth, max_depth, maxc=64):
+        uni = np.ascontiguousarray(uni, np.float32)
+        n = len(uni)
+        cs = np.zeros((n, maxc, 10), np.float32); nc = np.zeros(n, np.int32)
+        check(lib().drmlt_generate_bidir(self.h, C.c_int64(n), min_depth, max_depth, P(uni), uni.shape[1], maxc, P(cs), P(nc)))
+        return cs, nc
+
+    def grad_hess(self, states, fp64=False):
+        states = np.ascontiguousarray(states, PATH_REC)
+        n = len(states)
+        g = np.zeros((n, 32)); h = np.zeros((n, 32, 32)); ok = np.zeros(n, np.int32)
+        check(lib().drmlt_grad_hess(self.h, C.c_int64(n), P(states), int(fp64), P(g), P(h), P(ok)))
+        return g, h, ok
+
+
+def compute_gaussian(D, sc, sigma, grad, hess):
+    D = np.ascontiguousarray(D, np.int32)
+    n = len(D)
+    sc = np.ascontiguousarray(sc, np.float64)
+    grad = np.ascontiguousarray(grad, np.float64).reshape(n, 32)
+    hess = np.ascontiguousarray(hess, np.float64).reshape(n, 32, 32)
+    mean = np.zeros((n, 32)); covL = np.zeros((n, 24, 24)); inv = np.zeros((n, 24, 24)); ld = np.zeros(n)
+    check(lib().drmlt_compute_gaussian_f64(C.c_int64(n), P(D), P(sc), C.c_double(sigma), P(grad), P(hess), P(mean), P(covL),
+                                           P(inv), P(ld)))
+    return mean, covL, inv, ld
+
+
+def gaussian_logpdf(D, x, sign, mean, covL, inv, ld):
+    D = np.ascontiguousarray(D, np.int32)
+    n = len(D)
+    x = np.ascontiguousarray(x, np.float64).reshape(n, 32)
+    out = np.zeros(n)
+    check(lib().drmlt_gaussian_logpdf_f64(C.c_int64(n), P(D), P(x), C.c_double(sign), P(np.ascontiguousarray(mean)),
+                                          P(np.ascontiguousarray(covL)), P(np.ascontiguousarray(inv)),
+                                          P(np.ascontiguousarray(ld)), P(out)))
+    return out
+
+
+def dr_alpha2(rows):
+    rows = np.ascontiguousarray(rows, np.float64).reshape(-1, 6)
+    out = np.zeros(len(rows))
+    check(lib().drmlt_dr_alpha2_f64(C.c_int64(len(rows)), P(rows), P(out)))
+    return out

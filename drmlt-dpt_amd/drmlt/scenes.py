@@ -1,0 +1,328 @@
+"""Seeded procedural scenes for the five benchmark configurations.
+
+A scene is a dict of flat arrays, the format the C ABI (``drmlt_scene_desc`` in
+include/drmlt_amd.h) consumes:
+
+* ``tri_p``  float32 [nt, 9]  triangle vertex positions p0, p1, p2
+* ``tri_n``  float32 [nt, 9]  per-vertex shading normals n0, n1, n2
+* ``tri_mat`` int32 [nt]      material index
+* ``tri_light`` int32 [nt]    area-light index of an emissive triangle, else -1
+* ``mats``   float32 [nm, 16] material records (type, Kd/Ks, Ks/Kt, exponent,
+  alpha, eta, 1/eta, KsWeight, twoSided)
+* ``light_tris`` int32 [nl, 2] first triangle and triangle count of each light
+* ``light_emit`` float32 [nl, 4] radiance RGB and sampling weight
+* ``cam`` float32 [14]  position, target, up, fov (degrees, horizontal),
+  width, height, near clip, far clip
+
+plus the DR options of the configuration (the reference's ``<dpt>`` keys,
+src/dptoptions.h:8-37) under ``options``.
+
+Materials follow the reference's BSDFs: Lambertian (src/lambertian.cpp), Phong
+(src/phong.cpp, KsWeight = lum(Ks) / (lum(Ks) + lum(Kd)), src/phong.cpp:158-166)
+and rough dielectric (src/roughdielectric.cpp, eta = intIOR / extIOR,
+src/roughdielectric.h:17). Everything is derived from one 64-bit seed.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+MAT_LAMBERT, MAT_PHONG, MAT_ROUGHDIEL = 0, 1, 2
+LUM = np.array([0.212671, 0.715160, 0.072169])
+
+
+def lambert(kd, two_sided=False):
+    m = np.zeros(16, np.float32)
+    m[0] = MAT_LAMBERT
+    m[1:4] = kd
+    m[12] = 1.0 if two_sided else 0.0
+    return m
+
+
+def phong(kd, ks, exponent, two_sided=False):
+    m = np.zeros(16, np.float32)
+    m[0] = MAT_PHONG
+    m[1:4] = kd
+    m[4:7] = ks
+    m[7] = exponent
+    lks, lkd = float(LUM @ np.asarray(ks, np.float64)), float(LUM @ np.asarray(kd, np.float64))
+    m[11] = 0.0 if lks + lkd <= 0 else lks / (lks + lkd)
+    m[12] = 1.0 if two_sided else 0.0
+    return m
+
+
+def rough_dielectric(alpha, int_ior=1.5, ext_ior=1.0, ks=(1, 1, 1), kt=(1, 1, 1)):
+    m = np.zeros(16, np.float32)
+    m[0] = MAT_ROUGHDIEL
+    m[1:4] = ks
+    m[4:7] = kt
+    m[8] = alpha
+    eta = np.float32(int_ior) / np.float32(ext_ior)
+    m[9] = eta
+    m[10] = np.float32(1.0) / eta
+    return m
+
+
+class _Builder:
+    def __init__(self):
+        self.p, self.n, self.mat, self.light = [], [], [], []
+        self.mats = []
+        self.lights = []  # (first, count, emit rgb, weight)
+
+    def add_material(self, rec):
+        self.mats.append(np.asarray(rec, np.float32))
+        return len(self.mats) - 1
+
+    def add_tris(self, P, N, mat, light=-1):
+        P = np.asarray(P, np.float64).reshape(-1, 9)
+        N = np.asarray(N, np.float64).reshape(-1, 9)
+        self.p.append(P)
+        self.n.append(N)
+        self.mat.append(np.full(len(P), mat, np.int32))
+        self.light.append(np.full(len(P), light, np.int32))
+        return len(P)
+
+    def ntris(self):
+        return int(sum(len(x) for x in self.p))
+
+    def add_quad(self, corners, normal, mat, light=-1):
+        c = np.asarray(corners, np.float64)
+        n = np.asarray(normal, np.float64)
+        n = n / np.linalg.norm(n)
+        P = np.stack([np.concatenate([c[0], c[1], c[2]]), np.concatenate([c[0], c[2], c[3]])])
+        N = np.tile(np.concatenate([n, n, n]), (2, 1))
+        return self.add_tris(P, N, mat, light)
+
+    def add_light_quad(self, center, size, emission, normal=(0, -1, 0)):
+        first = self.ntris()
+        cx, cy, cz = center
+        h = size / 2.0
+        corners = [(cx - h, cy, cz - h), (cx + h, cy, cz - h), (cx + h, cy, cz + h), (cx - h, cy, cz + h)]
+        mat = self.add_material(lambert((0.0, 0.0, 0.0)))
+        lid = len(self.lights)
+        self.add_quad(corners, normal, mat, light=lid)
+        self.lights.append((first, 2, tuple(float(e) for e in emission), 1.0))
+        return lid
+
+    def add_mesh(self, V, F, VN, mat):
+        P = V[F].reshape(-1, 9)
+        N = VN[F].reshape(-1, 9)
+        return self.add_tris(P, N, mat)
+
+    def build(self, cam, options):
+        sc = {
+            "tri_p": np.ascontiguousarray(np.concatenate(self.p).astype(np.float32)),
+            "tri_n": np.ascontiguousarray(np.concatenate(self.n).astype(np.float32)),
+            "tri_mat": np.ascontiguousarray(np.concatenate(self.mat)),
+            "tri_light": np.ascontiguousarray(np.concatenate(self.light)),
+            "mats": np.ascontiguousarray(np.stack(self.mats).astype(np.float32)),
+            "light_tris": np.array([[l[0], l[1]] for l in self.lights], np.int32),
+            "light_emit": np.array([[*l[2], l[3]] for l in self.lights], np.float32),
+            "cam": np.asarray(cam, np.float32),
+            "options": dict(options),
+        }
+        return sc
+
+
+def box_mesh(center, size, angle):
+    """Axis-aligned box of extent ``size`` rotated by ``angle`` about +y; flat normals."""
+    sx, sy, sz = (s / 2.0 for s in size)
+    ca, sa = np.cos(angle), np.sin(angle)
+    R = np.array([[ca, 0, sa], [0, 1, 0], [-sa, 0, ca]])
+    faces = [  # (normal, 4 corners in local coords, ccw seen from outside)
+        ((1, 0, 0), [(sx, -sy, -sz), (sx, sy, -sz), (sx, sy, sz), (sx, -sy, sz)]),
+        ((-1, 0, 0), [(-sx, -sy, sz), (-sx, sy, sz), (-sx, sy, -sz), (-sx, -sy, -sz)]),
+        ((0, 1, 0), [(-sx, sy, -sz), (-sx, sy, sz), (sx, sy, sz), (sx, sy, -sz)]),
+        ((0, -1, 0), [(-sx, -sy, sz), (-sx, -sy, -sz), (sx, -sy, -sz), (sx, -sy, sz)]),
+        ((0, 0, 1), [(sx, -sy, sz), (sx, sy, sz), (-sx, sy, sz), (-sx, -sy, sz)]),
+        ((0, 0, -1), [(-sx, -sy, -sz), (-sx, sy, -sz), (sx, sy, -sz), (sx, -sy, -sz)]),
+    ]
+    P, N = [], []
+    c = np.asarray(center, np.float64)
+    for nrm, q in faces:
+        q = np.asarray(q, np.float64) @ R.T + c
+        n = R @ np.asarray(nrm, np.float64)
+        P += [np.concatenate([q[0], q[1], q[2]]), np.concatenate([q[0], q[2], q[3]])]
+        N += [np.concatenate([n, n, n])] * 2
+    return np.array(P), np.array(N)
+
+
+def icosphere(level):
+    t = (1.0 + 5 ** 0.5) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    V = [np.array(v, np.float64) / np.linalg.norm(v) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7),
+         (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = V[a] + V[b]
+                V.append(m / np.linalg.norm(m))
+                cache[key] = len(V) - 1
+            return cache[key]
+
+        NF = []
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            NF += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = NF
+    return np.array(V), np.array(F, np.int64)
+
+
+def displaced_grid(res, half, amp, rng, y0=0.0):
+    """(res x res)-vertex grid over [-half, half]^2 at height y0 + amp * smooth noise."""
+    xs = np.linspace(-half, half, res)
+    X, Z = np.meshgrid(xs, xs, indexing="ij")
+    Hh = np.zeros_like(X)
+    for _ in range(6):
+        k = rng.uniform(1.0, 6.0, 2) * np.pi / half
+        ph = rng.uniform(0, 2 * np.pi)
+        Hh += np.sin(k[0] * X + k[1] * Z + ph)
+    Hh = y0 + amp * Hh / 6.0
+    V = np.stack([X, Hh, Z], -1).reshape(-1, 3)
+    gx, gz = np.gradient(Hh, xs, xs)
+    VN = np.stack([-gx, np.ones_like(gx), -gz], -1).reshape(-1, 3)
+    VN /= np.linalg.norm(VN, axis=1, keepdims=True)
+    idx = np.arange(res * res).reshape(res, res)
+    a, b, c, d = idx[:-1, :-1], idx[1:, :-1], idx[1:, 1:], idx[:-1, 1:]
+    F = np.concatenate([np.stack([a, d, c], -1).reshape(-1, 3), np.stack([a, c, b], -1).reshape(-1, 3)])
+    return V, F, VN, Hh, xs
+
+
+def _height_at(Hh, xs, x, z):
+    i = int(np.clip(np.searchsorted(xs, x), 0, len(xs) - 1))
+    j = int(np.clip(np.searchsorted(xs, z), 0, len(xs) - 1))
+    return float(Hh[i, j])
+
+
+DEFAULT_OPTIONS = {  # src/dptoptions.h:10-36 defaults
+    "integrator": "drmlt", "spp": 16, "numinitsamples": 100000, "mindepth": -1, "maxdepth": -1, "directspp": 16,
+    "largestepprob": 0.3, "perturbstddev": 0.01, "roughnessthreshold": 0.03, "numchains": 1024, "seedoffset": 0,
+    "uselightcoordinatesampling": True, "largestepmultiplexed": True, "type": "green", "anisoperturbprob": 0.5,
+    "anisoperturbstddev": 0.0025, "acceptanceMap": False,
+}
+
+
+def config1(seed=1):
+    """Closed 2x2x2 m room (5 quads) + 2 boxes, Lambertian albedo U[0.2,0.8]^3,
+    0.5x0.5 m ceiling light (radiance 10), pinhole 40 deg, 256x256, max depth 5,
+    1024 chains, 64 mutations per pixel."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    walls = [
+        ([(-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1)], (0, 1, 0)),  # floor
+        ([(-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)], (0, -1, 0)),  # ceiling
+        ([(-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)], (0, 0, 1)),  # back
+        ([(-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1)], (1, 0, 0)),  # left
+        ([(1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)], (-1, 0, 0)),  # right
+    ]
+    b.add_light_quad((0.0, 1.99, 0.0), 0.5, (10.0, 10.0, 10.0))
+    for corners, n in walls:
+        m = b.add_material(lambert(rng.uniform(0.2, 0.8, 3)))
+        b.add_quad(corners, n, m)
+    for k in range(2):
+        size = rng.uniform(0.3, 0.7, 3)
+        cx = rng.uniform(-0.55, 0.55) + (0.0 if k == 0 else 0.0)
+        cz = rng.uniform(-0.55, 0.55)
+        P, N = box_mesh((cx, size[1] / 2 + 1e-3, cz), size, rng.uniform(0, np.pi))
+        m = b.add_material(lambert(rng.uniform(0.2, 0.8, 3)))
+        b.add_tris(P, N, m)
+    cam = [0, 1, 3.4, 0, 1, 0, 0, 1, 0, 40.0, 256, 256, 1e-2, 1000.0]
+    opts = dict(DEFAULT_OPTIONS, type="green", anisoperturbprob=0.5, anisoperturbstddev=0.0025, maxdepth=5,
+                numchains=1024, spp=64)
+    return b.build(cam, opts)
+
+
+def _random_material(rng, kind):
+    if kind == "lambert":
+        return lambert(rng.uniform(0.2, 0.8, 3))
+    if kind == "phong":
+        return phong(rng.uniform(0.1, 0.4, 3), rng.uniform(0.2, 0.5, 3), rng.uniform(10.0, 200.0))
+    return rough_dielectric(rng.uniform(0.05, 0.5), 1.5, 1.0)
+
+
+def config2(seed=2, sphere_level=4, grid_res=256):
+    """Displaced 256x256 grid floor (amplitude 0.05) + 64 icospheres (subdivision 4,
+    r ~ U[0.1,0.4]); materials 50% Lambertian / 30% Phong (n ~ U[10,200]) / 20%
+    rough dielectric (alpha ~ U[0.05,0.5], IOR 1.5); 0.5x0.5 m light (radiance
+    10); 512x512; path lengths up to 8; 65,536 chains."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    b.add_light_quad((0.0, 2.5, 0.0), 0.5, (10.0, 10.0, 10.0))
+    V, F, VN, Hh, xs = displaced_grid(grid_res, 2.5, 0.05, rng)
+    kinds = ["lambert"] * 50 + ["phong"] * 30 + ["dielectric"] * 20
+    b.add_mesh(V, F, VN, b.add_material(_random_material(rng, rng.choice(kinds))))
+    SV, SF = icosphere(sphere_level)
+    for _ in range(64):
+        r = rng.uniform(0.1, 0.4)
+        x, z = rng.uniform(-2.0, 2.0, 2)
+        y = _height_at(Hh, xs, x, z) + r
+        m = b.add_material(_random_material(rng, rng.choice(kinds)))
+        b.add_mesh(SV * r + np.array([x, y, z]), SF, SV, m)
+    cam = [0, 1.8, 4.6, 0, 0.25, 0, 0, 1, 0, 40.0, 512, 512, 1e-2, 1000.0]
+    opts = dict(DEFAULT_OPTIONS, maxdepth=8, numchains=65536, spp=16)
+    return b.build(cam, opts)
+
+
+def config4(seed=4, sphere_level=4, grid_res=512):
+    """512 seeded meshes (icospheres subdivision 4 and extruded boxes) on a
+    displaced floor, ~2M triangles; Phong with n log-uniform in [100,1000] on 60%
+    of surfaces, Lambertian elsewhere; 0.2x0.2 m light (radiance 50); 1024x1024;
+    max depth 10; 1,048,576 chains."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    b.add_light_quad((0.0, 3.0, 0.0), 0.2, (50.0, 50.0, 50.0))
+
+    def mat():
+        if rng.uniform() < 0.6:
+            return phong(rng.uniform(0.05, 0.3, 3), rng.uniform(0.3, 0.6, 3), float(np.exp(rng.uniform(np.log(100), np.log(1000)))))
+        return lambert(rng.uniform(0.2, 0.8, 3))
+
+    V, F, VN, Hh, xs = displaced_grid(grid_res, 4.0, 0.05, rng)
+    b.add_mesh(V, F, VN, b.add_material(mat()))
+    SV, SF = icosphere(sphere_level)
+    for k in range(512):
+        x, z = rng.uniform(-3.6, 3.6, 2)
+        if k % 2 == 0:
+            r = rng.uniform(0.08, 0.3)
+            y = _height_at(Hh, xs, x, z) + r
+            b.add_mesh(SV * r + np.array([x, y, z]), SF, SV, b.add_material(mat()))
+        else:
+            size = rng.uniform(0.1, 0.5, 3)
+            y = _height_at(Hh, xs, x, z) + size[1] / 2
+            P, N = box_mesh((x, y, z), size, rng.uniform(0, np.pi))
+            b.add_tris(P, N, b.add_material(mat()))
+    cam = [0, 2.2, 6.5, 0, 0.3, 0, 0, 1, 0, 40.0, 1024, 1024, 1e-2, 1000.0]
+    opts = dict(DEFAULT_OPTIONS, maxdepth=10, numchains=1048576, spp=16)
+    return b.build(cam, opts)
+
+
+def config5(seed=5, sphere_level=3, grid_res=128):
+    """128 rough dielectric spheres (alpha ~ U[0.02,0.1], IOR 1.5, r ~ U[0.05,0.2])
+    over a Lambertian floor (albedo 0.5); 0.1x0.1 m light (radiance 500);
+    acceptanceMap=true; max depth 12; 1,048,576 chains; 512x512."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    b.add_light_quad((0.0, 2.0, 0.0), 0.1, (500.0, 500.0, 500.0))
+    V, F, VN, Hh, xs = displaced_grid(grid_res, 2.0, 0.0, rng)
+    b.add_mesh(V, F, VN, b.add_material(lambert((0.5, 0.5, 0.5))))
+    SV, SF = icosphere(sphere_level)
+    for _ in range(128):
+        r = rng.uniform(0.05, 0.2)
+        x, z = rng.normal(0.0, 0.5, 2)
+        b.add_mesh(SV * r + np.array([x, r + 1e-3, z]), SF, SV, b.add_material(rough_dielectric(rng.uniform(0.02, 0.1))))
+    cam = [0, 1.5, 3.5, 0, 0.2, 0, 0, 1, 0, 40.0, 512, 512, 1e-2, 1000.0]
+    opts = dict(DEFAULT_OPTIONS, maxdepth=12, numchains=1048576, spp=16, acceptanceMap=True, type="green")
+    return b.build(cam, opts)
+
+
+CONFIGS = {1: config1, 2: config2, 4: config4, 5: config5}
+
+
+def scene_stats(sc):
+    return {"triangles": int(len(sc["tri_p"])), "materials": int(len(sc["mats"])), "lights": int(len(sc["light_tris"])),
+            "film": (int(sc["cam"][10]), int(sc["cam"][11]))}

@@ -1,0 +1,213 @@
+/*
+ * drmlt_amd.h -- C ABI of libdrmlt_hip.so, the MI355X (gfx950) implementation
+ * of the DRMLT data-parallel mutation loop of joeylitalien/drmlt-dpt.
+ *
+ * Two tiers:
+ *  1. Loop level: replaces the chain loop of DRIntegrator::Render
+ *     (src/integrators/delayed/delayed.cpp:334-673) together with its
+ *     pre-pass (delayed.cpp:14-173) and direct-lighting pass
+ *     (src/integrator.h:33-74). A host integrator creates a context from a
+ *     scene and the reference's DptOptions (src/dptoptions.h:8-37), seeds
+ *     the chains, runs iterations and reads back the film and statistics.
+ *  2. Function level: batched versions of the functions the loop is made of,
+ *     for parity tests and for callers that drive their own loop.
+ *
+ * Conventions: every function returns an int status (DRMLT_OK = 0, negative
+ * on error; drmlt_last_error() describes the last failure). Buffers are
+ * caller-owned host memory unless a name says otherwise; functions copy in and
+ * out. A context owns one HIP stream on the device current at creation and is
+ * not thread-safe (one host thread drives it). No function allocates on behalf
+ * of the caller beyond the context's own device state.
+ */
+#ifndef DRMLT_AMD_H
+#define DRMLT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRMLT_OK 0
+#define DRMLT_ERR_ARG (-1)
+#define DRMLT_ERR_HIP (-2)
+#define DRMLT_ERR_STATE (-3)
+
+typedef struct drmlt_ctx drmlt_ctx;
+
+/* Scene as flat arrays (the reference parses Mitsuba-style XML into
+ * TriangleMesh/BSDF/AreaLight objects, src/parsescene.cpp:136-503).
+ * tri_p, tri_n: 9 floats per triangle (three positions / three shading
+ * normals). materials: 16 floats each -- type (0 Lambertian, 1 Phong,
+ * 2 rough dielectric), Kd|Ks (3), Ks|Kt (3), Phong exponent, dielectric
+ * alpha, eta = intIOR/extIOR, 1/eta, Phong KsWeight, twoSided.
+ * light_tris: (first triangle, triangle count) per area light; light_emit:
+ * radiance RGB and sampling weight per light. camera: position, target, up,
+ * horizontal fov in degrees, width, height, near clip, far clip. */
+typedef struct {
+    int32_t num_triangles;
+    const float *tri_p;
+    const float *tri_n;
+    const int32_t *tri_mat;
+    const int32_t *tri_light; /* area-light index of an emissive triangle, else -1 */
+    int32_t num_materials;
+    const float *materials;
+    int32_t num_lights;
+    const int32_t *light_tris;
+    const float *light_emit;
+    const float *camera; /* 14 floats */
+} drmlt_scene_desc;
+
+/* The reference's DptOptions (src/dptoptions.h:8-37) plus the command-line
+ * knobs that reach the loop (src/main.cpp:28-42). */
+typedef struct {
+    int32_t spp;                 /* mutations per pixel */
+    int32_t num_init_samples;    /* pre-pass samples */
+    int32_t min_depth, max_depth;
+    int32_t direct_spp;
+    float large_step_prob;
+    float perturb_std_dev;       /* stage-1 sigma and H2MC prior sigma */
+    float roughness_threshold;
+    int32_t num_chains;
+    int32_t seed_offset;
+    int32_t use_light_coordinate_sampling;
+    int32_t large_step_multiplexed;  /* only multiplexed large steps are implemented */
+    float discrete_std_dev;
+    int32_t type;                /* 0 = "green", 1 = "mira" */
+    float aniso_perturb_prob;
+    float aniso_perturb_std_dev; /* the stage-2 isotropic sigma */
+    int32_t acceptance_map;
+    int32_t max_derivatives_depth;   /* --max-derivatives-depth (src/main.cpp:37, default 9) */
+    /* The reference stores the XML key anisoperturbstddev into lensPerturbStdDev
+     * (src/parsescene.cpp:553-554), so the loop always sees 0.0025. 1 keeps
+     * that behaviour (aniso_perturb_std_dev is ignored), 0 honours the option. */
+    int32_t quirk_ignore_aniso_std_dev;
+    /* The reference's derivative code weights area-light emission MIS as for a
+     * point light (src/path.cpp:2598-2604). 1 differentiates that function
+     * (what the reference's H2MC sees), 0 differentiates the primal log f. */
+    int32_t quirk_chad_mis;
+    uint64_t seed;
+} drmlt_options;
+
+/* One path state (the reference's Path + SubpathContrib, src/path.h:12-107):
+ * camera / light subpath vertex counts c, l (SubpathContrib camDepth /
+ * lightDepth); the emitting light and triangle (l >= 2) or the direct-light
+ * light and triangle (l == 1); the contribution's screen position; the
+ * primary sample u in GetPathPos order (src/path.cpp:2393-2439); per surface
+ * vertex the hit triangle, bsdfDiscrete and Russian-roulette weight. 536 bytes. */
+typedef struct {
+    int32_t c, l, light, lprim;
+    float screen[2];
+    float u[32];
+    int32_t cprim[16], lgprim[16];
+    float cdisc[16], ldisc[16];
+    float crr[16], lrr[16];
+} drmlt_path_rec;
+
+/* delayed.cpp:287-329 (Stats), counted per iteration */
+typedef struct {
+    uint64_t large_step, large_step_accepted;
+    uint64_t stage1, stage1_accepted;
+    uint64_t stage2_iso, stage2_iso_accepted;
+    uint64_t stage2_h2mc, stage2_h2mc_accepted;
+} drmlt_stats;
+
+/* Execution forms of drmlt_run */
+#define DRMLT_PLAIN 0          /* one kernel launch per step, one thread per chain iteration */
+#define DRMLT_BATCHED 1        /* persistent per-stage kernels over path-length bins */
+#define DRMLT_BATCHED_GRAPH 2  /* DRMLT_BATCHED with each step replayed from a captured hipGraph */
+
+const char *drmlt_last_error(void);
+int drmlt_options_default(drmlt_options *opt);
+
+/* ---------------- tier 1: the loop ---------------- */
+
+/* Replaces Scene::Scene + the integrator construction (src/main.cpp:101-128,
+ * src/scene.cpp:10-54): uploads the scene and builds the BVH. */
+int drmlt_ctx_create(const drmlt_scene_desc *scene, const drmlt_options *opt, drmlt_ctx **out);
+int drmlt_ctx_destroy(drmlt_ctx *ctx);
+
+/* DRIntegrator::PrePass (delayed.cpp:14-173) on the device: num_init_samples
+ * bidirectional paths, the normalisation b, the path-length distribution and
+ * num_chains equal-spaced seed states. */
+int drmlt_prepass(drmlt_ctx *ctx, double *b_out, float *length_func_out, int32_t n_len);
+
+/* Seeds the chains from caller-provided states instead (for example the
+ * states a reference pre-pass produced). contrib8 per state: valid, rgb,
+ * lsScore, ssScore, screen x, y. */
+int drmlt_seed_chains(drmlt_ctx *ctx, const drmlt_path_rec *states, const float *contrib8, int64_t n, float b,
+                      const float *length_func, int32_t n_len);
+
+/* DirectLighting (src/integrator.h:33-74) into the context's direct buffer. */
+int drmlt_direct_lighting(drmlt_ctx *ctx);
+
+/* Runs iters_per_chain DR iterations on every chain (delayed.cpp:396-656). */
+int drmlt_run(drmlt_ctx *ctx, int64_t iters_per_chain, int32_t mode);
+
+/* which: 0 merged film (direct/directSpp + indirect/spp with spp the
+ * mutations per pixel run so far, delayed.cpp:682-685), 1 raw indirect
+ * buffer, 2 raw direct buffer. rgb holds w*h*3 floats. */
+int drmlt_read_film(drmlt_ctx *ctx, float *rgb, int32_t w, int32_t h, int32_t which);
+int drmlt_read_stats(drmlt_ctx *ctx, drmlt_stats *stats);
+/* Device time of the last drmlt_run in milliseconds, total and per stage
+ * kernel (hipEvents on the context stream): out[0] total, out[1..7] stage 1,
+ * derivatives (current), Gaussian (current), stage-2 proposal, derivatives
+ * (proposal), Gaussian (proposal), stage-2 acceptance. n_out <= 8. */
+int drmlt_read_timing(drmlt_ctx *ctx, double *out, int32_t n_out);
+/* Counts traversal work of one stage-1 launch on the current chain states
+ * without advancing them: nodes visited, triangles tested, rays. */
+int drmlt_count_stage1_work(drmlt_ctx *ctx, uint64_t *counts3);
+
+/* ---------------- tier 2: batched functions ---------------- */
+
+/* Scene::Intersect / Occluded (src/scene.cpp:80-111). any = 0: closest hit,
+ * prim[i] = scene triangle index or -1 and t[i]; any = 1: prim[i] = 1 if any
+ * triangle is hit in (tmin, tmax). */
+int drmlt_trace(drmlt_ctx *ctx, int64_t n, const float *org, const float *dir, const float *tmin, const float *tmax, int32_t any,
+                int32_t *prim, float *t);
+
+/* PerturbPathBidir (src/path.cpp:1480-1628) on n states: off (n x 32) are the
+ * primary-sample offsets in GetPathPos order, disc (n x 32) the discrete
+ * offsets (camera vertex d at d, light vertex d at 16 + d). states are
+ * updated in place; contrib8 as in drmlt_seed_chains. */
+int drmlt_perturb(drmlt_ctx *ctx, int64_t n, drmlt_path_rec *states, const float *off, const float *disc, float *contrib8);
+
+/* GenerateSubpath (src/path.cpp:1123-1269) for strategies cl (n x 2: c, l)
+ * drawing from the uniforms uni (n x nuni) in the documented order. */
+int drmlt_generate_subpath(drmlt_ctx *ctx, int64_t n, const int32_t *cl, const float *uni, int32_t nuni, drmlt_path_rec *states,
+                           float *contrib8);
+
+/* GeneratePathBidir (src/path.cpp:978-1121): up to maxc contributions per
+ * sample, 10 floats each (c, l, rgb, lsScore, ssScore, screen x, y, MIS
+ * weight); ncontrib[i] is the number produced. */
+int drmlt_generate_bidir(drmlt_ctx *ctx, int64_t n, int32_t min_depth, int32_t max_depth, const float *uni, int32_t nuni, int32_t maxc,
+                         float *contribs, int32_t *ncontrib);
+
+/* Gradient and Hessian of log f (the function of evaluate_path_bidir_{c}_{l}
+ * _static_derv, src/path.cpp:2987-3175 / src/chad.cpp:494-551) at each state,
+ * triangles fixed. grad n x 32, hess n x 32 x 32 (row-major, leading
+ * dimension 32); ok[i] = 0 where the fixed-topology evaluation fails.
+ * fp64 = 1 evaluates in double. */
+int drmlt_grad_hess(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int32_t fp64, double *grad, double *hess, int32_t *ok);
+
+/* PH2MCSmallStep::ComputeGaussian (drsmallstep.cpp:18-93, 219-266) in FP64
+ * on the current device: D[i] dimensions, sc[i] the state's ssScore, sigma the
+ * prior, grad n x 32, hess n x 32 x 32. Outputs mean n x 32, covL and invCov
+ * n x 24 x 24 (leading dimension 24), logDet n. */
+int drmlt_compute_gaussian_f64(int64_t n, const int32_t *D, const double *sc, double sigma, const double *grad, const double *hess,
+                               double *mean, double *covL, double *invCov, double *logDet);
+
+/* GaussianLogPdf (gaussian.cpp:18-26) of sign * x[i] under the Gaussians
+ * laid out as drmlt_compute_gaussian_f64 returns them. */
+int drmlt_gaussian_logpdf_f64(int64_t n, const int32_t *D, const double *x, double sign, const double *mean, const double *covL,
+                              const double *invCov, const double *logDet, double *out);
+
+/* Second-stage acceptance of delayed.cpp:466-541: per row (type, a1, a2,
+ * aStar, transRatio, secondValid) -> alpha2. */
+int drmlt_dr_alpha2_f64(int64_t n, const double *in6, double *alpha2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DRMLT_AMD_H */

@@ -1,0 +1,454 @@
+// ORACLE (test infrastructure only). C entry points over the CPU restatement
+// in orc_*.h, loaded with ctypes by tests/, by bench.py's cpu_baseline leg and
+// by the smoke check. The product never links or loads this library.
+#include "orc_dr.h"
+#include <atomic>
+#include <chrono>
+#include <mutex>
+#include <thread>
+
+using namespace orc;
+
+// Flat path record shared with the product's parity surface (same layout as
+// drmlt_path_rec in include/drmlt_amd.h).
+struct PathRec {
+    int32_t c, l;
+    int32_t light, lprim;
+    float screen[2];
+    float u[32];
+    int32_t cprim[16], lgprim[16];
+    float cdisc[16], ldisc[16];
+    float crr[16], lrr[16];
+};
+static_assert(sizeof(PathRec) == 536, "record layout");
+
+struct OrcScene {
+    SceneT<float> f;
+    SceneT<double> d;
+};
+
+template <class F> static PathS<F> FromRec(const PathRec &r) {
+    PathS<F> p;
+    p.c = r.c; p.l = r.l;
+    p.cam.resize(std::max(r.c - 1, 0));
+    p.lgt.resize(std::max(r.l - 1, 0));
+    for (int i = 0; i < (int)p.cam.size(); i++) { p.cam[i].prim = r.cprim[i]; p.cam[i].disc = r.cdisc[i]; p.cam[i].rr = r.crr[i]; }
+    for (int i = 0; i < (int)p.lgt.size(); i++) { p.lgt[i].prim = r.lgprim[i]; p.lgt[i].disc = r.ldisc[i]; p.lgt[i].rr = r.lrr[i]; }
+    if (r.l >= 2) { p.lLight = r.light; p.lPrim = r.lprim; }
+    if (r.l == 1 && !p.cam.empty()) { p.cam.back().dlLight = r.light; p.cam.back().dlPrim = r.lprim; }
+    F u[32];
+    for (int i = 0; i < 32; i++) u[i] = F(r.u[i]);
+    SetPathPos(p, u);
+    if (r.c == 1) { p.screen[0] = r.screen[0]; p.screen[1] = r.screen[1]; }
+    return p;
+}
+template <class F> static void ToRec(const PathS<F> &p, PathRec &r) {
+    std::memset(&r, 0, sizeof(r));
+    r.c = p.c; r.l = p.l;
+    for (int i = 0; i < 16; i++) { r.cprim[i] = -1; r.lgprim[i] = -1; r.crr[i] = 1; r.lrr[i] = 1; }
+    for (int i = 0; i < (int)p.cam.size() && i < 16; i++) { r.cprim[i] = p.cam[i].prim; r.cdisc[i] = float(p.cam[i].disc); r.crr[i] = float(p.cam[i].rr); }
+    for (int i = 0; i < (int)p.lgt.size() && i < 16; i++) { r.lgprim[i] = p.lgt[i].prim; r.ldisc[i] = float(p.lgt[i].disc); r.lrr[i] = float(p.lgt[i].rr); }
+    r.light = -1; r.lprim = -1;
+    if (p.l >= 2) { r.light = p.lLight; r.lprim = p.lPrim; }
+    if (p.l == 1 && !p.cam.empty()) { r.light = p.cam.back().dlLight; r.lprim = p.cam.back().dlPrim; }
+    F u[DMAX] = {0};
+    GetPathPos(p, u);
+    const int D = Dim<F>(p.c, p.l);
+    for (int i = 0; i < D && i < 32; i++) r.u[i] = float(u[i]);
+    r.screen[0] = float(p.screen[0]);
+    r.screen[1] = float(p.screen[1]);
+}
+template <class F> static void ContribOut(const std::vector<Contrib<F>> &out, float *c8) {
+    std::memset(c8, 0, 8 * sizeof(float));
+    if (out.empty()) return;
+    const Contrib<F> &k = out[0];
+    c8[0] = 1; c8[1] = float(k.contrib.x); c8[2] = float(k.contrib.y); c8[3] = float(k.contrib.z);
+    c8[4] = float(k.lsScore); c8[5] = float(k.ssScore); c8[6] = float(k.screen[0]); c8[7] = float(k.screen[1]);
+}
+static Material<double> MatD(const float *m) {
+    Material<double> M;
+    M.type = int(m[0]);
+    M.a = V3<double>(m[1], m[2], m[3]);
+    M.b = V3<double>(m[4], m[5], m[6]);
+    M.expo = m[7]; M.alpha = m[8]; M.eta = m[9]; M.invEta = m[10]; M.ksWeight = m[11]; M.twoSided = m[12] != 0;
+    return M;
+}
+
+static void ParallelFor(int n, int nthreads, const std::function<void(int, int)> &fn) {
+    if (nthreads <= 1) { for (int i = 0; i < n; i++) fn(i, 0); return; }
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++)
+        th.emplace_back([&, t]() {
+            for (;;) {
+                int i = next.fetch_add(1);
+                if (i >= n) break;
+                fn(i, t);
+            }
+        });
+    for (auto &x : th) x.join();
+}
+
+extern "C" int orc_path_rec_size() { return int(sizeof(PathRec)); }
+
+extern "C" void *orc_scene_new(int nt, const float *triP, const float *triN, const int *triMat, const int *triLight, int nm, const float *mats,
+                    int nl, const int *lightTris, const float *lightEmit, const float *cam, int useBvh) {
+    OrcScene *s = new OrcScene();
+    LoadScene(s->f, nt, triP, triN, triMat, triLight, nm, mats, nl, lightTris, lightEmit, cam);
+    LoadScene(s->d, nt, triP, triN, triMat, triLight, nm, mats, nl, lightTris, lightEmit, cam);
+    s->f.useBvh = s->d.useBvh = useBvh != 0;
+    return s;
+}
+extern "C" void orc_scene_free(void *h) { delete (OrcScene *)h; }
+
+extern "C" void orc_trace(void *h, int n, const float *org, const float *dir, const float *tmin, const float *tmax, int *prim, float *t) {
+    OrcScene *s = (OrcScene *)h;
+    for (int i = 0; i < n; i++) prim[i] = TraceClosest(s->f, org + 3 * i, dir + 3 * i, tmin[i], tmax[i], t[i]);
+}
+extern "C" void orc_occluded(void *h, int n, const float *org, const float *dir, const float *tmin, const float *tmax, int *occ) {
+    OrcScene *s = (OrcScene *)h;
+    for (int i = 0; i < n; i++) occ[i] = TraceAny(s->f, org + 3 * i, dir + 3 * i, tmin[i], tmax[i]) ? 1 : 0;
+}
+
+extern "C" void orc_bsdf_eval(const float *mat, int adjoint, const double *wi, const double *nn, const double *wo, double *out6) {
+    Material<double> M = MatD(mat);
+    BsdfOut<double> o;
+    BsdfEval(M, adjoint != 0, V3<double>(wi[0], wi[1], wi[2]), V3<double>(nn[0], nn[1], nn[2]), V3<double>(wo[0], wo[1], wo[2]), o);
+    out6[0] = o.contrib.x; out6[1] = o.contrib.y; out6[2] = o.contrib.z;
+    out6[3] = o.cosWo; out6[4] = o.pdf; out6[5] = o.revPdf;
+}
+extern "C" int orc_bsdf_sample(const float *mat, int adjoint, const double *wi, const double *nn, double r0, double r1, double disc, double *wo3, double *out6) {
+    Material<double> M = MatD(mat);
+    BsdfOut<double> o;
+    V3<double> wo;
+    bool ok = BsdfSample(M, adjoint != 0, V3<double>(wi[0], wi[1], wi[2]), V3<double>(nn[0], nn[1], nn[2]), r0, r1, disc, wo, o);
+    wo3[0] = wo.x; wo3[1] = wo.y; wo3[2] = wo.z;
+    out6[0] = o.contrib.x; out6[1] = o.contrib.y; out6[2] = o.contrib.z;
+    out6[3] = o.cosWo; out6[4] = o.pdf; out6[5] = o.revPdf;
+    return ok ? 1 : 0;
+}
+extern "C" double orc_roughness(const float *mat, double disc) { return Roughness(MatD(mat), disc); }
+
+template <class F>
+static void PerturbT(const SceneT<F> &s, int n, const PathRec *in, const float *off, const float *disc, int occl, int fixed, int chadQuirk,
+                     PathRec *out, float *c8) {
+    Opts<F> o;
+    o.chadMisQuirk = chadQuirk != 0;
+    for (int i = 0; i < n; i++) {
+        PathS<F> p = FromRec<F>(in[i]);
+        F of[DMAX] = {0}, di[2 * VMAX] = {0};
+        for (int k = 0; k < 32; k++) of[k] = F(off[32 * i + k]);
+        for (int k = 0; k < 2 * VMAX; k++) di[k] = F(disc[32 * i + k]);
+        std::vector<Contrib<F>> res;
+        PerturbPathBidir(s, o, of, di, p, res, occl != 0, fixed != 0);
+        ToRec(p, out[i]);
+        ContribOut(res, c8 + 8 * i);
+        if (!res.empty()) { out[i].screen[0] = c8[8 * i + 6]; out[i].screen[1] = c8[8 * i + 7]; }
+    }
+}
+extern "C" void orc_perturb(void *h, int dbl, int n, const PathRec *in, const float *off, const float *disc, int occl, int fixed, int chadQuirk,
+                 PathRec *out, float *c8) {
+    OrcScene *s = (OrcScene *)h;
+    if (dbl) PerturbT(s->d, n, in, off, disc, occl, fixed, chadQuirk, out, c8);
+    else PerturbT(s->f, n, in, off, disc, occl, fixed, chadQuirk, out, c8);
+}
+
+template <class F>
+static void GenSubT(const SceneT<F> &s, int n, const int *cl, const float *uni, int nuni, PathRec *out, float *c8) {
+    Opts<F> o;
+    for (int i = 0; i < n; i++) {
+        ArraySrc<F> g(uni + size_t(i) * nuni, nuni);
+        PathS<F> p;
+        std::vector<Contrib<F>> res;
+        GenerateSubpath(s, o, cl[2 * i], cl[2 * i + 1], p, res, g);
+        Truncate(p, cl[2 * i], cl[2 * i + 1]);
+        ToRec(p, out[i]);
+        ContribOut(res, c8 + 8 * i);
+        if (!res.empty()) { out[i].screen[0] = c8[8 * i + 6]; out[i].screen[1] = c8[8 * i + 7]; }
+    }
+}
+extern "C" void orc_generate_subpath(void *h, int dbl, int n, const int *cl, const float *uni, int nuni, PathRec *out, float *c8) {
+    OrcScene *s = (OrcScene *)h;
+    if (dbl) GenSubT(s->d, n, cl, uni, nuni, out, c8);
+    else GenSubT(s->f, n, cl, uni, nuni, out, c8);
+}
+
+// Per sample up to maxc contributions, each 10 floats: c, l, r, g, b, lsScore, ssScore, sx, sy, mis.
+template <class F>
+static void GenBidirT(const SceneT<F> &s, int n, int minDepth, int maxDepth, const float *uni, int nuni, int maxc, float *cs, int *nc) {
+    Opts<F> o;
+    for (int i = 0; i < n; i++) {
+        ArraySrc<F> g(uni + size_t(i) * nuni, nuni);
+        PathS<F> p;
+        std::vector<Contrib<F>> res;
+        GeneratePathBidir(s, o, minDepth, maxDepth, p, res, g);
+        nc[i] = int(res.size());
+        for (int k = 0; k < (int)res.size() && k < maxc; k++) {
+            float *q = cs + (size_t(i) * maxc + k) * 10;
+            q[0] = float(res[k].c); q[1] = float(res[k].l);
+            q[2] = float(res[k].contrib.x); q[3] = float(res[k].contrib.y); q[4] = float(res[k].contrib.z);
+            q[5] = float(res[k].lsScore); q[6] = float(res[k].ssScore); q[7] = float(res[k].screen[0]); q[8] = float(res[k].screen[1]);
+            q[9] = float(res[k].mis);
+        }
+    }
+}
+extern "C" void orc_generate_bidir(void *h, int dbl, int n, int minDepth, int maxDepth, const float *uni, int nuni, int maxc, float *cs, int *nc) {
+    OrcScene *s = (OrcScene *)h;
+    if (dbl) GenBidirT(s->d, n, minDepth, maxDepth, uni, nuni, maxc, cs, nc);
+    else GenBidirT(s->f, n, minDepth, maxDepth, uni, nuni, maxc, cs, nc);
+}
+
+extern "C" void orc_grad_hess_fd(void *h, int n, const PathRec *in, double hstep, int chadQuirk, double *logf, double *grad, double *hess) {
+    OrcScene *s = (OrcScene *)h;
+    Opts<double> o;
+    o.chadMisQuirk = chadQuirk != 0;
+    for (int i = 0; i < n; i++) {
+        PathS<double> p = FromRec<double>(in[i]);
+        const int D = Dim<double>(p.c, p.l);
+        double zero[DMAX] = {0};
+        logf[i] = LogFFixed(s->d, o, p, zero);
+        std::vector<double> g(D), H(D * D);
+        GradHessFD(s->d, o, p, D, hstep, g.data(), H.data());
+        for (int a = 0; a < 32; a++) grad[32 * i + a] = a < D ? g[a] : 0.0;
+        for (int a = 0; a < 32 * 32; a++) hess[1024 * size_t(i) + a] = 0.0;
+        for (int a = 0; a < D; a++)
+            for (int b2 = 0; b2 < D; b2++) hess[1024 * size_t(i) + a * 32 + b2] = H[a * D + b2];
+    }
+}
+extern "C" void orc_logf_fixed(void *h, int n, const PathRec *in, const double *off, int chadQuirk, double *logf) {
+    OrcScene *s = (OrcScene *)h;
+    Opts<double> o;
+    o.chadMisQuirk = chadQuirk != 0;
+    for (int i = 0; i < n; i++) {
+        PathS<double> p = FromRec<double>(in[i]);
+        logf[i] = LogFFixed(s->d, o, p, off + 32 * size_t(i));
+    }
+}
+
+// grad: n x 32, hess: n x 32 x 32 (row-major, leading dimension 32)
+extern "C" void orc_compute_gaussian(int n, const int *Ds, const double *sc, double sigma, const double *grad, const double *hess, double *mean,
+                          double *covL, double *invCov, double *logDet) {
+    H2Param pr(sigma);
+    for (int i = 0; i < n; i++) {
+        const int D = Ds[i];
+        std::vector<double> g(D), H(D * D);
+        for (int a = 0; a < D; a++) g[a] = grad[32 * size_t(i) + a];
+        for (int a = 0; a < D; a++)
+            for (int b = 0; b < D; b++) H[a * D + b] = hess[1024 * size_t(i) + a * 32 + b];
+        Gauss G;
+        ComputeGaussian(pr, sc[i], D, g.data(), H.data(), G);
+        for (int a = 0; a < 32; a++) mean[32 * size_t(i) + a] = a < D ? G.mean[a] : 0.0;
+        for (int a = 0; a < 1024; a++) { covL[1024 * size_t(i) + a] = 0; invCov[1024 * size_t(i) + a] = 0; }
+        for (int a = 0; a < D; a++)
+            for (int b = 0; b < D; b++) {
+                covL[1024 * size_t(i) + a * 32 + b] = G.covL[a * D + b];
+                invCov[1024 * size_t(i) + a * 32 + b] = G.invCov[a * D + b];
+            }
+        logDet[i] = G.logDet;
+    }
+}
+extern "C" double orc_gaussian_logpdf(int D, const double *x, const double *mean, const double *invCov32, double logDet) {
+    Gauss G;
+    G.D = D;
+    G.mean.assign(mean, mean + D);
+    G.invCov.resize(D * D);
+    for (int a = 0; a < D; a++)
+        for (int b = 0; b < D; b++) G.invCov[a * D + b] = invCov32[a * 32 + b];
+    G.logDet = logDet;
+    return GaussianLogPdf(x, G);
+}
+extern "C" void orc_jacobi_eigen(int D, const double *A, double *lam, double *V) {
+    std::vector<double> a(A, A + D * D), v(D * D);
+    JacobiEigen(D, a.data(), v.data());
+    for (int i = 0; i < D; i++) lam[i] = a[i * D + i];
+    for (int i = 0; i < D * D; i++) V[i] = v[i];
+}
+
+// Direct-lighting pre-pass (integrator.h:33-74): film += contrib of the
+// unidirectional path of length <= 2, directSpp samples per pixel; the film is
+// returned un-normalised (divide by spp when merging).
+extern "C" void orc_direct_pass(void *h, int spp, uint64_t seed, int nthreads, float *film) {
+    OrcScene *s = (OrcScene *)h;
+    const int W = s->f.cam.W, H = s->f.cam.H;
+    std::vector<std::vector<float>> films(std::max(nthreads, 1), std::vector<float>(size_t(W) * H * 3, 0.f));
+    Opts<float> o;
+    ParallelFor(H, nthreads, [&](int y, int t) {
+        Philox rng(seed, uint64_t(y));
+        PhiloxSrc<float> g(&rng);
+        for (int x = 0; x < W; x++)
+            for (int k = 0; k < spp; k++) {
+                std::vector<Contrib<float>> out;
+                GeneratePathUni(s->f, o, x, y, -1, 2, out, g);
+                for (auto &c : out) {
+                    int ix = std::min(std::max(int(c.screen[0] * W), 0), W - 1);
+                    int iy = std::min(std::max(int(c.screen[1] * H), 0), H - 1);
+                    float *px = &films[t][(size_t(iy) * W + ix) * 3];
+                    if (std::isfinite(c.contrib.x) && std::isfinite(c.contrib.y) && std::isfinite(c.contrib.z)) {
+                        px[0] += c.contrib.x; px[1] += c.contrib.y; px[2] += c.contrib.z;
+                    }
+                }
+            }
+    });
+    for (size_t i = 0; i < size_t(W) * H * 3; i++) {
+        float a = 0;
+        for (auto &f : films) a += f[i];
+        film[i] = a;
+    }
+}
+
+// Pre-pass (delayed.cpp:14-173): numInit bidirectional samples with min
+// length 3, b = sum(lsScore)/numInit, lengthFunc[len] = sum of lsScore per
+// length, numChains equal-spaced seeds. Returns the number of candidates.
+extern "C" int orc_prepass(void *h, int numInit, int maxDepth, int numChains, uint64_t seed, int nthreads, double *bOut, float *lengthFunc, int nLen,
+                PathRec *seeds, float *seedC8) {
+    OrcScene *s = (OrcScene *)h;
+    Opts<float> o;
+    struct Cand { int sample, c, l; float ls; };
+    std::vector<std::vector<Cand>> per(numInit);
+    ParallelFor(numInit, nthreads, [&](int i, int) {
+        Philox rng(seed, uint64_t(i));
+        PhiloxSrc<float> g(&rng);
+        PathS<float> p;
+        std::vector<Contrib<float>> out;
+        GeneratePathBidir(s->f, o, 3, maxDepth, p, out, g);
+        for (auto &c : out) per[i].push_back({i, c.c, c.l, c.lsScore});
+    });
+    std::vector<Cand> all;
+    double total = 0;
+    std::vector<float> lf(nLen, 0.f);
+    for (auto &v : per)
+        for (auto &c : v) {
+            all.push_back(c);
+            total += c.ls;
+            int len = c.c + c.l - 1;
+            if (len < nLen) lf[len] += c.ls;
+        }
+    for (int i = 0; i < nLen; i++) lengthFunc[i] = lf[i];
+    *bOut = total / double(numInit);
+    if (all.empty()) return 0;
+    std::vector<double> cdf(all.size() + 1, 0.0);
+    for (size_t i = 0; i < all.size(); i++) cdf[i + 1] = cdf[i] + all[i].ls;
+    const double interval = cdf.back() / double(numChains);
+    Philox prng(seed ^ 0x5eed5eedull, 0);
+    double pos = prng.uni() * interval;
+    int cdfPos = 0;
+    for (int i = 0; i < numChains; i++) {
+        while (pos > cdf[cdfPos]) cdfPos = std::min(cdfPos + 1, int(all.size()) - 1);
+        const Cand &cd = all[std::max(cdfPos - 1, 0)];
+        Philox rng(seed, uint64_t(cd.sample));
+        PhiloxSrc<float> g(&rng);
+        PathS<float> p;
+        std::vector<Contrib<float>> out;
+        GeneratePathBidir(s->f, o, 3, maxDepth, p, out, g);
+        Contrib<float> sel;
+        for (auto &c : out)
+            if (c.c == cd.c && c.l == cd.l) sel = c;
+        Truncate(p, sel.c, sel.l);
+        ToRec(p, seeds[i]);
+        std::vector<Contrib<float>> one(1, sel);
+        ContribOut(one, seedC8 + 8 * i);
+        seeds[i].screen[0] = sel.screen[0];
+        seeds[i].screen[1] = sel.screen[1];
+        pos += interval;
+    }
+    return int(all.size());
+}
+
+// CPU DRMLT chain loop (delayed.cpp:334-673) over numChains chains, itersPerChain
+// iterations each, nthreads std::threads (chunk size 1, as ParallelFor). The
+// film returned is the indirect buffer (un-normalised). opts: type, largeStepProb,
+// perturbStdDev, anisoPerturbProb, anisoStdDev, discreteStdDev, roughThr,
+// maxDervDepth, acceptanceMap, chadMisQuirk.
+extern "C" void orc_render(void *h, const double *opts, const PathRec *seeds, const float *seedC8, int numChains, const float *lengthFunc, int nLen,
+                float b, int64_t itersPerChain, int nthreads, uint64_t seed, float *film, uint64_t *stats8, double *seconds) {
+    OrcScene *s = (OrcScene *)h;
+    DrOptions dro;
+    dro.type = int(opts[0]);
+    dro.largeStepProb = opts[1];
+    dro.perturbStdDev = opts[2];
+    dro.anisoPerturbProb = opts[3];
+    dro.anisoStdDev = opts[4];
+    dro.discreteStdDev = opts[5];
+    dro.roughThr = opts[6];
+    dro.maxDervDepth = int(opts[7]);
+    dro.acceptanceMap = opts[8] != 0;
+    dro.chadMisQuirk = opts[9] != 0;
+    Dist1D<float> ld;
+    ld.build(lengthFunc, nLen);
+    const int W = s->f.cam.W, H = s->f.cam.H;
+    const int nt = std::max(nthreads, 1);
+    std::vector<std::vector<float>> films(nt, std::vector<float>(size_t(W) * H * 3, 0.f));
+    std::vector<DrStats> st(nt);
+    auto t0 = std::chrono::steady_clock::now();
+    ParallelFor(numChains, nt, [&](int chain, int t) {
+        DrChainCtx<float> cx;
+        cx.s = &s->f;
+        cx.sd = &s->d;
+        cx.o.roughThr = float(dro.roughThr);
+        cx.od.roughThr = dro.roughThr;
+        cx.dro = dro;
+        cx.lengthDist = &ld;
+        cx.b = b;
+        cx.W = W;
+        cx.H = H;
+        float *fl = films[t].data();
+        cx.splat = [&](const float *sp, const V3<float> &v) {
+            int ix = std::min(std::max(int(sp[0] * W), 0), W - 1);
+            int iy = std::min(std::max(int(sp[1] * H), 0), H - 1);
+            if (std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z)) {
+                float *px = fl + (size_t(iy) * W + ix) * 3;
+                px[0] += v.x; px[1] += v.y; px[2] += v.z;
+            }
+        };
+        Philox rng(seed, uint64_t(chain) + 1000000007ull);
+        MState<float> cur, first, second, star;
+        cur.valid = true;
+        cur.path = FromRec<float>(seeds[chain]);
+        const float *c8 = seedC8 + 8 * chain;
+        cur.sp.c = seeds[chain].c; cur.sp.l = seeds[chain].l;
+        cur.sp.contrib = V3<float>(c8[1], c8[2], c8[3]);
+        cur.sp.lsScore = c8[4]; cur.sp.ssScore = c8[5];
+        cur.sp.screen[0] = c8[6]; cur.sp.screen[1] = c8[7];
+        for (int64_t it = 0; it < itersPerChain; it++) DrIteration(cx, cur, first, second, star, rng, st[t]);
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    *seconds = std::chrono::duration<double>(t1 - t0).count();
+    for (size_t i = 0; i < size_t(W) * H * 3; i++) {
+        float a = 0;
+        for (auto &f : films) a += f[i];
+        film[i] = a;
+    }
+    DrStats tot;
+    for (auto &x : st) {
+        tot.largeStep += x.largeStep; tot.largeStepAcc += x.largeStepAcc; tot.stage1 += x.stage1; tot.stage1Acc += x.stage1Acc;
+        tot.iso += x.iso; tot.isoAcc += x.isoAcc; tot.h2mc += x.h2mc; tot.h2mcAcc += x.h2mcAcc;
+    }
+    stats8[0] = tot.largeStep; stats8[1] = tot.largeStepAcc; stats8[2] = tot.stage1; stats8[3] = tot.stage1Acc;
+    stats8[4] = tot.iso; stats8[5] = tot.isoAcc; stats8[6] = tot.h2mc; stats8[7] = tot.h2mcAcc;
+}
+
+
+// Camera and area-sampling probes for the oracle's own tests.
+extern "C" void orc_camera_ray(void *h, double sx, double sy, double *org3, double *dir3, double *minmax) {
+    OrcScene *s = (OrcScene *)h;
+    Ray<double> r;
+    SamplePrimary(s->d.cam, sx, sy, r);
+    org3[0] = r.org.x; org3[1] = r.org.y; org3[2] = r.org.z;
+    dir3[0] = r.dir.x; dir3[1] = r.dir.y; dir3[2] = r.dir.z;
+    minmax[0] = r.minT; minmax[1] = r.maxT;
+}
+extern "C" int orc_project(void *h, const double *p3, double *s2) {
+    OrcScene *s = (OrcScene *)h;
+    return ProjectPoint(s->d.cam, V3<double>(p3[0], p3[1], p3[2]), s2[0], s2[1]) ? 1 : 0;
+}
+extern "C" void orc_sample_tri(void *h, int light, int prim, double u0, double u1, double *pos3, double *nrm3, double *pdf) {
+    OrcScene *s = (OrcScene *)h;
+    V3<double> p, n;
+    SampleTri(s->d, s->d.lights[light], prim, u0, u1, p, n, *pdf);
+    pos3[0] = p.x; pos3[1] = p.y; pos3[2] = p.z;
+    nrm3[0] = n.x; nrm3[1] = n.y; nrm3[2] = n.z;
+}
+extern "C" void orc_sample_param(void *h, int prim, const double *pos3, double *u2) {
+    OrcScene *s = (OrcScene *)h;
+    GetSampleParam(s->d, prim, V3<double>(pos3[0], pos3[1], pos3[2]), u2[0], u2[1]);
+}
