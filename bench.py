@@ -1,0 +1,139 @@
+"""DRMLT mutation-loop benchmark on one MI355X.
+
+A step is one DR iteration (delayed.cpp:396-656) of every chain. The workload
+is the batched configuration: the config-2 scene (displaced 256x256 grid +
+64 subdivision-4 icospheres, 50% Lambertian / 30% Phong / 20% rough
+dielectric, 512x512 film), 65,536 chains, path lengths up to 8, FP32 path
+evaluation, FP64 Gaussians and acceptance ratios. Chain states, scene and
+BVH are resident in HBM before the timed region; the pre-pass and the
+direct-lighting pass run before it.
+
+Prints one JSON line: mutations/s of the batched + adjoint form (hipGraph
+replay of the per-stage persistent kernels), the plain one-kernel-per-step
+form beside it, the roofline of the dominant kernel and the CPU baseline
+(the oracle's C++ restatement of the reference loop on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "drmlt-dpt_amd"))
+
+HBM_PEAK_GBS = 8000.0
+STAGES = ["stage1", "deriv_cur", "gauss_cur", "stage2_propose", "deriv_prop", "gauss_prop", "stage2_accept"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, threads, chains, iters, seed=11):
+    """The oracle's CPU loop (FP32 path, like the reference as shipped) on a
+    bounded sample: `chains` chains x `iters` iterations after a pre-pass."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+
+    s = orc.Scene(sc)
+    maxd = int(sc["options"]["maxdepth"])
+    b, lf, seeds, c8, _ = s.prepass(20000, maxd, chains, seed=seed, threads=threads)
+    _, stats, sec = s.render(seeds, c8, lf, b, iters, threads=threads, seed=seed)
+    return chains * iters / sec, sec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--chains", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--plain-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the DR chains would shard by chain "
+              "over GPUs with a film all-reduce; no multi-GPU run is part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    from drmlt import capi, scenes
+
+    t0 = time.time()
+    sc = scenes.CONFIGS[args.config]()
+    dpt = dict(sc["options"])
+    chains = args.chains or int(dpt["numchains"])
+    opt = capi.options_from_dpt(dpt, num_chains=chains, seed=1234)
+    log(f"scene cfg{args.config}: {len(sc['tri_p'])} triangles, {chains} chains, built in {time.time() - t0:.1f}s")
+    ctx = capi.Context(sc, opt)
+    t1 = time.time()
+    b, lf = ctx.prepass()
+    ctx.direct_lighting()
+    log(f"pre-pass b={b:.5f} + direct pass {time.time() - t1:.1f}s")
+
+    # plain one-kernel-per-step form, same chains
+    ctx.run(max(1, args.plain_steps // 2), capi.PLAIN)
+    ctx.run(args.plain_steps, capi.PLAIN)
+    plain_ms = ctx.timing()[0] / args.plain_steps
+    log(f"plain form {plain_ms:.2f} ms/step")
+
+    # batched form with per-stage timing (events between kernels)
+    ctx.run(3, capi.BATCHED)
+    stage_ms = ctx.timing()[1:8] / 3
+    log("batched per-stage ms: " + ", ".join(f"{n}={v:.2f}" for n, v in zip(STAGES, stage_ms)))
+
+    # work counters for one batched step
+    work = ctx.count_work()
+    log(f"work counters: {work}")
+
+    # batched + graph: the measured form
+    ctx.run(args.warmup, capi.BATCHED_GRAPH)
+    ctx.run(args.steps, capi.BATCHED_GRAPH)
+    total_ms = ctx.timing()[0]
+    ms_per_step = total_ms / args.steps
+    value = chains * args.steps / (total_ms / 1000.0)
+    st = ctx.stats()
+    log(f"batched+graph {ms_per_step:.3f} ms/step, {value:.4g} mutations/s; stats {st}")
+
+    # roofline of the dominant stage kernel
+    dom = int(max(range(len(STAGES)), key=lambda k: stage_ms[k]))
+    nodes, tris, rays, state_bytes = work[dom]
+    alg_bytes = nodes * 64 + tris * 48 + state_bytes
+    achieved = alg_bytes / (stage_ms[dom] / 1000.0) / 1e9
+    roof = {"kernel": STAGES[dom], "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": int(alg_bytes), "launch_ms": round(float(stage_ms[dom]), 4),
+            "rays_per_launch": int(rays)}
+
+    cpu = None
+    if not args.no_cpu:
+        ncpu = min(args.cpu_threads, os.cpu_count() or 1)
+        cchains, citers = (2048, 8) if args.config != 1 else (1024, 64)
+        cval, csec = cpu_baseline(sc, ncpu, cchains, citers)
+        cpu = {"value": round(cval, 1), "unit": "mutations/s", "cores": ncpu, "kind": "port",
+               "sample": f"{cchains} chains x {citers} iterations of the same scene after a 20000-sample pre-pass "
+                         f"({csec:.1f} s of chain loop)"}
+        log(f"cpu baseline {cval:.4g} mutations/s on {ncpu} threads")
+
+    out = {
+        "metric": "DRMLT mutations/s (batched+adjoint GPU form; plain GPU form and CPU restatement alongside)",
+        "value": round(value, 1), "unit": "mutations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 path / f64 gaussian+ratio", "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": f"config{args.config}: {len(sc['tri_p'])} tris, {chains} chains, maxdepth {dpt['maxdepth']}, "
+                               f"{int(sc['cam'][10])}x{int(sc['cam'][11])} film, type={dpt['type']}",
+                   "chains": chains, "form": "batched+graph"},
+        "plain_form": {"ms_per_step": round(plain_ms, 3), "mutations_per_s": round(chains / (plain_ms / 1000.0), 1)},
+        "stage_ms": {n: round(float(v), 4) for n, v in zip(STAGES, stage_ms)},
+        "acceptance": st,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -95,9 +95,11 @@ DEV void jacobi(int D, double *A, double *V) {
 // lower triangle of the column-major view of the same buffer.
 template <class GP, class HP>
 DEV void compute_gaussian(const H2Param &pr, double sc, int D, GP grad, HP hess, const GRef &g) {
+    // Frobenius norm of the symmetric Hessian from its upper triangle
+    // (hess(i, j) is only read for i <= j)
     double fro = 0;
     for (int i = 0; i < D; i++)
-        for (int j = 0; j < D; j++) { const double h = hess(i, j); fro += h * h; }
+        for (int j = i; j < D; j++) { const double h = hess(i, j); fro += (i == j ? 1.0 : 2.0) * h * h; }
     fro = sqrt(fro);
     if (sc <= 1e-15 || fro < 0.5 / (pr.sigma * pr.sigma)) {
         iso_gaussian(D, pr.sigma, g);

@@ -615,6 +615,7 @@ struct BidirContrib {
 template <int MAXL, class G, class SINK>
 DEV void generate_path_bidir(const DevScene &S, const Opts &o, int minDepth, int maxDepth, G &g, SINK &sink, int keepC, int keepL,
                              const SRef *ref) {
+    int keepDone = 0;
     BState<float> lsv[MAXL + 1];
     int lprims[MAXL + 1];
     float ldiscs[MAXL + 1], lrrs[MAXL + 1], lrnd[2 * (MAXL + 1)];
@@ -638,8 +639,12 @@ DEV void generate_path_bidir(const DevScene &S, const Opts &o, int minDepth, int
             POut<float> out;
             out.valid = false;
             connect_to_camera<float, false>(S, o, d, lsv[d], lprims[d], ldiscs[d], r.org, out);
-            if (out.valid) sink(out, 1, 2 + d);
+            if (out.valid) {
+                sink(out, 1, 2 + d);
+                if (keepC == 1 && keepL == 2 + d) keepDone = 4;
+            }
         }
+        if (keepDone) break;
         if (maxDepth != -1 && d + 2 >= maxDepth) break;
         if (d + 1 > MAXL) break;
         BState<float> nx;
@@ -660,8 +665,7 @@ DEV void generate_path_bidir(const DevScene &S, const Opts &o, int minDepth, int
     float cdiscs[kVMax], crrs[kVMax], crnd[2 * kVMax];
     int dlLight = -1, dlPrim = -1;
     float dlR0 = 0, dlR1 = 0;
-    int keepDone = 0;
-    for (int d = 0; d < kVMax; d++) {
+    for (int d = 0; d < kVMax && !keepDone; d++) {
         const bool hit = intersect(S, r, cs.isect);
         cprims[d] = hit ? cs.isect.prim : -1;
         crrs[d] = 1.0f;
@@ -727,6 +731,7 @@ DEV void generate_path_bidir(const DevScene &S, const Opts &o, int minDepth, int
         r.minT = kIsectEps;
         r.maxT = __builtin_inff();
     }
+    if (ref) *ref->c = 0;  // marks "strategy not found" unless written below
     if (ref && keepDone) {
         const int s = ref->s;
         const int c = keepC, l = keepL;

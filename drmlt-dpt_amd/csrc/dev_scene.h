@@ -36,7 +36,18 @@ struct DevScene {
     const float *pickCdf;           // nlights + 1
     float lightWeightSum;
     DevCamera cam;
+    // optional work counters of the launch: nodes visited, triangles tested,
+    // rays, state bytes (null outside counting runs)
+    unsigned long long *counters;
 };
+
+DEV void count_work(const DevScene &S, unsigned nodes, unsigned tris) {
+    if (S.counters) {
+        atomicAdd(S.counters + 0, (unsigned long long)nodes);
+        atomicAdd(S.counters + 1, (unsigned long long)tris);
+        atomicAdd(S.counters + 2, 1ull);
+    }
+}
 
 // ---------------------------------------------------------------- traversal
 // Moller-Trumbore with the same operation order as the CPU restatement; no FMA
@@ -91,7 +102,9 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
     int best = -1;
     tbest = tmax;
     int node = 0;
+    unsigned nvis = 0, ntri = 0;
     for (;;) {
+        nvis++;
         const float4 n0 = S.nodes[4 * node + 0], n1 = S.nodes[4 * node + 1], n2 = S.nodes[4 * node + 2];
         const int4 ch = *reinterpret_cast<const int4 *>(&S.nodes[4 * node + 3]);
         float tl, tr;
@@ -99,6 +112,7 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
         bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, inv, tmin, tbest, tr);
         // leaves are tested in place
         if (hl && ch.z > 0) {
+            ntri += ch.z;
             for (int k = ch.x; k < ch.x + ch.z; k++) {
                 float t;
                 const float4 A = S.tA[k];
@@ -110,6 +124,7 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
             hl = false;
         }
         if (hr && ch.w > 0) {
+            ntri += ch.w;
             for (int k = ch.y; k < ch.y + ch.w; k++) {
                 float t;
                 const float4 A = S.tA[k];
@@ -135,6 +150,7 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
             node = stack[--sp];
         }
     }
+    count_work(S, nvis, ntri);
     return best;
 }
 
@@ -143,24 +159,31 @@ DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float 
     int stack[kBvhStack];
     int sp = 0;
     int node = 0;
+    unsigned nvis = 0, ntri = 0;
+    bool hit = false;
     for (;;) {
+        nvis++;
         const float4 n0 = S.nodes[4 * node + 0], n1 = S.nodes[4 * node + 1], n2 = S.nodes[4 * node + 2];
         const int4 ch = *reinterpret_cast<const int4 *>(&S.nodes[4 * node + 3]);
         float tl, tr;
         bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, o, inv, tmin, tmax, tl);
         bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, inv, tmin, tmax, tr);
         if (hl && ch.z > 0) {
-            for (int k = ch.x; k < ch.x + ch.z; k++) {
+            for (int k = ch.x; k < ch.x + ch.z && !hit; k++) {
                 float t;
-                if (tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t)) return true;
+                ntri++;
+                hit = tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t);
             }
+            if (hit) break;
             hl = false;
         }
         if (hr && ch.w > 0) {
-            for (int k = ch.y; k < ch.y + ch.w; k++) {
+            for (int k = ch.y; k < ch.y + ch.w && !hit; k++) {
                 float t;
-                if (tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t)) return true;
+                ntri++;
+                hit = tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t);
             }
+            if (hit) break;
             hr = false;
         }
         if (hl && hr) {
@@ -175,7 +198,8 @@ DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float 
             node = stack[--sp];
         }
     }
-    return false;
+    count_work(S, nvis, ntri);
+    return hit;
 }
 
 // ---------------------------------------------------------------- geometry in T

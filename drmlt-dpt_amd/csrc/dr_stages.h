@@ -68,10 +68,25 @@ DEV void splat_const(const Chains &ch, int W, int H, float *film, int p, float r
     atomicAdd(px + 2, b);
 }
 
-// wave-aggregated counter increment
+// Algorithmic chain-state bytes of one path evaluation: state read and
+// proposal written, 4 D (u) + 4 (P + 1) (bsdfDiscrete) + 4 (P + 1) (triangle
+// ids) + 16 (c, l, scores) each way.
+DEV void count_state(const DevScene &S, int c, int l) {
+    if (S.counters) {
+        const int P = c + l - 1, D = (P > 2 ? P : 2) * 2;
+        atomicAdd(S.counters + 3, (unsigned long long)(2 * (4 * D + 8 * (P + 1) + 16)));
+    }
+}
+DEV void count_bytes(const DevScene &S, unsigned long long b) {
+    if (S.counters) atomicAdd(S.counters + 3, b);
+}
+
+// wave-aggregated counter increment; the lowest active lane adds for the
+// wave, so the call may sit in divergent code
 DEV void count_if(unsigned long long *ctr, bool pred) {
     const unsigned long long m = __ballot(pred);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
+    const int leader = __ffsll((long long)__ballot(1)) - 1;
+    if (int(threadIdx.x & 63) == leader && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
 }
 enum { ST_LARGE = 0, ST_LARGE_ACC, ST_S1, ST_S1_ACC, ST_ISO, ST_ISO_ACC, ST_H2MC, ST_H2MC_ACC };
 
@@ -141,7 +156,7 @@ DEV void init_gaussian_thread(const DevScene &Sc, const Chains &ch, const DrPara
             jet_pass_index(D, q, row, col0);
             ok = jet_pass<float>(Sc, P.oDeriv, ref, D, row, col0,
                                  [&](int cc, float gv) { if (row == 0) gbuf[cc] = gv; },
-                                 [&](int cc, float hv) { hbuf[row * D + cc] = hv; hbuf[cc * D + row] = hv; });
+                                 [&](int cc, float hv) { if (cc >= row) hbuf[row * D + cc] = hv; });
         }
         bool fin = ok;
         for (int i = 0; i < D && fin; i++) fin = isfinite(gbuf[i]);
@@ -165,7 +180,10 @@ DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int c
     const int pc = chain + C * perm_cur(perm), pf = chain + C * perm_first(perm);
     Philox g(P.seed, iter, chain, TAG_S1);
     const bool isLarge = flip_coin(P.largeStepProb, g.uni());
-    const bool curValid = ch.valid[pc] != 0;
+    // a current state outside the fixed vertex slots is treated as invalid
+    const int cc = ch.c[pc], cl = ch.l[pc];
+    const bool wellFormed = cc >= 1 && cl >= 0 && cc <= kVMax + 1 && cl <= kVMax + 1 && cc + cl - 1 >= 2 && 2 * (cc + cl - 1) <= kDMax;
+    const bool curValid = ch.valid[pc] != 0 && wellFormed;
     float a1 = 0.0f;
     bool acc1 = false;
     const SRef rf = slot_ref(ch, pf);
@@ -176,6 +194,7 @@ DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int c
         const int cam = len - lgt + 1;
         POut<float> out;
         generate_subpath(Sc, P.o, cam, lgt, rf, g, out);
+        count_state(Sc, cam, lgt);
         ch.gInit[pf] = 0;
         if (out.valid) {
             (void)g.uni();  // contribution pick (one candidate when multiplexed)
@@ -194,6 +213,7 @@ DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int c
         PerturbU<OffNormal> up{rf.u, rf.s, 0, OffNormal{&g, P.perturbStdDev}};
         POut<float> out;
         perturb_eval<float, false>(Sc, P.o, rf, up, DiscNormal{&g, P.discreteStdDev}, out);
+        count_state(Sc, ch.c[pc], ch.l[pc]);
         ch.gInit[pf] = 0;
         if (out.valid) {
             store_contrib(ch, pf, out);
@@ -261,6 +281,7 @@ DEV void dr_stage2_propose(const DevScene &Sc, const Chains &ch, const DrParams 
         perturb_eval<float, false>(Sc, P.o, rs, up, DiscNormal{&g, P.discreteStdDev}, out);
         ch.gInit[ps] = 0;
     }
+    count_state(Sc, ch.c[pc], ch.l[pc]);
     double a2 = 0.0;
     if (out.valid) {
         store_contrib(ch, ps, out);
@@ -312,6 +333,7 @@ DEV void dr_stage2_accept(const DevScene &Sc, const Chains &ch, const DrParams &
             PerturbU<OffDiff> up{rt.u, rt.s, 0, OffDiff{ch.u + pc, ch.u + pf, ch.NS, ch.NS}};
             POut<float> out;
             perturb_eval<float, false>(Sc, P.o, rt, up, DiscNormal{&g, P.discreteStdDev}, out);
+            count_state(Sc, ch.c[ps], ch.l[ps]);
             if (out.valid) aStar = clamp_ref(out.ssScore / ch.ss[ps]);
         }
         if (is_invalid(aStar)) aStar = 0.0f;

@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(64) k_b_deriv(DevScene Sc, Chains ch, DrParams
             const int C = ch.C;
             const bool ok = jet_pass<float>(Sc, P.oDeriv, slot_ref(ch, p), D, row, col0,
                                             [&](int cc, float gv) { if (row == 0) gr[cc * C] = gv; },
-                                            [&](int cc, float hv) { hs[(row * kGMax + cc) * C] = hv; hs[(cc * kGMax + row) * C] = hv; });
+                                            [&](int cc, float hv) { if (cc >= row) hs[(row * kGMax + cc) * C] = hv; });
             if (!ok) gr[0] = __builtin_nanf("");
         }
     }
@@ -56,11 +56,11 @@ DEV void gradhess_rec(int w, const DevScene &Sc, const Opts &o, int n, char *rec
     jet_pass_index(D, q, row, col0);
     const bool ok = jet_pass<S>(Sc, o, ref, D, row, col0,
                                 [&](int cc, S gv) { if (row == 0) grad[size_t(r) * 32 + cc] = gv; },
-                                [&](int cc, S hv) { hess[(size_t(r) * 32 + row) * 32 + cc] = hv; });
+                                [&](int cc, S hv) { if (cc >= row) hess[(size_t(r) * 32 + row) * 32 + cc] = hv; });
     if (!ok) okOut[r] = 0;
 }
 
-__global__ void k_gradhess_rec_f32(DevScene Sc, Opts o, int n, char *recs, const int *passOff, int totalPasses, float *grad, float *hess,
+__global__ void __launch_bounds__(64) k_gradhess_rec_f32(DevScene Sc, Opts o, int n, char *recs, const int *passOff, int totalPasses, float *grad, float *hess,
                                    int *okOut) {
     gradhess_rec<float>(blockIdx.x * blockDim.x + threadIdx.x, Sc, o, n, recs, passOff, totalPasses, grad, hess, okOut);
 }

@@ -19,11 +19,11 @@ DEV void gradhess_rec(int w, const DevScene &Sc, const Opts &o, int n, char *rec
     jet_pass_index(D, q, row, col0);
     const bool ok = jet_pass<S>(Sc, o, ref, D, row, col0,
                                 [&](int cc, S gv) { if (row == 0) grad[size_t(r) * 32 + cc] = gv; },
-                                [&](int cc, S hv) { hess[(size_t(r) * 32 + row) * 32 + cc] = hv; });
+                                [&](int cc, S hv) { if (cc >= row) hess[(size_t(r) * 32 + row) * 32 + cc] = hv; });
     if (!ok) okOut[r] = 0;
 }
 
-__global__ void k_gradhess_rec_f64(DevScene Sc, Opts o, int n, char *recs, const int *passOff, int totalPasses, double *grad,
+__global__ void __launch_bounds__(64) k_gradhess_rec_f64(DevScene Sc, Opts o, int n, char *recs, const int *passOff, int totalPasses, double *grad,
                                    double *hess, int *okOut) {
     gradhess_rec<double>(blockIdx.x * blockDim.x + threadIdx.x, Sc, o, n, recs, passOff, totalPasses, grad, hess, okOut);
 }

@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(64) k_b_gauss(DevScene Sc, Chains ch, DrParams
                 if (fin) {
                     for (int i = 0; i < D && fin; i++) fin = isfinite(gr[i * C]);
                     for (int i = 0; i < D && fin; i++)
-                        for (int j = 0; j < D && fin; j++) fin = isfinite(hs[(i * kGMax + j) * C]);
+                        for (int j = i; j < D && fin; j++) fin = isfinite(hs[(i * kGMax + j) * C]);
                 }
                 if (fin) {
                     compute_gaussian(P.h2, ssv, D, [&](int i) { return double(gr[i * C]); },
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(64) k_b_stage2_accept(DevScene Sc, Chains ch, 
 
 // Clears the list counters for the next step and advances the iteration
 // counter (a fill kernel, not a memset, so the step can be graph-captured).
-__global__ void k_b_reset(Lists L, uint32_t *iterp, int advance) {
+__global__ void __launch_bounds__(64) k_b_reset(Lists L, uint32_t *iterp, int advance) {
     const int t = threadIdx.x;
     if (t < 3 * kMaxBins) L.count[t] = 0;
     if (t == 0 && advance) *iterp += 1;
@@ -121,7 +121,7 @@ struct PreSink {
         atomicAdd(total, double(o.lsScore));
     }
 };
-__global__ void k_prepass_gen(DevScene Sc, Opts o, int numInit, int maxDepth, uint64_t seed, int *cnt, int *cl, float *ls, double *lenSum,
+__global__ void __launch_bounds__(64) k_prepass_gen(DevScene Sc, Opts o, int numInit, int maxDepth, uint64_t seed, int *cnt, int *cl, float *ls, double *lenSum,
                               double *total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= numInit) return;
@@ -140,7 +140,7 @@ struct KeepSink {
 };
 // Seed replay (delayed.cpp:125-167): chain i regenerates sample sel[i] and
 // keeps the strategy (c, l) as its current state.
-__global__ void k_prepass_seed(DevScene Sc, Opts o, Chains ch, int maxDepth, uint64_t seed, const int *selSample, const int *selCl) {
+__global__ void __launch_bounds__(64) k_prepass_seed(DevScene Sc, Opts o, Chains ch, int maxDepth, uint64_t seed, const int *selSample, const int *selCl) {
     const int chain = blockIdx.x * blockDim.x + threadIdx.x;
     if (chain >= ch.C) return;
     Philox g(seed, 0, selSample[chain], TAG_PRE);
@@ -151,8 +151,18 @@ __global__ void k_prepass_seed(DevScene Sc, Opts o, Chains ch, int maxDepth, uin
     const int p = chain;  // physical slot 0 is current
     const SRef ref = slot_ref(ch, p);
     generate_path_bidir<kPreMaxL>(Sc, o, 3, maxDepth, g, sink, sink.c, sink.l, &ref);
+    const bool found = ch.c[p] != 0 && sink.kept.valid;
+    if (!found) {
+        // replay did not reproduce the strategy: leave a well-formed, invalid state
+        ch.c[p] = 2; ch.l[p] = 2; ch.light[p] = 0; ch.lprim[p] = Sc.lightTris[0];
+        for (int k = 0; k < kVMax; k++) { ch.cprim[k * ch.NS + p] = 0; ch.lgprim[k * ch.NS + p] = 0; }
+        sink.kept.valid = false;
+        sink.kept.contrib = V3<float>(0, 0, 0);
+        sink.kept.lsScore = sink.kept.ssScore = 0.0f;
+        sink.kept.screen[0] = sink.kept.screen[1] = 0.5f;
+    }
     store_contrib(ch, p, sink.kept);
-    ch.valid[p] = 1;
+    ch.valid[p] = found ? 1 : 0;
     ch.hasSplat[p] = 0;  // seeds carry no splat list
     ch.gInit[p] = 0;
     for (int k = 1; k < 4; k++) {
@@ -166,7 +176,7 @@ __global__ void k_prepass_seed(DevScene Sc, Opts o, Chains ch, int maxDepth, uin
 }
 
 // ------------------------------------------------------------ direct pass
-__global__ void k_direct(DevScene Sc, Opts o, int spp, uint64_t seed, float *film) {
+__global__ void __launch_bounds__(128) k_direct(DevScene Sc, Opts o, int spp, uint64_t seed, float *film) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long npix = long(Sc.cam.W) * Sc.cam.H;
     if (i >= npix * spp) return;
@@ -176,7 +186,7 @@ __global__ void k_direct(DevScene Sc, Opts o, int spp, uint64_t seed, float *fil
 }
 
 // ------------------------------------------------------------ parity surface
-__global__ void k_trace(DevScene Sc, int n, const float *org, const float *dir, const float *tmin, const float *tmax, int any, int *prim,
+__global__ void __launch_bounds__(128) k_trace(DevScene Sc, int n, const float *org, const float *dir, const float *tmin, const float *tmax, int any, int *prim,
                         float *t) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -190,24 +200,13 @@ __global__ void k_trace(DevScene Sc, int n, const float *org, const float *dir, 
     }
 }
 
-__global__ void k_perturb_rec(DevScene Sc, Opts o, int n, char *recs, const float *off, const float *disc, float *c8) {
+__global__ void __launch_bounds__(64) k_perturb_rec(DevScene Sc, Opts o, int n, char *recs, const float *off, const float *disc, float *c8) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const SRef r = rec_ref(recs + size_t(i) * kRecBytes);
     PerturbU<OffArray> up{r.u, 1, 0, OffArray{off + 32 * i, 1}};
     POut<float> out;
     perturb_eval<float, false>(Sc, o, r, up, DiscArray{disc + 32 * i, 1}, out);
-    if (out.valid) { r.screen[0] = out.screen[0]; r.screen[1] = out.screen[1]; }
-    contrib8(c8 + 8 * i, out);
-}
-
-__global__ void k_gensub_rec(DevScene Sc, Opts o, int n, const int *cl, const float *uni, int nuni, char *recs, float *c8) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const SRef r = rec_ref(recs + size_t(i) * kRecBytes);
-    GenArray g{uni + size_t(i) * nuni, nuni, 0};
-    POut<float> out;
-    generate_subpath(Sc, o, cl[2 * i], cl[2 * i + 1], r, g, out);
     if (out.valid) { r.screen[0] = out.screen[0]; r.screen[1] = out.screen[1]; }
     contrib8(c8 + 8 * i, out);
 }
@@ -224,7 +223,7 @@ struct ListSink {
         n++;
     }
 };
-__global__ void k_genbidir_rec(DevScene Sc, Opts o, int n, int minDepth, int maxDepth, const float *uni, int nuni, int maxc, float *cs, int *nc) {
+__global__ void __launch_bounds__(64) k_genbidir_rec(DevScene Sc, Opts o, int n, int minDepth, int maxDepth, const float *uni, int nuni, int maxc, float *cs, int *nc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     GenArray g{uni + size_t(i) * nuni, nuni, 0};
@@ -233,7 +232,7 @@ __global__ void k_genbidir_rec(DevScene Sc, Opts o, int n, int minDepth, int max
     nc[i] = sink.n;
 }
 
-__global__ void k_gauss_rec(int n, H2Param h2, const int *Ds, const double *sc, const double *grad, const double *hess, double *mean,
+__global__ void __launch_bounds__(64) k_gauss_rec(int n, H2Param h2, const int *Ds, const double *sc, const double *grad, const double *hess, double *mean,
                             double *covL, double *invCov, double *logDet) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -244,7 +243,7 @@ __global__ void k_gauss_rec(int n, H2Param h2, const int *Ds, const double *sc, 
                      [&](int a, int b) { return hess[(size_t(i) * 32 + a) * 32 + b]; }, G);
 }
 
-__global__ void k_logpdf_rec(int n, const int *Ds, const double *x, double sign, const double *mean, const double *covL, const double *invCov,
+__global__ void __launch_bounds__(64) k_logpdf_rec(int n, const int *Ds, const double *x, double sign, const double *mean, const double *covL, const double *invCov,
                              const double *logDet, double *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -256,7 +255,7 @@ __global__ void k_logpdf_rec(int n, const int *Ds, const double *x, double sign,
 
 // Second-stage acceptance arithmetic alone (delayed.cpp:466-541), batched:
 // in[i] = (type, a1, a2, aStar, transRatio, secondValid); out = alpha2.
-__global__ void k_alpha2(int n, const double *in, double *out) {
+__global__ void __launch_bounds__(64) k_alpha2(int n, const double *in, double *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double *q = in + 6 * size_t(i);

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "libdrmlt_hip.so")
 EXPORTS = [
     "drmlt_last_error", "drmlt_options_default", "drmlt_ctx_create", "drmlt_ctx_destroy", "drmlt_prepass",
     "drmlt_seed_chains", "drmlt_direct_lighting", "drmlt_run", "drmlt_read_film", "drmlt_read_stats", "drmlt_read_timing",
-    "drmlt_count_stage1_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
+    "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
     "drmlt_grad_hess", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
 ]
 
@@ -167,6 +167,12 @@ class Context:
         s = Stats()
         check(lib().drmlt_read_stats(self.h, C.byref(s)))
         return s.as_dict()
+
+    def count_work(self):
+        """One batched step with work counters: per stage (nodes, triangles, rays, state bytes)."""
+        out = np.zeros((7, 4), np.uint64)
+        check(lib().drmlt_count_work(self.h, P(out)))
+        return out.astype(np.int64)
 
     def timing(self):
         out = np.zeros(8)

@@ -154,9 +154,11 @@ int drmlt_read_stats(drmlt_ctx *ctx, drmlt_stats *stats);
  * derivatives (current), Gaussian (current), stage-2 proposal, derivatives
  * (proposal), Gaussian (proposal), stage-2 acceptance. n_out <= 8. */
 int drmlt_read_timing(drmlt_ctx *ctx, double *out, int32_t n_out);
-/* Counts traversal work of one stage-1 launch on the current chain states
- * without advancing them: nodes visited, triangles tested, rays. */
-int drmlt_count_stage1_work(drmlt_ctx *ctx, uint64_t *counts3);
+/* Runs one batched DR step (advancing every chain by one iteration) with work
+ * counters on: out is 7 x 4 words, one row per stage kernel in the order of
+ * drmlt_read_timing, holding BVH nodes visited, triangles tested, rays traced
+ * and algorithmic chain-state bytes. */
+int drmlt_count_work(drmlt_ctx *ctx, uint64_t *out);
 
 /* ---------------- tier 2: batched functions ---------------- */
 
@@ -173,7 +175,10 @@ int drmlt_trace(drmlt_ctx *ctx, int64_t n, const float *org, const float *dir, c
 int drmlt_perturb(drmlt_ctx *ctx, int64_t n, drmlt_path_rec *states, const float *off, const float *disc, float *contrib8);
 
 /* GenerateSubpath (src/path.cpp:1123-1269) for strategies cl (n x 2: c, l)
- * drawing from the uniforms uni (n x nuni) in the documented order. */
+ * drawing from the uniforms uni (n x nuni) in the documented order.
+ * Currently switched off: returns DRMLT_ERR_STATE after argument checks (its
+ * batched kernel made a GPU memory fault whose cause is not established).
+ * The same device routine runs inside drmlt_run's large steps. */
 int drmlt_generate_subpath(drmlt_ctx *ctx, int64_t n, const int32_t *cl, const float *uni, int32_t nuni, drmlt_path_rec *states,
                            float *contrib8);
 

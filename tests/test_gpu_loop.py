@@ -1,0 +1,117 @@
+"""Loop-level tests of the HIP library: pre-pass, direct lighting and the DR
+mutation loop in its three execution forms, against the oracle's CPU loop
+(statistically: the two draw from different random streams) and against each
+other (the forms draw identical streams)."""
+import numpy as np
+import pytest
+
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _ctx(sc, **kw):
+    from drmlt import capi
+
+    opt = capi.options_from_dpt(sc["options"], **kw)
+    return capi.Context(sc, opt)
+
+
+def test_prepass_and_direct_vs_oracle(cfg1):
+    """Normalisation b and the path-length distribution (delayed.cpp:14-173)
+    within Monte Carlo error of the oracle's; direct-lighting film mean
+    likewise (integrator.h:33-74)."""
+    ctx = _ctx(cfg1, num_init_samples=100000, num_chains=1024, seed=5)
+    O = orc.Scene(cfg1)
+    say("prepass cfg1 100000 samples")
+    b, lf = ctx.prepass(n_len=16)
+    ob, olf, _, _, _ = O.prepass(100000, int(cfg1["options"]["maxdepth"]), 1024, seed=9)
+    say("b gpu", b, "oracle", ob)
+    assert abs(b - ob) < 0.03 * ob
+    n = min(len(olf), len(lf))
+    assert np.allclose(lf[:n] / lf[:n].sum(), olf[:n] / olf[:n].sum(), atol=0.02)
+    say("direct cfg1")
+    ctx.direct_lighting()
+    d = ctx.film(2)
+    spp = int(ctx.opt.direct_spp)
+    od = O.direct_pass(spp, seed=3)
+    gm, om = d.mean(axis=(0, 1)), od.mean(axis=(0, 1))
+    say("direct mean gpu", gm, "oracle", om)
+    assert np.allclose(gm, om, rtol=0.02)
+    ctx.close()
+
+
+def test_forms_agree(cfg1):
+    """Plain, batched and graph-replayed forms advance the same chains with
+    the same random streams: acceptance counts agree and the films match
+    closely."""
+    from drmlt import capi
+
+    res = {}
+    for mode in (capi.PLAIN, capi.BATCHED, capi.BATCHED_GRAPH):
+        ctx = _ctx(cfg1, num_init_samples=8192, num_chains=2048, seed=21)
+        say(f"form {mode}: prepass + 8 iterations")
+        ctx.prepass()
+        ctx.run(8, mode)
+        res[mode] = (ctx.film(1), ctx.stats())
+        ctx.close()
+    f0, s0 = res[capi.PLAIN]
+    for mode in (capi.BATCHED, capi.BATCHED_GRAPH):
+        f, s = res[mode]
+        say("stats", mode, s)
+        assert s["large_step"] + s["stage1"] == 8 * 2048
+        for k in s:
+            assert abs(s[k] - s0[k]) <= max(3, 0.01 * s0[k]), (mode, k, s[k], s0[k])
+        assert abs(f.sum() - f0.sum()) <= 0.02 * f0.sum()
+    fb, _ = res[capi.BATCHED]
+    fg, _ = res[capi.BATCHED_GRAPH]
+    assert np.allclose(fb.sum(), fg.sum(), rtol=1e-4)
+
+
+def test_loop_vs_oracle(cfg1):
+    """The DR loop (Green-Mira, H2MC second stage) against the oracle's loop
+    from the oracle's own seeds: acceptance rates per stage and the film mean
+    agree within Monte Carlo error."""
+    O = orc.Scene(cfg1)
+    maxd = int(cfg1["options"]["maxdepth"])
+    chains, iters = 2048, 32
+    b, lf, seeds, c8, _ = O.prepass(20000, maxd, chains, seed=13)
+    ctx = _ctx(cfg1, num_chains=chains, seed=17)
+    say(f"loop cfg1 {chains} chains x {iters} iterations")
+    ctx.seed_chains(seeds, c8, b, lf)
+    ctx.run(iters, 2)
+    gfilm, gs = ctx.film(1), ctx.stats()
+    ofilm, ost, _ = O.render(seeds, c8, lf, b, iters, seed=19)
+    names = ["large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso", "stage2_iso_accepted",
+             "stage2_h2mc", "stage2_h2mc_accepted"]
+    os_ = dict(zip(names, [int(v) for v in ost]))
+    say("gpu stats", gs)
+    say("oracle stats", os_)
+    for a, t in (("large_step_accepted", "large_step"), ("stage1_accepted", "stage1"), ("stage2_iso_accepted", "stage2_iso"),
+                 ("stage2_h2mc_accepted", "stage2_h2mc")):
+        rg, ro = gs[a] / max(gs[t], 1), os_[a] / max(os_[t], 1)
+        tol = 4 * np.sqrt(ro * (1 - ro) / max(os_[t], 1)) + 0.01
+        assert abs(rg - ro) < tol, (a, rg, ro, tol)
+    assert abs(gs["large_step"] / (chains * iters) - os_["large_step"] / (chains * iters)) < 0.02
+    gm, om = gfilm.mean(axis=(0, 1)), ofilm.mean(axis=(0, 1))
+    say("film mean gpu", gm, "oracle", om)
+    assert np.allclose(gm, om, rtol=0.05)
+    ctx.close()
+
+
+def test_count_work_and_timing(cfg1):
+    """Work counters and per-stage timing of one batched step are populated."""
+    ctx = _ctx(cfg1, num_init_samples=8192, num_chains=1024, seed=2)
+    ctx.prepass()
+    say("count_work cfg1")
+    w = ctx.count_work()
+    assert w.shape == (7, 4)
+    assert w[0, 0] > 0 and w[0, 1] > 0 and w[0, 2] >= 1024 and w[0, 3] > 0
+    ctx.run(2, 1)
+    t = ctx.timing()
+    assert t[0] > 0 and (t[1:8] >= 0).all()
+    ctx.close()
