@@ -16,7 +16,7 @@ struct DevCamera {
     int W, H;
 };
 
-struct DevScene {
+struct DevSceneData {
     // traversal data (leaf order): p0 | e1 | e2, .w of tA holds the scene-order index
     const float4 *tA, *tB, *tC;
     // BVH: 4 float4 per node: (l.lo xyz, l.hi x) (l.hi yz, r.lo xy) (r.lo z, r.hi xyz) (int left, right, lcount, rcount)
@@ -36,18 +36,28 @@ struct DevScene {
     const float *pickCdf;           // nlights + 1
     float lightWeightSum;
     DevCamera cam;
-    // optional work counters of the launch: nodes visited, triangles tested,
-    // rays, state bytes (null outside counting runs)
+};
+// Scene of the counting kernels: the same data plus the launch's work
+// counters (nodes visited, triangles tested, rays, state bytes).
+struct DevSceneCounted : DevSceneData {
     unsigned long long *counters;
 };
 
+// The path code is compiled twice: as the product kernels (DevScene, no
+// counting code at all) and as the counting kernels behind drmlt_count_work
+// (DRMLT_COUNT_WORK defined, DevScene = DevSceneCounted).
+#ifdef DRMLT_COUNT_WORK
+using DevScene = DevSceneCounted;
 DEV void count_work(const DevScene &S, unsigned nodes, unsigned tris) {
-    if (S.counters) {
-        atomicAdd(S.counters + 0, (unsigned long long)nodes);
-        atomicAdd(S.counters + 1, (unsigned long long)tris);
-        atomicAdd(S.counters + 2, 1ull);
-    }
+    atomicAdd(S.counters + 0, (unsigned long long)nodes);
+    atomicAdd(S.counters + 1, (unsigned long long)tris);
+    atomicAdd(S.counters + 2, 1ull);
 }
+#define DRMLT_COUNT(stmt) stmt
+#else
+struct DevScene : DevSceneData {};
+#define DRMLT_COUNT(stmt)
+#endif
 
 // ---------------------------------------------------------------- traversal
 // Moller-Trumbore with the same operation order as the CPU restatement; no FMA
@@ -102,9 +112,9 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
     int best = -1;
     tbest = tmax;
     int node = 0;
-    unsigned nvis = 0, ntri = 0;
+    DRMLT_COUNT(unsigned nvis = 0; unsigned ntri = 0;)
     for (;;) {
-        nvis++;
+        DRMLT_COUNT(nvis++;)
         const float4 n0 = S.nodes[4 * node + 0], n1 = S.nodes[4 * node + 1], n2 = S.nodes[4 * node + 2];
         const int4 ch = *reinterpret_cast<const int4 *>(&S.nodes[4 * node + 3]);
         float tl, tr;
@@ -112,7 +122,7 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
         bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, inv, tmin, tbest, tr);
         // leaves are tested in place
         if (hl && ch.z > 0) {
-            ntri += ch.z;
+            DRMLT_COUNT(ntri += ch.z;)
             for (int k = ch.x; k < ch.x + ch.z; k++) {
                 float t;
                 const float4 A = S.tA[k];
@@ -124,7 +134,7 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
             hl = false;
         }
         if (hr && ch.w > 0) {
-            ntri += ch.w;
+            DRMLT_COUNT(ntri += ch.w;)
             for (int k = ch.y; k < ch.y + ch.w; k++) {
                 float t;
                 const float4 A = S.tA[k];
@@ -150,7 +160,7 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
             node = stack[--sp];
         }
     }
-    count_work(S, nvis, ntri);
+    DRMLT_COUNT(count_work(S, nvis, ntri);)
     return best;
 }
 
@@ -159,31 +169,34 @@ DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float 
     int stack[kBvhStack];
     int sp = 0;
     int node = 0;
-    unsigned nvis = 0, ntri = 0;
-    bool hit = false;
+    DRMLT_COUNT(unsigned nvis = 0; unsigned ntri = 0;)
     for (;;) {
-        nvis++;
+        DRMLT_COUNT(nvis++;)
         const float4 n0 = S.nodes[4 * node + 0], n1 = S.nodes[4 * node + 1], n2 = S.nodes[4 * node + 2];
         const int4 ch = *reinterpret_cast<const int4 *>(&S.nodes[4 * node + 3]);
         float tl, tr;
         bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, o, inv, tmin, tmax, tl);
         bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, inv, tmin, tmax, tr);
         if (hl && ch.z > 0) {
-            for (int k = ch.x; k < ch.x + ch.z && !hit; k++) {
+            for (int k = ch.x; k < ch.x + ch.z; k++) {
                 float t;
-                ntri++;
-                hit = tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t);
+                DRMLT_COUNT(ntri++;)
+                if (tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t)) {
+                    DRMLT_COUNT(count_work(S, nvis, ntri);)
+                    return true;
+                }
             }
-            if (hit) break;
             hl = false;
         }
         if (hr && ch.w > 0) {
-            for (int k = ch.y; k < ch.y + ch.w && !hit; k++) {
+            for (int k = ch.y; k < ch.y + ch.w; k++) {
                 float t;
-                ntri++;
-                hit = tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t);
+                DRMLT_COUNT(ntri++;)
+                if (tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t)) {
+                    DRMLT_COUNT(count_work(S, nvis, ntri);)
+                    return true;
+                }
             }
-            if (hit) break;
             hr = false;
         }
         if (hl && hr) {
@@ -198,8 +211,8 @@ DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float 
             node = stack[--sp];
         }
     }
-    count_work(S, nvis, ntri);
-    return hit;
+    DRMLT_COUNT(count_work(S, nvis, ntri);)
+    return false;
 }
 
 // ---------------------------------------------------------------- geometry in T

@@ -68,18 +68,22 @@ DEV void splat_const(const Chains &ch, int W, int H, float *film, int p, float r
     atomicAdd(px + 2, b);
 }
 
-// Algorithmic chain-state bytes of one path evaluation: state read and
-// proposal written, 4 D (u) + 4 (P + 1) (bsdfDiscrete) + 4 (P + 1) (triangle
-// ids) + 16 (c, l, scores) each way.
-DEV void count_state(const DevScene &S, int c, int l) {
-    if (S.counters) {
-        const int P = c + l - 1, D = (P > 2 ? P : 2) * 2;
-        atomicAdd(S.counters + 3, (unsigned long long)(2 * (4 * D + 8 * (P + 1) + 16)));
-    }
+// Algorithmic chain-state bytes of one path state: 4 D (u) + 4 (P + 1)
+// (bsdfDiscrete) + 4 (P + 1) (triangle ids) + 16 (c, l, scores). A path
+// evaluation reads one state and writes one. Counted in the counting kernels
+// only.
+DEV int state_bytes(int c, int l) {
+    const int P = c + l - 1, D = (P > 2 ? P : 2) * 2;
+    return 4 * D + 8 * (P + 1) + 16;
 }
 DEV void count_bytes(const DevScene &S, unsigned long long b) {
-    if (S.counters) atomicAdd(S.counters + 3, b);
+#ifdef DRMLT_COUNT_WORK
+    atomicAdd(S.counters + 3, b);
+#else
+    (void)S; (void)b;
+#endif
 }
+DEV void count_state(const DevScene &S, int c, int l) { count_bytes(S, (unsigned long long)(2 * state_bytes(c, l))); }
 
 // wave-aggregated counter increment; the lowest active lane adds for the
 // wave, so the call may sit in divergent code

@@ -335,7 +335,6 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     S.lightCdf = d_cdf; S.lightCdfOff = d_cdfOff; S.lightFunc = d_func; S.pickCdf = d_pick;
     S.lightWeightSum = wsum;
     S.cam = make_camera(sc->camera);
-    S.counters = nullptr;
     c->W = S.cam.W;
     c->H = S.cam.H;
     const size_t filmN = size_t(c->W) * c->H * 3;
@@ -558,32 +557,44 @@ extern "C" int drmlt_direct_lighting(drmlt_ctx *c) {
 static int persistent_blocks(const drmlt_ctx *c) { return c->cus * 16; }
 
 // One batched DR step: seven stage kernels and the list reset. With
-// `counters` (7 x 4 words) every stage kernel accumulates its traversal work
-// and state bytes into its own row.
+// `counters` (7 x 4 words) the counting forms of the stage kernels run
+// instead, each accumulating its traversal work and state bytes into its own
+// row.
 static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, unsigned long long *counters = nullptr) {
     hipStream_t s = c->stream;
     const int C = c->ch.C;
     const int pb = persistent_blocks(c);
     const int g1 = std::min((C + 63) / 64, pb);
-    DevScene Sk[7];
-    for (int k = 0; k < 7; k++) {
-        Sk[k] = c->S;
-        Sk[k].counters = counters ? counters + 4 * k : nullptr;
+    if (counters) {
+        DevSceneCounted Sk[7];
+        for (int k = 0; k < 7; k++) {
+            static_cast<DevSceneData &>(Sk[k]) = c->S;
+            Sk[k].counters = counters + 4 * k;
+        }
+        hipLaunchKernelGGL(k_b_stage1_counted, dim3(g1), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+        hipLaunchKernelGGL(k_b_deriv_counted, dim3(pb), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 0, 0);
+        hipLaunchKernelGGL(k_b_gauss_counted, dim3(g1), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0);
+        hipLaunchKernelGGL(k_b_stage2_propose_counted, dim3(g1), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
+        hipLaunchKernelGGL(k_b_deriv_counted, dim3(pb), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
+        hipLaunchKernelGGL(k_b_gauss_counted, dim3(g1), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1);
+        hipLaunchKernelGGL(k_b_stage2_accept_counted, dim3(g1), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+        hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
+        return;
     }
     if (timed) (void)hipEventRecord(c->ev[0], s);
-    hipLaunchKernelGGL(k_b_stage1, dim3(g1), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    hipLaunchKernelGGL(k_b_stage1, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[1], s);
-    hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 0, 0);
+    hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
     if (timed) (void)hipEventRecord(c->ev[2], s);
-    hipLaunchKernelGGL(k_b_gauss, dim3(g1), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0);
+    hipLaunchKernelGGL(k_b_gauss, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0);
     if (timed) (void)hipEventRecord(c->ev[3], s);
-    hipLaunchKernelGGL(k_b_stage2_propose, dim3(g1), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
+    hipLaunchKernelGGL(k_b_stage2_propose, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
     if (timed) (void)hipEventRecord(c->ev[4], s);
-    hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
+    hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
     if (timed) (void)hipEventRecord(c->ev[5], s);
-    hipLaunchKernelGGL(k_b_gauss, dim3(g1), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1);
+    hipLaunchKernelGGL(k_b_gauss, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1);
     if (timed) (void)hipEventRecord(c->ev[6], s);
-    hipLaunchKernelGGL(k_b_stage2_accept, dim3(g1), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    hipLaunchKernelGGL(k_b_stage2_accept, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[7], s);
     hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
     if (timed) (void)hipEventRecord(c->ev[8], s);
@@ -753,18 +764,10 @@ static bool recs_fixed_ok(const drmlt_path_rec *r, int64_t n) {
 extern "C" int drmlt_perturb(drmlt_ctx *c, int64_t n, drmlt_path_rec *st, const float *off, const float *disc, float *c8) {
     if (!c || n < 0 || !st || !off || !disc || !c8) return fail(DRMLT_ERR_ARG, "bad argument");
     if (!recs_ok(c, st, n)) return fail(DRMLT_ERR_ARG, "path record out of range");
-    if (n == 0) return DRMLT_OK;
-    hipStream_t s = c->stream;
-    DBuf<char> r;
-    DBuf<float> o, d, k8;
-    DUP(r, (const char *)st, n * sizeof(drmlt_path_rec)); DUP(o, off, 32 * n); DUP(d, disc, 32 * n); DNEW(k8, 8 * n);
-    const DrParams P = make_params(c);
-    hipLaunchKernelGGL(k_perturb_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.o, int(n), r.p, o.p, d.p, k8.p);
-    HIPCHK(hipGetLastError());
-    DGET((char *)st, r, n * sizeof(drmlt_path_rec));
-    DGET(c8, k8, 8 * n);
-    HIPCHK(hipStreamSynchronize(s));
-    return DRMLT_OK;
+    // The batched PerturbPathBidir kernel made a GPU memory fault whose cause
+    // is not established; it stays switched off until it is. Perturbation
+    // itself runs inside the loop's small steps.
+    return fail(DRMLT_ERR_STATE, "drmlt_perturb is switched off in this build (unexplained GPU memory fault of its kernel)");
 }
 
 extern "C" int drmlt_generate_subpath(drmlt_ctx *c, int64_t n, const int32_t *cl, const float *uni, int32_t nuni, drmlt_path_rec *st,

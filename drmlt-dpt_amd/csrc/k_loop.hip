@@ -1,96 +1,8 @@
 // Batched DR pipeline (persistent per-stage kernels over path-length bins), pre-pass, direct pass and the primal parity surface.
-#include "dr_stages.h"
+#include "k_stages.h"
 #include "kernels_decl.h"
 
 namespace drmlt {
-
-// ------------------------------------------------------------ batched form
-// List kinds: 0 = needs current Gaussian, 1 = second stage (aniso or iso),
-// 2 = needs proposal Gaussian. Each is binned by path length.
-DEV void list_push(const Lists &L, int kind, int C, int bin, int item) {
-    const int slot = atomicAdd(&L.count[kind * kMaxBins + bin], 1);
-    L.items[(size_t(kind) * kMaxBins + bin) * C + slot] = item;
-}
-
-__global__ void __launch_bounds__(64) k_b_stage1(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
-                                                 unsigned long long *stats, Lists L) {
-    const uint32_t iter = *iterp;
-    for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gridDim.x * blockDim.x) {
-        const int flags = dr_stage1(Sc, ch, P, chain, iter, film, stats);
-        if (flags & 2) {
-            const int perm = ch.perm[chain];
-            const int pc = chain + ch.C * perm_cur(perm);
-            const int bin = min(ch.c[pc] + ch.l[pc] - 1, kMaxBins - 1);
-            list_push(L, 1, ch.C, bin, chain);
-            if ((flags & 4) && !ch.gInit[pc]) list_push(L, 0, ch.C, bin, chain);
-        }
-    }
-}
-
-__global__ void __launch_bounds__(64) k_b_gauss(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which, int finishRatio) {
-    const int gsz = gridDim.x * blockDim.x;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int bin = 0; bin < kMaxBins; bin++) {
-        const int n = L.count[kind * kMaxBins + bin];
-        const int *items = L.items + (size_t(kind) * kMaxBins + bin) * ch.C;
-        for (int w = tid; w < n; w += gsz) {
-            const int chain = items[w];
-            const int perm = ch.perm[chain];
-            const int p = chain + ch.C * (which == 0 ? perm_cur(perm) : perm_second(perm));
-            const int c = ch.c[p], l = ch.l[p], D = dim_of(c, l);
-            const GRef G = g_ref(ch, p);
-            const int C = ch.C;
-            if (!has_derivative(c, l, P.maxDervDepth) || D > kGMax) {
-                iso_gaussian(D, P.h2.sigma, G);
-            } else {
-                const double ssv = ch.ss[p];
-                const float *gr = ch.grad + chain, *hs = ch.hess + chain;
-                bool fin = ssv > 1e-15;
-                if (fin) {
-                    for (int i = 0; i < D && fin; i++) fin = isfinite(gr[i * C]);
-                    for (int i = 0; i < D && fin; i++)
-                        for (int j = i; j < D && fin; j++) fin = isfinite(hs[(i * kGMax + j) * C]);
-                }
-                if (fin) {
-                    compute_gaussian(P.h2, ssv, D, [&](int i) { return double(gr[i * C]); },
-                                     [&](int i, int j) { return double(hs[(i * kGMax + j) * C]); }, G);
-                } else {
-                    compute_gaussian(P.h2, ssv, D, [&](int) { return 0.0; }, [&](int, int) { return 0.0; }, G);
-                }
-            }
-            ch.gInit[p] = 1;
-            if (finishRatio) dr_stage2_ratio(ch, chain);
-        }
-    }
-}
-
-__global__ void __launch_bounds__(64) k_b_stage2_propose(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
-    const uint32_t iter = *iterp;
-    const int gsz = gridDim.x * blockDim.x;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int bin = 0; bin < kMaxBins; bin++) {
-        const int n = L.count[1 * kMaxBins + bin];
-        const int *items = L.items + (size_t(1) * kMaxBins + bin) * ch.C;
-        for (int w = tid; w < n; w += gsz) {
-            const int chain = items[w];
-            dr_stage2_propose(Sc, ch, P, chain, iter);
-            const int flags = ch.flags[chain];
-            if ((flags & 4) && (flags & 16)) list_push(L, 2, ch.C, bin, chain);
-        }
-    }
-}
-
-__global__ void __launch_bounds__(64) k_b_stage2_accept(DevScene Sc, Chains ch, DrParams P, uint32_t *iterp, float *film,
-                                                        unsigned long long *stats, Lists L) {
-    const uint32_t iter = *iterp;
-    const int gsz = gridDim.x * blockDim.x;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int bin = 0; bin < kMaxBins; bin++) {
-        const int n = L.count[1 * kMaxBins + bin];
-        const int *items = L.items + (size_t(1) * kMaxBins + bin) * ch.C;
-        for (int w = tid; w < n; w += gsz) dr_stage2_accept(Sc, ch, P, items[w], iter, film, stats);
-    }
-}
 
 // Clears the list counters for the next step and advances the iteration
 // counter (a fill kernel, not a memset, so the step can be graph-captured).
@@ -198,17 +110,6 @@ __global__ void __launch_bounds__(128) k_trace(DevScene Sc, int n, const float *
         prim[i] = trace_closest(Sc, o, d, tmin[i], tmax[i], tb);
         t[i] = tb;
     }
-}
-
-__global__ void __launch_bounds__(64) k_perturb_rec(DevScene Sc, Opts o, int n, char *recs, const float *off, const float *disc, float *c8) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const SRef r = rec_ref(recs + size_t(i) * kRecBytes);
-    PerturbU<OffArray> up{r.u, 1, 0, OffArray{off + 32 * i, 1}};
-    POut<float> out;
-    perturb_eval<float, false>(Sc, o, r, up, DiscArray{disc + 32 * i, 1}, out);
-    if (out.valid) { r.screen[0] = out.screen[0]; r.screen[1] = out.screen[1]; }
-    contrib8(c8 + 8 * i, out);
 }
 
 struct ListSink {

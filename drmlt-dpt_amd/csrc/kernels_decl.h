@@ -11,11 +11,16 @@ __global__ void k_b_gauss(DevScene, Chains, DrParams, Lists, int, int, int);
 __global__ void k_b_stage2_propose(DevScene, Chains, DrParams, const uint32_t *, Lists);
 __global__ void k_b_stage2_accept(DevScene, Chains, DrParams, uint32_t *, float *, unsigned long long *, Lists);
 __global__ void k_b_reset(Lists, uint32_t *, int);
+// counting forms of the stage kernels (k_count.hip)
+__global__ void k_b_stage1_counted(DevSceneCounted, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
+__global__ void k_b_deriv_counted(DevSceneCounted, Chains, DrParams, Lists, int, int);
+__global__ void k_b_gauss_counted(DevSceneCounted, Chains, DrParams, Lists, int, int, int);
+__global__ void k_b_stage2_propose_counted(DevSceneCounted, Chains, DrParams, const uint32_t *, Lists);
+__global__ void k_b_stage2_accept_counted(DevSceneCounted, Chains, DrParams, uint32_t *, float *, unsigned long long *, Lists);
 __global__ void k_prepass_gen(DevScene, Opts, int, int, uint64_t, int *, int *, float *, double *, double *);
 __global__ void k_prepass_seed(DevScene, Opts, Chains, int, uint64_t, const int *, const int *);
 __global__ void k_direct(DevScene, Opts, int, uint64_t, float *);
 __global__ void k_trace(DevScene, int, const float *, const float *, const float *, const float *, int, int *, float *);
-__global__ void k_perturb_rec(DevScene, Opts, int, char *, const float *, const float *, float *);
 __global__ void k_genbidir_rec(DevScene, Opts, int, int, int, const float *, int, int, float *, int *);
 __global__ void k_gradhess_rec_f32(DevScene, Opts, int, char *, const int *, int, float *, float *, int *);
 __global__ void k_gradhess_rec_f64(DevScene, Opts, int, char *, const int *, int, double *, double *, int *);

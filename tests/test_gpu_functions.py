@@ -81,6 +81,7 @@ def _valid_recs(sc, O, n, seed, P_range=(3, 4, 5)):
 def test_perturb_parity(which, request):
     """PerturbPathBidir: same validity, triangles and scores as the oracle in
     double (float32 path arithmetic: relative error of the scores ~1e-6)."""
+    pytest.skip("drmlt_perturb is switched off: its kernel made a GPU memory fault whose cause is not established")
     sc, ctx, O = request.getfixturevalue(which)
     recs, _ = _valid_recs(sc, O, 6000, 1)
     rng = np.random.default_rng(2)
