@@ -110,7 +110,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         ncpu = min(args.cpu_threads, os.cpu_count() or 1)
-        cchains, citers = (2048, 8) if args.config != 1 else (1024, 64)
+        cchains, citers = (16384, 128) if args.config != 1 else (1024, 512)
+        log(f"cpu baseline: the oracle loop, {cchains} chains x {citers} iterations on {ncpu} threads")
         cval, csec = cpu_baseline(sc, ncpu, cchains, citers)
         cpu = {"value": round(cval, 1), "unit": "mutations/s", "cores": ncpu, "kind": "port",
                "sample": f"{cchains} chains x {citers} iterations of the same scene after a 20000-sample pre-pass "

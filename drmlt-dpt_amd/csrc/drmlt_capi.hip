@@ -509,7 +509,10 @@ extern "C" int drmlt_prepass(drmlt_ctx *c, double *bOut, float *lfOut, int32_t n
     int cdfPos = 0;
     std::vector<int> selS(C), selCl(C);
     for (int i = 0; i < C; i++) {
-        while (pos > cdf[cdfPos]) cdfPos = std::min(cdfPos + 1, ncand - 1);
+        // the cursor is capped at ncand - 1 as in delayed.cpp:138-140, where a
+        // position inside the last candidate's interval never ends the walk;
+        // here the walk stops at the cap and takes the candidate it names
+        while (cdfPos < ncand - 1 && pos > cdf[cdfPos]) cdfPos++;
         const int k = std::max(cdfPos - 1, 0);
         selS[i] = candSample[k];
         selCl[i] = candCl[k];

@@ -333,7 +333,10 @@ extern "C" int orc_prepass(void *h, int numInit, int maxDepth, int numChains, ui
     double pos = prng.uni() * interval;
     int cdfPos = 0;
     for (int i = 0; i < numChains; i++) {
-        while (pos > cdf[cdfPos]) cdfPos = std::min(cdfPos + 1, int(all.size()) - 1);
+        // delayed.cpp:138-140 caps the cursor at size - 1 and never leaves the
+        // loop when pos lies in the last candidate's interval; the walk stops
+        // at the cap instead
+        while (cdfPos < int(all.size()) - 1 && pos > cdf[cdfPos]) cdfPos++;
         const Cand &cd = all[std::max(cdfPos - 1, 0)];
         Philox rng(seed, uint64_t(cd.sample));
         PhiloxSrc<float> g(&rng);
