@@ -1,25 +1,41 @@
 // FP64 H2MC Gaussian on the device: ComputeGaussian (drsmallstep.cpp:18-93,
 // 219-266), IsotropicGaussian / GaussianLogPdf / GenerateSample
-// (gaussian.cpp:4-47). The symmetric eigen-decomposition is a cyclic Jacobi
-// iteration (Golub & Van Loan 8.5) run by one thread per matrix in double.
-// Gaussian fields live in HBM as structure-of-arrays: element e of slot k at
-// [e * gs + k] (gs = number of Gaussian slots), so a wave's stores coalesce.
+// (gaussian.cpp:4-47).
+//
+// A Gaussian is stored factored, one contiguous slot of kGSlot doubles per
+// state: the eigenvectors V of the Hessian (column k = k-th eigenvector), the
+// precisions p along them, the mean and logDet. The reference's matrices are
+// products of these: invCov = V diag(p) V^T, covL = V diag(p^-1/2)
+// (drsmallstep.cpp:70-92).
+//
+// Every routine is written once over an executor X that names the lanes
+// working on one Gaussian: SeqExec (one thread does everything, the plain
+// form) or WaveExec (the 64 lanes of a one-wave workgroup share the work, the
+// matrices in LDS). Work is split by index, every element is computed by the
+// same expression in the same order in both, and reductions run in index
+// order on one lane, so both executors produce bit-identical results.
+// Floating-point contraction is off in these routines for the same reason.
+//
+// The eigen-decomposition is a parallel-order cyclic Jacobi iteration
+// (Golub & Van Loan, Matrix Computations, 4th ed., 8.5.4-8.5.5: round-robin
+// ordering; each round applies D/2 disjoint rotations, rows then columns).
 #pragma once
 #include "dev_math.h"
 #include <cmath>
 
 namespace drmlt {
 
-constexpr int kGMax = 24;  // largest D with derivatives (path length 12)
+constexpr int kGMax = 24;          // largest D with derivatives (path length 12)
+constexpr int kGS = kGMax + 1;     // work-matrix row stride (odd, so rows fall on different LDS banks)
+constexpr int kSlotP = kGMax * kGMax, kSlotM = kSlotP + kGMax, kSlotLD = kSlotM + kGMax;
+constexpr int kGSlot = 640;        // doubles per Gaussian slot (V, p, mean, logDet, padding to 64 B)
 
-struct GRef {
-    double *mean, *covL, *invCov, *logDet;  // [e * gs + k]
-    long gs;
-    long k;
-    DEV double &m(int i) const { return mean[i * gs + k]; }
-    DEV double &L(int i, int j) const { return covL[(i * kGMax + j) * gs + k]; }
-    DEV double &Q(int i, int j) const { return invCov[(i * kGMax + j) * gs + k]; }
-    DEV double &ld() const { return logDet[k]; }
+struct GSlot {
+    double *b;
+    DEV double &V(int i, int k) const { return b[i * kGMax + k]; }
+    DEV double &p(int k) const { return b[kSlotP + k]; }
+    DEV double &m(int i) const { return b[kSlotM + i]; }
+    DEV double &ld() const { return b[kSlotLD]; }
 };
 
 // drsmallstep.h:26-42; posOffset is 0.5 * (e^L + e^-L - 1) as computed there
@@ -37,128 +53,213 @@ inline H2Param make_h2param(double sigma) {
     return p;
 }
 
-// gaussian.cpp:4-16
-DEV void iso_gaussian(int D, double sigma, const GRef &g) {
-    double ld = 0;
-    for (int i = 0; i < D; i++) {
-        g.m(i) = 0;
-        for (int j = 0; j < D; j++) {
-            g.L(i, j) = i == j ? sigma : 0.0;
-            g.Q(i, j) = i == j ? 1.0 / (sigma * sigma) : 0.0;
-        }
-        ld += log(1.0 / (sigma * sigma));
-    }
-    g.ld() = ld;
+struct SeqExec {
+    int lane = 0;
+    static constexpr int n = 1;
+    DEV void sync() const {}
+};
+struct WaveExec {
+    int lane;
+    static constexpr int n = 64;
+    DEV void sync() const { __syncthreads(); }
+};
+
+// Scratch of one Gaussian computation (private for SeqExec, LDS for WaveExec).
+struct GWork {
+    double A[kGMax * kGS], V[kGMax * kGS];
+    double vec[kGMax], vec2[kGMax], part[kGMax];
+    double rc[kGMax / 2], rs[kGMax / 2];
+    int rp[kGMax / 2], rq[kGMax / 2];
+    int flag;
+};
+
+// Pair k of round r of the round-robin schedule over D (even) indices:
+// position 0 stays, positions 1..D-1 rotate.
+DEV void rr_pair(int D, int r, int k, int &p, int &q) {
+    auto pos = [&](int i) { return i == 0 ? 0 : 1 + (i - 1 + r) % (D - 1); };
+    const int a = pos(k), b = pos(D - 1 - k);
+    p = a < b ? a : b;
+    q = a < b ? b : a;
 }
 
-// Cyclic Jacobi on A (D x D, row-major, private), eigenvectors in V columns.
-DEV void jacobi(int D, double *A, double *V) {
-    for (int i = 0; i < D * D; i++) V[i] = 0;
-    for (int i = 0; i < D; i++) V[i * D + i] = 1;
+// gaussian.cpp:4-16 in factored form: V = I, p = 1/sigma^2, mean 0.
+template <class X> DEV void gauss_iso(const X &x, int D, double sigma, const GSlot &g) {
+#pragma clang fp contract(off)
+    for (int t = x.lane; t < D * D; t += x.n) g.V(t / D, t % D) = (t / D == t % D) ? 1.0 : 0.0;
+    for (int i = x.lane; i < D; i += x.n) {
+        g.p(i) = 1.0 / (sigma * sigma);
+        g.m(i) = 0.0;
+    }
+    if (x.lane == 0) {
+        double ld = 0;
+        for (int i = 0; i < D; i++) ld += log(1.0 / (sigma * sigma));
+        g.ld() = ld;
+    }
+}
+
+// Eigen-decomposition of w.A (D x D symmetric, stride kGS) into w.V; the
+// eigenvalues end on the diagonal of w.A. Converged when the off-diagonal
+// mass is below 1e-30 of the diagonal mass.
+template <class X> DEV void gauss_eigen(const X &x, int D, GWork &w) {
+#pragma clang fp contract(off)
+    for (int t = x.lane; t < D * D; t += x.n) w.V[(t / D) * kGS + t % D] = (t / D == t % D) ? 1.0 : 0.0;
+    const int half = D / 2;
     for (int sweep = 0; sweep < 60; sweep++) {
-        double off = 0, diag = 0;
-        for (int i = 0; i < D; i++)
-            for (int j = 0; j < D; j++) {
-                const double a = A[i * D + j];
-                if (i == j) diag += a * a; else off += a * a;
+        x.sync();
+        for (int i = x.lane; i < D; i += x.n) {
+            double o = 0;
+            for (int j = 0; j < D; j++)
+                if (j != i) o += w.A[i * kGS + j] * w.A[i * kGS + j];
+            w.part[i] = o;
+            w.vec[i] = w.A[i * kGS + i] * w.A[i * kGS + i];
+        }
+        x.sync();
+        if (x.lane == 0) {
+            double off = 0, diag = 0;
+            for (int i = 0; i < D; i++) { off += w.part[i]; diag += w.vec[i]; }
+            w.flag = (off <= 1e-30 * diag || off == 0) ? 1 : 0;
+        }
+        x.sync();
+        if (w.flag) break;
+        for (int r = 0; r < D - 1; r++) {
+            for (int k = x.lane; k < half; k += x.n) {
+                int p, q;
+                rr_pair(D, r, k, p, q);
+                const double apq = w.A[p * kGS + q];
+                double c = 1.0, s = 0.0;
+                if (apq != 0) {
+                    const double app = w.A[p * kGS + p], aqq = w.A[q * kGS + q];
+                    const double theta = (aqq - app) / (2 * apq);
+                    const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                    c = 1 / sqrt(t * t + 1);
+                    s = t * c;
+                }
+                w.rp[k] = p; w.rq[k] = q; w.rc[k] = c; w.rs[k] = s;
             }
-        if (off <= 1e-30 * diag || off == 0) break;
-        for (int p = 0; p < D - 1; p++)
-            for (int q = p + 1; q < D; q++) {
-                const double apq = A[p * D + q];
-                if (apq == 0) continue;
-                const double app = A[p * D + p], aqq = A[q * D + q];
-                const double theta = (aqq - app) / (2 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
-                const double c = 1 / sqrt(t * t + 1), s = t * c;
-                for (int k = 0; k < D; k++) {
-                    const double akp = A[k * D + p], akq = A[k * D + q];
-                    A[k * D + p] = c * akp - s * akq;
-                    A[k * D + q] = s * akp + c * akq;
-                }
-                for (int k = 0; k < D; k++) {
-                    const double apk = A[p * D + k], aqk = A[q * D + k];
-                    A[p * D + k] = c * apk - s * aqk;
-                    A[q * D + k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < D; k++) {
-                    const double vkp = V[k * D + p], vkq = V[k * D + q];
-                    V[k * D + p] = c * vkp - s * vkq;
-                    V[k * D + q] = s * vkp + c * vkq;
-                }
+            x.sync();
+            for (int t = x.lane; t < half * D; t += x.n) {
+                const int k = t / D, j = t % D, p = w.rp[k], q = w.rq[k];
+                const double c = w.rc[k], s = w.rs[k];
+                const double a = w.A[p * kGS + j], b = w.A[q * kGS + j];
+                w.A[p * kGS + j] = c * a - s * b;
+                w.A[q * kGS + j] = s * a + c * b;
             }
+            x.sync();
+            for (int t = x.lane; t < half * D; t += x.n) {
+                const int k = t / D, i = t % D, p = w.rp[k], q = w.rq[k];
+                const double c = w.rc[k], s = w.rs[k];
+                const double a = w.A[i * kGS + p], b = w.A[i * kGS + q];
+                w.A[i * kGS + p] = c * a - s * b;
+                w.A[i * kGS + q] = s * a + c * b;
+                const double va = w.V[i * kGS + p], vb = w.V[i * kGS + q];
+                w.V[i * kGS + p] = c * va - s * vb;
+                w.V[i * kGS + q] = s * va + c * vb;
+            }
+            x.sync();
+        }
     }
 }
 
-// drsmallstep.cpp:219-266 + 18-93. grad[i * gst], hess[(i * hld + j) * hst]
-// (row-major); the eigen-solver reads the upper triangle, as Eigen reads the
-// lower triangle of the column-major view of the same buffer.
-template <class GP, class HP>
-DEV void compute_gaussian(const H2Param &pr, double sc, int D, GP grad, HP hess, const GRef &g) {
-    // Frobenius norm of the symmetric Hessian from its upper triangle
-    // (hess(i, j) is only read for i <= j)
-    double fro = 0;
-    for (int i = 0; i < D; i++)
-        for (int j = i; j < D; j++) { const double h = hess(i, j); fro += (i == j ? 1.0 : 2.0) * h * h; }
-    fro = sqrt(fro);
-    if (sc <= 1e-15 || fro < 0.5 / (pr.sigma * pr.sigma)) {
-        iso_gaussian(D, pr.sigma, g);
+// drsmallstep.cpp:219-266 + 18-93. grad(i) and hess(i, j) for i <= j (the
+// upper triangle of the row-major Hessian; Eigen reads the lower triangle of
+// the column-major view of the same buffer). Writes the Gaussian to g.
+template <class X, class GP, class HP>
+DEV void gauss_compute(const X &x, const H2Param &pr, double sc, int D, GP grad, HP hess, GWork &w, const GSlot &g) {
+#pragma clang fp contract(off)
+    for (int t = x.lane; t < D * D; t += x.n) {
+        const int i = t / D, j = t % D;
+        w.A[i * kGS + j] = i <= j ? hess(i, j) : hess(j, i);
+    }
+    for (int i = x.lane; i < D; i += x.n) {
+        double fr = 0;
+        for (int j = i; j < D; j++) { const double h = hess(i, j); fr += (i == j ? 1.0 : 2.0) * h * h; }
+        w.part[i] = fr;
+    }
+    x.sync();
+    if (x.lane == 0) {
+        double fro = 0;
+        for (int i = 0; i < D; i++) fro += w.part[i];
+        fro = sqrt(fro);
+        w.flag = (sc <= 1e-15 || fro < 0.5 / (pr.sigma * pr.sigma)) ? 1 : 0;
+    }
+    x.sync();
+    if (w.flag) {
+        gauss_iso(x, D, pr.sigma, g);
+        x.sync();
         return;
     }
-    double A[kGMax * kGMax], V[kGMax * kGMax];
-    for (int i = 0; i < D; i++)
-        for (int j = 0; j < D; j++) A[i * D + j] = i <= j ? hess(i, j) : hess(j, i);
-    jacobi(D, A, V);
-    double e[kGMax], off[kGMax], p[kGMax], lam[kGMax];
+    gauss_eigen(x, D, w);
     const double invS2 = 1.0 / (pr.sigma * pr.sigma);
-    for (int i = 0; i < D; i++) {
-        lam[i] = A[i * D + i];
-        e[i] = fabs(lam[i]) > 1e-10 ? 1.0 / fabs(lam[i]) : 0.0;
+    for (int i = x.lane; i < D; i += x.n) {
+        const double lam = w.A[i * kGS + i];
+        double e = fabs(lam) > 1e-10 ? 1.0 / fabs(lam) : 0.0;
         double vg = 0;
-        for (int k = 0; k < D; k++) vg += V[k * D + i] * grad(k);
-        off[i] = e[i] * vg;
-    }
-    double ld = 0;
-    for (int i = 0; i < D; i++) {
+        for (int k = 0; k < D; k++) vg += w.V[k * kGS + i] * grad(k);
+        const double off = e * vg;
         double s2, o;
-        if (fabs(lam[i]) > 1e-10) {
-            o = off[i];
-            if (lam[i] > 0) { s2 = pr.posScale; o *= pr.posOffset; }
+        if (fabs(lam) > 1e-10) {
+            o = off;
+            if (lam > 0) { s2 = pr.posScale; o *= pr.posOffset; }
             else { s2 = pr.negScale; o *= pr.negOffset; }
         } else {
             s2 = pr.L * pr.L;
-            o = 0.5 * off[i] * pr.L * pr.L;
+            o = 0.5 * off * pr.L * pr.L;
         }
-        e[i] *= s2;
-        e[i] = e[i] > 1e-10 ? 1.0 / e[i] : 0.0;
-        off[i] = o;
-        p[i] = e[i] + invS2;
-        ld += log(p[i]);
+        e *= s2;
+        e = e > 1e-10 ? 1.0 / e : 0.0;
+        const double p = e + invS2;
+        w.vec[i] = p;
+        w.vec2[i] = (e / p) * o;
+        g.p(i) = p;
     }
-    for (int i = 0; i < D; i++) {
+    x.sync();
+    for (int i = x.lane; i < D; i += x.n) {
         double mi = 0;
-        for (int k = 0; k < D; k++) mi += V[i * D + k] * (e[k] / p[k]) * off[k];
+        for (int k = 0; k < D; k++) mi += w.V[i * kGS + k] * w.vec2[k];
         g.m(i) = mi;
-        for (int j = 0; j < D; j++) {
-            double q = 0;
-            for (int k = 0; k < D; k++) q += V[i * D + k] * p[k] * V[j * D + k];
-            g.Q(i, j) = q;
-            g.L(i, j) = V[i * D + j] / sqrt(p[j]);
-        }
     }
-    g.ld() = ld;
+    for (int t = x.lane; t < D * D; t += x.n) g.V(t / D, t % D) = w.V[(t / D) * kGS + t % D];
+    if (x.lane == 0) {
+        double ld = 0;
+        for (int i = 0; i < D; i++) ld += log(w.vec[i]);
+        g.ld() = ld;
+    }
+    x.sync();
 }
 
-// gaussian.cpp:18-26 (x given by functor, sign applied to x)
-template <class XP> DEV double gaussian_logpdf(int D, XP x, double sign, const GRef &g) {
-    double q = 0;
-    for (int i = 0; i < D; i++) {
-        double si = 0;
-        const double di = sign * x(i) - g.m(i);
-        for (int j = 0; j < D; j++) si += g.Q(i, j) * (sign * x(j) - g.m(j));
-        q += di * si;
+// GenerateSample (gaussian.cpp:36-47): x = mean + covL n with
+// covL = V diag(p^-1/2); nrm(j) supplies the j-th standard normal.
+template <class X, class NP, class OW>
+DEV void gauss_sample(const X &x, int D, const GSlot &g, NP nrm, GWork &w, OW out) {
+#pragma clang fp contract(off)
+    for (int j = x.lane; j < D; j += x.n) w.vec[j] = nrm(j) / sqrt(g.p(j));
+    x.sync();
+    for (int i = x.lane; i < D; i += x.n) {
+        double xi = g.m(i);
+        for (int j = 0; j < D; j++) xi += g.V(i, j) * w.vec[j];
+        out(i, xi);
     }
-    return double(D) * (-0.5 * log(2.0 * 3.14159265358979323846)) + 0.5 * g.ld() - 0.5 * q;
+    x.sync();
+}
+
+// GaussianLogPdf (gaussian.cpp:18-26) of sign * xv(i):
+// -D/2 log 2pi + logDet/2 - 1/2 sum_k p_k (V^T (x - mean))_k^2.
+template <class X, class XP>
+DEV double gauss_logpdf(const X &x, int D, XP xv, double sign, const GSlot &g, GWork &w) {
+#pragma clang fp contract(off)
+    for (int i = x.lane; i < D; i += x.n) w.vec[i] = sign * xv(i) - g.m(i);
+    x.sync();
+    for (int k = x.lane; k < D; k += x.n) {
+        double s = 0;
+        for (int i = 0; i < D; i++) s += g.V(i, k) * w.vec[i];
+        w.vec2[k] = g.p(k) * s * s;
+    }
+    x.sync();
+    double q = 0;
+    for (int k = 0; k < D; k++) q += w.vec2[k];
+    const double r = double(D) * (-0.5 * log(2.0 * 3.14159265358979323846)) + 0.5 * g.ld() - 0.5 * q;
+    x.sync();
+    return r;
 }
 
 }  // namespace drmlt

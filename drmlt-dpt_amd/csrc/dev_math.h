@@ -189,10 +189,33 @@ struct Philox {
         return buf[--have];
     }
     DEV float uni() { return float(next() >> 8) * (1.0f / 16777216.0f); }
-    DEV double normal() {
-        double u1 = (double(next() >> 8) + 1.0) * (1.0 / 16777216.0);
-        double u2 = double(next() >> 8) * (1.0 / 16777216.0);
+    DEV static double box_muller(uint32_t w1, uint32_t w2) {
+        double u1 = (double(w1 >> 8) + 1.0) * (1.0 / 16777216.0);
+        double u2 = double(w2 >> 8) * (1.0 / 16777216.0);
         return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+    DEV double normal() {
+        const uint32_t w1 = next();
+        return box_muller(w1, next());
+    }
+    // The value the j-th normal() call returns on this stream when it is at a
+    // block boundary (have == 0): normal() takes two words, next() hands out
+    // words 3, 2, 1, 0 of each block.
+    DEV double normal_at(int j) const {
+        Philox t = *this;
+        t.c0 = c0 + uint32_t(j / 2);
+        t.block();
+        return (j & 1) ? box_muller(t.buf[1], t.buf[0]) : box_muller(t.buf[3], t.buf[2]);
+    }
+    // Advances a stream at a block boundary past `count` normal() calls.
+    DEV void skip_normals(int count) {
+        const int words = 2 * count;
+        c0 += uint32_t(words / 4);
+        have = 0;
+        if (words % 4) {
+            block();
+            have -= words % 4;
+        }
     }
 };
 

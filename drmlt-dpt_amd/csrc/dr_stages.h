@@ -139,39 +139,82 @@ DEV bool jet_pass(const DevScene &Sc, const Opts &o, const SRef &ref, int D, int
 // Derivative function availability (path.cpp:3186-3206 with maxDervDepth)
 DEV bool has_derivative(int c, int l, int maxD) { return c >= 1 && l >= 0 && c + l >= 3 && c + l - 1 <= maxD; }
 
-// initGaussian (drsmallstep.cpp:129-166) for state slot p, all in one thread.
-DEV void init_gaussian_thread(const DevScene &Sc, const Chains &ch, const DrParams &P, int p, float *gbuf, float *hbuf) {
+// Derivative passes of state slot p into the derivative buffers of `chain`,
+// all in one thread (the plain form; k_b_deriv spreads the same passes over
+// threads). A failed fixed-topology evaluation marks the gradient non-finite.
+DEV void derivs_thread(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, int p) {
     const int c = ch.c[p], l = ch.l[p], D = dim_of(c, l);
-    const GRef G = g_ref(ch, p);
-    if (!has_derivative(c, l, P.maxDervDepth) || D > kGMax) {
-        iso_gaussian(D, P.h2.sigma, G);
-        ch.gInit[p] = 1;
-        return;
+    if (!has_derivative(c, l, P.maxDervDepth) || D > kGMax || !(double(ch.ss[p]) > 1e-15)) return;
+    float *gr = ch.grad + size_t(chain) * kGMax, *hs = ch.hess + size_t(chain) * kGMax * kGMax;
+    const SRef ref = slot_ref(ch, p);
+    const int np = jet_passes(D);
+    for (int q = 0; q < np; q++) {
+        int row, col0;
+        jet_pass_index(D, q, row, col0);
+        const bool ok = jet_pass<float>(Sc, P.oDeriv, ref, D, row, col0,
+                                        [&](int cc, float gv) { if (row == 0) gr[cc] = gv; },
+                                        [&](int cc, float hv) { if (cc >= row) hs[row * kGMax + cc] = hv; });
+        if (!ok) { gr[0] = __builtin_nanf(""); return; }
     }
-    for (int i = 0; i < D; i++) gbuf[i] = 0.0f;
-    for (int i = 0; i < D * D; i++) hbuf[i] = 0.0f;
+}
+
+// initGaussian (drsmallstep.cpp:129-166) of state slot p from the derivative
+// buffers of `chain`: isotropic without derivatives (path too long, f <= 1e-15
+// or a non-finite gradient / Hessian, drsmallstep.cpp:135-163), else
+// ComputeGaussian. Marks the slot's Gaussian as initialised.
+template <class X> DEV void gaussian_from_derivs(const X &x, const Chains &ch, const DrParams &P, int chain, int p, GWork &w) {
+    const int c = ch.c[p], l = ch.l[p], D = dim_of(c, l);
+    const GSlot G = g_ref(ch, p);
     const double ssv = ch.ss[p];
-    if (ssv > 1e-15) {
-        const SRef ref = slot_ref(ch, p);
-        bool ok = true;
-        const int np = jet_passes(D);
-        for (int q = 0; q < np && ok; q++) {
-            int row, col0;
-            jet_pass_index(D, q, row, col0);
-            ok = jet_pass<float>(Sc, P.oDeriv, ref, D, row, col0,
-                                 [&](int cc, float gv) { if (row == 0) gbuf[cc] = gv; },
-                                 [&](int cc, float hv) { if (cc >= row) hbuf[row * D + cc] = hv; });
+    const float *gr = ch.grad + size_t(chain) * kGMax, *hs = ch.hess + size_t(chain) * kGMax * kGMax;
+    bool derivs = has_derivative(c, l, P.maxDervDepth) && D <= kGMax;
+    if (derivs) {
+        if (x.lane == 0) w.flag = ssv > 1e-15 ? 1 : 0;
+        x.sync();
+        if (w.flag) {
+            for (int t = x.lane; t < D + D * D; t += x.n) {
+                const int i = (t - D) / D, j = (t - D) % D;
+                const float v = t < D ? gr[t] : (i <= j ? hs[i * kGMax + j] : 0.0f);
+                if (!isfinite(v)) w.flag = 0;
+            }
         }
-        bool fin = ok;
-        for (int i = 0; i < D && fin; i++) fin = isfinite(gbuf[i]);
-        for (int i = 0; i < D * D && fin; i++) fin = isfinite(hbuf[i]);
-        if (!fin) {
-            for (int i = 0; i < D; i++) gbuf[i] = 0.0f;
-            for (int i = 0; i < D * D; i++) hbuf[i] = 0.0f;
-        }
+        x.sync();
+        derivs = w.flag != 0;
+        x.sync();
     }
-    compute_gaussian(P.h2, ssv, D, [&](int i) { return double(gbuf[i]); }, [&](int i, int j) { return double(hbuf[i * D + j]); }, G);
-    ch.gInit[p] = 1;
+    if (derivs) {
+        gauss_compute(x, P.h2, ssv, D, [&](int i) { return double(gr[i]); }, [&](int i, int j) { return double(hs[i * kGMax + j]); },
+                      w, G);
+    } else {
+        gauss_iso(x, D, P.h2.sigma, G);
+        x.sync();
+    }
+    if (x.lane == 0) ch.gInit[p] = 1;
+}
+
+// H2MC proposal offset, drawn from the current state's Gaussian with the
+// stage-2 stream's first D normals (drsmallstep.cpp:171-179,
+// gaussian.cpp:36-47); dr_stage2_propose skips those normals.
+template <class X> DEV void h2mc_offset(const X &x, const Chains &ch, const DrParams &P, int chain, uint32_t iter, GWork &w) {
+    const int C = ch.C;
+    const int pc = chain + C * perm_cur(ch.perm[chain]);
+    const int D = dim_of(ch.c[pc], ch.l[pc]);
+    const Philox g(P.seed, iter, chain, TAG_S2);
+    gauss_sample(x, D, g_ref(ch, pc), [&](int j) { return g.normal_at(j); }, w,
+                 [&](int i, double v) { ch.off[i * C + chain] = float(v); });
+}
+
+// H2MC acceptance before the delayed-rejection correction
+// (drsmallstep.cpp:181-196): N_z(-off) / N_x(off) * f_z / f_x, unclamped.
+template <class X> DEV void h2mc_ratio(const X &x, const Chains &ch, int chain, GWork &w) {
+    const int C = ch.C;
+    const int perm = ch.perm[chain];
+    const int pc = chain + C * perm_cur(perm), ps = chain + C * perm_second(perm);
+    const int D = dim_of(ch.c[pc], ch.l[pc]);
+    auto off = [&](int i) { return double(ch.off[i * C + chain]); };
+    const double py = gauss_logpdf(x, D, off, 1.0, g_ref(ch, pc), w);
+    const double px = gauss_logpdf(x, D, off, -1.0, g_ref(ch, ps), w);
+    if (x.lane == 0) ch.a2d[chain] = exp(px - py) * double(ch.ss[ps]) / double(ch.ss[pc]);
 }
 
 // ------------------------------------------------------------ DR stages
@@ -255,9 +298,8 @@ DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int c
 
 // Stage-2 proposal (delayed.cpp:448-463): H2MC (drsmallstep.cpp:113-212) or
 // isotropic with the second-stage sigma (mutation.cpp:167-208). For H2MC the
-// current state's Gaussian must be initialised. Stores a2 (double) for aniso
-// before the proposal's Gaussian is known: the kernel ratio is finished in
-// dr_stage2_ratio.
+// offset must already be drawn (h2mc_offset); its ratio needs the proposal's
+// Gaussian and is finished by h2mc_ratio.
 DEV void dr_stage2_propose(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter) {
     const int C = ch.C;
     const int perm = ch.perm[chain];
@@ -269,15 +311,8 @@ DEV void dr_stage2_propose(const DevScene &Sc, const Chains &ch, const DrParams 
     copy_state(slot_ref(ch, pc), rs);
     POut<float> out;
     if (isAniso) {
-        const int D = dim_of(ch.c[pc], ch.l[pc]);
-        const GRef G = g_ref(ch, pc);
-        double n[kGMax];
-        for (int i = 0; i < D; i++) n[i] = g.normal();
-        for (int i = 0; i < D; i++) {
-            double x = G.m(i);
-            for (int j = 0; j < D; j++) x += G.L(i, j) * n[j];
-            ch.off[i * C + chain] = float(x);
-        }
+        // the offset was drawn by h2mc_offset from this stream's first D normals
+        g.skip_normals(dim_of(ch.c[pc], ch.l[pc]));
         PerturbU<OffArray> up{rs.u, rs.s, 0, OffArray{ch.off + chain, C}};
         perturb_eval<float, false>(Sc, P.o, rs, up, DiscNormal{&g, P.discreteStdDev}, out);
     } else {
@@ -301,16 +336,6 @@ DEV void dr_stage2_propose(const DevScene &Sc, const Chains &ch, const DrParams 
 }
 
 // H2MC kernel ratio once both Gaussians exist (drsmallstep.cpp:185-196)
-DEV void dr_stage2_ratio(const Chains &ch, int chain) {
-    const int C = ch.C;
-    const int perm = ch.perm[chain];
-    const int pc = chain + C * perm_cur(perm), ps = chain + C * perm_second(perm);
-    const int D = dim_of(ch.c[pc], ch.l[pc]);
-    auto off = [&](int i) { return double(ch.off[i * C + chain]); };
-    const double py = gaussian_logpdf(D, off, 1.0, g_ref(ch, pc));
-    const double px = gaussian_logpdf(D, off, -1.0, g_ref(ch, ps));
-    ch.a2d[chain] = exp(px - py) * double(ch.ss[ps]) / double(ch.ss[pc]);
-}
 
 // Second-stage acceptance (delayed.cpp:466-541), splats (548-561) and state
 // swap (565-606).

@@ -137,7 +137,6 @@ struct drmlt_ctx {
     Chains ch;
     std::vector<void *> chainAllocs;
     std::vector<size_t> chainBytes;
-    float *plainScratch = nullptr;
     Lists L;
     uint32_t *iterDev = nullptr;
     uint32_t iterHost = 0;
@@ -373,11 +372,10 @@ static int alloc_chains(drmlt_ctx *c, int C) {
     bool ok = A(&ch.c, NS) && A(&ch.l, NS) && A(&ch.light, NS) && A(&ch.lprim, NS) && A(&ch.screen, 2 * NS) && A(&ch.u, 32 * NS) &&
               A(&ch.cprim, 16 * NS) && A(&ch.lgprim, 16 * NS) && A(&ch.cdisc, 16 * NS) && A(&ch.ldisc, 16 * NS) && A(&ch.crr, 16 * NS) &&
               A(&ch.lrr, 16 * NS) && A(&ch.valid, NS) && A(&ch.hasSplat, NS) && A(&ch.gInit, NS) && A(&ch.rgb, 3 * NS) &&
-              A(&ch.ls, NS) && A(&ch.ss, NS) && A(&ch.perm, size_t(C)) && A(&ch.gmean, size_t(kGMax) * GS) &&
-              A(&ch.gcovL, size_t(kGMax * kGMax) * GS) && A(&ch.ginv, size_t(kGMax * kGMax) * GS) && A(&ch.gld, GS) &&
+              A(&ch.ls, NS) && A(&ch.ss, NS) && A(&ch.perm, size_t(C)) && A(&ch.gauss, size_t(kGSlot) * GS) &&
               A(&ch.off, 32 * size_t(C)) && A(&ch.grad, size_t(kGMax) * C) && A(&ch.hess, size_t(kGMax * kGMax) * C) &&
               A(&ch.a1, size_t(C)) && A(&ch.a2, size_t(C)) && A(&ch.a2d, size_t(C)) && A(&ch.flags, size_t(C)) &&
-              A(&c->plainScratch, size_t(C) * (kGMax + kGMax * kGMax)) && A(&c->L.count, 3 * kMaxBins) &&
+              A(&c->L.count, 3 * kMaxBins) &&
               A(&c->L.items, size_t(3) * kMaxBins * C);
     if (!ok) return fail(DRMLT_ERR_HIP, "chain state allocation");
     hipStream_t s = c->stream;
@@ -576,10 +574,10 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
         }
         hipLaunchKernelGGL(k_b_stage1_counted, dim3(g1), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
         hipLaunchKernelGGL(k_b_deriv_counted, dim3(pb), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 0, 0);
-        hipLaunchKernelGGL(k_b_gauss_counted, dim3(g1), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0);
+        hipLaunchKernelGGL(k_b_gauss_counted, dim3(pb), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0, c->iterDev);
         hipLaunchKernelGGL(k_b_stage2_propose_counted, dim3(g1), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
         hipLaunchKernelGGL(k_b_deriv_counted, dim3(pb), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
-        hipLaunchKernelGGL(k_b_gauss_counted, dim3(g1), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1);
+        hipLaunchKernelGGL(k_b_gauss_counted, dim3(pb), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1, c->iterDev);
         hipLaunchKernelGGL(k_b_stage2_accept_counted, dim3(g1), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
         return;
@@ -589,13 +587,13 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
     if (timed) (void)hipEventRecord(c->ev[1], s);
     hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
     if (timed) (void)hipEventRecord(c->ev[2], s);
-    hipLaunchKernelGGL(k_b_gauss, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0);
+    hipLaunchKernelGGL(k_b_gauss, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[3], s);
     hipLaunchKernelGGL(k_b_stage2_propose, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
     if (timed) (void)hipEventRecord(c->ev[4], s);
     hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
     if (timed) (void)hipEventRecord(c->ev[5], s);
-    hipLaunchKernelGGL(k_b_gauss, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1);
+    hipLaunchKernelGGL(k_b_gauss, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[6], s);
     hipLaunchKernelGGL(k_b_stage2_accept, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[7], s);
@@ -622,7 +620,7 @@ extern "C" int drmlt_run(drmlt_ctx *c, int64_t iters, int32_t mode) {
     if (mode == DRMLT_PLAIN) {
         for (int64_t it = 0; it < iters; it++) {
             hipLaunchKernelGGL(k_dr_plain, dim3((C + 63) / 64), dim3(64), 0, s, c->S, c->ch, P, c->iterHost + uint32_t(it), c->filmInd,
-                               c->stats, c->plainScratch);
+                               c->stats);
         }
         HIPCHK(hipGetLastError());
     } else if (mode == DRMLT_BATCHED) {
@@ -866,26 +864,22 @@ extern "C" int drmlt_compute_gaussian_f64(int64_t n, const int32_t *D, const dou
                                           double *mean, double *covL, double *invCov, double *logDet) {
     if (n < 0 || !D || !sc || !grad || !hess || !mean || !covL || !invCov || !logDet) return fail(DRMLT_ERR_ARG, "bad argument");
     for (int64_t i = 0; i < n; i++)
-        if (D[i] < 1 || D[i] > kGMax) return fail(DRMLT_ERR_ARG, "D out of range");
+        if (D[i] < 4 || D[i] > kGMax || D[i] % 2) return fail(DRMLT_ERR_ARG, "D must be even, 4 <= D <= 24");
     if (n == 0) return DRMLT_OK;
     hipStream_t s = 0;
     DBuf<int> d;
-    DBuf<double> a, g, h, m, L, Q, ld;
+    DBuf<double> a, g, h, sl, m, L, Q, ld;
     DUP(d, D, n); DUP(a, sc, n); DUP(g, grad, 32 * n); DUP(h, hess, 1024 * n);
-    DNEW(m, size_t(kGMax) * n); DNEW(L, size_t(kGMax * kGMax) * n); DNEW(Q, size_t(kGMax * kGMax) * n); DNEW(ld, n);
-    hipLaunchKernelGGL(k_gauss_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, int(n), make_h2param(sigma), d.p, a.p, g.p, h.p, m.p, L.p,
-                       Q.p, ld.p);
+    DNEW(sl, size_t(kGSlot) * n); DNEW(m, 32 * size_t(n)); DNEW(L, size_t(kGMax * kGMax) * n); DNEW(Q, size_t(kGMax * kGMax) * n);
+    DNEW(ld, n);
+    const unsigned blocks = unsigned(std::min<int64_t>(n, 65535));
+    hipLaunchKernelGGL(k_gauss_rec, dim3(blocks), dim3(64), 0, s, int(n), make_h2param(sigma), d.p, a.p, g.p, h.p, sl.p);
     HIPCHK(hipGetLastError());
-    std::vector<double> hm(size_t(kGMax) * n), hL(size_t(kGMax * kGMax) * n), hQ(size_t(kGMax * kGMax) * n);
-    DGET(hm.data(), m, hm.size()); DGET(hL.data(), L, hL.size()); DGET(hQ.data(), Q, hQ.size()); DGET(logDet, ld, n);
+    const int64_t ne = n * kGMax * kGMax;
+    hipLaunchKernelGGL(k_gauss_expand, dim3(unsigned((ne + 63) / 64)), dim3(64), 0, s, int(n), d.p, sl.p, m.p, L.p, Q.p, ld.p);
+    HIPCHK(hipGetLastError());
+    DGET(mean, m, 32 * n); DGET(covL, L, size_t(kGMax * kGMax) * n); DGET(invCov, Q, size_t(kGMax * kGMax) * n); DGET(logDet, ld, n);
     HIPCHK(hipStreamSynchronize(s));
-    for (int64_t i = 0; i < n; i++) {
-        for (int e = 0; e < 32; e++) mean[32 * i + e] = e < kGMax ? hm[size_t(e) * n + i] : 0.0;
-        for (int e = 0; e < kGMax * kGMax; e++) {
-            covL[size_t(kGMax * kGMax) * i + e] = hL[size_t(e) * n + i];
-            invCov[size_t(kGMax * kGMax) * i + e] = hQ[size_t(e) * n + i];
-        }
-    }
     return DRMLT_OK;
 }
 
@@ -895,21 +889,11 @@ extern "C" int drmlt_gaussian_logpdf_f64(int64_t n, const int32_t *D, const doub
     if (n == 0) return DRMLT_OK;
     for (int64_t i = 0; i < n; i++)
         if (D[i] < 1 || D[i] > kGMax) return fail(DRMLT_ERR_ARG, "D out of range");
-    // back to structure-of-arrays
-    std::vector<double> m(size_t(kGMax) * n), L(size_t(kGMax * kGMax) * n), Q(size_t(kGMax * kGMax) * n);
-    for (int64_t i = 0; i < n; i++) {
-        for (int e = 0; e < kGMax; e++) m[size_t(e) * n + i] = mean[32 * i + e];
-        for (int e = 0; e < kGMax * kGMax; e++) {
-            L[size_t(e) * n + i] = covL[size_t(kGMax * kGMax) * i + e];
-            Q[size_t(e) * n + i] = invCov[size_t(kGMax * kGMax) * i + e];
-        }
-    }
     hipStream_t s = 0;
     DBuf<int> d;
-    DBuf<double> xx, dm, dL, dQ, dl, o;
-    DUP(d, D, n); DUP(xx, x, 32 * n); DUP(dm, m.data(), m.size()); DUP(dL, L.data(), L.size()); DUP(dQ, Q.data(), Q.size());
-    DUP(dl, logDet, n); DNEW(o, n);
-    hipLaunchKernelGGL(k_logpdf_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, int(n), d.p, xx.p, sign, dm.p, dL.p, dQ.p, dl.p, o.p);
+    DBuf<double> xx, dm, dQ, dl, o;
+    DUP(d, D, n); DUP(xx, x, 32 * n); DUP(dm, mean, 32 * n); DUP(dQ, invCov, size_t(kGMax * kGMax) * n); DUP(dl, logDet, n); DNEW(o, n);
+    hipLaunchKernelGGL(k_logpdf_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, int(n), d.p, xx.p, sign, dm.p, dQ.p, dl.p, o.p);
     HIPCHK(hipGetLastError());
     DGET(out, o, n);
     HIPCHK(hipStreamSynchronize(s));

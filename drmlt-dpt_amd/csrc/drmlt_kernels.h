@@ -22,10 +22,10 @@ struct Chains {
     int *valid, *hasSplat, *gInit;
     float *rgb, *ls, *ss;  // rgb [3][NS]
     int *perm;
-    double *gmean, *gcovL, *ginv, *gld;
-    // per chain scratch for the batched pipeline
+    double *gauss;         // [3C][kGSlot] Gaussian slots (dev_gauss.h), one per state slot 0..3C-1
+    // per chain scratch of the DR step
     float *off;            // [32][C] H2MC offset
-    float *grad, *hess;    // [24][C], [24*24][C] derivative output
+    float *grad, *hess;    // [C][24], [C][24*24] derivative output (Hessian: upper triangle)
     float *a1, *a2;        // stage acceptances
     double *a2d;
     int *flags;            // bit0 acc1, bit1 doSecond, bit2 isAniso, bit3 large, bit4 z valid
@@ -40,13 +40,7 @@ DEV SRef slot_ref(const Chains &ch, int p) {
     r.s = ch.NS;
     return r;
 }
-DEV GRef g_ref(const Chains &ch, int p) {
-    GRef g;
-    g.mean = ch.gmean; g.covL = ch.gcovL; g.invCov = ch.ginv; g.logDet = ch.gld;
-    g.gs = ch.GS;
-    g.k = p;
-    return g;
-}
+DEV GSlot g_ref(const Chains &ch, int p) { return GSlot{ch.gauss + size_t(p) * kGSlot}; }
 
 // DR options the kernels read (src/dptoptions.h:8-37)
 struct DrParams {

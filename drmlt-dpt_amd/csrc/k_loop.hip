@@ -133,25 +133,57 @@ __global__ void __launch_bounds__(64) k_genbidir_rec(DevScene Sc, Opts o, int n,
     nc[i] = sink.n;
 }
 
-__global__ void __launch_bounds__(64) k_gauss_rec(int n, H2Param h2, const int *Ds, const double *sc, const double *grad, const double *hess, double *mean,
-                            double *covL, double *invCov, double *logDet) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int D = Ds[i];
-    GRef G;
-    G.mean = mean; G.covL = covL; G.invCov = invCov; G.logDet = logDet; G.gs = n; G.k = i;
-    compute_gaussian(h2, sc[i], D, [&](int k) { return grad[size_t(i) * 32 + k]; },
-                     [&](int a, int b) { return hess[(size_t(i) * 32 + a) * 32 + b]; }, G);
+// ComputeGaussian of one record per one-wave workgroup, the same code the
+// batched step runs; slots [n][kGSlot].
+__global__ void __launch_bounds__(64) k_gauss_rec(int n, H2Param h2, const int *Ds, const double *sc, const double *grad, const double *hess,
+                                                  double *slots) {
+    __shared__ GWork w;
+    const WaveExec x{int(threadIdx.x)};
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        x.sync();
+        gauss_compute(x, h2, sc[i], Ds[i], [&](int k) { return grad[size_t(i) * 32 + k]; },
+                      [&](int a, int b) { return hess[(size_t(i) * 32 + a) * 32 + b]; }, w, GSlot{slots + size_t(i) * kGSlot});
+    }
+}
+// The reference's representation of a factored Gaussian: mean (n x 32),
+// covL = V diag(p^-1/2) and invCov = V diag(p) V^T (n x 24 x 24), logDet.
+__global__ void __launch_bounds__(64) k_gauss_expand(int n, const int *Ds, const double *slots, double *mean, double *covL, double *invCov,
+                                                     double *logDet) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= long(n) * kGMax * kGMax) return;
+    const int r = int(t / (kGMax * kGMax)), e = int(t % (kGMax * kGMax)), i = e / kGMax, j = e % kGMax;
+    const int D = Ds[r];
+    const GSlot g{const_cast<double *>(slots) + size_t(r) * kGSlot};
+    double L = 0, Q = 0;
+    if (i < D && j < D) {
+        L = g.V(i, j) / sqrt(g.p(j));
+        for (int k = 0; k < D; k++) Q += g.V(i, k) * g.p(k) * g.V(j, k);
+    }
+    covL[t] = L;
+    invCov[t] = Q;
+    if (j == 0) mean[size_t(r) * 32 + i] = i < D ? g.m(i) : 0.0;
+    if (e == 0) {
+        logDet[r] = g.ld();
+        for (int k = kGMax; k < 32; k++) mean[size_t(r) * 32 + k] = 0.0;
+    }
 }
 
-__global__ void __launch_bounds__(64) k_logpdf_rec(int n, const int *Ds, const double *x, double sign, const double *mean, const double *covL, const double *invCov,
-                             const double *logDet, double *out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    GRef G;
-    G.mean = const_cast<double *>(mean); G.covL = const_cast<double *>(covL); G.invCov = const_cast<double *>(invCov);
-    G.logDet = const_cast<double *>(logDet); G.gs = n; G.k = i;
-    out[i] = gaussian_logpdf(Ds[i], [&](int k) { return x[size_t(i) * 32 + k]; }, sign, G);
+// GaussianLogPdf (gaussian.cpp:18-26) over the reference's representation:
+// mean (n x 32), invCov (n x 24 x 24), logDet, of sign * x.
+__global__ void __launch_bounds__(64) k_logpdf_rec(int n, const int *Ds, const double *x, double sign, const double *mean,
+                                                   const double *invCov, const double *logDet, double *out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int D = Ds[r];
+    const double *m = mean + size_t(r) * 32, *xv = x + size_t(r) * 32, *Q = invCov + size_t(r) * kGMax * kGMax;
+    double q = 0;
+    for (int i = 0; i < D; i++) {
+        double si = 0;
+        const double di = sign * xv[i] - m[i];
+        for (int j = 0; j < D; j++) si += Q[i * kGMax + j] * (sign * xv[j] - m[j]);
+        q += di * si;
+    }
+    out[r] = double(D) * (-0.5 * log(2.0 * 3.14159265358979323846)) + 0.5 * logDet[r] - 0.5 * q;
 }
 
 // Second-stage acceptance arithmetic alone (delayed.cpp:466-541), batched:

@@ -4,23 +4,27 @@
 namespace drmlt {
 
 // ------------------------------------------------------------ plain form
-__global__ void __launch_bounds__(64) k_dr_plain(DevScene Sc, Chains ch, DrParams P, uint32_t iter, float *film, unsigned long long *stats,
-                                                 float *scratch) {
+__global__ void __launch_bounds__(64) k_dr_plain(DevScene Sc, Chains ch, DrParams P, uint32_t iter, float *film, unsigned long long *stats) {
     const int chain = blockIdx.x * blockDim.x + threadIdx.x;
     if (chain >= ch.C) return;
     const int flags = dr_stage1(Sc, ch, P, chain, iter, film, stats);
     if (!(flags & 2)) return;
-    float *gbuf = scratch + size_t(chain) * (kGMax + kGMax * kGMax);
-    float *hbuf = gbuf + kGMax;
     const int perm = ch.perm[chain];
     const int pc = chain + ch.C * perm_cur(perm), ps = chain + ch.C * perm_second(perm);
+    const SeqExec x;
+    GWork w;
     if (flags & 4) {
-        if (!ch.gInit[pc]) init_gaussian_thread(Sc, ch, P, pc, gbuf, hbuf);
+        if (!ch.gInit[pc]) {
+            derivs_thread(Sc, ch, P, chain, pc);
+            gaussian_from_derivs(x, ch, P, chain, pc, w);
+        }
+        h2mc_offset(x, ch, P, chain, iter, w);
     }
     dr_stage2_propose(Sc, ch, P, chain, iter);
     if ((flags & 4) && (ch.flags[chain] & 16)) {
-        init_gaussian_thread(Sc, ch, P, ps, gbuf, hbuf);
-        dr_stage2_ratio(ch, chain);
+        derivs_thread(Sc, ch, P, chain, ps);
+        gaussian_from_derivs(x, ch, P, chain, ps, w);
+        h2mc_ratio(x, ch, chain, w);
     }
     dr_stage2_accept(Sc, ch, P, chain, iter, film, stats);
 }

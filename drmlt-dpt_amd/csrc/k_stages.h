@@ -31,50 +31,42 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chain
             const int pc = chain + ch.C * perm_cur(perm);
             const int bin = min(ch.c[pc] + ch.l[pc] - 1, kMaxBins - 1);
             list_push(L, 1, ch.C, bin, chain);
-            if ((flags & 4) && !ch.gInit[pc]) list_push(L, 0, ch.C, bin, chain);
+            if (flags & 4) list_push(L, 0, ch.C, bin, chain);
         }
     }
 }
 
-__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which, int finishRatio) {
-    const int gsz = gridDim.x * blockDim.x;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per Gaussian (one-wave workgroups, the work matrices in LDS).
+// kind 0 (which 0): every H2MC chain; the current state's Gaussian unless it
+// is cached, then the proposal offset. kind 2 (which 1): the proposal's
+// Gaussian, then (finishRatio) the H2MC ratio.
+__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which,
+                                                             int finishRatio, const uint32_t *iterp) {
+    __shared__ GWork w;
+    const WaveExec x{int(threadIdx.x)};
+    const uint32_t iter = *iterp;
     for (int bin = 0; bin < kMaxBins; bin++) {
         const int n = L.count[kind * kMaxBins + bin];
         const int *items = L.items + (size_t(kind) * kMaxBins + bin) * ch.C;
-        for (int w = tid; w < n; w += gsz) {
-            const int chain = items[w];
+        for (int it = blockIdx.x; it < n; it += gridDim.x) {
+            const int chain = items[it];
             const int perm = ch.perm[chain];
             const int p = chain + ch.C * (which == 0 ? perm_cur(perm) : perm_second(perm));
-            const int c = ch.c[p], l = ch.l[p], D = dim_of(c, l);
-            const GRef G = g_ref(ch, p);
-            const int C = ch.C;
-            if (!has_derivative(c, l, P.maxDervDepth) || D > kGMax) {
-                iso_gaussian(D, P.h2.sigma, G);
-            } else {
-                const double ssv = ch.ss[p];
-                const float *gr = ch.grad + chain, *hs = ch.hess + chain;
-                bool fin = ssv > 1e-15;
-                if (fin) {
-                    for (int i = 0; i < D && fin; i++) fin = isfinite(gr[i * C]);
-                    for (int i = 0; i < D && fin; i++)
-                        for (int j = i; j < D && fin; j++) fin = isfinite(hs[(i * kGMax + j) * C]);
-                }
-                if (fin) {
-                    compute_gaussian(P.h2, ssv, D, [&](int i) { return double(gr[i * C]); },
-                                     [&](int i, int j) { return double(hs[(i * kGMax + j) * C]); }, G);
-                } else {
-                    compute_gaussian(P.h2, ssv, D, [&](int) { return 0.0; }, [&](int, int) { return 0.0; }, G);
+            const bool cached = which == 0 && ch.gInit[p] != 0;
+            x.sync();
+            if (!cached) {
+                gaussian_from_derivs(x, ch, P, chain, p, w);
+                if (x.lane == 0) {
+                    // gradient and upper Hessian read (float); V, p, mean, logDet written (double)
+                    const int D = dim_of(ch.c[p], ch.l[p]);
+                    count_bytes(Sc, (unsigned long long)(4 * (D + D * (D + 1) / 2) + 8 * (D * D + 2 * D + 1)));
                 }
             }
-            ch.gInit[p] = 1;
-            // gradient and upper Hessian read (float), mean, covL, invCov, logDet written (double)
-            count_bytes(Sc, (unsigned long long)(4 * (D + D * (D + 1) / 2) + 8 * (D + 2 * D * D + 1)));
-            if (finishRatio) dr_stage2_ratio(ch, chain);
+            if (which == 0) h2mc_offset(x, ch, P, chain, iter, w);
+            if (finishRatio) h2mc_ratio(x, ch, chain, w);
         }
     }
 }
-
 __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_propose)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
     const uint32_t iter = *iterp;
     const int gsz = gridDim.x * blockDim.x;
@@ -128,13 +120,13 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_deriv)(DevScene Sc, Chains
             const int perm = ch.perm[chain];
             const int p = chain + ch.C * (which == 0 ? perm_cur(perm) : perm_second(perm));
             if (!has_derivative(ch.c[p], ch.l[p], P.maxDervDepth) || !(double(ch.ss[p]) > 1e-15)) continue;
+            if (which == 0 && ch.gInit[p]) continue;  // cached Gaussian
             int row, col0;
             jet_pass_index(D, q, row, col0);
-            float *gr = ch.grad + chain, *hs = ch.hess + chain;
-            const int C = ch.C;
+            float *gr = ch.grad + size_t(chain) * kGMax, *hs = ch.hess + size_t(chain) * kGMax * kGMax;
             const bool ok = jet_pass<float>(Sc, P.oDeriv, slot_ref(ch, p), D, row, col0,
-                                            [&](int cc, float gv) { if (row == 0) gr[cc * C] = gv; },
-                                            [&](int cc, float hv) { if (cc >= row) hs[(row * kGMax + cc) * C] = hv; });
+                                            [&](int cc, float gv) { if (row == 0) gr[cc] = gv; },
+                                            [&](int cc, float hv) { if (cc >= row) hs[row * kGMax + cc] = hv; });
             if (!ok) gr[0] = __builtin_nanf("");
             // state read by the pass, gradient (row 0) and Hessian values written
             count_bytes(Sc, (unsigned long long)(state_bytes(ch.c[p], ch.l[p]) + 4 * kJetK * (row == 0 ? 2 : 1)));
