@@ -20,6 +20,21 @@ DEV void list_push(const Lists &L, int kind, int C, int bin, int item) {
     const int slot = atomicAdd(&L.count[kind * kMaxBins + bin], 1);
     L.items[(size_t(kind) * kMaxBins + bin) * C + slot] = item;
 }
+// The bins of a list concatenated in bin order: its length, and item w of it.
+// Consecutive items share a bin (a path length) except at bin boundaries.
+DEV int list_length(const Lists &L, int kind) {
+    int n = 0;
+    for (int b = 0; b < kMaxBins; b++) n += L.count[kind * kMaxBins + b];
+    return n;
+}
+DEV int list_item(const Lists &L, int kind, int C, int w) {
+    for (int b = 0; b < kMaxBins; b++) {
+        const int n = L.count[kind * kMaxBins + b];
+        if (w < n) return L.items[(size_t(kind) * kMaxBins + b) * C + w];
+        w -= n;
+    }
+    return -1;
+}
 
 __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
                                                  unsigned long long *stats, Lists L) {
@@ -45,11 +60,10 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains
     __shared__ GWork w;
     const WaveExec x{int(threadIdx.x)};
     const uint32_t iter = *iterp;
-    for (int bin = 0; bin < kMaxBins; bin++) {
-        const int n = L.count[kind * kMaxBins + bin];
-        const int *items = L.items + (size_t(kind) * kMaxBins + bin) * ch.C;
+    const int n = list_length(L, kind);
+    {
         for (int it = blockIdx.x; it < n; it += gridDim.x) {
-            const int chain = items[it];
+            const int chain = list_item(L, kind, ch.C, it);
             const int perm = ch.perm[chain];
             const int p = chain + ch.C * (which == 0 ? perm_cur(perm) : perm_second(perm));
             const bool cached = which == 0 && ch.gInit[p] != 0;
@@ -70,15 +84,14 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains
 __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_propose)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
     const uint32_t iter = *iterp;
     const int gsz = gridDim.x * blockDim.x;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int bin = 0; bin < kMaxBins; bin++) {
-        const int n = L.count[1 * kMaxBins + bin];
-        const int *items = L.items + (size_t(1) * kMaxBins + bin) * ch.C;
-        for (int w = tid; w < n; w += gsz) {
-            const int chain = items[w];
-            dr_stage2_propose(Sc, ch, P, chain, iter);
-            const int flags = ch.flags[chain];
-            if ((flags & 4) && (flags & 16)) list_push(L, 2, ch.C, bin, chain);
+    const int n = list_length(L, 1);
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gsz) {
+        const int chain = list_item(L, 1, ch.C, w);
+        dr_stage2_propose(Sc, ch, P, chain, iter);
+        const int flags = ch.flags[chain];
+        if ((flags & 4) && (flags & 16)) {
+            const int pc = chain + ch.C * perm_cur(ch.perm[chain]);
+            list_push(L, 2, ch.C, min(ch.c[pc] + ch.l[pc] - 1, kMaxBins - 1), chain);
         }
     }
 }
@@ -87,50 +100,46 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_accept)(DevScene Sc
                                                         unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
     const int gsz = gridDim.x * blockDim.x;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int bin = 0; bin < kMaxBins; bin++) {
-        const int n = L.count[1 * kMaxBins + bin];
-        const int *items = L.items + (size_t(1) * kMaxBins + bin) * ch.C;
-        for (int w = tid; w < n; w += gsz) dr_stage2_accept(Sc, ch, P, items[w], iter, film, stats);
-    }
+    const int n = list_length(L, 1);
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gsz) dr_stage2_accept(Sc, ch, P, list_item(L, 1, ch.C, w), iter, film, stats);
 }
 
 // Derivative passes over list `kind` (slot: 0 current, 2 second), persistent
 // over (chain, pass) work items bin by bin.
 __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_deriv)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which) {
-    const int gsz = gridDim.x * blockDim.x;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int bin = 0; bin < kMaxBins; bin++) {
-        const int n = L.count[kind * kMaxBins + bin];
-        if (n == 0) continue;
-        const int *items = L.items + (size_t(kind) * kMaxBins + bin) * ch.C;
-        const int chain0 = items[0];
-        const int perm0 = ch.perm[chain0];
-        const int p0 = chain0 + ch.C * (which == 0 ? perm_cur(perm0) : perm_second(perm0));
-        const int c0 = ch.c[p0], l0 = ch.l[p0];
-        (void)l0;
-        // every chain in a bin has the same path length, hence the same D
-        const int D = dim_of(c0, ch.l[p0]);
-        if (D > kGMax) continue;
-        const int np = jet_passes(D);
-        const long total = long(n) * np;
-        for (long w = tid; w < total; w += gsz) {
-            const int chain = items[w / np];
-            const int q = int(w % np);
-            const int perm = ch.perm[chain];
-            const int p = chain + ch.C * (which == 0 ? perm_cur(perm) : perm_second(perm));
-            if (!has_derivative(ch.c[p], ch.l[p], P.maxDervDepth) || !(double(ch.ss[p]) > 1e-15)) continue;
-            if (which == 0 && ch.gInit[p]) continue;  // cached Gaussian
-            int row, col0;
-            jet_pass_index(D, q, row, col0);
-            float *gr = ch.grad + size_t(chain) * kGMax, *hs = ch.hess + size_t(chain) * kGMax * kGMax;
-            const bool ok = jet_pass<float>(Sc, P.oDeriv, slot_ref(ch, p), D, row, col0,
-                                            [&](int cc, float gv) { if (row == 0) gr[cc] = gv; },
-                                            [&](int cc, float hv) { if (cc >= row) hs[row * kGMax + cc] = hv; });
-            if (!ok) gr[0] = __builtin_nanf("");
-            // state read by the pass, gradient (row 0) and Hessian values written
-            count_bytes(Sc, (unsigned long long)(state_bytes(ch.c[p], ch.l[p]) + 4 * kJetK * (row == 0 ? 2 : 1)));
+    // bin b holds paths of length b (D = 2 max(b, 2)); its work items are
+    // (chain, pass) pairs, all bins concatenated
+    long total = 0;
+    for (int b = 0; b < kMaxBins; b++) {
+        const int D = 2 * (b > 2 ? b : 2);
+        if (D <= kGMax) total += long(L.count[kind * kMaxBins + b]) * jet_passes(D);
+    }
+    const long gsz = long(gridDim.x) * blockDim.x;
+    for (long w = long(blockIdx.x) * blockDim.x + threadIdx.x; w < total; w += gsz) {
+        long r = w;
+        int bin = 0, D = 0, np = 0;
+        for (; bin < kMaxBins; bin++) {
+            D = 2 * (bin > 2 ? bin : 2);
+            np = D <= kGMax ? jet_passes(D) : 0;
+            const long nb = long(L.count[kind * kMaxBins + bin]) * np;
+            if (r < nb) break;
+            r -= nb;
         }
+        const int chain = L.items[(size_t(kind) * kMaxBins + bin) * ch.C + r / np];
+        const int q = int(r % np);
+        const int perm = ch.perm[chain];
+        const int p = chain + ch.C * (which == 0 ? perm_cur(perm) : perm_second(perm));
+        if (!has_derivative(ch.c[p], ch.l[p], P.maxDervDepth) || !(double(ch.ss[p]) > 1e-15)) continue;
+        if (which == 0 && ch.gInit[p]) continue;  // cached Gaussian
+        int row, col0;
+        jet_pass_index(D, q, row, col0);
+        float *gr = ch.grad + size_t(chain) * kGMax, *hs = ch.hess + size_t(chain) * kGMax * kGMax;
+        const bool ok = jet_pass<float>(Sc, P.oDeriv, slot_ref(ch, p), D, row, col0,
+                                        [&](int cc, float gv) { if (row == 0) gr[cc] = gv; },
+                                        [&](int cc, float hv) { if (cc >= row) hs[row * kGMax + cc] = hv; });
+        if (!ok) gr[0] = __builtin_nanf("");
+        // state read by the pass, gradient (row 0) and Hessian values written
+        count_bytes(Sc, (unsigned long long)(state_bytes(ch.c[p], ch.l[p]) + 4 * kJetK * (row == 0 ? 2 : 1)));
     }
 }
 

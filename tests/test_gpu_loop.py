@@ -60,6 +60,8 @@ def test_forms_agree(cfg1):
         res[mode] = (ctx.film(1), ctx.stats())
         ctx.close()
     f0, s0 = res[capi.PLAIN]
+    say("stats", capi.PLAIN, s0)
+    assert res[capi.BATCHED][1] == res[capi.BATCHED_GRAPH][1]
     for mode in (capi.BATCHED, capi.BATCHED_GRAPH):
         f, s = res[mode]
         say("stats", mode, s)
