@@ -43,6 +43,22 @@ def cpu_baseline(sc, threads, chains, iters, seed=11):
     return chains * iters / sec, sec
 
 
+def copy_calibration(mb):
+    """One device-to-device hipMemcpy of `mb` MB (read mb, write mb)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {mb} MB device to device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +69,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="first copy MB megabytes device to device (hipMemcpy), the known-size kernel that scales "
+                         "rocprof FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the DR chains would shard by chain "
@@ -61,6 +80,8 @@ def main():
 
     from drmlt import capi, scenes
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     t0 = time.time()
     sc = scenes.CONFIGS[args.config]()
     dpt = dict(sc["options"])
