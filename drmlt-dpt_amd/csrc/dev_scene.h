@@ -8,7 +8,13 @@
 
 namespace drmlt {
 
-constexpr int kBvhStack = 48;  // the builder caps tree depth below this
+constexpr int kBvhStack = 48;    // the builder caps tree depth below this
+constexpr int kTraceLanes = 64;  // workgroup size of every kernel that traces rays
+
+// Traversal stacks of a workgroup's lanes in LDS, interleaved by lane (entry k
+// of lane t at [k * kTraceLanes + t]). Every traversal of a lane reuses its one
+// stack, whatever the number of call sites the path code inlines.
+__shared__ int g_trace_stack[kBvhStack * kTraceLanes];
 
 struct DevCamera {
     float s2c[16], c2s[16], c2w[16], w2c[16];
@@ -107,7 +113,7 @@ DEV float next_up(float x) { return __int_as_float(__float_as_int(x) + 1); }
 // scene-order triangle index or -1.
 DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], float tmin, float tmax, float &tbest) {
     float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
-    int stack[kBvhStack];
+    int *stack = g_trace_stack + threadIdx.x;  // entry k at stack[k * kTraceLanes]
     int sp = 0;
     int best = -1;
     tbest = tmax;
@@ -147,7 +153,7 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
         }
         if (hl && hr) {
             int nearC = tl <= tr ? ch.x : ch.y, farC = tl <= tr ? ch.y : ch.x;
-            stack[sp++] = farC;
+            stack[kTraceLanes * sp++] = farC;
             node = nearC;
         } else if (hl) {
             node = ch.x;
@@ -157,7 +163,7 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
             // pop, skipping entries whose box now lies beyond the best hit is
             // not possible without their distance; pop unconditionally
             if (sp == 0) break;
-            node = stack[--sp];
+            node = stack[kTraceLanes * --sp];
         }
     }
     DRMLT_COUNT(count_work(S, nvis, ntri);)
@@ -166,7 +172,7 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
 
 DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float tmin, float tmax) {
     float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
-    int stack[kBvhStack];
+    int *stack = g_trace_stack + threadIdx.x;  // entry k at stack[k * kTraceLanes]
     int sp = 0;
     int node = 0;
     DRMLT_COUNT(unsigned nvis = 0; unsigned ntri = 0;)
@@ -200,7 +206,7 @@ DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float 
             hr = false;
         }
         if (hl && hr) {
-            stack[sp++] = ch.y;
+            stack[kTraceLanes * sp++] = ch.y;
             node = ch.x;
         } else if (hl) {
             node = ch.x;
@@ -208,7 +214,7 @@ DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float 
             node = ch.y;
         } else {
             if (sp == 0) break;
-            node = stack[--sp];
+            node = stack[kTraceLanes * --sp];
         }
     }
     DRMLT_COUNT(count_work(S, nvis, ntri);)

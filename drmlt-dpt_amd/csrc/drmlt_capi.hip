@@ -547,7 +547,7 @@ extern "C" int drmlt_direct_lighting(drmlt_ctx *c) {
     if (spp > 0 && c->opt.min_depth <= 2) {
         const long n = long(c->W) * c->H * spp;
         DrParams P = make_params(c);
-        hipLaunchKernelGGL(k_direct, dim3(unsigned((n + 127) / 128)), dim3(128), 0, s, c->S, P.o, spp, P.seed ^ 0xD1EC7ull, c->filmDir);
+        hipLaunchKernelGGL(k_direct, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.o, spp, P.seed ^ 0xD1EC7ull, c->filmDir);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -727,7 +727,7 @@ extern "C" int drmlt_trace(drmlt_ctx *c, int64_t n, const float *org, const floa
     DBuf<float> o, d, a, b, tt;
     DBuf<int> pr;
     DUP(o, org, 3 * n); DUP(d, dir, 3 * n); DUP(a, tmin, n); DUP(b, tmax, n); DNEW(pr, n); DNEW(tt, n);
-    hipLaunchKernelGGL(k_trace, dim3(unsigned((n + 127) / 128)), dim3(128), 0, s, c->S, int(n), o.p, d.p, a.p, b.p, any, pr.p, tt.p);
+    hipLaunchKernelGGL(k_trace, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, int(n), o.p, d.p, a.p, b.p, any, pr.p, tt.p);
     HIPCHK(hipGetLastError());
     DGET(prim, pr, n);
     if (!any) DGET(t, tt, n);

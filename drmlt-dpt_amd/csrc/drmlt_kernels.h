@@ -6,7 +6,7 @@
 
 namespace drmlt {
 
-constexpr int kJetK = 8;     // Hessian columns per derivative pass
+constexpr int kJetK = 2;     // Hessian columns per derivative pass
 constexpr int kMaxBins = 16; // path-length bins (path length 0..15)
 
 // Chain state: 4 physical slots per chain (current, first- and second-stage

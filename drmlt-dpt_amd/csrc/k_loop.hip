@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(64) k_prepass_seed(DevScene Sc, Opts o, Chains
 }
 
 // ------------------------------------------------------------ direct pass
-__global__ void __launch_bounds__(128) k_direct(DevScene Sc, Opts o, int spp, uint64_t seed, float *film) {
+__global__ void __launch_bounds__(64) k_direct(DevScene Sc, Opts o, int spp, uint64_t seed, float *film) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long npix = long(Sc.cam.W) * Sc.cam.H;
     if (i >= npix * spp) return;
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(128) k_direct(DevScene Sc, Opts o, int spp, ui
 }
 
 // ------------------------------------------------------------ parity surface
-__global__ void __launch_bounds__(128) k_trace(DevScene Sc, int n, const float *org, const float *dir, const float *tmin, const float *tmax, int any, int *prim,
+__global__ void __launch_bounds__(64) k_trace(DevScene Sc, int n, const float *org, const float *dir, const float *tmin, const float *tmax, int any, int *prim,
                         float *t) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

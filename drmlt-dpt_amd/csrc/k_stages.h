@@ -14,8 +14,9 @@ namespace drmlt {
 #endif
 
 // ------------------------------------------------------------ batched form
-// List kinds: 0 = needs current Gaussian, 1 = second stage (aniso or iso),
-// 2 = needs proposal Gaussian. Each is binned by path length.
+// Work lists, binned by path length: kind 0 = H2MC chains (current Gaussian
+// and offset), kind 2 = H2MC chains with a valid proposal (its Gaussian and
+// the ratio). The stage kernels proper run over all chains in index order.
 DEV void list_push(const Lists &L, int kind, int C, int bin, int item) {
     const int slot = atomicAdd(&L.count[kind * kMaxBins + bin], 1);
     L.items[(size_t(kind) * kMaxBins + bin) * C + slot] = item;
@@ -45,7 +46,6 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chain
             const int perm = ch.perm[chain];
             const int pc = chain + ch.C * perm_cur(perm);
             const int bin = min(ch.c[pc] + ch.l[pc] - 1, kMaxBins - 1);
-            list_push(L, 1, ch.C, bin, chain);
             if (flags & 4) list_push(L, 0, ch.C, bin, chain);
         }
     }
@@ -84,9 +84,9 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains
 __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_propose)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
     const uint32_t iter = *iterp;
     const int gsz = gridDim.x * blockDim.x;
-    const int n = list_length(L, 1);
-    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gsz) {
-        const int chain = list_item(L, 1, ch.C, w);
+    // chains in index order (coalesced state access); those without a second stage idle
+    for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gsz) {
+        if (!(ch.flags[chain] & 2)) continue;
         dr_stage2_propose(Sc, ch, P, chain, iter);
         const int flags = ch.flags[chain];
         if ((flags & 4) && (flags & 16)) {
@@ -100,8 +100,8 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_accept)(DevScene Sc
                                                         unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
     const int gsz = gridDim.x * blockDim.x;
-    const int n = list_length(L, 1);
-    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gsz) dr_stage2_accept(Sc, ch, P, list_item(L, 1, ch.C, w), iter, film, stats);
+    for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gsz)
+        if (ch.flags[chain] & 2) dr_stage2_accept(Sc, ch, P, chain, iter, film, stats);
 }
 
 // Derivative passes over list `kind` (slot: 0 current, 2 second), persistent
