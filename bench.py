@@ -24,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "drmlt-dpt_amd"))
 
 HBM_PEAK_GBS = 8000.0
 STAGES = ["stage1", "deriv_cur", "gauss_cur", "stage2_propose", "deriv_prop", "gauss_prop", "stage2_accept"]
+STAGE_KERNELS = ["k_b_stage1", "k_b_deriv", "k_b_gauss", "k_b_stage2_propose", "k_b_deriv", "k_b_gauss", "k_b_stage2_accept"]
 
 
 def log(*a):
@@ -123,15 +124,28 @@ def main():
     nodes, tris, rays, state_bytes = work[dom]
     alg_bytes = nodes * 64 + tris * 48 + state_bytes
     achieved = alg_bytes / (stage_ms[dom] / 1000.0) / 1e9
+    traffic, traffic_src = None, None
+    prof = os.path.join(ROOT, "profiles", f"dr_step_config{args.config}_hbm_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            pt = json.load(f)
+        k = STAGE_KERNELS[dom]
+        if k in pt.get("kernels", {}):
+            traffic = int(pt["kernels"][k]["hbm_bytes_per_launch"])
+            traffic_src = (f"rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of {k} at commit {pt['commit']}, scaled by a "
+                           f"256 MiB device copy (x{pt['fetch_scale']:.2f}, x{pt['write_scale']:.2f}); "
+                           f"profiles/dr_step_config{args.config}_hbm_traffic.md")
     roof = {"kernel": STAGES[dom], "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": int(alg_bytes), "launch_ms": round(float(stage_ms[dom]), 4),
             "rays_per_launch": int(rays)}
+    if traffic_src:
+        roof["traffic_source"] = traffic_src
 
     cpu = None
     if not args.no_cpu:
         ncpu = min(args.cpu_threads, os.cpu_count() or 1)
-        cchains, citers = (16384, 128) if args.config != 1 else (1024, 512)
+        cchains, citers = (16384, 512) if args.config != 1 else (1024, 2048)
         log(f"cpu baseline: the oracle loop, {cchains} chains x {citers} iterations on {ncpu} threads")
         cval, csec = cpu_baseline(sc, ncpu, cchains, citers)
         cpu = {"value": round(cval, 1), "unit": "mutations/s", "cores": ncpu, "kind": "port",
@@ -148,6 +162,9 @@ def main():
                                f"{int(sc['cam'][10])}x{int(sc['cam'][11])} film, type={dpt['type']}",
                    "chains": chains, "form": "batched+graph"},
         "plain_form": {"ms_per_step": round(plain_ms, 3), "mutations_per_s": round(chains / (plain_ms / 1000.0), 1)},
+        # the forms advance identical chains (identical acceptance statistics, tests/test_gpu_loop.py), so
+        # equal RMSE is reached after equal iteration counts and the time ratio is the step-time ratio
+        "equal_rmse_time_ratio_plain_over_batched": round(plain_ms / ms_per_step, 3),
         "stage_ms": {n: round(float(v), 4) for n, v in zip(STAGES, stage_ms)},
         "acceptance": st,
         "roofline": roof,

@@ -7,6 +7,12 @@
 
 namespace drmlt {
 
+// Minimum waves per SIMD the path-stage kernels are compiled for (bounds
+// their registers); more resident waves hide traversal latency.
+#ifndef DRMLT_STAGE_WAVES
+#define DRMLT_STAGE_WAVES 1
+#endif
+
 #ifdef DRMLT_COUNT_WORK
 #define DRMLT_STAGE(name) name##_counted
 #else
@@ -37,7 +43,7 @@ DEV int list_item(const Lists &L, int kind, int C, int w) {
     return -1;
 }
 
-__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
+__global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
                                                  unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
     for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gridDim.x * blockDim.x) {
@@ -81,7 +87,7 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains
         }
     }
 }
-__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_propose)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
+__global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_propose)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
     const uint32_t iter = *iterp;
     const int gsz = gridDim.x * blockDim.x;
     // chains in index order (coalesced state access); those without a second stage idle
@@ -96,7 +102,7 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_propose)(DevScene S
     }
 }
 
-__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_accept)(DevScene Sc, Chains ch, DrParams P, uint32_t *iterp, float *film,
+__global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_accept)(DevScene Sc, Chains ch, DrParams P, uint32_t *iterp, float *film,
                                                         unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
     const int gsz = gridDim.x * blockDim.x;
