@@ -122,7 +122,8 @@ def main():
     # roofline of the dominant stage kernel
     dom = int(max(range(len(STAGES)), key=lambda k: stage_ms[k]))
     nodes, tris, rays, state_bytes = work[dom]
-    alg_bytes = nodes * 64 + tris * 48 + state_bytes
+    # 128 B per 4-wide BVH node visited (one cache line), 48 B per triangle tested
+    alg_bytes = nodes * 128 + tris * 48 + state_bytes
     achieved = alg_bytes / (stage_ms[dom] / 1000.0) / 1e9
     traffic, traffic_src = None, None
     prof = os.path.join(ROOT, "profiles", f"dr_step_config{args.config}_hbm_traffic.json")

@@ -1,9 +1,11 @@
 // Binned-SAH BVH builder (host, once per scene) replacing the reference's
-// Embree build (scene.cpp:42-54). Output layout is what the gfx950 traversal
-// reads: 4 float4 per internal node -- both child boxes plus
-// (left, right, leftCount, rightCount); a child with count > 0 is a leaf whose
-// triangles are [index, index + count) in the reordered triangle arrays.
-// Tree depth is capped below the device traversal stack (kBvhStack).
+// Embree build (scene.cpp:42-54). A binary SAH tree is built first and then
+// collapsed into a 4-wide tree, which is what the gfx950 traversal reads:
+// 8 float4 (128 bytes, one cache line) per node -- the four child boxes as
+// structure of arrays (lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4])
+// followed by child[4] and count[4]: count > 0 is a leaf whose triangles are
+// [child, child + count) in the reordered triangle arrays, count == 0 an
+// internal node, count < 0 an empty slot.
 #include "bvh_build.h"
 
 #include <algorithm>
@@ -25,12 +27,18 @@ struct Box {
     }
 };
 
+// binary tree of the first pass
+struct BNode {
+    Box box[2];
+    int index[2], count[2];  // count > 0: leaf range; 0: internal node index
+};
+
 struct Builder {
     const float *triP;
     std::vector<int> idx;
     std::vector<Box> tb;
     std::vector<float> cen;
-    std::vector<float> nodes;  // 16 floats per node
+    std::vector<BNode> bnodes;
     int maxDepth;
     int maxLeaf = 4;
     int depthReached = 0;
@@ -108,22 +116,64 @@ struct Builder {
             std::nth_element(idx.begin() + first, idx.begin() + mid, idx.begin() + first + n,
                              [&](int a, int b) { return cen[3 * a + axis] < cen[3 * b + axis]; });
         }
-        int node = int(nodes.size() / 16);
-        nodes.resize(nodes.size() + 16);
+        const int node = int(bnodes.size());
+        bnodes.emplace_back();
         int li, lc, ri, rc;
         Box lb, rb;
         build(first, mid - first, depth + 1, li, lc, lb);
         build(mid, first + n - mid, depth + 1, ri, rc, rb);
         pad(lb);
         pad(rb);
-        float *nd = &nodes[16 * node];
-        nd[0] = lb.lo[0]; nd[1] = lb.lo[1]; nd[2] = lb.lo[2]; nd[3] = lb.hi[0];
-        nd[4] = lb.hi[1]; nd[5] = lb.hi[2]; nd[6] = rb.lo[0]; nd[7] = rb.lo[1];
-        nd[8] = rb.lo[2]; nd[9] = rb.hi[0]; nd[10] = rb.hi[1]; nd[11] = rb.hi[2];
-        int ch[4] = {li, ri, lc, rc};
-        std::memcpy(nd + 12, ch, 16);
+        BNode &bn = bnodes[node];
+        bn.box[0] = lb; bn.box[1] = rb;
+        bn.index[0] = li; bn.index[1] = ri;
+        bn.count[0] = lc; bn.count[1] = rc;
         outIndex = node;
         outCount = 0;
+    }
+
+    // 4-wide collapse: a node takes the two children of binary node b and
+    // repeatedly opens its internal child of largest area until it holds four.
+    struct Slot { Box box; int index, count; };
+    std::vector<float> nodes;  // 32 floats per 4-wide node
+    int depth4 = 0, stackNeed = 0;
+    int collapse(int b, int depth, int pushed) {
+        Slot s[4];
+        int n = 2;
+        for (int k = 0; k < 2; k++) s[k] = Slot{bnodes[b].box[k], bnodes[b].index[k], bnodes[b].count[k]};
+        while (n < 4) {
+            int open = -1;
+            float best = -1;
+            for (int k = 0; k < n; k++)
+                if (s[k].count == 0 && s[k].box.area() > best) { best = s[k].box.area(); open = k; }
+            if (open < 0) break;
+            const BNode &c = bnodes[s[open].index];
+            s[open] = Slot{c.box[0], c.index[0], c.count[0]};
+            s[n++] = Slot{c.box[1], c.index[1], c.count[1]};
+        }
+        const int me = int(nodes.size() / 32);
+        nodes.resize(nodes.size() + 32);
+        depth4 = std::max(depth4, depth);
+        // a traversal visiting this node pushes at most n - 1 siblings
+        const int pushHere = pushed + (n - 1);
+        stackNeed = std::max(stackNeed, pushHere);
+        int child[4], count[4];
+        for (int k = 0; k < 4; k++) {
+            if (k >= n) { child[k] = 0; count[k] = -1; continue; }
+            if (s[k].count > 0) { child[k] = s[k].index; count[k] = s[k].count; }
+            else { child[k] = collapse(s[k].index, depth + 1, pushHere); count[k] = 0; }
+        }
+        float *nd = &nodes[32 * size_t(me)];
+        for (int k = 0; k < 4; k++) {
+            const Box e;  // empty slot: an inverted box
+            const Box &bx = k < n ? s[k].box : e;
+            nd[0 + k] = bx.lo[0]; nd[4 + k] = bx.hi[0];
+            nd[8 + k] = bx.lo[1]; nd[12 + k] = bx.hi[1];
+            nd[16 + k] = bx.lo[2]; nd[20 + k] = bx.hi[2];
+        }
+        std::memcpy(nd + 24, child, 16);
+        std::memcpy(nd + 28, count, 16);
+        return me;
     }
 };
 }  // namespace
@@ -142,27 +192,29 @@ int BuildBvh(int nt, const float *triP, int maxDepth, BvhOut &out) {
         b.tb[i] = bx;
         for (int k = 0; k < 3; k++) b.cen[3 * size_t(i) + k] = 0.5f * (bx.lo[k] + bx.hi[k]);
     }
-    b.nodes.reserve(size_t(nt) * 16 / 2 + 16);
+    b.bnodes.reserve(size_t(nt) / 2 + 2);
     if (nt <= b.maxLeaf) {
         // a root whose two children are the same leaf (testing a triangle twice
-        // gives the same hit; every child slot of a node is then a leaf)
-        b.nodes.resize(16);
+        // gives the same hit)
+        BNode r;
         Box lb = b.rangeBox(0, nt);
         Builder::pad(lb);
-        float *nd = &b.nodes[0];
-        nd[0] = lb.lo[0]; nd[1] = lb.lo[1]; nd[2] = lb.lo[2]; nd[3] = lb.hi[0];
-        nd[4] = lb.hi[1]; nd[5] = lb.hi[2];
-        nd[6] = lb.lo[0]; nd[7] = lb.lo[1]; nd[8] = lb.lo[2]; nd[9] = lb.hi[0]; nd[10] = lb.hi[1]; nd[11] = lb.hi[2];
-        int ch[4] = {0, 0, nt, nt};
-        std::memcpy(nd + 12, ch, 16);
+        r.box[0] = r.box[1] = lb;
+        r.index[0] = r.index[1] = 0;
+        r.count[0] = r.count[1] = nt;
+        b.bnodes.push_back(r);
     } else {
         int ri, rc;
         Box rb;
         b.build(0, nt, 0, ri, rc, rb);
     }
+    b.nodes.reserve(b.bnodes.size() * 32 / 2 + 32);
+    b.collapse(0, 0, 0);
     out.nodes = std::move(b.nodes);
     out.order = std::move(b.idx);
     out.depth = b.depthReached;
+    out.depth4 = b.depth4;
+    out.stackNeed = b.stackNeed;
     return 0;
 }
 

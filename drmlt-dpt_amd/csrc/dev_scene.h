@@ -8,7 +8,7 @@
 
 namespace drmlt {
 
-constexpr int kBvhStack = 48;    // the builder caps tree depth below this
+constexpr int kBvhStack = 48;    // traversal stack entries per lane (the builder keeps every ray within it)
 constexpr int kTraceLanes = 64;  // workgroup size of every kernel that traces rays
 
 // Traversal stacks of a workgroup's lanes in LDS, interleaved by lane (entry k
@@ -25,7 +25,8 @@ struct DevCamera {
 struct DevSceneData {
     // traversal data (leaf order): p0 | e1 | e2, .w of tA holds the scene-order index
     const float4 *tA, *tB, *tC;
-    // BVH: 4 float4 per node: (l.lo xyz, l.hi x) (l.hi yz, r.lo xy) (r.lo z, r.hi xyz) (int left, right, lcount, rcount)
+    // 4-wide BVH: 8 float4 per node: lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4], child[4], count[4]
+    // (count > 0 leaf triangle range, 0 internal node, < 0 empty slot)
     const float4 *nodes;
     // scene order
     const float4 *sp0, *se1, *se2;  // p0, e1 = p1 - p0, e2 = p2 - p0 (float, as the reference computes them)
@@ -111,6 +112,25 @@ DEV float next_up(float x) { return __int_as_float(__float_as_int(x) + 1); }
 
 // Closest hit; ties in t go to the smaller scene-order index. Returns the
 // scene-order triangle index or -1.
+// One 4-wide node: the ray's interval against each child box (slab) and the
+// child records.
+struct Node4 {
+    float4 lx, hx, ly, hy, lz, hz;
+    int4 child, count;
+};
+DEV Node4 load_node4(const DevScene &S, int node) {
+    const float4 *n = S.nodes + 8 * size_t(node);
+    Node4 r;
+    r.lx = n[0]; r.hx = n[1]; r.ly = n[2]; r.hy = n[3]; r.lz = n[4]; r.hz = n[5];
+    r.child = *reinterpret_cast<const int4 *>(n + 6);
+    r.count = *reinterpret_cast<const int4 *>(n + 7);
+    return r;
+}
+DEV float f4at(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+DEV int i4at(const int4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+// Closest hit; ties in t go to the smaller scene-order index. Returns the
+// scene-order triangle index or -1.
 DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], float tmin, float tmax, float &tbest) {
     float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
     int *stack = g_trace_stack + threadIdx.x;  // entry k at stack[k * kTraceLanes]
@@ -121,44 +141,48 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
     DRMLT_COUNT(unsigned nvis = 0; unsigned ntri = 0;)
     for (;;) {
         DRMLT_COUNT(nvis++;)
-        const float4 n0 = S.nodes[4 * node + 0], n1 = S.nodes[4 * node + 1], n2 = S.nodes[4 * node + 2];
-        const int4 ch = *reinterpret_cast<const int4 *>(&S.nodes[4 * node + 3]);
-        float tl, tr;
-        bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, o, inv, tmin, tbest, tl);
-        bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, inv, tmin, tbest, tr);
+        const Node4 N = load_node4(S, node);
+        float tn[4];
+        bool hit[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            hit[k] = i4at(N.count, k) >= 0 &&
+                     slab(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tbest, tn[k]);
+        }
         // leaves are tested in place
-        if (hl && ch.z > 0) {
-            DRMLT_COUNT(ntri += ch.z;)
-            for (int k = ch.x; k < ch.x + ch.z; k++) {
-                float t;
-                const float4 A = S.tA[k];
-                if (tri_hit(A, S.tB[k], S.tC[k], o, d, tmin, best < 0 ? tmax : next_up(tbest), t)) {
-                    int id = __float_as_int(A.w);
-                    if (t < tbest || (t == tbest && (best < 0 || id < best))) { tbest = t; best = id; }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int cnt = i4at(N.count, k);
+            if (hit[k] && cnt > 0) {
+                DRMLT_COUNT(ntri += cnt;)
+                const int first = i4at(N.child, k);
+                for (int t = first; t < first + cnt; t++) {
+                    float th;
+                    const float4 A = S.tA[t];
+                    if (tri_hit(A, S.tB[t], S.tC[t], o, d, tmin, best < 0 ? tmax : next_up(tbest), th)) {
+                        const int id = __float_as_int(A.w);
+                        if (th < tbest || (th == tbest && (best < 0 || id < best))) { tbest = th; best = id; }
+                    }
                 }
+                hit[k] = false;
             }
-            hl = false;
         }
-        if (hr && ch.w > 0) {
-            DRMLT_COUNT(ntri += ch.w;)
-            for (int k = ch.y; k < ch.y + ch.w; k++) {
-                float t;
-                const float4 A = S.tA[k];
-                if (tri_hit(A, S.tB[k], S.tC[k], o, d, tmin, best < 0 ? tmax : next_up(tbest), t)) {
-                    int id = __float_as_int(A.w);
-                    if (t < tbest || (t == tbest && (best < 0 || id < best))) { tbest = t; best = id; }
-                }
+        // internal children still in front of the best hit, nearest first
+        int cn[4];
+        float ct[4];
+        int nh = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (hit[k] && tn[k] <= tbest) {
+                int j = nh++;
+                while (j > 0 && ct[j - 1] > tn[k]) { ct[j] = ct[j - 1]; cn[j] = cn[j - 1]; j--; }
+                ct[j] = tn[k];
+                cn[j] = i4at(N.child, k);
             }
-            hr = false;
         }
-        if (hl && hr) {
-            int nearC = tl <= tr ? ch.x : ch.y, farC = tl <= tr ? ch.y : ch.x;
-            stack[kTraceLanes * sp++] = farC;
-            node = nearC;
-        } else if (hl) {
-            node = ch.x;
-        } else if (hr) {
-            node = ch.y;
+        if (nh > 0) {
+            for (int j = nh - 1; j > 0; j--) stack[kTraceLanes * sp++] = cn[j];
+            node = cn[0];
         } else {
             // pop, skipping entries whose box now lies beyond the best hit is
             // not possible without their distance; pop unconditionally
@@ -178,40 +202,33 @@ DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float 
     DRMLT_COUNT(unsigned nvis = 0; unsigned ntri = 0;)
     for (;;) {
         DRMLT_COUNT(nvis++;)
-        const float4 n0 = S.nodes[4 * node + 0], n1 = S.nodes[4 * node + 1], n2 = S.nodes[4 * node + 2];
-        const int4 ch = *reinterpret_cast<const int4 *>(&S.nodes[4 * node + 3]);
-        float tl, tr;
-        bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, o, inv, tmin, tmax, tl);
-        bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, inv, tmin, tmax, tr);
-        if (hl && ch.z > 0) {
-            for (int k = ch.x; k < ch.x + ch.z; k++) {
-                float t;
-                DRMLT_COUNT(ntri++;)
-                if (tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t)) {
-                    DRMLT_COUNT(count_work(S, nvis, ntri);)
-                    return true;
+        const Node4 N = load_node4(S, node);
+        int next = -1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int cnt = i4at(N.count, k);
+            float tn;
+            if (cnt < 0 ||
+                !slab(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tmax, tn))
+                continue;
+            const int ch = i4at(N.child, k);
+            if (cnt > 0) {
+                for (int t = ch; t < ch + cnt; t++) {
+                    float th;
+                    DRMLT_COUNT(ntri++;)
+                    if (tri_hit(S.tA[t], S.tB[t], S.tC[t], o, d, tmin, tmax, th)) {
+                        DRMLT_COUNT(count_work(S, nvis, ntri);)
+                        return true;
+                    }
                 }
+            } else if (next < 0) {
+                next = ch;
+            } else {
+                stack[kTraceLanes * sp++] = ch;
             }
-            hl = false;
         }
-        if (hr && ch.w > 0) {
-            for (int k = ch.y; k < ch.y + ch.w; k++) {
-                float t;
-                DRMLT_COUNT(ntri++;)
-                if (tri_hit(S.tA[k], S.tB[k], S.tC[k], o, d, tmin, tmax, t)) {
-                    DRMLT_COUNT(count_work(S, nvis, ntri);)
-                    return true;
-                }
-            }
-            hr = false;
-        }
-        if (hl && hr) {
-            stack[kTraceLanes * sp++] = ch.y;
-            node = ch.x;
-        } else if (hl) {
-            node = ch.x;
-        } else if (hr) {
-            node = ch.y;
+        if (next >= 0) {
+            node = next;
         } else {
             if (sp == 0) break;
             node = stack[kTraceLanes * --sp];

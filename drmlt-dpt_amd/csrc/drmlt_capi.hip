@@ -253,9 +253,15 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     const int nt = sc->num_triangles;
     c->nt = nt;
     // BVH over the triangles (host build, leaf-order copy for traversal)
+    // the binary depth cap is lowered until no ray can need more than the
+    // device traversal stack
     BvhOut bvh;
-    BuildBvh(nt, sc->tri_p, kBvhStack - 2, bvh);
-    c->bvhDepth = bvh.depth;
+    for (int cap = 48;; cap -= 4) {
+        BuildBvh(nt, sc->tri_p, cap, bvh);
+        if (bvh.stackNeed <= kBvhStack) break;
+        if (cap <= 8) return bail(fail(DRMLT_ERR_ARG, "BVH does not fit the traversal stack"));
+    }
+    c->bvhDepth = bvh.depth4;
     std::vector<float4> tA(nt), tB(nt), tC(nt), s0(nt), s1(nt), s2(nt);
     for (int i = 0; i < nt; i++) {
         const float *p = sc->tri_p + 9 * size_t(i);
