@@ -10,7 +10,7 @@ namespace drmlt {
 // Minimum waves per SIMD the path-stage kernels are compiled for (bounds
 // their registers); more resident waves hide traversal latency.
 #ifndef DRMLT_STAGE_WAVES
-#define DRMLT_STAGE_WAVES 1
+#define DRMLT_STAGE_WAVES 2
 #endif
 
 #ifdef DRMLT_COUNT_WORK
