@@ -68,6 +68,23 @@ template <class T> DEV T snc(bool adjoint, const V3<T> &wi, const Isect<T> &is, 
 DEV Mat mat_of(const DevScene &S, int prim) { return load_mat(S.mats + 16 * S.tmat[prim]); }
 DEV float modulo1(float a) { float r = fmodf(a, 1.0f); return r < 0.0f ? r + 1.0f : r; }  // utils.h:383-386
 
+// A visibility test the caller takes over: the segment from o along the unit
+// direction d, occluded as occluded() decides it for distance dist. The walk
+// of dev_walk.h passes one so that the test is traced at its shared trace
+// site; with none the test is made on the spot.
+struct ShadowQ {
+    V3<float> o, d;
+    float dist;
+};
+template <class T> DEV bool occluded_or_defer(const DevScene &S, ShadowQ *sh, const V3<T> &org, const V3<T> &dir, const T &dist) {
+    if (sh) {
+        sh->o = lower(org);
+        sh->d = lower(dir);
+        sh->dist = float(val(dist));
+        return false;
+    }
+    return occluded(S, org, dir, dist);
+}
 // path.cpp:455-474
 template <class T> DEV void emit_from_camera(const DevScene &S, const T &sx, const T &sy, RayT<T> &r, BState<T> &st) {
     RayT<T> cr;
@@ -126,7 +143,7 @@ template <class T> DEV void emit(POut<T> &out, int c, int l, const V3<T> &contri
 
 // path.cpp:519-613. Returns false when the reference clears the contribution list.
 template <class T, bool FIXED> DEV bool connect_to_camera(const DevScene &S, const Opts &o, int lgtDepth, const BState<T> &st, int prim,
-                                                          float disc, const V3<T> &prevPos, POut<T> &out) {
+                                                          float disc, const V3<T> &prevPos, POut<T> &out, ShadowQ *sh = nullptr) {
     RayT<T> cr;
     sample_primary(S.cam, T(0.5f), T(0.5f), cr);
     V3<T> toCam = cr.org - st.isect.pos;
@@ -135,7 +152,7 @@ template <class T, bool FIXED> DEV bool connect_to_camera(const DevScene &S, con
     if (!project_point(S.cam, st.isect.pos, sx, sy)) return true;
     const T d2 = length2(toCam), d = s_sqrt(d2);
     toCam = toCam * (T(1) / d);
-    if (!FIXED && occluded(S, st.isect.pos, toCam, d)) return true;
+    if (!FIXED && occluded_or_defer(S, sh, st.isect.pos, toCam, d)) return true;
     const Mat m = mat_of(S, prim);
     BOut<T> b;
     bsdf_eval(m, true, st.wi, st.isect.sn, toCam, b);
@@ -210,13 +227,14 @@ template <class T> DEV bool bsdf_sampling(const DevScene &S, const Opts &o, bool
 
 // path.cpp:726-810 (bidirectional DirectLighting)
 template <class T, bool FIXED> DEV void direct_lighting(const DevScene &S, const Opts &o, int camDepth, const BState<T> &st, float sx,
-                                                        float sy, int li, int lprim, const T &r0, const T &r1, int prim, POut<T> &out) {
+                                                        float sy, int li, int lprim, const T &r0, const T &r1, int prim, POut<T> &out,
+                                                        ShadowQ *sh = nullptr) {
     const float pick = pick_light_prob(S, li);
     const Mat m = mat_of(S, prim);
     V3<T> toL, lc;
     T dist, cosL, dPdf, emPdf;
     if (!light_sample_direct(S, li, lprim, st.isect.pos, r0, r1, toL, dist, lc, cosL, dPdf, emPdf)) return;
-    if (!FIXED && occluded(S, st.isect.pos, toL, dist)) return;
+    if (!FIXED && occluded_or_defer(S, sh, st.isect.pos, toL, dist)) return;
     BOut<T> b;
     bsdf_eval(m, false, st.wi, st.isect.sn, toL, b);
     if (is_zero(b.contrib)) return;
@@ -255,11 +273,12 @@ template <class T> DEV void handle_hit_light(const DevScene &S, int camDepth, in
 }
 // path.cpp:862-976
 template <class T, bool FIXED> DEV void connect_vertex(const DevScene &S, int camDepth, int lgtDepth, const BState<T> &ls, int lprimV,
-                                                       const BState<T> &cs, int cprimV, float sx, float sy, POut<T> &out) {
+                                                       const BState<T> &cs, int cprimV, float sx, float sy, POut<T> &out,
+                                                       ShadowQ *sh = nullptr) {
     V3<T> toL = ls.isect.pos - cs.isect.pos;
     const T d2 = length2(toL), d = s_sqrt(d2);
     toL = toL * (T(1) / d);
-    if (!FIXED && occluded(S, cs.isect.pos, toL, d)) return;
+    if (!FIXED && occluded_or_defer(S, sh, cs.isect.pos, toL, d)) return;
     const Mat cm = mat_of(S, cprimV);
     BOut<T> cb;
     bsdf_eval(cm, false, cs.wi, cs.isect.sn, toL, cb);
@@ -288,7 +307,7 @@ template <class T, bool FIXED> DEV void connect_vertex(const DevScene &S, int ca
 }
 // path.cpp:1430-1478 (bidirectional LightCoordinateSampling)
 template <class T, bool FIXED> DEV bool light_coordinate_sampling(const DevScene &S, int prim, const T &r0, const T &r1, int nprim,
-                                                                  BState<T> &st, V3<T> &dir) {
+                                                                  BState<T> &st, V3<T> &dir, ShadowQ *sh = nullptr) {
     const Mat m = mat_of(S, prim);
     const int li = S.tlight[nprim];
     V3<T> np, nn;
@@ -297,7 +316,7 @@ template <class T, bool FIXED> DEV bool light_coordinate_sampling(const DevScene
     dir = np - st.isect.pos;
     const T d2 = length2(dir), d = s_sqrt(d2);
     dir = dir * (T(1) / d);
-    if (!FIXED && occluded(S, st.isect.pos, dir, d)) return false;
+    if (!FIXED && occluded_or_defer(S, sh, st.isect.pos, dir, d)) return false;
     BOut<T> b;
     bsdf_eval(m, false, st.wi, st.isect.sn, dir, b);
     if (is_zero(b.contrib)) return false;

@@ -55,15 +55,30 @@ struct DevSceneCounted : DevSceneData {
 // (DRMLT_COUNT_WORK defined, DevScene = DevSceneCounted).
 #ifdef DRMLT_COUNT_WORK
 using DevScene = DevSceneCounted;
-DEV void count_work(const DevScene &S, unsigned nodes, unsigned tris) {
-    atomicAdd(S.counters + 0, (unsigned long long)nodes);
-    atomicAdd(S.counters + 1, (unsigned long long)tris);
-    atomicAdd(S.counters + 2, 1ull);
-}
 #define DRMLT_COUNT(stmt) stmt
 #else
 struct DevScene : DevSceneData {};
 #define DRMLT_COUNT(stmt)
+#endif
+
+// What a traversal reads: the 4-wide nodes and the leaf-order triangles (and,
+// in the counting build, the work counters).
+struct BvhRef {
+    const float4 *nodes, *tA, *tB, *tC;
+    DRMLT_COUNT(unsigned long long *counters;)
+};
+DEV BvhRef bvh_of(const DevScene &S) {
+    BvhRef b;
+    b.nodes = S.nodes; b.tA = S.tA; b.tB = S.tB; b.tC = S.tC;
+    DRMLT_COUNT(b.counters = S.counters;)
+    return b;
+}
+#ifdef DRMLT_COUNT_WORK
+DEV void count_work(const BvhRef &S, unsigned nodes, unsigned tris) {
+    atomicAdd(S.counters + 0, (unsigned long long)nodes);
+    atomicAdd(S.counters + 1, (unsigned long long)tris);
+    atomicAdd(S.counters + 2, 1ull);
+}
 #endif
 
 // ---------------------------------------------------------------- traversal
@@ -118,7 +133,7 @@ struct Node4 {
     float4 lx, hx, ly, hy, lz, hz;
     int4 child, count;
 };
-DEV Node4 load_node4(const DevScene &S, int node) {
+DEV Node4 load_node4(const BvhRef &S, int node) {
     const float4 *n = S.nodes + 8 * size_t(node);
     Node4 r;
     r.lx = n[0]; r.hx = n[1]; r.ly = n[2]; r.hy = n[3]; r.lz = n[4]; r.hz = n[5];
@@ -129,9 +144,15 @@ DEV Node4 load_node4(const DevScene &S, int node) {
 DEV float f4at(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 DEV int i4at(const int4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
-// Closest hit; ties in t go to the smaller scene-order index. Returns the
-// scene-order triangle index or -1.
-DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], float tmin, float tmax, float &tbest) {
+// One traversal for both query kinds, so that lanes asking for a closest hit
+// and lanes asking for visibility share the loop.
+// any = false: closest hit in (tmin, tmax); ties in t go to the smaller
+// scene-order index. Returns the scene-order triangle index or -1 and the hit
+// distance in tbest.
+// any = true: returns a triangle with a hit in (tmin, tmax) (the first found)
+// or -1; the ray interval is not narrowed, so the answer is the same boolean
+// as a search of every triangle.
+DEV int trace_impl(const BvhRef &S, const float o[3], const float d[3], float tmin, float tmax, bool any, float &tbest) {
     float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
     int *stack = g_trace_stack + threadIdx.x;  // entry k at stack[k * kTraceLanes]
     int sp = 0;
@@ -150,23 +171,26 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
                      slab(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tbest, tn[k]);
         }
         // leaves are tested in place
+        bool found = false;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int cnt = i4at(N.count, k);
-            if (hit[k] && cnt > 0) {
+            if (!found && hit[k] && cnt > 0) {
                 DRMLT_COUNT(ntri += cnt;)
                 const int first = i4at(N.child, k);
                 for (int t = first; t < first + cnt; t++) {
                     float th;
                     const float4 A = S.tA[t];
-                    if (tri_hit(A, S.tB[t], S.tC[t], o, d, tmin, best < 0 ? tmax : next_up(tbest), th)) {
+                    if (tri_hit(A, S.tB[t], S.tC[t], o, d, tmin, (any || best < 0) ? tmax : next_up(tbest), th)) {
                         const int id = __float_as_int(A.w);
+                        if (any) { best = id; found = true; break; }
                         if (th < tbest || (th == tbest && (best < 0 || id < best))) { tbest = th; best = id; }
                     }
                 }
-                hit[k] = false;
             }
+            hit[k] = hit[k] && cnt == 0;
         }
+        if (found) break;
         // internal children still in front of the best hit, nearest first
         int cn[4];
         float ct[4];
@@ -184,8 +208,8 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
             for (int j = nh - 1; j > 0; j--) stack[kTraceLanes * sp++] = cn[j];
             node = cn[0];
         } else {
-            // pop, skipping entries whose box now lies beyond the best hit is
-            // not possible without their distance; pop unconditionally
+            // an entry whose box now lies beyond the best hit is popped and
+            // visited anyway (its distance is not kept)
             if (sp == 0) break;
             node = stack[kTraceLanes * --sp];
         }
@@ -194,48 +218,33 @@ DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], flo
     return best;
 }
 
+// The traversal is emitted once per kernel as a real call (not inlined at
+// each of the path code's call sites), so every trace of a kernel runs the
+// same code; the stage kernels additionally reach it from a single site per
+// round (dev_walk.h). The counting build inlines it (it needs the counters
+// and is not timed).
+struct TraceHit {
+    int prim;
+    float t;
+};
+#ifdef DRMLT_COUNT_WORK
+DEV TraceHit trace_call(const BvhRef S, float ox, float oy, float oz, float dx, float dy, float dz, float tmin, float tmax, int any) {
+#else
+__device__ __noinline__ TraceHit trace_call(const BvhRef S, float ox, float oy, float oz, float dx, float dy, float dz, float tmin,
+                                            float tmax, int any) {
+#endif
+    const float o[3] = {ox, oy, oz}, d[3] = {dx, dy, dz};
+    TraceHit h;
+    h.prim = trace_impl(S, o, d, tmin, tmax, any != 0, h.t);
+    return h;
+}
+DEV int trace_closest(const DevScene &S, const float o[3], const float d[3], float tmin, float tmax, float &tbest) {
+    const TraceHit h = trace_call(bvh_of(S), o[0], o[1], o[2], d[0], d[1], d[2], tmin, tmax, 0);
+    tbest = h.t;
+    return h.prim;
+}
 DEV bool trace_any(const DevScene &S, const float o[3], const float d[3], float tmin, float tmax) {
-    float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
-    int *stack = g_trace_stack + threadIdx.x;  // entry k at stack[k * kTraceLanes]
-    int sp = 0;
-    int node = 0;
-    DRMLT_COUNT(unsigned nvis = 0; unsigned ntri = 0;)
-    for (;;) {
-        DRMLT_COUNT(nvis++;)
-        const Node4 N = load_node4(S, node);
-        int next = -1;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int cnt = i4at(N.count, k);
-            float tn;
-            if (cnt < 0 ||
-                !slab(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tmax, tn))
-                continue;
-            const int ch = i4at(N.child, k);
-            if (cnt > 0) {
-                for (int t = ch; t < ch + cnt; t++) {
-                    float th;
-                    DRMLT_COUNT(ntri++;)
-                    if (tri_hit(S.tA[t], S.tB[t], S.tC[t], o, d, tmin, tmax, th)) {
-                        DRMLT_COUNT(count_work(S, nvis, ntri);)
-                        return true;
-                    }
-                }
-            } else if (next < 0) {
-                next = ch;
-            } else {
-                stack[kTraceLanes * sp++] = ch;
-            }
-        }
-        if (next >= 0) {
-            node = next;
-        } else {
-            if (sp == 0) break;
-            node = stack[kTraceLanes * --sp];
-        }
-    }
-    DRMLT_COUNT(count_work(S, nvis, ntri);)
-    return false;
+    return trace_call(bvh_of(S), o[0], o[1], o[2], d[0], d[1], d[2], tmin, tmax, 1).prim >= 0;
 }
 
 // ---------------------------------------------------------------- geometry in T

@@ -11,6 +11,7 @@
 // chains; only the order of the floating-point film additions differs.
 #pragma once
 #include "drmlt_kernels.h"
+#include "dev_walk.h"
 
 namespace drmlt {
 
@@ -218,48 +219,80 @@ template <class X> DEV void h2mc_ratio(const X &x, const Chains &ch, int chain, 
 }
 
 // ------------------------------------------------------------ DR stages
-// Stage 1 (delayed.cpp:404-445). Returns the chain's flags; finishes the
-// iteration (splats, swap) when no second stage follows.
-DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter, float *film,
-                  unsigned long long *stats) {
+// Each stage is split into a beginning (random draws that choose the move,
+// the proposal slot), the path evaluation, and an end (acceptance, splats,
+// state swap). The plain form evaluates the path with perturb_eval /
+// generate_subpath on the spot; the batched form runs the same evaluation as
+// a walk (dev_walk.h) whose rays the lanes of a wave trace together.
+
+// Stage 1 (delayed.cpp:404-445): a large step (LargeStep::Mutate,
+// mutation.cpp:7-124, multiplexed) or an isotropic small step
+// (SmallStep::Mutate, mutation.cpp:126-165) into the first-proposal slot.
+struct Stage1 {
+    int perm, pc, pf;
+    bool isLarge, curValid;
+    int cam, lgt;  // large step: the sampled strategy
+};
+DEV Stage1 stage1_begin(const Chains &ch, const DrParams &P, int chain, Philox &g) {
+    Stage1 a;
     const int C = ch.C;
-    const int perm = ch.perm[chain];
-    const int pc = chain + C * perm_cur(perm), pf = chain + C * perm_first(perm);
-    Philox g(P.seed, iter, chain, TAG_S1);
-    const bool isLarge = flip_coin(P.largeStepProb, g.uni());
+    a.perm = ch.perm[chain];
+    a.pc = chain + C * perm_cur(a.perm);
+    a.pf = chain + C * perm_first(a.perm);
+    a.isLarge = flip_coin(P.largeStepProb, g.uni());
     // a current state outside the fixed vertex slots is treated as invalid
-    const int cc = ch.c[pc], cl = ch.l[pc];
+    const int cc = ch.c[a.pc], cl = ch.l[a.pc];
     const bool wellFormed = cc >= 1 && cl >= 0 && cc <= kVMax + 1 && cl <= kVMax + 1 && cc + cl - 1 >= 2 && 2 * (cc + cl - 1) <= kDMax;
-    const bool curValid = ch.valid[pc] != 0 && wellFormed;
+    a.curValid = ch.valid[a.pc] != 0 && wellFormed;
+    a.cam = a.lgt = 0;
+    if (!a.curValid || a.isLarge) {
+        const int len = sample_discrete(P.lenCdf, P.nLen, g.uni());
+        a.lgt = min(max(int(g.uni() * float(len + 1)), 0), len);
+        a.cam = len - a.lgt + 1;
+    } else {
+        copy_state(slot_ref(ch, a.pc), slot_ref(ch, a.pf));
+    }
+    return a;
+}
+// The walk of stage 1 on slot pf (the small step's state was copied there).
+DEV void stage1_walk(const Chains &ch, const DrParams &P, const Stage1 &a, Philox &g, Walk &w) {
+    w.ref = slot_ref(ch, a.pf);
+    w.g = &g;
+    w.dsigma = P.discreteStdDev;
+    if (!a.curValid || a.isLarge) {
+        w.mode = WK_GEN;
+        w.c = a.cam;
+        w.l = a.lgt;
+    } else {
+        w.mode = WK_OFF_NORMAL;
+        w.sigma = P.perturbStdDev;
+        w.c = ch.c[a.pc];
+        w.l = ch.l[a.pc];
+    }
+}
+// Returns the chain's flags; finishes the iteration (splats, swap) when no
+// second stage follows.
+DEV int stage1_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, const Stage1 &a, const POut<float> &out, Philox &g,
+                   float *film, unsigned long long *stats) {
+    const int perm = a.perm, pc = a.pc, pf = a.pf;
+    const bool isLarge = a.isLarge, curValid = a.curValid;
     float a1 = 0.0f;
     bool acc1 = false;
-    const SRef rf = slot_ref(ch, pf);
     if (!curValid || isLarge) {
-        // LargeStep::Mutate, mutation.cpp:7-124 (multiplexed)
-        const int len = sample_discrete(P.lenCdf, P.nLen, g.uni());
-        const int lgt = min(max(int(g.uni() * float(len + 1)), 0), len);
-        const int cam = len - lgt + 1;
-        POut<float> out;
-        generate_subpath(Sc, P.o, cam, lgt, rf, g, out);
-        count_state(Sc, cam, lgt);
+        count_state(Sc, a.cam, a.lgt);
         ch.gInit[pf] = 0;
         if (out.valid) {
             (void)g.uni();  // contribution pick (one candidate when multiplexed)
             store_contrib(ch, pf, out);
             ch.hasSplat[pf] = 1;
             if (curValid) {
-                const int lc = ch.c[pc] + ch.l[pc] - 1, lp = cam + lgt - 1;
+                const int lc = ch.c[pc] + ch.l[pc] - 1, lp = a.cam + a.lgt - 1;
                 const float pmfP = P.lenFunc[lp] / (P.lenFuncInt * float(P.nLen));
                 const float pmfC = P.lenFunc[lc] / (P.lenFuncInt * float(P.nLen));
                 a1 = clamp_ref((float(lp + 1) * out.lsScore / pmfP) / (float(lc + 1) * ch.ls[pc] / pmfC));
             }
         }
     } else {
-        // SmallStep::Mutate, mutation.cpp:126-165
-        copy_state(slot_ref(ch, pc), rf);
-        PerturbU<OffNormal> up{rf.u, rf.s, 0, OffNormal{&g, P.perturbStdDev}};
-        POut<float> out;
-        perturb_eval<float, false>(Sc, P.o, rf, up, DiscNormal{&g, P.discreteStdDev}, out);
         count_state(Sc, ch.c[pc], ch.l[pc]);
         ch.gInit[pf] = 0;
         if (out.valid) {
@@ -295,36 +328,67 @@ DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int c
     }
     return flags;
 }
+// Plain form of stage 1.
+DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter, float *film,
+                  unsigned long long *stats) {
+    Philox g(P.seed, iter, chain, TAG_S1);
+    const Stage1 a = stage1_begin(ch, P, chain, g);
+    const SRef rf = slot_ref(ch, a.pf);
+    POut<float> out;
+    if (!a.curValid || a.isLarge) {
+        generate_subpath(Sc, P.o, a.cam, a.lgt, rf, g, out);
+    } else {
+        PerturbU<OffNormal> up{rf.u, rf.s, 0, OffNormal{&g, P.perturbStdDev}};
+        perturb_eval<float, false>(Sc, P.o, rf, up, DiscNormal{&g, P.discreteStdDev}, out);
+    }
+    return stage1_end(Sc, ch, P, chain, a, out, g, film, stats);
+}
 
 // Stage-2 proposal (delayed.cpp:448-463): H2MC (drsmallstep.cpp:113-212) or
-// isotropic with the second-stage sigma (mutation.cpp:167-208). For H2MC the
-// offset must already be drawn (h2mc_offset); its ratio needs the proposal's
-// Gaussian and is finished by h2mc_ratio.
-DEV void dr_stage2_propose(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter) {
+// isotropic with the second-stage sigma (mutation.cpp:167-208) into the
+// second-proposal slot. For H2MC the offset must already be drawn
+// (h2mc_offset); its ratio needs the proposal's Gaussian and is finished by
+// h2mc_ratio.
+struct Stage2 {
+    int pc, ps, flags;
+    bool isAniso;
+};
+DEV Stage2 stage2_begin(const Chains &ch, int chain, Philox &g) {
+    Stage2 a;
     const int C = ch.C;
     const int perm = ch.perm[chain];
-    const int pc = chain + C * perm_cur(perm), ps = chain + C * perm_second(perm);
-    const int flags = ch.flags[chain];
-    const bool isAniso = (flags & 4) != 0;
-    Philox g(P.seed, iter, chain, TAG_S2);
-    const SRef rs = slot_ref(ch, ps);
-    copy_state(slot_ref(ch, pc), rs);
-    POut<float> out;
-    if (isAniso) {
-        // the offset was drawn by h2mc_offset from this stream's first D normals
-        g.skip_normals(dim_of(ch.c[pc], ch.l[pc]));
-        PerturbU<OffArray> up{rs.u, rs.s, 0, OffArray{ch.off + chain, C}};
-        perturb_eval<float, false>(Sc, P.o, rs, up, DiscNormal{&g, P.discreteStdDev}, out);
+    a.pc = chain + C * perm_cur(perm);
+    a.ps = chain + C * perm_second(perm);
+    a.flags = ch.flags[chain];
+    a.isAniso = (a.flags & 4) != 0;
+    copy_state(slot_ref(ch, a.pc), slot_ref(ch, a.ps));
+    // the H2MC offset was drawn by h2mc_offset from this stream's first D normals
+    if (a.isAniso) g.skip_normals(dim_of(ch.c[a.pc], ch.l[a.pc]));
+    return a;
+}
+DEV void stage2_walk(const Chains &ch, const DrParams &P, int chain, const Stage2 &a, Philox &g, Walk &w) {
+    w.ref = slot_ref(ch, a.ps);
+    w.g = &g;
+    w.dsigma = P.discreteStdDev;
+    w.c = ch.c[a.pc];
+    w.l = ch.l[a.pc];
+    if (a.isAniso) {
+        w.mode = WK_OFF_ARRAY;
+        w.oa = ch.off + chain;
+        w.os = ch.C;
     } else {
-        PerturbU<OffNormal> up{rs.u, rs.s, 0, OffNormal{&g, P.anisoStdDev}};
-        perturb_eval<float, false>(Sc, P.o, rs, up, DiscNormal{&g, P.discreteStdDev}, out);
-        ch.gInit[ps] = 0;
+        w.mode = WK_OFF_NORMAL;
+        w.sigma = P.anisoStdDev;
     }
+}
+DEV void stage2_end(const DevScene &Sc, const Chains &ch, int chain, const Stage2 &a, const POut<float> &out) {
+    const int pc = a.pc, ps = a.ps;
+    if (!a.isAniso) ch.gInit[ps] = 0;
     count_state(Sc, ch.c[pc], ch.l[pc]);
     double a2 = 0.0;
     if (out.valid) {
         store_contrib(ch, ps, out);
-        if (isAniso) {
+        if (a.isAniso) {
             ch.hasSplat[ps] = (out.lsScore > 0.0f && isfinite(out.lsScore)) ? 1 : 0;
         } else {
             ch.hasSplat[ps] = 1;
@@ -332,38 +396,76 @@ DEV void dr_stage2_propose(const DevScene &Sc, const Chains &ch, const DrParams 
         }
     }
     ch.a2d[chain] = a2;
-    ch.flags[chain] = flags | (out.valid ? 16 : 0);
+    ch.flags[chain] = a.flags | (out.valid ? 16 : 0);
+}
+// Plain form of the stage-2 proposal.
+DEV void dr_stage2_propose(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter) {
+    Philox g(P.seed, iter, chain, TAG_S2);
+    const Stage2 a = stage2_begin(ch, chain, g);
+    const SRef rs = slot_ref(ch, a.ps);
+    POut<float> out;
+    if (a.isAniso) {
+        PerturbU<OffArray> up{rs.u, rs.s, 0, OffArray{ch.off + chain, ch.C}};
+        perturb_eval<float, false>(Sc, P.o, rs, up, DiscNormal{&g, P.discreteStdDev}, out);
+    } else {
+        PerturbU<OffNormal> up{rs.u, rs.s, 0, OffNormal{&g, P.anisoStdDev}};
+        perturb_eval<float, false>(Sc, P.o, rs, up, DiscNormal{&g, P.discreteStdDev}, out);
+    }
+    stage2_end(Sc, ch, chain, a, out);
 }
 
-// H2MC kernel ratio once both Gaussians exist (drsmallstep.cpp:185-196)
-
 // Second-stage acceptance (delayed.cpp:466-541), splats (548-561) and state
-// swap (565-606).
-DEV void dr_stage2_accept(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter, float *film,
-                          unsigned long long *stats) {
+// swap (565-606). For Green & Mira the y* = z + (x - y) state
+// (mutation.cpp:210-237) is evaluated in the spare fourth slot.
+struct Stage3 {
+    int perm, pc, pf, ps, pstar, flags;
+    bool isAniso, traceStar;
+    float a1;
+    double a2;
+};
+DEV Stage3 stage3_begin(const Chains &ch, const DrParams &P, int chain) {
+    Stage3 a;
     const int C = ch.C;
-    const int perm = ch.perm[chain];
-    const int pc = chain + C * perm_cur(perm), pf = chain + C * perm_first(perm), ps = chain + C * perm_second(perm);
-    const int pstar = chain + 3 * C;
-    const int flags = ch.flags[chain];
-    const bool isAniso = (flags & 4) != 0, zValid = (flags & 16) != 0;
-    const float a1 = ch.a1[chain];
-    double a2 = zValid ? ch.a2d[chain] : 0.0;
+    a.perm = ch.perm[chain];
+    a.pc = chain + C * perm_cur(a.perm);
+    a.pf = chain + C * perm_first(a.perm);
+    a.ps = chain + C * perm_second(a.perm);
+    a.pstar = chain + 3 * C;
+    a.flags = ch.flags[chain];
+    a.isAniso = (a.flags & 4) != 0;
+    const bool zValid = (a.flags & 16) != 0;
+    a.a1 = ch.a1[chain];
+    a.a2 = zValid ? ch.a2d[chain] : 0.0;
+    a.traceStar = !is_invalid(a.a2) && P.type == 0 && a.a2 != 0.0 && ch.valid[a.ps];
+    if (a.traceStar) copy_state(slot_ref(ch, a.ps), slot_ref(ch, a.pstar));
+    return a;
+}
+DEV void stage3_walk(const Chains &ch, const DrParams &P, const Stage3 &a, Philox &g, Walk &w) {
+    w.ref = slot_ref(ch, a.pstar);
+    w.g = &g;
+    w.dsigma = P.discreteStdDev;
+    w.mode = WK_OFF_DIFF;
+    w.ux = ch.u + a.pc;
+    w.uy = ch.u + a.pf;
+    w.us = ch.NS;
+    w.c = ch.c[a.ps];
+    w.l = ch.l[a.ps];
+}
+DEV void stage3_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, const Stage3 &a, const POut<float> &starOut,
+                    Philox &g, float *film, unsigned long long *stats) {
+    const int perm = a.perm, pc = a.pc, pf = a.pf, ps = a.ps;
+    const bool isAniso = a.isAniso;
+    const float a1 = a.a1;
+    double a2 = a.a2;
     bool acc2 = false;
-    Philox g(P.seed, iter, chain, TAG_S3);
     if (is_invalid(a2)) {
         a2 = 0.0;
     } else if (P.type == 0) {
-        // Green & Mira: y* = z + (x - y) (mutation.cpp:210-237)
+        // Green & Mira
         float aStar = 0.0f;
-        if (a2 != 0.0 && ch.valid[ps]) {
-            const SRef rz = slot_ref(ch, ps), rt = slot_ref(ch, pstar);
-            copy_state(rz, rt);
-            PerturbU<OffDiff> up{rt.u, rt.s, 0, OffDiff{ch.u + pc, ch.u + pf, ch.NS, ch.NS}};
-            POut<float> out;
-            perturb_eval<float, false>(Sc, P.o, rt, up, DiscNormal{&g, P.discreteStdDev}, out);
+        if (a.traceStar) {
             count_state(Sc, ch.c[ps], ch.l[ps]);
-            if (out.valid) aStar = clamp_ref(out.ssScore / ch.ss[ps]);
+            if (starOut.valid) aStar = clamp_ref(starOut.ssScore / ch.ss[ps]);
         }
         if (is_invalid(aStar)) aStar = 0.0f;
         if (aStar == 1.0f) {
@@ -412,6 +514,20 @@ DEV void dr_stage2_accept(const DevScene &Sc, const Chains &ch, const DrParams &
         if (P.acceptanceMap) splat_const(ch, W, H, film, pc, 0.0f, 1.0f, 0.0f);
         ch.valid[ps] = 1;
     }
+}
+// Plain form of the second-stage acceptance.
+DEV void dr_stage2_accept(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter, float *film,
+                          unsigned long long *stats) {
+    Philox g(P.seed, iter, chain, TAG_S3);
+    const Stage3 a = stage3_begin(ch, P, chain);
+    POut<float> out;
+    out.valid = false;
+    if (a.traceStar) {
+        const SRef rt = slot_ref(ch, a.pstar);
+        PerturbU<OffDiff> up{rt.u, rt.s, 0, OffDiff{ch.u + a.pc, ch.u + a.pf, ch.NS, ch.NS}};
+        perturb_eval<float, false>(Sc, P.o, rt, up, DiscNormal{&g, P.discreteStdDev}, out);
+    }
+    stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
 }
 
 constexpr int kPreMaxL = 16;  // light-subpath vertices of a pre-pass sample

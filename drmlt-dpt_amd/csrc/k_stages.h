@@ -47,7 +47,13 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)
                                                  unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
     for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gridDim.x * blockDim.x) {
-        const int flags = dr_stage1(Sc, ch, P, chain, iter, film, stats);
+        Philox g(P.seed, iter, chain, TAG_S1);
+        const Stage1 a = stage1_begin(ch, P, chain, g);
+        Walk w;
+        stage1_walk(ch, P, a, g, w);
+        POut<float> out;
+        walk_run(Sc, P.o, w, true, out);
+        const int flags = stage1_end(Sc, ch, P, chain, a, out, g, film, stats);
         if (flags & 2) {
             const int perm = ch.perm[chain];
             const int pc = chain + ch.C * perm_cur(perm);
@@ -92,13 +98,23 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
     const int gsz = gridDim.x * blockDim.x;
     // chains in index order (coalesced state access); those without a second stage idle
     for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gsz) {
-        if (!(ch.flags[chain] & 2)) continue;
-        dr_stage2_propose(Sc, ch, P, chain, iter);
-        const int flags = ch.flags[chain];
-        if ((flags & 4) && (flags & 16)) {
-            const int pc = chain + ch.C * perm_cur(ch.perm[chain]);
-            list_push(L, 2, ch.C, min(ch.c[pc] + ch.l[pc] - 1, kMaxBins - 1), chain);
+        // every lane of the wave takes part in the walk rounds; those without
+        // a second stage have no walk
+        const bool act = (ch.flags[chain] & 2) != 0;
+        Philox g(P.seed, iter, chain, TAG_S2);
+        Stage2 a;
+        Walk w;
+        if (act) {
+            a = stage2_begin(ch, chain, g);
+            stage2_walk(ch, P, chain, a, g, w);
         }
+        POut<float> out;
+        out.valid = false;
+        walk_run(Sc, P.o, w, act, out);
+        if (!act) continue;
+        stage2_end(Sc, ch, chain, a, out);
+        const int flags = ch.flags[chain];
+        if ((flags & 4) && (flags & 16)) list_push(L, 2, ch.C, min(ch.c[a.pc] + ch.l[a.pc] - 1, kMaxBins - 1), chain);
     }
 }
 
@@ -106,8 +122,21 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
                                                         unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
     const int gsz = gridDim.x * blockDim.x;
-    for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gsz)
-        if (ch.flags[chain] & 2) dr_stage2_accept(Sc, ch, P, chain, iter, film, stats);
+    for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gsz) {
+        const bool act = (ch.flags[chain] & 2) != 0;
+        Philox g(P.seed, iter, chain, TAG_S3);
+        Stage3 a;
+        Walk w;
+        a.traceStar = false;
+        if (act) {
+            a = stage3_begin(ch, P, chain);
+            if (a.traceStar) stage3_walk(ch, P, a, g, w);
+        }
+        POut<float> out;
+        out.valid = false;
+        walk_run(Sc, P.o, w, act && a.traceStar, out);
+        if (act) stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
+    }
 }
 
 // Derivative passes over list `kind` (slot: 0 current, 2 second), persistent

@@ -74,6 +74,31 @@ def test_forms_agree(cfg1):
     assert np.allclose(fb.sum(), fg.sum(), rtol=1e-4)
 
 
+@pytest.mark.parametrize("which", ["cfg1", "cfg2_small"])
+def test_batched_paths_equal_plain_paths(which, request):
+    """One iteration from identical seeds: the batched stage kernels evaluate
+    paths as walks that share one trace site per round, the plain form with
+    the sequential path code. Stage-1 moves (large and small steps) and the
+    isotropic second stage (with Green's y*) depend on nothing but those
+    evaluations and the random streams, so their counts must be identical."""
+    from drmlt import capi
+
+    sc = request.getfixturevalue(which)
+    res = {}
+    for mode in (capi.PLAIN, capi.BATCHED):
+        ctx = _ctx(sc, num_init_samples=8192, num_chains=4096, seed=5)
+        say(f"{which} form {mode}: prepass + 1 iteration")
+        ctx.prepass()
+        ctx.run(1, mode)
+        res[mode] = ctx.stats()
+        ctx.close()
+    say("plain", res[capi.PLAIN])
+    say("batched", res[capi.BATCHED])
+    for k in ("large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso", "stage2_iso_accepted", "stage2_h2mc"):
+        assert res[capi.PLAIN][k] == res[capi.BATCHED][k], k
+    assert res[capi.PLAIN]["stage1_accepted"] > 0 and res[capi.PLAIN]["stage2_iso_accepted"] > 0
+
+
 def test_loop_vs_oracle(cfg1):
     """The DR loop (Green-Mira, H2MC second stage) against the oracle's loop
     from the oracle's own seeds: acceptance rates per stage and the film mean
