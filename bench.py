@@ -70,6 +70,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--tune-steps", type=int, default=3, help="timed steps per candidate schedule of the autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device to device (hipMemcpy), the known-size kernel that scales "
                          "rocprof FETCH_SIZE / WRITE_SIZE readings")
@@ -100,6 +101,10 @@ def main():
     ctx.run(args.plain_steps, capi.PLAIN)
     plain_ms = ctx.timing()[0] / args.plain_steps
     log(f"plain form {plain_ms:.2f} ms/step")
+
+    # launch schedule of the batched form chosen by the autotuner on these chains
+    sched, tuned_ms = ctx.autotune(steps=args.tune_steps)
+    log(f"autotuned schedule {sched}: {tuned_ms:.3f} ms/step")
 
     # batched form with per-stage timing (events between kernels)
     ctx.run(3, capi.BATCHED)
@@ -161,7 +166,7 @@ def main():
         "dtype": "f32 path / f64 gaussian+ratio", "data": "synthetic (seeded procedural scene)",
         "config": {"workload": f"config{args.config}: {len(sc['tri_p'])} tris, {chains} chains, maxdepth {dpt['maxdepth']}, "
                                f"{int(sc['cam'][10])}x{int(sc['cam'][11])} film, type={dpt['type']}",
-                   "chains": chains, "form": "batched+graph"},
+                   "chains": chains, "form": "batched+graph", "schedule": sched},
         "plain_form": {"ms_per_step": round(plain_ms, 3), "mutations_per_s": round(chains / (plain_ms / 1000.0), 1)},
         # the forms advance identical chains (identical acceptance statistics, tests/test_gpu_loop.py), so
         # equal RMSE is reached after equal iteration counts and the time ratio is the step-time ratio

@@ -233,7 +233,9 @@ struct Stage1 {
     bool isLarge, curValid;
     int cam, lgt;  // large step: the sampled strategy
 };
-DEV Stage1 stage1_begin(const Chains &ch, const DrParams &P, int chain, Philox &g) {
+// copyState: copy the current state into the proposal slot for a small step
+// (the batched form does it while binning, and begins again without it).
+DEV Stage1 stage1_begin(const Chains &ch, const DrParams &P, int chain, Philox &g, bool copyState = true) {
     Stage1 a;
     const int C = ch.C;
     a.perm = ch.perm[chain];
@@ -249,10 +251,14 @@ DEV Stage1 stage1_begin(const Chains &ch, const DrParams &P, int chain, Philox &
         const int len = sample_discrete(P.lenCdf, P.nLen, g.uni());
         a.lgt = min(max(int(g.uni() * float(len + 1)), 0), len);
         a.cam = len - a.lgt + 1;
-    } else {
+    } else if (copyState) {
         copy_state(slot_ref(ch, a.pc), slot_ref(ch, a.pf));
     }
     return a;
+}
+// Length (edges) of the path stage 1 evaluates.
+DEV int stage1_length(const Chains &ch, const Stage1 &a) {
+    return (!a.curValid || a.isLarge) ? a.cam + a.lgt - 1 : ch.c[a.pc] + ch.l[a.pc] - 1;
 }
 // The walk of stage 1 on slot pf (the small step's state was copied there).
 DEV void stage1_walk(const Chains &ch, const DrParams &P, const Stage1 &a, Philox &g, Walk &w) {

@@ -156,6 +156,8 @@ struct drmlt_ctx {
     double timing[8] = {0};
     hipEvent_t ev[9];
     bool evInit = false;
+    // launch schedule of the batched step (drmlt_set_schedule / drmlt_autotune)
+    drmlt_schedule sched = {0, 0, 0, 0};
 };
 
 static DrParams make_params(const drmlt_ctx *c) {
@@ -181,6 +183,7 @@ static DrParams make_params(const drmlt_ctx *c) {
     P.o.chadMisQuirk = 0;
     P.oDeriv = P.o;
     P.oDeriv.chadMisQuirk = o.quirk_chad_mis;
+    P.binStages = c->sched.bin_stages;
     return P;
 }
 
@@ -381,14 +384,14 @@ static int alloc_chains(drmlt_ctx *c, int C) {
               A(&ch.ls, NS) && A(&ch.ss, NS) && A(&ch.perm, size_t(C)) && A(&ch.gauss, size_t(kGSlot) * GS) &&
               A(&ch.off, 32 * size_t(C)) && A(&ch.grad, size_t(kGMax) * C) && A(&ch.hess, size_t(kGMax * kGMax) * C) &&
               A(&ch.a1, size_t(C)) && A(&ch.a2, size_t(C)) && A(&ch.a2d, size_t(C)) && A(&ch.flags, size_t(C)) &&
-              A(&c->L.count, 3 * kMaxBins) &&
-              A(&c->L.items, size_t(3) * kMaxBins * C);
+              A(&c->L.count, kListKinds * kMaxBins) &&
+              A(&c->L.items, size_t(kListKinds) * kMaxBins * C);
     if (!ok) return fail(DRMLT_ERR_HIP, "chain state allocation");
     hipStream_t s = c->stream;
     // every state field starts zeroed: c = l = 0 is an empty path
     for (size_t k = 0; k < c->chainAllocs.size(); k++)
         if (c->chainAllocs[k]) HIPCHK(hipMemsetAsync(c->chainAllocs[k], 0, c->chainBytes[k], s));
-    (void)hipMemsetAsync(c->L.count, 0, 3 * kMaxBins * 4, s);
+    (void)hipMemsetAsync(c->L.count, 0, kListKinds * kMaxBins * 4, s);
     c->haveChains = true;
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -562,46 +565,61 @@ extern "C" int drmlt_direct_lighting(drmlt_ctx *c) {
 }
 
 static int persistent_blocks(const drmlt_ctx *c) { return c->cus * 16; }
+// Workgroup counts of the batched step's kernels under the context's schedule.
+struct StepGrid {
+    int stage, deriv, gauss;
+};
+static StepGrid step_grid(const drmlt_ctx *c) {
+    const int C = c->ch.C;
+    const int pb = persistent_blocks(c);
+    StepGrid g;
+    const int perChain = (C + 63) / 64;
+    g.stage = c->sched.stage_blocks > 0 ? std::min(c->sched.stage_blocks, perChain) : std::min(perChain, pb);
+    g.deriv = c->sched.deriv_blocks > 0 ? c->sched.deriv_blocks : pb;
+    g.gauss = c->sched.gauss_blocks > 0 ? c->sched.gauss_blocks : pb;
+    return g;
+}
 
-// One batched DR step: seven stage kernels and the list reset. With
+// One batched DR step: the stage-1 binning kernel, seven stage kernels and
+// the list reset (the binning kernel is timed with stage 1). With
 // `counters` (7 x 4 words) the counting forms of the stage kernels run
 // instead, each accumulating its traversal work and state bytes into its own
 // row.
 static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, unsigned long long *counters = nullptr) {
     hipStream_t s = c->stream;
-    const int C = c->ch.C;
-    const int pb = persistent_blocks(c);
-    const int g1 = std::min((C + 63) / 64, pb);
+    const StepGrid G = step_grid(c);
     if (counters) {
         DevSceneCounted Sk[7];
         for (int k = 0; k < 7; k++) {
             static_cast<DevSceneData &>(Sk[k]) = c->S;
             Sk[k].counters = counters + 4 * k;
         }
-        hipLaunchKernelGGL(k_b_stage1_counted, dim3(g1), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
-        hipLaunchKernelGGL(k_b_deriv_counted, dim3(pb), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 0, 0);
-        hipLaunchKernelGGL(k_b_gauss_counted, dim3(pb), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0, c->iterDev);
-        hipLaunchKernelGGL(k_b_stage2_propose_counted, dim3(g1), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
-        hipLaunchKernelGGL(k_b_deriv_counted, dim3(pb), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
-        hipLaunchKernelGGL(k_b_gauss_counted, dim3(pb), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1, c->iterDev);
-        hipLaunchKernelGGL(k_b_stage2_accept_counted, dim3(g1), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+        if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin_counted, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
+        hipLaunchKernelGGL(k_b_stage1_counted, dim3(G.stage), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+        hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 0, 0);
+        hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0, c->iterDev);
+        hipLaunchKernelGGL(k_b_stage2_propose_counted, dim3(G.stage), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
+        hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
+        hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1, c->iterDev);
+        hipLaunchKernelGGL(k_b_stage2_accept_counted, dim3(G.stage), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
         return;
     }
     if (timed) (void)hipEventRecord(c->ev[0], s);
-    hipLaunchKernelGGL(k_b_stage1, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
+    hipLaunchKernelGGL(k_b_stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[1], s);
-    hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
+    hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
     if (timed) (void)hipEventRecord(c->ev[2], s);
-    hipLaunchKernelGGL(k_b_gauss, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0, c->iterDev);
+    hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[3], s);
-    hipLaunchKernelGGL(k_b_stage2_propose, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
+    hipLaunchKernelGGL(k_b_stage2_propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
     if (timed) (void)hipEventRecord(c->ev[4], s);
-    hipLaunchKernelGGL(k_b_deriv, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
+    hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
     if (timed) (void)hipEventRecord(c->ev[5], s);
-    hipLaunchKernelGGL(k_b_gauss, dim3(pb), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1, c->iterDev);
+    hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[6], s);
-    hipLaunchKernelGGL(k_b_stage2_accept, dim3(g1), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    hipLaunchKernelGGL(k_b_stage2_accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[7], s);
     hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
     if (timed) (void)hipEventRecord(c->ev[8], s);
@@ -620,7 +638,7 @@ extern "C" int drmlt_run(drmlt_ctx *c, int64_t iters, int32_t mode) {
     HIPCHK(hipEventCreate(&e1));
     // the device iteration counter follows the host one (plain form reads the host one)
     HIPCHK(hipMemcpyAsync(c->iterDev, &c->iterHost, 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemsetAsync(c->L.count, 0, 3 * kMaxBins * 4, s));
+    HIPCHK(hipMemsetAsync(c->L.count, 0, kListKinds * kMaxBins * 4, s));
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipEventRecord(e0, s));
     if (mode == DRMLT_PLAIN) {
@@ -668,6 +686,76 @@ extern "C" int drmlt_read_timing(drmlt_ctx *c, double *out, int32_t n) {
     return DRMLT_OK;
 }
 
+extern "C" int drmlt_get_schedule(drmlt_ctx *c, drmlt_schedule *out) {
+    if (!c || !out) return fail(DRMLT_ERR_ARG, "bad argument");
+    *out = c->sched;
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_set_schedule(drmlt_ctx *c, const drmlt_schedule *sc) {
+    if (!c || !sc) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (sc->bin_stages < 0 || sc->bin_stages > 1 || sc->stage_blocks < 0 || sc->deriv_blocks < 0 || sc->gauss_blocks < 0)
+        return fail(DRMLT_ERR_ARG, "schedule out of range");
+    c->sched = *sc;
+    drop_graph(c);  // the captured step bakes in the launch geometry
+    return DRMLT_OK;
+}
+
+// Coordinate search over the schedule: first the path-stage scheduling
+// (binned or chain order, with the stage kernels' workgroup count) by the
+// whole step's time, then the derivative and Gaussian kernels' workgroup
+// counts by their own kernel times. Every candidate runs one untimed and
+// `steps` timed batched steps of the context's own chains; those are ordinary
+// DR iterations (they advance the chains and splat into the film).
+extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best, double *bestMs) {
+    if (!c || steps <= 0) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!c->haveChains || !c->lenCdf) return fail(DRMLT_ERR_STATE, "chains are not seeded (call drmlt_prepass or drmlt_seed_chains)");
+    const int C = c->ch.C, cus = c->cus, perChain = (C + 63) / 64;
+    // per-step milliseconds of stage kernels k0..k1 (timing slots) under schedule sc
+    auto measure = [&](const drmlt_schedule &sc, int k0, int k1, double &ms) -> int {
+        c->sched = sc;
+        int r = drmlt_run(c, 1, DRMLT_BATCHED);
+        if (r) return r;
+        r = drmlt_run(c, steps, DRMLT_BATCHED);
+        if (r) return r;
+        ms = 0;
+        for (int k = k0; k <= k1; k++) ms += c->timing[k] / steps;
+        return DRMLT_OK;
+    };
+    drmlt_schedule cur = c->sched, cand;
+    double curMs = 0, ms = 0;
+    int r = measure(cur, 1, 7, curMs);
+    if (r) return r;
+    std::vector<int> stageOpts = {cus * 4, cus * 8, cus * 16, perChain};
+    for (int bin = 0; bin <= 1; bin++)
+        for (int sb : stageOpts) {
+            cand = cur;
+            cand.bin_stages = bin;
+            cand.stage_blocks = std::min(sb, perChain);
+            if ((r = measure(cand, 1, 7, ms))) return r;
+            if (ms < curMs) { cur = cand; curMs = ms; }
+        }
+    const int kernelSlots[2][2] = {{2, 5}, {3, 6}};  // derivative and Gaussian kernels (current + proposal)
+    for (int which = 0; which < 2; which++) {
+        double kMs = 0;
+        if ((r = measure(cur, kernelSlots[which][0], kernelSlots[which][0], kMs))) return r;
+        kMs += c->timing[kernelSlots[which][1]] / steps;
+        for (int mult : {4, 8, 16, 32}) {
+            cand = cur;
+            (which == 0 ? cand.deriv_blocks : cand.gauss_blocks) = cus * mult;
+            if ((r = measure(cand, kernelSlots[which][0], kernelSlots[which][0], ms))) return r;
+            ms += c->timing[kernelSlots[which][1]] / steps;
+            if (ms < kMs) { cur = cand; kMs = ms; }
+        }
+    }
+    if ((r = measure(cur, 1, 7, curMs))) return r;
+    c->sched = cur;
+    drop_graph(c);
+    if (best) *best = cur;
+    if (bestMs) *bestMs = curMs;
+    return DRMLT_OK;
+}
+
 extern "C" int drmlt_read_film(drmlt_ctx *c, float *rgb, int32_t w, int32_t h, int32_t which) {
     if (!c || !rgb || w != c->W || h != c->H) return fail(DRMLT_ERR_ARG, "film size mismatch");
     const size_t n = size_t(w) * h * 3;
@@ -704,7 +792,7 @@ extern "C" int drmlt_count_work(drmlt_ctx *c, uint64_t *out) {
     HIPCHK(dalloc(&d, 28));
     HIPCHK(hipMemsetAsync(d, 0, 28 * 8, s));
     HIPCHK(hipMemcpyAsync(c->iterDev, &c->iterHost, 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemsetAsync(c->L.count, 0, 3 * kMaxBins * 4, s));
+    HIPCHK(hipMemsetAsync(c->L.count, 0, kListKinds * kMaxBins * 4, s));
     const DrParams P = make_params(c);
     launch_batched_step(c, P, false, d);
     HIPCHK(hipGetLastError());

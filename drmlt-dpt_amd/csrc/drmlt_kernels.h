@@ -56,9 +56,17 @@ struct DrParams {
     H2Param h2;
     Opts o;
     Opts oDeriv;  // options for the derivative evaluation (chad MIS quirk)
+    int binStages;  // path-stage kernels take their chains from the length bins (1) or in chain order (0)
 };
 
-// Work lists of the batched pipeline, each binned by path length.
+// Work lists of the batched pipeline, each binned by path length:
+// kind 0: H2MC chains (the current state's Gaussian, then the offset);
+// kind 1: chains with a second stage (binned by the current path length);
+// kind 2: H2MC chains whose proposal is valid (its Gaussian and the ratio);
+// kind 3: every chain, binned by the length of the path its first stage
+//         evaluates (the current length for a small step, the sampled one
+//         for a large step).
+constexpr int kListKinds = 4;
 struct Lists {
     int *count;  // [kMaxBins] per list kind
     int *items;  // [kMaxBins][C]

@@ -8,7 +8,7 @@ namespace drmlt {
 // counter (a fill kernel, not a memset, so the step can be graph-captured).
 __global__ void __launch_bounds__(64) k_b_reset(Lists L, uint32_t *iterp, int advance) {
     const int t = threadIdx.x;
-    if (t < 3 * kMaxBins) L.count[t] = 0;
+    for (int k = t; k < kListKinds * kMaxBins; k += blockDim.x) L.count[k] = 0;
     if (t == 0 && advance) *iterp += 1;
 }
 

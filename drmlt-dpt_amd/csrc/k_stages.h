@@ -20,13 +20,31 @@ namespace drmlt {
 #endif
 
 // ------------------------------------------------------------ batched form
-// Work lists, binned by path length: kind 0 = H2MC chains (current Gaussian
-// and offset), kind 2 = H2MC chains with a valid proposal (its Gaussian and
-// the ratio). The stage kernels proper run over all chains in index order.
-DEV void list_push(const Lists &L, int kind, int C, int bin, int item) {
-    const int slot = atomicAdd(&L.count[kind * kMaxBins + bin], 1);
-    L.items[(size_t(kind) * kMaxBins + bin) * C + slot] = item;
+// The step's work lists (drmlt_kernels.h: Lists) are filled by the kernels as
+// they go, binned by path length; each kernel takes its items from the bins
+// concatenated in bin order, so the lanes of a wave mostly evaluate paths of
+// one length and take the same number of walk rounds.
+// Wave-aggregated list_push for every active lane with `pred`: one atomic per
+// bin present in the wave, and the wave's items of one bin are stored next to
+// each other in lane order (neighbouring chains stay neighbours in the list).
+// Called by all active lanes of a wave together.
+DEV void list_push_wave(const Lists &L, int kind, int C, int bin, int item, bool pred) {
+    const int lane = int(threadIdx.x & 63);
+    unsigned long long todo = __ballot(pred);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int b = __shfl(bin, leader);
+        const bool mine = pred && bin == b;
+        const unsigned long long m = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&L.count[kind * kMaxBins + b], __popcll(m));
+        base = __shfl(base, leader);
+        if (mine) L.items[(size_t(kind) * kMaxBins + b) * C + base + __popcll(m & ((1ull << lane) - 1))] = item;
+        todo &= ~m;
+    }
 }
+DEV int length_bin(int len) { return min(max(len, 0), kMaxBins - 1); }
+
 // The bins of a list concatenated in bin order: its length, and item w of it.
 // Consecutive items share a bin (a path length) except at bin boundaries.
 DEV int list_length(const Lists &L, int kind) {
@@ -43,23 +61,36 @@ DEV int list_item(const Lists &L, int kind, int C, int w) {
     return -1;
 }
 
-__global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
-                                                 unsigned long long *stats, Lists L) {
+// Stage-1 moves binned by the length of the path each evaluates (list 3); a
+// small step's state is copied into its proposal slot here.
+__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage1_bin)(Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
     const uint32_t iter = *iterp;
     for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gridDim.x * blockDim.x) {
         Philox g(P.seed, iter, chain, TAG_S1);
         const Stage1 a = stage1_begin(ch, P, chain, g);
+        list_push_wave(L, 3, ch.C, length_bin(stage1_length(ch, a)), chain, true);
+    }
+}
+
+// Stage 1 over every chain: from list 3 (binned) or in chain order.
+__global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
+                                                 unsigned long long *stats, Lists L) {
+    const uint32_t iter = *iterp;
+    const int n = P.binStages ? list_length(L, 3) : ch.C;
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
+        const int chain = P.binStages ? list_item(L, 3, ch.C, it) : it;
+        Philox g(P.seed, iter, chain, TAG_S1);
+        const Stage1 a = stage1_begin(ch, P, chain, g, !P.binStages);
         Walk w;
         stage1_walk(ch, P, a, g, w);
         POut<float> out;
         walk_run(Sc, P.o, w, true, out);
         const int flags = stage1_end(Sc, ch, P, chain, a, out, g, film, stats);
-        if (flags & 2) {
-            const int perm = ch.perm[chain];
-            const int pc = chain + ch.C * perm_cur(perm);
-            const int bin = min(ch.c[pc] + ch.l[pc] - 1, kMaxBins - 1);
-            if (flags & 4) list_push(L, 0, ch.C, bin, chain);
-        }
+        // second stage: list 1 (every chain with one), list 0 (H2MC)
+        const bool second = (flags & 2) != 0;
+        const int bin = length_bin(ch.c[a.pc] + ch.l[a.pc] - 1);
+        list_push_wave(L, 1, ch.C, bin, chain, second);
+        list_push_wave(L, 0, ch.C, bin, chain, second && (flags & 4));
     }
 }
 
@@ -93,14 +124,14 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains
         }
     }
 }
+// Second-stage proposals: list 1 (binned), or every chain in chain order with
+// the lanes of chains without a second stage idle in the walk rounds.
 __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_propose)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
     const uint32_t iter = *iterp;
-    const int gsz = gridDim.x * blockDim.x;
-    // chains in index order (coalesced state access); those without a second stage idle
-    for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gsz) {
-        // every lane of the wave takes part in the walk rounds; those without
-        // a second stage have no walk
-        const bool act = (ch.flags[chain] & 2) != 0;
+    const int n = P.binStages ? list_length(L, 1) : ch.C;
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
+        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : it;
+        const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
         Philox g(P.seed, iter, chain, TAG_S2);
         Stage2 a;
         Walk w;
@@ -111,19 +142,25 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
         POut<float> out;
         out.valid = false;
         walk_run(Sc, P.o, w, act, out);
-        if (!act) continue;
-        stage2_end(Sc, ch, chain, a, out);
-        const int flags = ch.flags[chain];
-        if ((flags & 4) && (flags & 16)) list_push(L, 2, ch.C, min(ch.c[a.pc] + ch.l[a.pc] - 1, kMaxBins - 1), chain);
+        int bin = 0, flags = 0;
+        if (act) {
+            stage2_end(Sc, ch, chain, a, out);
+            flags = ch.flags[chain];
+            bin = length_bin(ch.c[a.pc] + ch.l[a.pc] - 1);
+        }
+        list_push_wave(L, 2, ch.C, bin, chain, (flags & 4) && (flags & 16));
     }
 }
 
+// Second-stage acceptance over the same chains; Green's y* walks where they
+// are needed.
 __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_accept)(DevScene Sc, Chains ch, DrParams P, uint32_t *iterp, float *film,
                                                         unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
-    const int gsz = gridDim.x * blockDim.x;
-    for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gsz) {
-        const bool act = (ch.flags[chain] & 2) != 0;
+    const int n = P.binStages ? list_length(L, 1) : ch.C;
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
+        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : it;
+        const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
         Philox g(P.seed, iter, chain, TAG_S3);
         Stage3 a;
         Walk w;
@@ -134,7 +171,7 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
         }
         POut<float> out;
         out.valid = false;
-        walk_run(Sc, P.o, w, act && a.traceStar, out);
+        walk_run(Sc, P.o, w, a.traceStar, out);
         if (act) stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
     }
 }

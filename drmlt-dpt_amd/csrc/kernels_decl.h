@@ -5,6 +5,7 @@
 namespace drmlt {
 constexpr int kPreMaxC = 64;  // pre-pass contributions kept per sample (k_loop.hip)
 __global__ void k_dr_plain(DevScene, Chains, DrParams, uint32_t, float *, unsigned long long *);
+__global__ void k_b_stage1_bin(Chains, DrParams, const uint32_t *, Lists);
 __global__ void k_b_stage1(DevScene, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
 __global__ void k_b_deriv(DevScene, Chains, DrParams, Lists, int, int);
 __global__ void k_b_gauss(DevScene, Chains, DrParams, Lists, int, int, int, const uint32_t *);
@@ -12,6 +13,7 @@ __global__ void k_b_stage2_propose(DevScene, Chains, DrParams, const uint32_t *,
 __global__ void k_b_stage2_accept(DevScene, Chains, DrParams, uint32_t *, float *, unsigned long long *, Lists);
 __global__ void k_b_reset(Lists, uint32_t *, int);
 // counting forms of the stage kernels (k_count.hip)
+__global__ void k_b_stage1_bin_counted(Chains, DrParams, const uint32_t *, Lists);
 __global__ void k_b_stage1_counted(DevSceneCounted, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
 __global__ void k_b_deriv_counted(DevSceneCounted, Chains, DrParams, Lists, int, int);
 __global__ void k_b_gauss_counted(DevSceneCounted, Chains, DrParams, Lists, int, int, int, const uint32_t *);

@@ -18,6 +18,7 @@ EXPORTS = [
     "drmlt_seed_chains", "drmlt_direct_lighting", "drmlt_run", "drmlt_read_film", "drmlt_read_stats", "drmlt_read_timing",
     "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
     "drmlt_grad_hess", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
+    "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune",
 ]
 
 PLAIN, BATCHED, BATCHED_GRAPH = 0, 1, 2
@@ -46,6 +47,14 @@ class Options(C.Structure):
                 ("aniso_perturb_std_dev", C.c_float), ("acceptance_map", C.c_int32),
                 ("max_derivatives_depth", C.c_int32), ("quirk_ignore_aniso_std_dev", C.c_int32),
                 ("quirk_chad_mis", C.c_int32), ("seed", C.c_uint64)]
+
+
+class Schedule(C.Structure):
+    _fields_ = [("bin_stages", C.c_int32), ("stage_blocks", C.c_int32), ("deriv_blocks", C.c_int32),
+                ("gauss_blocks", C.c_int32)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
 class Stats(C.Structure):
@@ -173,6 +182,26 @@ class Context:
         out = np.zeros((7, 4), np.uint64)
         check(lib().drmlt_count_work(self.h, P(out)))
         return out.astype(np.int64)
+
+    def schedule(self):
+        sc = Schedule()
+        check(lib().drmlt_get_schedule(self.h, C.byref(sc)))
+        return sc.as_dict()
+
+    def set_schedule(self, **fields):
+        sc = Schedule()
+        check(lib().drmlt_get_schedule(self.h, C.byref(sc)))
+        for k, v in fields.items():
+            setattr(sc, k, int(v))
+        check(lib().drmlt_set_schedule(self.h, C.byref(sc)))
+
+    def autotune(self, steps=3):
+        """Times candidate launch schedules on this context's chains and keeps
+        the fastest; returns (schedule dict, ms per step)."""
+        sc = Schedule()
+        ms = C.c_double(0)
+        check(lib().drmlt_autotune(self.h, int(steps), C.byref(sc), C.byref(ms)))
+        return sc.as_dict(), ms.value
 
     def timing(self):
         out = np.zeros(8)

@@ -112,9 +112,21 @@ typedef struct {
     uint64_t stage2_h2mc, stage2_h2mc_accepted;
 } drmlt_stats;
 
+/* Launch schedule of the batched form. The chains advance identically under
+ * every schedule (the same random streams, the same arithmetic); a schedule
+ * only changes speed. The reference has no counterpart: its chains run as
+ * CPU threads (src/parallel.cpp). Zero fields take the defaults. */
+typedef struct {
+    int32_t bin_stages;    /* 1: the path-stage kernels take their chains from path-length bins
+                              (waves of equal-length paths); 0 (default): in chain order */
+    int32_t stage_blocks;  /* 64-lane workgroups of the path-stage kernels (default: one per 64 chains, at most 16 per CU) */
+    int32_t deriv_blocks;  /* workgroups of the derivative kernel (default 16 per CU) */
+    int32_t gauss_blocks;  /* one-wave workgroups of the Gaussian kernel (default 16 per CU) */
+} drmlt_schedule;
+
 /* Execution forms of drmlt_run */
 #define DRMLT_PLAIN 0          /* one kernel launch per step, one thread per chain iteration */
-#define DRMLT_BATCHED 1        /* persistent per-stage kernels over path-length bins */
+#define DRMLT_BATCHED 1        /* one persistent kernel per DR stage, under the context's schedule */
 #define DRMLT_BATCHED_GRAPH 2  /* DRMLT_BATCHED with each step replayed from a captured hipGraph */
 
 const char *drmlt_last_error(void);
@@ -159,6 +171,17 @@ int drmlt_read_timing(drmlt_ctx *ctx, double *out, int32_t n_out);
  * drmlt_read_timing, holding BVH nodes visited, triangles tested, rays traced
  * and algorithmic chain-state bytes. */
 int drmlt_count_work(drmlt_ctx *ctx, uint64_t *out);
+
+/* The batched form's launch schedule. Setting one drops the captured step
+ * graph (it is captured again at the next DRMLT_BATCHED_GRAPH run). */
+int drmlt_get_schedule(drmlt_ctx *ctx, drmlt_schedule *out);
+int drmlt_set_schedule(drmlt_ctx *ctx, const drmlt_schedule *sched);
+/* Autotuner: times candidate schedules on the context's own chains (each
+ * candidate one untimed and `steps` timed batched steps, hipEvents on the
+ * context stream) and keeps the fastest. The steps are ordinary DR iterations:
+ * they advance the chains, count in the statistics and splat into the film.
+ * best and best_ms (both optional) receive the schedule and its time per step. */
+int drmlt_autotune(drmlt_ctx *ctx, int32_t steps, drmlt_schedule *best, double *best_ms);
 
 /* ---------------- tier 2: batched functions ---------------- */
 

@@ -34,6 +34,7 @@ def test_struct_layouts_match_header(tmp_path):
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "drmlt_amd.h"\n'
                    "int main(void){printf(\"%zu %zu %zu %zu %zu %zu\\n\", sizeof(drmlt_options), sizeof(drmlt_scene_desc),"
                    " sizeof(drmlt_path_rec), sizeof(drmlt_stats), offsetof(drmlt_options, seed), offsetof(drmlt_path_rec, lrr));"
+                   "printf(\"%zu\\n\", sizeof(drmlt_schedule));"
                    "return 0;}\n")
     exe = tmp_path / "probe"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -44,6 +45,7 @@ def test_struct_layouts_match_header(tmp_path):
     assert out[3] == C.sizeof(capi.Stats)
     assert out[4] == capi.Options.seed.offset
     assert out[5] == capi.PATH_REC.fields["lrr"][1]
+    assert out[6] == C.sizeof(capi.Schedule)
 
 
 def test_defaults_are_the_references():

@@ -99,6 +99,37 @@ def test_batched_paths_equal_plain_paths(which, request):
     assert res[capi.PLAIN]["stage1_accepted"] > 0 and res[capi.PLAIN]["stage2_iso_accepted"] > 0
 
 
+def test_schedules_agree_and_autotune(cfg2_small):
+    """Binned and chain-order scheduling of the path-stage kernels, and other
+    workgroup counts, advance the chains identically (one iteration from the
+    same seeds gives the same counts); the autotuner returns a schedule the
+    context then runs under, graph-replayed."""
+    from drmlt import capi
+
+    res = []
+    for sched in ({}, {"bin_stages": 1}, {"bin_stages": 1, "stage_blocks": 16, "deriv_blocks": 64, "gauss_blocks": 32}):
+        ctx = _ctx(cfg2_small, num_init_samples=8192, num_chains=4096, seed=5)
+        ctx.prepass()
+        ctx.set_schedule(**sched)
+        say(f"schedule {ctx.schedule()}: 2 iterations")
+        ctx.run(2, capi.BATCHED)
+        res.append(ctx.stats())
+        ctx.close()
+    say(res)
+    assert res[0] == res[1] == res[2]
+    ctx = _ctx(cfg2_small, num_init_samples=8192, num_chains=4096, seed=5)
+    ctx.prepass()
+    say("autotune cfg2_small 4096 chains")
+    best, ms = ctx.autotune(steps=1)
+    say("autotune", best, ms)
+    assert ms > 0 and best == ctx.schedule() and best["bin_stages"] in (0, 1)
+    before = ctx.stats()
+    ctx.run(3, capi.BATCHED_GRAPH)
+    after = ctx.stats()
+    assert after["stage1"] + after["large_step"] - before["stage1"] - before["large_step"] == 3 * 4096
+    ctx.close()
+
+
 def test_loop_vs_oracle(cfg1):
     """The DR loop (Green-Mira, H2MC second stage) against the oracle's loop
     from the oracle's own seeds: acceptance rates per stage and the film mean
