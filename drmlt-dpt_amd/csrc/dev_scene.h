@@ -125,8 +125,6 @@ DEV bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz, 
 // next representable float above a positive finite x
 DEV float next_up(float x) { return __int_as_float(__float_as_int(x) + 1); }
 
-// Closest hit; ties in t go to the smaller scene-order index. Returns the
-// scene-order triangle index or -1.
 // One 4-wide node: the ray's interval against each child box (slab) and the
 // child records.
 struct Node4 {
@@ -191,22 +189,38 @@ DEV int trace_impl(const BvhRef &S, const float o[3], const float d[3], float tm
             hit[k] = hit[k] && cnt == 0;
         }
         if (found) break;
-        // internal children still in front of the best hit, nearest first
-        int cn[4];
-        float ct[4];
-        int nh = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (hit[k] && tn[k] <= tbest) {
-                int j = nh++;
-                while (j > 0 && ct[j - 1] > tn[k]) { ct[j] = ct[j - 1]; cn[j] = cn[j - 1]; j--; }
-                ct[j] = tn[k];
-                cn[j] = i4at(N.child, k);
-            }
-        }
+        // internal children still in front of the best hit, nearest first:
+        // a sorting network on (entry distance, child) in registers (an
+        // insertion sort over arrays would index them dynamically and put
+        // them in scratch memory). Other slots get a key above every
+        // distance and sort last.
+        const float kNone = __int_as_float(0x7f800000);
+        float k0 = (hit[0] && tn[0] <= tbest) ? fminf(tn[0], 3.4e38f) : kNone;
+        float k1 = (hit[1] && tn[1] <= tbest) ? fminf(tn[1], 3.4e38f) : kNone;
+        float k2 = (hit[2] && tn[2] <= tbest) ? fminf(tn[2], 3.4e38f) : kNone;
+        float k3 = (hit[3] && tn[3] <= tbest) ? fminf(tn[3], 3.4e38f) : kNone;
+        int c0 = N.child.x, c1 = N.child.y, c2 = N.child.z, c3 = N.child.w;
+        const int nh = int(k0 != kNone) + int(k1 != kNone) + int(k2 != kNone) + int(k3 != kNone);
+#define DRMLT_CSWAP(ka, ca, kb, cb)                          \
+    {                                                        \
+        const bool sw = kb < ka;                             \
+        const float tk = sw ? kb : ka;                       \
+        const int tc = sw ? cb : ca;                         \
+        kb = sw ? ka : kb; cb = sw ? ca : cb;                \
+        ka = tk; ca = tc;                                    \
+    }
+        DRMLT_CSWAP(k0, c0, k1, c1)
+        DRMLT_CSWAP(k2, c2, k3, c3)
+        DRMLT_CSWAP(k0, c0, k2, c2)
+        DRMLT_CSWAP(k1, c1, k3, c3)
+        DRMLT_CSWAP(k1, c1, k2, c2)
+#undef DRMLT_CSWAP
         if (nh > 0) {
-            for (int j = nh - 1; j > 0; j--) stack[kTraceLanes * sp++] = cn[j];
-            node = cn[0];
+            // farthest first onto the stack, visit the nearest
+            if (nh > 3) stack[kTraceLanes * sp++] = c3;
+            if (nh > 2) stack[kTraceLanes * sp++] = c2;
+            if (nh > 1) stack[kTraceLanes * sp++] = c1;
+            node = c0;
         } else {
             // an entry whose box now lies beyond the best hit is popped and
             // visited anyway (its distance is not kept)
