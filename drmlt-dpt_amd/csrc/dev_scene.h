@@ -23,8 +23,9 @@ struct DevCamera {
 };
 
 struct DevSceneData {
-    // traversal data (leaf order): p0 | e1 | e2, .w of tA holds the scene-order index
-    const float4 *tA, *tB, *tC;
+    // traversal data (leaf order): per triangle three float4 p0 | e1 | e2,
+    // interleaved (48 bytes), .w of p0 holding the scene-order index
+    const float4 *tri;
     // 4-wide BVH: 8 float4 per node: lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4], child[4], count[4]
     // (count > 0 leaf triangle range, 0 internal node, < 0 empty slot)
     const float4 *nodes;
@@ -64,12 +65,12 @@ struct DevScene : DevSceneData {};
 // What a traversal reads: the 4-wide nodes and the leaf-order triangles (and,
 // in the counting build, the work counters).
 struct BvhRef {
-    const float4 *nodes, *tA, *tB, *tC;
+    const float4 *nodes, *tri;
     DRMLT_COUNT(unsigned long long *counters;)
 };
 DEV BvhRef bvh_of(const DevScene &S) {
     BvhRef b;
-    b.nodes = S.nodes; b.tA = S.tA; b.tB = S.tB; b.tC = S.tC;
+    b.nodes = S.nodes; b.tri = S.tri;
     DRMLT_COUNT(b.counters = S.counters;)
     return b;
 }
@@ -82,38 +83,49 @@ DEV void count_work(const BvhRef &S, unsigned nodes, unsigned tris) {
 #endif
 
 // ---------------------------------------------------------------- traversal
+// The traversal reads through global-address-space pointers: it is reached
+// through a function call, where generic pointers would become flat loads
+// (slower, and their completion is waited for together with LDS traffic).
+typedef float vfloat4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) vfloat4 gvfloat4;
+DEV float4 gload(const float4 *p, size_t i) {
+    const vfloat4 v = ((gvfloat4 *)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Moller-Trumbore with the same operation order as the CPU restatement; no FMA
 // contraction so that the hit search is bit-identical to a float CPU search.
+// Written without branches (every quantity is computed, the tests combined at
+// the end), which gives the same decision and t as testing in sequence, and
+// keeps the triangle's loads from being deferred into a branch.
 DEV bool tri_hit(const float4 &A, const float4 &B, const float4 &Cc, const float o[3], const float d[3], float tmin, float tmax,
                  float &t) {
 #pragma clang fp contract(off)
     const float e1x = B.x, e1y = B.y, e1z = B.z, e2x = Cc.x, e2y = Cc.y, e2z = Cc.z;
-    float s1x = d[1] * e2z - d[2] * e2y, s1y = d[2] * e2x - d[0] * e2z, s1z = d[0] * e2y - d[1] * e2x;
-    float div = s1x * e1x + s1y * e1y + s1z * e1z;
-    if (div == 0.0f) return false;
-    float inv = 1.0f / div;
-    float sx = o[0] - A.x, sy = o[1] - A.y, sz = o[2] - A.z;
-    float u = (sx * s1x + sy * s1y + sz * s1z) * inv;
-    if (!(u >= 0.0f)) return false;
-    float s2x = sy * e1z - sz * e1y, s2y = sz * e1x - sx * e1z, s2z = sx * e1y - sy * e1x;
-    float v = (d[0] * s2x + d[1] * s2y + d[2] * s2z) * inv;
-    if (!(v >= 0.0f) || !(u + v <= 1.0f)) return false;
+    const float s1x = d[1] * e2z - d[2] * e2y, s1y = d[2] * e2x - d[0] * e2z, s1z = d[0] * e2y - d[1] * e2x;
+    const float div = s1x * e1x + s1y * e1y + s1z * e1z;
+    const float inv = 1.0f / div;
+    const float sx = o[0] - A.x, sy = o[1] - A.y, sz = o[2] - A.z;
+    const float u = (sx * s1x + sy * s1y + sz * s1z) * inv;
+    const float s2x = sy * e1z - sz * e1y, s2y = sz * e1x - sx * e1z, s2z = sx * e1y - sy * e1x;
+    const float v = (d[0] * s2x + d[1] * s2y + d[2] * s2z) * inv;
     t = (e2x * s2x + e2y * s2y + e2z * s2z) * inv;
-    return t > tmin && t < tmax;
+    return div != 0.0f && u >= 0.0f && v >= 0.0f && u + v <= 1.0f && t > tmin && t < tmax;
 }
 
+// Ray interval against a box; a slab whose bounds come out NaN (origin on
+// the plane of a zero direction component) does not narrow it.
 DEV bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz, const float o[3], const float inv[3], float tmin,
               float tmax, float &tnear) {
     float t0 = tmin, t1 = tmax;
     float a, b;
-#define DRMLT_SLAB(lo, hi, k)                          \
-    a = (lo - o[k]) * inv[k];                          \
-    b = (hi - o[k]) * inv[k];                          \
-    if (!(a != a) && !(b != b)) {                      \
-        float lo_ = fminf(a, b), hi_ = fmaxf(a, b);    \
-        t0 = fmaxf(t0, lo_);                           \
-        t1 = fminf(t1, hi_);                           \
-    }
+    bool ok;
+#define DRMLT_SLAB(lo, hi, k)                             \
+    a = (lo - o[k]) * inv[k];                             \
+    b = (hi - o[k]) * inv[k];                             \
+    ok = !(a != a) && !(b != b);                          \
+    t0 = ok ? fmaxf(t0, fminf(a, b)) : t0;                \
+    t1 = ok ? fminf(t1, fmaxf(a, b)) : t1;
     DRMLT_SLAB(lox, hix, 0)
     DRMLT_SLAB(loy, hiy, 1)
     DRMLT_SLAB(loz, hiz, 2)
@@ -132,15 +144,20 @@ struct Node4 {
     int4 child, count;
 };
 DEV Node4 load_node4(const BvhRef &S, int node) {
-    const float4 *n = S.nodes + 8 * size_t(node);
+    const size_t n = 8 * size_t(node);
     Node4 r;
-    r.lx = n[0]; r.hx = n[1]; r.ly = n[2]; r.hy = n[3]; r.lz = n[4]; r.hz = n[5];
-    r.child = *reinterpret_cast<const int4 *>(n + 6);
-    r.count = *reinterpret_cast<const int4 *>(n + 7);
+    r.lx = gload(S.nodes, n); r.hx = gload(S.nodes, n + 1); r.ly = gload(S.nodes, n + 2);
+    r.hy = gload(S.nodes, n + 3); r.lz = gload(S.nodes, n + 4); r.hz = gload(S.nodes, n + 5);
+    const float4 c = gload(S.nodes, n + 6), k = gload(S.nodes, n + 7);
+    r.child = make_int4(__float_as_int(c.x), __float_as_int(c.y), __float_as_int(c.z), __float_as_int(c.w));
+    r.count = make_int4(__float_as_int(k.x), __float_as_int(k.y), __float_as_int(k.z), __float_as_int(k.w));
     return r;
 }
 DEV float f4at(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 DEV int i4at(const int4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+// Triangles tested per batch of a leaf: their loads are issued together.
+constexpr int kTriBatch = 2;
 
 // One traversal for both query kinds, so that lanes asking for a closest hit
 // and lanes asking for visibility share the loop.
@@ -150,17 +167,23 @@ DEV int i4at(const int4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2
 // any = true: returns a triangle with a hit in (tmin, tmax) (the first found)
 // or -1; the ray interval is not narrowed, so the answer is the same boolean
 // as a search of every triangle.
+// Per node: the four child boxes are tested, the next node to visit is chosen
+// (the nearest internal child in front of the best hit so far, the others
+// pushed farthest first, else the top of the stack) and its fetch issued, and
+// only then the node's leaves are tested, so the two memory latencies
+// overlap. A child chosen before the leaves shortened the ray may turn out to
+// lie behind the best hit: visiting it costs one fetch, its own box tests
+// cull it.
 DEV int trace_impl(const BvhRef &S, const float o[3], const float d[3], float tmin, float tmax, bool any, float &tbest) {
-    float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    const float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
     int *stack = g_trace_stack + threadIdx.x;  // entry k at stack[k * kTraceLanes]
     int sp = 0;
     int best = -1;
     tbest = tmax;
-    int node = 0;
     DRMLT_COUNT(unsigned nvis = 0; unsigned ntri = 0;)
+    Node4 N = load_node4(S, 0);
     for (;;) {
         DRMLT_COUNT(nvis++;)
-        const Node4 N = load_node4(S, node);
         float tn[4];
         bool hit[4];
 #pragma unroll
@@ -168,37 +191,16 @@ DEV int trace_impl(const BvhRef &S, const float o[3], const float d[3], float tm
             hit[k] = i4at(N.count, k) >= 0 &&
                      slab(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tbest, tn[k]);
         }
-        // leaves are tested in place
-        bool found = false;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int cnt = i4at(N.count, k);
-            if (!found && hit[k] && cnt > 0) {
-                DRMLT_COUNT(ntri += cnt;)
-                const int first = i4at(N.child, k);
-                for (int t = first; t < first + cnt; t++) {
-                    float th;
-                    const float4 A = S.tA[t];
-                    if (tri_hit(A, S.tB[t], S.tC[t], o, d, tmin, (any || best < 0) ? tmax : next_up(tbest), th)) {
-                        const int id = __float_as_int(A.w);
-                        if (any) { best = id; found = true; break; }
-                        if (th < tbest || (th == tbest && (best < 0 || id < best))) { tbest = th; best = id; }
-                    }
-                }
-            }
-            hit[k] = hit[k] && cnt == 0;
-        }
-        if (found) break;
-        // internal children still in front of the best hit, nearest first:
-        // a sorting network on (entry distance, child) in registers (an
+        // internal children in front of the best hit, nearest first: a
+        // sorting network on (entry distance, child) in registers (an
         // insertion sort over arrays would index them dynamically and put
-        // them in scratch memory). Other slots get a key above every
-        // distance and sort last.
+        // them in scratch memory); other slots get a key above every
+        // distance and sort last
         const float kNone = __int_as_float(0x7f800000);
-        float k0 = (hit[0] && tn[0] <= tbest) ? fminf(tn[0], 3.4e38f) : kNone;
-        float k1 = (hit[1] && tn[1] <= tbest) ? fminf(tn[1], 3.4e38f) : kNone;
-        float k2 = (hit[2] && tn[2] <= tbest) ? fminf(tn[2], 3.4e38f) : kNone;
-        float k3 = (hit[3] && tn[3] <= tbest) ? fminf(tn[3], 3.4e38f) : kNone;
+        float k0 = (hit[0] && N.count.x == 0 && tn[0] <= tbest) ? fminf(tn[0], 3.4e38f) : kNone;
+        float k1 = (hit[1] && N.count.y == 0 && tn[1] <= tbest) ? fminf(tn[1], 3.4e38f) : kNone;
+        float k2 = (hit[2] && N.count.z == 0 && tn[2] <= tbest) ? fminf(tn[2], 3.4e38f) : kNone;
+        float k3 = (hit[3] && N.count.w == 0 && tn[3] <= tbest) ? fminf(tn[3], 3.4e38f) : kNone;
         int c0 = N.child.x, c1 = N.child.y, c2 = N.child.z, c3 = N.child.w;
         const int nh = int(k0 != kNone) + int(k1 != kNone) + int(k2 != kNone) + int(k3 != kNone);
 #define DRMLT_CSWAP(ka, ca, kb, cb)                          \
@@ -215,18 +217,75 @@ DEV int trace_impl(const BvhRef &S, const float o[3], const float d[3], float tm
         DRMLT_CSWAP(k1, c1, k3, c3)
         DRMLT_CSWAP(k1, c1, k2, c2)
 #undef DRMLT_CSWAP
+        int next = -1;
         if (nh > 0) {
             // farthest first onto the stack, visit the nearest
             if (nh > 3) stack[kTraceLanes * sp++] = c3;
             if (nh > 2) stack[kTraceLanes * sp++] = c2;
             if (nh > 1) stack[kTraceLanes * sp++] = c1;
-            node = c0;
-        } else {
-            // an entry whose box now lies beyond the best hit is popped and
-            // visited anyway (its distance is not kept)
-            if (sp == 0) break;
-            node = stack[kTraceLanes * --sp];
+            next = c0;
+        } else if (sp > 0) {
+            next = stack[kTraceLanes * --sp];
         }
+        // the hit leaves of the node as a queue of up to four triangle
+        // ranges in registers (named, not an array, so nothing is indexed
+        // dynamically)
+        int qf0 = 0, qe0 = 0, qf1 = 0, qe1 = 0, qf2 = 0, qe2 = 0, qf3 = 0, qe3 = 0;
+#define DRMLT_ENQ(k, cnt, first)                                                                        \
+    if (hit[k] && (cnt) > 0) {                                                                          \
+        DRMLT_COUNT(ntri += (cnt);)                                                                     \
+        if (qe0 == 0) { qf0 = (first); qe0 = (first) + (cnt); }                                         \
+        else if (qe1 == 0) { qf1 = (first); qe1 = (first) + (cnt); }                                    \
+        else if (qe2 == 0) { qf2 = (first); qe2 = (first) + (cnt); }                                    \
+        else { qf3 = (first); qe3 = (first) + (cnt); }                                                  \
+    }
+        DRMLT_ENQ(0, N.count.x, N.child.x)
+        DRMLT_ENQ(1, N.count.y, N.child.y)
+        DRMLT_ENQ(2, N.count.z, N.child.z)
+        DRMLT_ENQ(3, N.count.w, N.child.w)
+#undef DRMLT_ENQ
+        // the next node's fetch and the first triangle batch go out together
+        Node4 NN;
+        if (next >= 0) NN = load_node4(S, next);
+        float4 A[kTriBatch], B[kTriBatch], Cc[kTriBatch];
+        int t0 = qf0, end = qe0;
+        bool have = end > 0;
+#define DRMLT_LOAD_BATCH()                                                                              \
+    _Pragma("unroll") for (int j = 0; j < kTriBatch; j++) {                                             \
+        const int t = min(t0 + j, end - 1); /* past the end: the last triangle again, not tested */     \
+        A[j] = gload(S.tri, 3 * size_t(t));                                                             \
+        B[j] = gload(S.tri, 3 * size_t(t) + 1);                                                         \
+        Cc[j] = gload(S.tri, 3 * size_t(t) + 2);                                                        \
+    }
+        if (have) { DRMLT_LOAD_BATCH() }
+        bool found = false;
+        while (have) {
+#pragma unroll
+            for (int j = 0; j < kTriBatch; j++) {
+                float th;
+                if (!found && t0 + j < end && tri_hit(A[j], B[j], Cc[j], o, d, tmin, (any || best < 0) ? tmax : next_up(tbest), th)) {
+                    const int id = __float_as_int(A[j].w);
+                    if (any) {
+                        best = id;
+                        found = true;
+                    } else if (th < tbest || (th == tbest && (best < 0 || id < best))) {
+                        tbest = th;
+                        best = id;
+                    }
+                }
+            }
+            // the next batch: further in this leaf, else the next leaf of the queue
+            t0 += kTriBatch;
+            if (t0 >= end) {
+                t0 = qf1; end = qe1;
+                qf1 = qf2; qe1 = qe2; qf2 = qf3; qe2 = qe3; qe3 = 0;
+            }
+            have = !found && end > 0;
+            if (have) { DRMLT_LOAD_BATCH() }
+        }
+#undef DRMLT_LOAD_BATCH
+        if (found || next < 0) break;
+        N = NN;
     }
     DRMLT_COUNT(count_work(S, nvis, ntri);)
     return best;

@@ -265,7 +265,7 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
         if (cap <= 8) return bail(fail(DRMLT_ERR_ARG, "BVH does not fit the traversal stack"));
     }
     c->bvhDepth = bvh.depth4;
-    std::vector<float4> tA(nt), tB(nt), tC(nt), s0(nt), s1(nt), s2(nt);
+    std::vector<float4> tri(3 * size_t(nt)), s0(nt), s1(nt), s2(nt);
     for (int i = 0; i < nt; i++) {
         const float *p = sc->tri_p + 9 * size_t(i);
         s0[i] = make_float4(p[0], p[1], p[2], 0.0f);
@@ -278,9 +278,9 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
         std::memcpy(&bits, &i, 4);
         float w;
         std::memcpy(&w, &bits, 4);
-        tA[k] = make_float4(s0[i].x, s0[i].y, s0[i].z, w);
-        tB[k] = s1[i];
-        tC[k] = s2[i];
+        tri[3 * size_t(k)] = make_float4(s0[i].x, s0[i].y, s0[i].z, w);
+        tri[3 * size_t(k) + 1] = s1[i];
+        tri[3 * size_t(k) + 2] = s2[i];
     }
     // lights: area cdf per light (TriangleMesh::SetAreaLight, trianglemesh.cpp:274-290)
     const int nl = sc->num_lights;
@@ -319,10 +319,10 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
         c->sceneAllocs.push_back((void *)*p);
         return e;
     };
-    float4 *d_tA, *d_tB, *d_tC, *d_nodes, *d_s0, *d_s1, *d_s2;
+    float4 *d_tri, *d_nodes, *d_s0, *d_s1, *d_s2;
     float *d_sn, *d_mats, *d_le, *d_area, *d_cdf, *d_func, *d_pick;
     int *d_mat, *d_light, *d_lt, *d_cdfOff;
-    bool ok = up(&d_tA, tA.data(), nt) == hipSuccess && up(&d_tB, tB.data(), nt) == hipSuccess && up(&d_tC, tC.data(), nt) == hipSuccess &&
+    bool ok = up(&d_tri, tri.data(), 3 * size_t(nt)) == hipSuccess &&
               up(&d_nodes, (const float4 *)bvh.nodes.data(), bvh.nodes.size() / 4) == hipSuccess &&
               up(&d_s0, s0.data(), nt) == hipSuccess && up(&d_s1, s1.data(), nt) == hipSuccess && up(&d_s2, s2.data(), nt) == hipSuccess &&
               up(&d_sn, sc->tri_n, size_t(nt) * 9) == hipSuccess && up(&d_mat, sc->tri_mat, nt) == hipSuccess &&
@@ -335,7 +335,7 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     for (int i = 0; i < nt; i++)
         if (sc->tri_mat[i] < 0 || sc->tri_mat[i] >= sc->num_materials || sc->tri_light[i] >= nl)
             return bail(fail(DRMLT_ERR_ARG, "triangle material / light index out of range"));
-    S.tA = d_tA; S.tB = d_tB; S.tC = d_tC; S.nodes = d_nodes;
+    S.tri = d_tri; S.nodes = d_nodes;
     S.sp0 = d_s0; S.se1 = d_s1; S.se2 = d_s2; S.sn = d_sn;
     S.tmat = d_mat; S.tlight = d_light; S.mats = d_mats;
     S.nt = nt; S.nlights = nl;
