@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--tune-steps", type=int, default=3, help="timed steps per candidate schedule of the autotuner")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch schedule (profiled runs: no candidate schedules among the launches)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device to device (hipMemcpy), the known-size kernel that scales "
                          "rocprof FETCH_SIZE / WRITE_SIZE readings")
@@ -103,8 +105,12 @@ def main():
     log(f"plain form {plain_ms:.2f} ms/step")
 
     # launch schedule of the batched form chosen by the autotuner on these chains
-    sched, tuned_ms = ctx.autotune(steps=args.tune_steps)
-    log(f"autotuned schedule {sched}: {tuned_ms:.3f} ms/step")
+    if args.no_tune:
+        sched = ctx.schedule()
+        log(f"default schedule {sched}")
+    else:
+        sched, tuned_ms = ctx.autotune(steps=args.tune_steps)
+        log(f"autotuned schedule {sched}: {tuned_ms:.3f} ms/step")
 
     # batched form with per-stage timing (events between kernels)
     ctx.run(3, capi.BATCHED)
@@ -130,7 +136,7 @@ def main():
     # 128 B per 4-wide BVH node visited (one cache line), 48 B per triangle tested
     alg_bytes = nodes * 128 + tris * 48 + state_bytes
     achieved = alg_bytes / (stage_ms[dom] / 1000.0) / 1e9
-    traffic, traffic_src = None, None
+    traffic, traffic_src, valu = None, None, None
     prof = os.path.join(ROOT, "profiles", f"dr_step_config{args.config}_hbm_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
@@ -138,6 +144,7 @@ def main():
         k = STAGE_KERNELS[dom]
         if k in pt.get("kernels", {}):
             traffic = int(pt["kernels"][k]["hbm_bytes_per_launch"])
+            valu = pt["kernels"][k].get("valu_util")
             traffic_src = (f"rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of {k} at commit {pt['commit']}, scaled by a "
                            f"256 MiB device copy (x{pt['fetch_scale']:.2f}, x{pt['write_scale']:.2f}); "
                            f"profiles/dr_step_config{args.config}_hbm_traffic.md")
@@ -147,6 +154,10 @@ def main():
             "rays_per_launch": int(rays)}
     if traffic_src:
         roof["traffic_source"] = traffic_src
+    if valu is not None:
+        roof["valu_util"] = valu
+        roof["valu_util_source"] = (f"4 x SQ_ACTIVE_INST_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8) of {STAGE_KERNELS[dom]}, "
+                                    f"profiles/dr_step_config{args.config}_valu.md")
 
     cpu = None
     if not args.no_cpu:
