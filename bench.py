@@ -44,6 +44,52 @@ def cpu_baseline(sc, threads, chains, iters, seed=11):
     return chains * iters / sec, sec
 
 
+def derivative_workload(n=262144, seed=4321):
+    """The derivative kernels alone: gradient and full Hessian of log f at n
+    path states of the config-2 scene, in FP32 with an FP64 check. The states
+    are realised by the loop itself (pre-pass seeds plus two DR iterations at
+    maximum depth 9, so D = 2 * path length reaches 18), read back with
+    drmlt_read_chains and kept where the reference computes derivatives
+    (drsmallstep.cpp:135-139: f > 1e-15, c + l >= 3, path length <= 9)."""
+    import numpy as np
+    from drmlt import capi, scenes
+
+    sc = scenes.config2()
+    opt = capi.options_from_dpt(dict(sc["options"], maxdepth=9), num_chains=n, seed=seed)
+    ctx = capi.Context(sc, opt)
+    ctx.prepass()
+    ctx.run(2, capi.BATCHED_GRAPH)
+    recs, c8 = ctx.read_chains()
+    P = recs["c"] + recs["l"] - 1
+    keep = (c8[:, 0] > 0) & (c8[:, 5] > 1e-15) & (recs["c"] + recs["l"] >= 3) & (P <= int(opt.max_derivatives_depth))
+    recs = recs[keep]
+    D = 2 * np.maximum(recs["c"] + recs["l"] - 1, 2)
+    log(f"derivative workload: {len(recs)} of {n} chain states, D histogram {np.bincount(D).tolist()}")
+    g32, h32, ok32 = ctx.grad_hess(recs, fp64=False)
+    ms32 = float(ctx.timing()[0])
+    g64, h64, ok64 = ctx.grad_hess(recs, fp64=True)
+    ms64 = float(ctx.timing()[0])
+    _, okl = ctx.eval_logf(recs, fp64=False)
+    msl = float(ctx.timing()[0])
+    ctx.close()
+    both = (ok32 == 1) & (ok64 == 1)
+    eg, eh = [], []
+    for i in np.nonzero(both)[0][:20000]:
+        d = D[i]
+        eg.append(np.linalg.norm(g32[i, :d] - g64[i, :d]) / max(np.linalg.norm(g64[i, :d]), 1e-30))
+        eh.append(np.linalg.norm(h32[i, :d, :d] - h64[i, :d, :d]) / max(np.linalg.norm(h64[i, :d, :d]), 1e-30))
+    return {"workload": f"config3: gradient + Hessian of log f at {len(recs)} states of the config-2 scene "
+                        f"(D {int(D.min())}-{int(D.max())}), states from the DR loop",
+            "states": int(len(recs)), "d_histogram": np.bincount(D).tolist(),
+            "fp32_ms": round(ms32, 3), "fp32_states_per_s": round(len(recs) / (ms32 / 1000.0), 1),
+            "fp64_ms": round(ms64, 3), "fp64_states_per_s": round(len(recs) / (ms64 / 1000.0), 1),
+            "logf_fp32_ms": round(msl, 3),
+            "fp32_vs_fp64_rel_err": {"grad_median": float(np.median(eg)), "grad_p99": float(np.percentile(eg, 99)),
+                                     "hess_median": float(np.median(eh)), "hess_p99": float(np.percentile(eh, 99)),
+                                     "states_compared": len(eg)},
+            "ok_fraction": float(both.mean())}
+
+
 def copy_calibration(mb):
     """One device-to-device hipMemcpy of `mb` MB (read mb, write mb)."""
     import ctypes as C
@@ -71,6 +117,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--tune-steps", type=int, default=3, help="timed steps per candidate schedule of the autotuner")
+    ap.add_argument("--no-deriv", action="store_true", help="skip the derivative-kernel workload (config 3)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch schedule (profiled runs: no candidate schedules among the launches)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -159,6 +206,12 @@ def main():
         roof["valu_util_source"] = (f"4 x SQ_ACTIVE_INST_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8) of {STAGE_KERNELS[dom]}, "
                                     f"profiles/dr_step_config{args.config}_valu.md")
 
+    ctx.close()
+    deriv = None
+    if not args.no_deriv:
+        deriv = derivative_workload()
+        log(f"derivative workload: {deriv}")
+
     cpu = None
     if not args.no_cpu:
         ncpu = min(args.cpu_threads, os.cpu_count() or 1)
@@ -186,9 +239,9 @@ def main():
         "acceptance": st,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "derivatives": deriv,
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

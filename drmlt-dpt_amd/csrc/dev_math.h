@@ -158,6 +158,7 @@ template <class T> DEV T luminance(const V3<T> &v) { return v.x * T(0.212671f) +
 template <class T> DEV V3<T> vconst(float a, float b, float c) { return V3<T>(T(a), T(b), T(c)); }
 template <class T> DEV V3<T> lift(const V3<float> &a) { return V3<T>(T(a.x), T(a.y), T(a.z)); }
 DEV V3<float> lower(const V3<float> &a) { return a; }
+DEV V3<float> lower(const V3<double> &a) { return V3<float>(float(a.x), float(a.y), float(a.z)); }
 template <class S, int K> DEV V3<float> lower(const V3<Jet<S, K>> &a) { return V3<float>(float(a.x.v), float(a.y.v), float(a.z.v)); }
 
 // ---------------------------------------------------------------- Philox4x32-10

@@ -369,6 +369,14 @@ template <class S, int K> struct JetU {
     }
 };
 
+// The state's own u, unperturbed, in T: the primal of the fixed-topology
+// function (evaluate_path_bidir_{c}_{l}_static, src/chad.cpp:183-238).
+template <class T> struct PlainU {
+    const float *u;
+    int s, k;
+    DEV T next() { return T(u[k++ * s]); }
+};
+
 // Discrete-dimension perturbation (path.cpp:1482 etc.): bsdfDiscrete <- (d + n) mod 1
 struct DiscArray {
     const float *d;
@@ -516,6 +524,10 @@ template <class G> DEV void generate_subpath(const DevScene &S, const Opts &o, i
         ref.cdisc[k * s] = 0.0f; ref.ldisc[k * s] = 0.0f;
     }
     for (int k = 0; k < kDMax; k++) ref.u[k * s] = 0.0f;
+    // the camera subpath needs two vertices unless a light subpath of two or
+    // more ends in the camera connection (the reference asserts camLength > 1
+    // there, path.cpp:1186); anything else is no path
+    if (camLen < 2 && lgtLen < 2) return;
     UWriter uw{ref.u, s, 0};
     RayT<float> r;
     BState<float> ls;

@@ -157,7 +157,10 @@ DEV float f4at(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k 
 DEV int i4at(const int4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
 // Triangles tested per batch of a leaf: their loads are issued together.
-constexpr int kTriBatch = 2;
+#ifndef DRMLT_TRI_BATCH
+#define DRMLT_TRI_BATCH 2
+#endif
+constexpr int kTriBatch = DRMLT_TRI_BATCH;
 
 // One traversal for both query kinds, so that lanes asking for a closest hit
 // and lanes asking for visibility share the loop.
@@ -300,7 +303,7 @@ struct TraceHit {
     int prim;
     float t;
 };
-#ifdef DRMLT_COUNT_WORK
+#if defined(DRMLT_COUNT_WORK) || defined(DRMLT_TRACE_INLINE)
 DEV TraceHit trace_call(const BvhRef S, float ox, float oy, float oz, float dx, float dy, float dz, float tmin, float tmax, int any) {
 #else
 __device__ __noinline__ TraceHit trace_call(const BvhRef S, float ox, float oy, float oz, float dx, float dy, float dz, float tmin,

@@ -124,6 +124,9 @@ DEV bool walk_start(const DevScene &S, const Opts &o, Walk &w, POut<float> &out)
             ref.cdisc[k * s] = 0.0f; ref.ldisc[k * s] = 0.0f;
         }
         for (int k = 0; k < kDMax; k++) ref.u[k * s] = 0.0f;
+        // as generate_subpath: no path without two camera vertices or a light
+        // subpath that ends in the camera connection
+        if (w.c < 2 && w.l < 2) return false;
     }
     if (w.l > 1) {
         float p0, p1, d0, d1;

@@ -900,6 +900,78 @@ static int jet_passes_host(int D) {
     return np;
 }
 
+// Device time of the function-level kernel bracketed by ev[0] and ev[1], kept
+// as timing[0] (drmlt_read_timing).
+static void kernel_time(drmlt_ctx *c) {
+    float ms = 0;
+    if (hipEventSynchronize(c->ev[1]) == hipSuccess && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
+        for (double &t : c->timing) t = 0;
+        c->timing[0] = ms;
+    }
+}
+
+extern "C" int drmlt_eval_logf(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st, int32_t fp64, double *logf, int32_t *okOut) {
+    if (!c || n < 0 || !st || !logf || !okOut) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!recs_ok(c, st, n) || !recs_fixed_ok(st, n)) return fail(DRMLT_ERR_ARG, "path record out of range");
+    if (n == 0) return DRMLT_OK;
+    hipStream_t s = c->stream;
+    DBuf<char> r;
+    DBuf<double> lf;
+    DBuf<int> ok;
+    DUP(r, (const char *)st, n * sizeof(drmlt_path_rec));
+    DNEW(lf, n); DNEW(ok, n);
+    const DrParams P = make_params(c);
+    HIPCHK(hipEventRecord(c->ev[0], s));
+    if (fp64) hipLaunchKernelGGL(k_logf_rec_f64, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.oDeriv, int(n), r.p, lf.p, ok.p);
+    else hipLaunchKernelGGL(k_logf_rec_f32, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.oDeriv, int(n), r.p, lf.p, ok.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[1], s));
+    DGET(logf, lf, n);
+    DGET(okOut, ok, n);
+    HIPCHK(hipStreamSynchronize(s));
+    kernel_time(c);
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_read_chains(drmlt_ctx *c, int64_t n, drmlt_path_rec *st, float *c8) {
+    if (!c || n < 0 || !st || !c8) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!c->haveChains) return fail(DRMLT_ERR_STATE, "chains are not seeded (call drmlt_prepass or drmlt_seed_chains)");
+    if (n > c->ch.C) return fail(DRMLT_ERR_ARG, "more states asked for than there are chains");
+    if (n == 0) return DRMLT_OK;
+    const Chains &ch = c->ch;
+    const size_t C = size_t(ch.C), NS = size_t(ch.NS);
+    hipStream_t s = c->stream;
+    auto get = [&](auto *dev, size_t count, auto &host) {
+        host.resize(count);
+        return hipMemcpyAsync(host.data(), dev, count * sizeof(host[0]), hipMemcpyDeviceToHost, s);
+    };
+    std::vector<int> perm, ci, li, lig, lpr, valid, cp, lp;
+    std::vector<float> scr, u, cd, ld, crr, lrr, rgb, ls, ss;
+    HIPCHK(get(ch.perm, C, perm)); HIPCHK(get(ch.c, NS, ci)); HIPCHK(get(ch.l, NS, li)); HIPCHK(get(ch.light, NS, lig));
+    HIPCHK(get(ch.lprim, NS, lpr)); HIPCHK(get(ch.valid, NS, valid)); HIPCHK(get(ch.cprim, 16 * NS, cp));
+    HIPCHK(get(ch.lgprim, 16 * NS, lp)); HIPCHK(get(ch.screen, 2 * NS, scr)); HIPCHK(get(ch.u, 32 * NS, u));
+    HIPCHK(get(ch.cdisc, 16 * NS, cd)); HIPCHK(get(ch.ldisc, 16 * NS, ld)); HIPCHK(get(ch.crr, 16 * NS, crr));
+    HIPCHK(get(ch.lrr, 16 * NS, lrr)); HIPCHK(get(ch.rgb, 3 * NS, rgb)); HIPCHK(get(ch.ls, NS, ls)); HIPCHK(get(ch.ss, NS, ss));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; i++) {
+        const size_t p = size_t(i) + C * size_t(perm[i] & 3);
+        drmlt_path_rec &R = st[i];
+        R.c = ci[p]; R.l = li[p]; R.light = lig[p]; R.lprim = lpr[p];
+        R.screen[0] = scr[p]; R.screen[1] = scr[p + NS];
+        for (int k = 0; k < 32; k++) R.u[k] = u[k * NS + p];
+        for (int k = 0; k < 16; k++) {
+            R.cprim[k] = cp[k * NS + p]; R.lgprim[k] = lp[k * NS + p];
+            R.cdisc[k] = cd[k * NS + p]; R.ldisc[k] = ld[k * NS + p];
+            R.crr[k] = crr[k * NS + p]; R.lrr[k] = lrr[k * NS + p];
+        }
+        float *o = c8 + 8 * size_t(i);
+        o[0] = valid[p] ? 1.0f : 0.0f;
+        o[1] = rgb[p]; o[2] = rgb[p + NS]; o[3] = rgb[p + 2 * NS];
+        o[4] = ls[p]; o[5] = ss[p]; o[6] = scr[p]; o[7] = scr[p + NS];
+    }
+    return DRMLT_OK;
+}
+
 extern "C" int drmlt_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st, int32_t fp64, double *grad, double *hess, int32_t *okOut) {
     if (!c || n < 0 || !st || !grad || !hess || !okOut) return fail(DRMLT_ERR_ARG, "bad argument");
     if (!recs_ok(c, st, n) || !recs_fixed_ok(st, n)) return fail(DRMLT_ERR_ARG, "path record out of range");
@@ -923,9 +995,11 @@ extern "C" int drmlt_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st
         DNEW(g, 32 * n); DNEW(h, 1024 * n);
         HIPCHK(hipMemsetAsync(g.p, 0, 32 * n * 8, s));
         HIPCHK(hipMemsetAsync(h.p, 0, 1024 * n * 8, s));
+        HIPCHK(hipEventRecord(c->ev[0], s));
         hipLaunchKernelGGL(k_gradhess_rec_f64, dim3(unsigned((total + 63) / 64)), dim3(64), 0, s, c->S, P.oDeriv, int(n), r.p, po.p,
                            total, g.p, h.p, ok.p);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[1], s));
         DGET(grad, g, 32 * n);
         DGET(hess, h, 1024 * n);
     } else {
@@ -933,9 +1007,11 @@ extern "C" int drmlt_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st
         DNEW(g, 32 * n); DNEW(h, 1024 * n);
         HIPCHK(hipMemsetAsync(g.p, 0, 32 * n * 4, s));
         HIPCHK(hipMemsetAsync(h.p, 0, 1024 * n * 4, s));
+        HIPCHK(hipEventRecord(c->ev[0], s));
         hipLaunchKernelGGL(k_gradhess_rec_f32, dim3(unsigned((total + 63) / 64)), dim3(64), 0, s, c->S, P.oDeriv, int(n), r.p, po.p,
                            total, g.p, h.p, ok.p);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[1], s));
         std::vector<float> hg(32 * n), hh(1024 * n);
         DGET(hg.data(), g, 32 * n);
         DGET(hh.data(), h, 1024 * n);
@@ -945,6 +1021,7 @@ extern "C" int drmlt_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st
     }
     DGET(okOut, ok, n);
     HIPCHK(hipStreamSynchronize(s));
+    kernel_time(c);
     // the passes write the upper triangle (column >= row); mirror it
     for (int64_t i = 0; i < n; i++) {
         double *H = hess + size_t(i) * 1024;

@@ -23,6 +23,26 @@ DEV void gradhess_rec(int w, const DevScene &Sc, const Opts &o, int n, char *rec
     if (!ok) okOut[r] = 0;
 }
 
+// log f at each record with its triangles fixed (the function the derivative
+// passes differentiate), in T; ok[i] = 0 where it has no contribution.
+template <class T>
+DEV void logf_rec(int i, const DevScene &Sc, const Opts &o, int n, const char *recs, double *logf, int *okOut) {
+    if (i >= n) return;
+    const SRef ref = rec_ref(const_cast<char *>(recs) + size_t(i) * kRecBytes);
+    PlainU<T> up{ref.u, ref.s, 0};
+    POut<T> out;
+    perturb_eval<T, true>(Sc, o, ref, up, DiscNone{}, out);
+    const bool ok = out.valid && out.ssScore > T(0);
+    logf[i] = ok ? double(log(out.ssScore)) : 0.0;
+    okOut[i] = ok ? 1 : 0;
+}
+__global__ void __launch_bounds__(64) k_logf_rec_f32(DevScene Sc, Opts o, int n, const char *recs, double *logf, int *okOut) {
+    logf_rec<float>(blockIdx.x * blockDim.x + threadIdx.x, Sc, o, n, recs, logf, okOut);
+}
+__global__ void __launch_bounds__(64) k_logf_rec_f64(DevScene Sc, Opts o, int n, const char *recs, double *logf, int *okOut) {
+    logf_rec<double>(blockIdx.x * blockDim.x + threadIdx.x, Sc, o, n, recs, logf, okOut);
+}
+
 __global__ void __launch_bounds__(64) k_gradhess_rec_f32(DevScene Sc, Opts o, int n, char *recs, const int *passOff, int totalPasses, float *grad, float *hess,
                                    int *okOut) {
     gradhess_rec<float>(blockIdx.x * blockDim.x + threadIdx.x, Sc, o, n, recs, passOff, totalPasses, grad, hess, okOut);

@@ -18,7 +18,7 @@ EXPORTS = [
     "drmlt_seed_chains", "drmlt_direct_lighting", "drmlt_run", "drmlt_read_film", "drmlt_read_stats", "drmlt_read_timing",
     "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
     "drmlt_grad_hess", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
-    "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune",
+    "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains",
 ]
 
 PLAIN, BATCHED, BATCHED_GRAPH = 0, 1, 2
@@ -240,6 +240,24 @@ class Context:
         cs = np.zeros((n, maxc, 10), np.float32); nc = np.zeros(n, np.int32)
         check(lib().drmlt_generate_bidir(self.h, C.c_int64(n), min_depth, max_depth, P(uni), uni.shape[1], maxc, P(cs), P(nc)))
         return cs, nc
+
+    def read_chains(self, n=None):
+        """Current states of chains 0..n-1 as records, with contrib8 rows."""
+        n = int(self.opt.num_chains if n is None else n)
+        st = np.zeros(n, PATH_REC)
+        c8 = np.zeros((n, 8), np.float32)
+        check(lib().drmlt_read_chains(self.h, C.c_int64(n), P(st), P(c8)))
+        return st, c8
+
+    def eval_logf(self, states, fp64=False):
+        """log f at each state with its triangles fixed (the function
+        grad_hess differentiates); returns (logf, ok)."""
+        states = np.ascontiguousarray(states, PATH_REC)
+        n = len(states)
+        lf = np.zeros(n)
+        ok = np.zeros(n, np.int32)
+        check(lib().drmlt_eval_logf(self.h, C.c_int64(n), P(states), int(fp64), P(lf), P(ok)))
+        return lf, ok
 
     def grad_hess(self, states, fp64=False):
         states = np.ascontiguousarray(states, PATH_REC)

@@ -161,7 +161,8 @@ int drmlt_run(drmlt_ctx *ctx, int64_t iters_per_chain, int32_t mode);
  * buffer, 2 raw direct buffer. rgb holds w*h*3 floats. */
 int drmlt_read_film(drmlt_ctx *ctx, float *rgb, int32_t w, int32_t h, int32_t which);
 int drmlt_read_stats(drmlt_ctx *ctx, drmlt_stats *stats);
-/* Device time of the last drmlt_run in milliseconds, total and per stage
+/* Device time of the last drmlt_run in milliseconds (after a function-level
+ * call that says so: of its kernel, in out[0]), total and per stage
  * kernel (hipEvents on the context stream): out[0] total, out[1..7] stage 1,
  * derivatives (current), Gaussian (current), stage-2 proposal, derivatives
  * (proposal), Gaussian (proposal), stage-2 acceptance. n_out <= 8. */
@@ -182,6 +183,11 @@ int drmlt_set_schedule(drmlt_ctx *ctx, const drmlt_schedule *sched);
  * they advance the chains, count in the statistics and splat into the film.
  * best and best_ms (both optional) receive the schedule and its time per step. */
 int drmlt_autotune(drmlt_ctx *ctx, int32_t steps, drmlt_schedule *best, double *best_ms);
+
+/* The current state of chains 0..n-1 as records (the Markov states the
+ * loop holds; delayed.cpp:343-345 keeps them per chain), contrib8 as in
+ * drmlt_seed_chains (valid flag, rgb, lsScore, ssScore, screen x, y). */
+int drmlt_read_chains(drmlt_ctx *ctx, int64_t n, drmlt_path_rec *states, float *contrib8);
 
 /* ---------------- tier 2: batched functions ---------------- */
 
@@ -211,11 +217,20 @@ int drmlt_generate_subpath(drmlt_ctx *ctx, int64_t n, const int32_t *cl, const f
 int drmlt_generate_bidir(drmlt_ctx *ctx, int64_t n, int32_t min_depth, int32_t max_depth, const float *uni, int32_t nuni, int32_t maxc,
                          float *contribs, int32_t *ncontrib);
 
+/* log f at each state with its triangles fixed: the primal of the
+ * reference's chad library, evaluate_path_bidir_{c}_{l}_static
+ * (src/chad.cpp:183-238, logLumValue of src/path.cpp:3159), i.e. the
+ * function drmlt_grad_hess differentiates. ok[i] = 0 where the state has no
+ * contribution. fp64 = 1 evaluates in double. Afterwards drmlt_read_timing's
+ * out[0] is the device time of the kernel. */
+int drmlt_eval_logf(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int32_t fp64, double *logf, int32_t *ok);
+
 /* Gradient and Hessian of log f (the function of evaluate_path_bidir_{c}_{l}
  * _static_derv, src/path.cpp:2987-3175 / src/chad.cpp:494-551) at each state,
  * triangles fixed. grad n x 32, hess n x 32 x 32 (row-major, leading
  * dimension 32); ok[i] = 0 where the fixed-topology evaluation fails.
- * fp64 = 1 evaluates in double. */
+ * fp64 = 1 evaluates in double. Afterwards drmlt_read_timing's out[0] is the
+ * device time of the derivative kernel. */
 int drmlt_grad_hess(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int32_t fp64, double *grad, double *hess, int32_t *ok);
 
 /* PH2MCSmallStep::ComputeGaussian (drsmallstep.cpp:18-93, 219-266) in FP64
