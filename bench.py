@@ -90,6 +90,93 @@ def derivative_workload(n=262144, seed=4321):
             "ok_fraction": float(both.mean())}
 
 
+def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
+    """Equal-time image RMSE against a long reference render.
+
+    The reference is a batched render from independent seeds with 256 direct
+    samples per pixel. The plain form, the batched form (under the tuned
+    schedule `sched`) and, if cpu_threads > 0, the oracle's CPU loop on the same
+    number of chains each render for `budget_s` seconds of their own time
+    (device time for the GPU forms); the RMSE of the full image and of the DR
+    (indirect) part are taken over the film against the reference's. The
+    batched form's curve gives the time it needs to reach the plain form's
+    indirect RMSE at the budget."""
+    import numpy as np
+    from drmlt import capi, scenes
+
+    sc = scenes.CONFIGS[config]()
+    dpt = dict(sc["options"])
+    chains = int(dpt["numchains"])
+    W, H = int(sc["cam"][10]), int(sc["cam"][11])
+
+    def make(seed_, **kw):
+        ctx = capi.Context(sc, capi.options_from_dpt(dpt, num_chains=chains, seed=seed_, **kw))
+        ctx.set_schedule(**sched)
+        ctx.prepass()
+        ctx.direct_lighting()
+        return ctx
+
+    def images(ctx):
+        full = ctx.film(0)
+        return full, full - ctx.film(2) / float(ctx.opt.direct_spp)
+
+    rmse = lambda a, b: float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)))
+    t0 = time.time()
+    ref = make(seed + 7777, direct_spp=256)
+    chunk = 50
+    for k in range(0, ref_steps, chunk):
+        ref.run(min(chunk, ref_steps - k), capi.BATCHED_GRAPH)
+        if time.time() - t0 > 10:
+            log(f"rmse study: reference {k + chunk}/{ref_steps} steps")
+            t0 = time.time()
+    rfull, rind = images(ref)
+    ref.close()
+
+    def curve(mode, step_chunk):
+        ctx = make(seed)
+        t, steps, pts = 0.0, 0, []
+        t1 = time.time()
+        while t < budget_s * 1000.0:
+            ctx.run(step_chunk, mode)
+            t += float(ctx.timing()[0])
+            steps += step_chunk
+            full, ind = images(ctx)
+            pts.append((t / 1000.0, steps, rmse(full, rfull), rmse(ind, rind)))
+            if time.time() - t1 > 10:
+                log(f"rmse study: mode {mode} {t / 1000.0:.1f}/{budget_s} s")
+                t1 = time.time()
+        ctx.close()
+        return pts
+
+    plain = curve(capi.PLAIN, 1)
+    batched = curve(capi.BATCHED_GRAPH, 2)
+    target = plain[-1][3]
+    reach = next((p for p in batched if p[3] <= target), None)
+    pick = lambda pts: {"time_s": round(pts[-1][0], 3), "steps": pts[-1][1], "rmse": pts[-1][2], "rmse_indirect": pts[-1][3]}
+    out = {"workload": f"config{config}: {len(sc['tri_p'])} tris, {chains} chains, {W}x{H}",
+           "budget_s": budget_s, "reference": {"steps": ref_steps, "direct_spp": 256, "seed": "independent"},
+           "plain": pick(plain), "batched": pick(batched),
+           "batched_time_to_plain_rmse_s": round(reach[0], 4) if reach else None,
+           "batched_steps_to_plain_rmse": reach[1] if reach else None,
+           "time_ratio_plain_over_batched_at_equal_rmse": round(plain[-1][0] / reach[0], 2) if reach else None}
+    if cpu_threads > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import orc
+
+        O = orc.Scene(sc)
+        b, lf, seeds, c8, _ = O.prepass(int(dpt["numinitsamples"]), int(dpt["maxdepth"]), chains, seed=seed, threads=cpu_threads)
+        _, _, sec1 = O.render(seeds, c8, lf, b, 1, threads=cpu_threads, seed=seed)
+        iters = max(1, int(budget_s / max(sec1, 1e-6)))
+        film, _, sec = O.render(seeds, c8, lf, b, iters, threads=cpu_threads, seed=seed)
+        ind = film / (iters * chains / float(W * H))
+        dctx = make(seed)  # the same direct pass as the GPU forms' films
+        direct = dctx.film(2) / float(dctx.opt.direct_spp)
+        dctx.close()
+        out["cpu"] = {"time_s": round(sec, 3), "steps": iters, "threads": cpu_threads, "kind": "port",
+                      "rmse": rmse(ind + direct, rfull), "rmse_indirect": rmse(ind, rind)}
+    return out
+
+
 def copy_calibration(mb):
     """One device-to-device hipMemcpy of `mb` MB (read mb, write mb)."""
     import ctypes as C
@@ -118,6 +205,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--tune-steps", type=int, default=3, help="timed steps per candidate schedule of the autotuner")
     ap.add_argument("--no-deriv", action="store_true", help="skip the derivative-kernel workload (config 3)")
+    ap.add_argument("--rmse-config", type=int, default=2, help="scene configuration of the equal-time RMSE study (0: none)")
+    ap.add_argument("--rmse-budget", type=float, default=2.0, help="seconds each form renders in the RMSE study")
+    ap.add_argument("--rmse-ref-steps", type=int, default=2000, help="batched steps of the RMSE study's reference render")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch schedule (profiled runs: no candidate schedules among the launches)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -207,6 +297,11 @@ def main():
                                     f"profiles/dr_step_config{args.config}_valu.md")
 
     ctx.close()
+    study = None
+    if args.rmse_config:
+        study = rmse_study(args.rmse_config, args.rmse_budget, args.rmse_ref_steps, sched,
+                           cpu_threads=0 if args.no_cpu else min(args.cpu_threads, os.cpu_count() or 1))
+        log(f"rmse study: {study}")
     deriv = None
     if not args.no_deriv:
         deriv = derivative_workload()
@@ -240,6 +335,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "derivatives": deriv,
+        "equal_time_rmse": study,
     }
     print(json.dumps(out), flush=True)
 
