@@ -109,20 +109,33 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
     chains = int(dpt["numchains"])
     W, H = int(sc["cam"][10]), int(sc["cam"][11])
 
-    def make(seed_, **kw):
+    def make(seed_, b_common=None, **kw):
         ctx = capi.Context(sc, capi.options_from_dpt(dpt, num_chains=chains, seed=seed_, **kw))
         ctx.set_schedule(**sched)
-        ctx.prepass()
+        b, lf = ctx.prepass()
+        if b_common is not None:
+            # the chains' own seed states, the reference's normalisation: the
+            # pre-pass estimate of b scales the whole DR image, and its
+            # sampling error would otherwise put a floor under every RMSE
+            st, c8 = ctx.read_chains()
+            ctx.seed_chains(st, c8, b_common, lf)
         ctx.direct_lighting()
-        return ctx
+        return ctx, b
 
     def images(ctx):
         full = ctx.film(0)
         return full, full - ctx.film(2) / float(ctx.opt.direct_spp)
 
     rmse = lambda a, b: float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)))
+    # relative MSE (per pixel and channel, against the reference squared plus 0.01)
+    relmse = lambda a, b: float(np.mean((a.astype(np.float64) - b) ** 2 / (b.astype(np.float64) ** 2 + 1e-2)))
     t0 = time.time()
-    ref = make(seed + 7777, direct_spp=256)
+    ref, b_ref = make(seed + 7777, direct_spp=256)
+    if config != 2 or not sched:
+        # a schedule for this workload, tuned on the reference's chains (its
+        # tuning steps only add to the reference render)
+        sched, _ = ref.autotune(steps=2)
+        log(f"rmse study: schedule {sched}")
     chunk = 50
     for k in range(0, ref_steps, chunk):
         ref.run(min(chunk, ref_steps - k), capi.BATCHED_GRAPH)
@@ -133,7 +146,7 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
     ref.close()
 
     def curve(mode, step_chunk):
-        ctx = make(seed)
+        ctx, _ = make(seed, b_common=b_ref)
         t, steps, pts = 0.0, 0, []
         t1 = time.time()
         while t < budget_s * 1000.0:
@@ -141,7 +154,8 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
             t += float(ctx.timing()[0])
             steps += step_chunk
             full, ind = images(ctx)
-            pts.append((t / 1000.0, steps, rmse(full, rfull), rmse(ind, rind)))
+            pts.append((t / 1000.0, steps, rmse(full, rfull), rmse(ind, rind), relmse(ind, rind),
+                        float(ind.mean() / max(rind.mean(), 1e-30))))
             if time.time() - t1 > 10:
                 log(f"rmse study: mode {mode} {t / 1000.0:.1f}/{budget_s} s")
                 t1 = time.time()
@@ -150,11 +164,22 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
 
     plain = curve(capi.PLAIN, 1)
     batched = curve(capi.BATCHED_GRAPH, 2)
+    # the batched form's time to the plain form's indirect RMSE at the budget:
+    # the first point from which its curve stays at or below it (an almost
+    # empty early film can lie below it before the estimate has settled)
     target = plain[-1][3]
-    reach = next((p for p in batched if p[3] <= target), None)
-    pick = lambda pts: {"time_s": round(pts[-1][0], 3), "steps": pts[-1][1], "rmse": pts[-1][2], "rmse_indirect": pts[-1][3]}
+    reach = None
+    for i in range(len(batched) - 1, -1, -1):
+        if batched[i][3] > target:
+            break
+        reach = batched[i]
+    pick = lambda pts: {"time_s": round(pts[-1][0], 3), "steps": pts[-1][1], "rmse": pts[-1][2], "rmse_indirect": pts[-1][3],
+                        "relmse_indirect": pts[-1][4], "mean_ratio_indirect": round(pts[-1][5], 4),
+                        "curve_s_rmse_indirect": [(round(p[0], 3), p[3]) for p in pts[:: max(1, len(pts) // 12)]]}
     out = {"workload": f"config{config}: {len(sc['tri_p'])} tris, {chains} chains, {W}x{H}",
-           "budget_s": budget_s, "reference": {"steps": ref_steps, "direct_spp": 256, "seed": "independent"},
+           "budget_s": budget_s, "schedule": sched,
+           "reference": {"steps": ref_steps, "direct_spp": 256, "seed": "independent", "b": b_ref,
+                         "note": "test renders keep their own seed states and take the reference's normalisation b"},
            "plain": pick(plain), "batched": pick(batched),
            "batched_time_to_plain_rmse_s": round(reach[0], 4) if reach else None,
            "batched_steps_to_plain_rmse": reach[1] if reach else None,
@@ -167,13 +192,14 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
         b, lf, seeds, c8, _ = O.prepass(int(dpt["numinitsamples"]), int(dpt["maxdepth"]), chains, seed=seed, threads=cpu_threads)
         _, _, sec1 = O.render(seeds, c8, lf, b, 1, threads=cpu_threads, seed=seed)
         iters = max(1, int(budget_s / max(sec1, 1e-6)))
-        film, _, sec = O.render(seeds, c8, lf, b, iters, threads=cpu_threads, seed=seed)
+        film, _, sec = O.render(seeds, c8, lf, b_ref, iters, threads=cpu_threads, seed=seed)
         ind = film / (iters * chains / float(W * H))
-        dctx = make(seed)  # the same direct pass as the GPU forms' films
+        dctx, _ = make(seed)  # the same direct pass as the GPU forms' films
         direct = dctx.film(2) / float(dctx.opt.direct_spp)
         dctx.close()
         out["cpu"] = {"time_s": round(sec, 3), "steps": iters, "threads": cpu_threads, "kind": "port",
-                      "rmse": rmse(ind + direct, rfull), "rmse_indirect": rmse(ind, rind)}
+                      "rmse": rmse(ind + direct, rfull), "rmse_indirect": rmse(ind, rind), "relmse_indirect": relmse(ind, rind),
+                      "mean_ratio_indirect": round(float(ind.mean() / max(rind.mean(), 1e-30)), 4)}
     return out
 
 

@@ -424,9 +424,13 @@ static int set_length_dist(drmlt_ctx *c, const float *lf, int n) {
 // Host-side mirror of the seed records into chain slot 0 (structure of arrays).
 extern "C" int drmlt_seed_chains(drmlt_ctx *c, const drmlt_path_rec *st, const float *c8, int64_t n, float b, const float *lf, int32_t nLen) {
     if (!c || !st || !c8 || !lf || n <= 0 || nLen <= 0 || nLen > kMaxBins) return fail(DRMLT_ERR_ARG, "bad argument");
-    if (!recs_ok(c, st, n) || !recs_fixed_ok(st, n)) return fail(DRMLT_ERR_ARG, "seed record out of range");
-    for (int64_t i = 0; i < n; i++)
+    // records flagged invalid (contrib8[0] == 0) seed an empty invalid state,
+    // which the first iteration replaces by a large step
+    for (int64_t i = 0; i < n; i++) {
+        if (c8[8 * i] == 0.0f) continue;
+        if (!recs_ok(c, st + i, 1) || !recs_fixed_ok(st + i, 1)) return fail(DRMLT_ERR_ARG, "seed record out of range");
         if (st[i].c + st[i].l - 1 > c->opt.max_depth) return fail(DRMLT_ERR_ARG, "seed path longer than max_depth");
+    }
     int r = alloc_chains(c, int(n));
     if (r) return r;
     Chains &ch = c->ch;
@@ -436,6 +440,8 @@ extern "C" int drmlt_seed_chains(drmlt_ctx *c, const drmlt_path_rec *st, const f
         ls(NS, 0), ss(NS, 0);
     std::vector<int> cp(16 * NS, -1), lp(16 * NS, -1);
     for (size_t i = 0; i < C; i++) {
+        perm[i] = 0 | (1 << 2) | (2 << 4);
+        if (c8[8 * i] == 0.0f) continue;  // empty (c = l = 0) and invalid
         const drmlt_path_rec &R = st[i];
         for (int k = 0; k < 4; k++) { ci[i + k * C] = R.c; li[i + k * C] = R.l; }
         lig[i] = R.light; lpr[i] = R.lprim;

@@ -156,4 +156,17 @@ def test_read_chains_matches_loop_scores(cfg1):
     # hold a score with the relative light parameterisation (the light area
     # factor); every other state reproduces exactly up to float rounding
     assert np.median(rel) < 1e-5 and np.mean(rel < 1e-5) > 0.6
+    # the read-back seeds another context (invalid states included: they
+    # seed empty states that the first iteration replaces by a large step)
+    allrecs, allc8 = ctx.read_chains()
+    b = 0.5
+    ctx2 = capi.Context(cfg1, opt)
+    ctx2.seed_chains(allrecs, allc8, b, np.ones(16, np.float32))
+    say("seed_chains from read_chains: 2 iterations")
+    ctx2.run(2, capi.BATCHED_GRAPH)
+    st = ctx2.stats()
+    assert st["large_step"] + st["stage1"] == 2 * len(allrecs)
+    back, back8 = ctx2.read_chains()
+    assert (back8[:, 0] > 0).mean() > 0.9
+    ctx2.close()
     ctx.close()
