@@ -376,6 +376,7 @@ extern "C" void orc_render(void *h, const double *opts, const PathRec *seeds, co
     dro.maxDervDepth = int(opts[7]);
     dro.acceptanceMap = opts[8] != 0;
     dro.chadMisQuirk = opts[9] != 0;
+    dro.fdStep = opts[10];
     Dist1D<float> ld;
     ld.build(lengthFunc, nLen);
     const int W = s->f.cam.W, H = s->f.cam.H;

@@ -69,6 +69,13 @@ struct DrOptions {
     double discreteStdDev = 0.01, roughThr = 0.03;
     int maxDervDepth = 9;
     bool lightCoord = true, acceptanceMap = false, chadMisQuirk = true;
+    // derivatives of log f for the H2MC Gaussians (the reference takes them
+    // from chad's exact AD): central differences of this step in primary-
+    // sample units, in FP64. At 1e-4 the Hessians of glossy (Phong,
+    // rough dielectric) vertices are off enough to lower the H2MC
+    // acceptance visibly; from 3e-6 down the acceptance no longer moves and
+    // equals the one exact derivatives give.
+    double fdStep = 2e-6;
 };
 
 struct DrStats {
@@ -156,7 +163,7 @@ template <class F> inline void InitGaussian(DrChainCtx<F> &cx, MState<F> &st) {
             PathS<double> pd = ToDouble(st.path);
             Opts<double> od = cx.od;
             od.chadMisQuirk = cx.dro.chadMisQuirk;
-            GradHessFD(*cx.sd, od, pd, D, 1e-4, g.data(), h.data());
+            GradHessFD(*cx.sd, od, pd, D, cx.dro.fdStep, g.data(), h.data());
             bool fin = true;
             for (double x : g) fin = fin && std::isfinite(x);
             for (double x : h) fin = fin && std::isfinite(x);

@@ -130,16 +130,19 @@ def test_schedules_agree_and_autotune(cfg2_small):
     ctx.close()
 
 
-def test_loop_vs_oracle(cfg1):
+@pytest.mark.parametrize("which", ["cfg1", "cfg2_small"])
+def test_loop_vs_oracle(which, request):
     """The DR loop (Green-Mira, H2MC second stage) against the oracle's loop
     from the oracle's own seeds: acceptance rates per stage and the film mean
-    agree within Monte Carlo error."""
+    agree within Monte Carlo error (cfg1: Lambertian room; cfg2_small: Phong
+    and rough dielectric surfaces)."""
+    cfg1 = request.getfixturevalue(which)
     O = orc.Scene(cfg1)
     maxd = int(cfg1["options"]["maxdepth"])
     chains, iters = 2048, 32
     b, lf, seeds, c8, _ = O.prepass(20000, maxd, chains, seed=13)
     ctx = _ctx(cfg1, num_chains=chains, seed=17)
-    say(f"loop cfg1 {chains} chains x {iters} iterations")
+    say(f"loop {which} {chains} chains x {iters} iterations")
     ctx.seed_chains(seeds, c8, b, lf)
     ctx.run(iters, 2)
     gfilm, gs = ctx.film(1), ctx.stats()

@@ -70,3 +70,27 @@ def test_stats_structure(small):
     assert iso + h2 == s1 - s1_acc
     assert h2 / (iso + h2) == pytest.approx(0.5, abs=0.03)
     assert 0 < s1_acc < s1 and 0 < h2_acc < h2 and 0 < iso_acc < iso
+
+
+def test_h2mc_derivative_step_is_converged(cfg2_small):
+    """The oracle's H2MC Gaussians take their derivatives from central
+    differences of its FP64 fixed-topology log f (the reference differentiates
+    exactly, with chad). On glossy and refractive vertices a step of 1e-4 is
+    too coarse: the H2MC acceptance rate moves measurably. At the oracle's
+    step (2e-6) halving it leaves the rate unchanged within Monte Carlo
+    error, while 1e-4 lies clearly away from it."""
+    O = orc.Scene(cfg2_small)
+    maxd = int(cfg2_small["options"]["maxdepth"])
+    b, lf, seeds, c8, _ = O.prepass(8000, maxd, 1024, seed=13, threads=8)
+    rate = {}
+    for h in (1e-4, 2e-6, 1e-6):
+        acc = tot = 0
+        for seed in (19, 23):
+            _, st, _ = O.render(seeds, c8, lf, b, 16, threads=8, seed=seed, opts={"fdstep": h})
+            acc += int(st[7]); tot += int(st[6])
+        rate[h] = (acc / tot, tot)
+    sd = lambda r, n: np.sqrt(r * (1 - r) / n)
+    (r2, n2), (r1, n1), (rc, nc) = rate[2e-6], rate[1e-6], rate[1e-4]
+    tol = 4 * np.hypot(sd(r2, n2), sd(r1, n1))
+    assert abs(r2 - r1) < tol, rate
+    assert r2 - rc > tol, rate
