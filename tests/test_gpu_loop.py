@@ -130,23 +130,26 @@ def test_schedules_agree_and_autotune(cfg2_small):
     ctx.close()
 
 
-@pytest.mark.parametrize("which", ["cfg1", "cfg2_small"])
-def test_loop_vs_oracle(which, request):
-    """The DR loop (Green-Mira, H2MC second stage) against the oracle's loop
-    from the oracle's own seeds: acceptance rates per stage and the film mean
-    agree within Monte Carlo error (cfg1: Lambertian room; cfg2_small: Phong
-    and rough dielectric surfaces)."""
+@pytest.mark.parametrize("which,kind", [("cfg1", "green"), ("cfg2_small", "green"), ("cfg1", "mira"), ("cfg2_small", "mira")])
+def test_loop_vs_oracle(which, kind, request):
+    """The DR loop (H2MC second stage; Green-Mira or Tierney-Mira acceptance)
+    against the oracle's loop from the oracle's own seeds: acceptance rates per
+    stage and the film mean agree within Monte Carlo error (cfg1: Lambertian
+    room; cfg2_small: Phong and rough dielectric surfaces)."""
     cfg1 = request.getfixturevalue(which)
     O = orc.Scene(cfg1)
     maxd = int(cfg1["options"]["maxdepth"])
     chains, iters = 2048, 32
     b, lf, seeds, c8, _ = O.prepass(20000, maxd, chains, seed=13)
-    ctx = _ctx(cfg1, num_chains=chains, seed=17)
-    say(f"loop {which} {chains} chains x {iters} iterations")
+    from drmlt import capi
+
+    opt = capi.options_from_dpt(dict(cfg1["options"], type=kind), num_chains=chains, seed=17)
+    ctx = capi.Context(cfg1, opt)
+    say(f"loop {which} {kind} {chains} chains x {iters} iterations")
     ctx.seed_chains(seeds, c8, b, lf)
     ctx.run(iters, 2)
     gfilm, gs = ctx.film(1), ctx.stats()
-    ofilm, ost, _ = O.render(seeds, c8, lf, b, iters, seed=19)
+    ofilm, ost, _ = O.render(seeds, c8, lf, b, iters, seed=19, opts={"type": 1 if kind == "mira" else 0})
     names = ["large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso", "stage2_iso_accepted",
              "stage2_h2mc", "stage2_h2mc_accepted"]
     os_ = dict(zip(names, [int(v) for v in ost]))
