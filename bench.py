@@ -41,7 +41,8 @@ def cpu_baseline(sc, threads, chains, iters, seed=11):
     maxd = int(sc["options"]["maxdepth"])
     b, lf, seeds, c8, _ = s.prepass(20000, maxd, chains, seed=seed, threads=threads)
     _, stats, sec = s.render(seeds, c8, lf, b, iters, threads=threads, seed=seed)
-    return chains * iters / sec, sec
+    acc = int(stats[1]) + int(stats[3]) + int(stats[5]) + int(stats[7])
+    return chains * iters / sec, sec, acc / sec
 
 
 def derivative_workload(n=262144, seed=4321):
@@ -203,6 +204,12 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
     return out
 
 
+def accepted(st):
+    """Accepted mutations in a stats dict: accept.first or accept.second of an
+    iteration (delayed.cpp:565), counted per stage."""
+    return st["large_step_accepted"] + st["stage1_accepted"] + st["stage2_iso_accepted"] + st["stage2_h2mc_accepted"]
+
+
 def copy_calibration(mb):
     """One device-to-device hipMemcpy of `mb` MB (read mb, write mb)."""
     import ctypes as C
@@ -263,8 +270,10 @@ def main():
 
     # plain one-kernel-per-step form, same chains
     ctx.run(max(1, args.plain_steps // 2), capi.PLAIN)
+    s0 = ctx.stats()
     ctx.run(args.plain_steps, capi.PLAIN)
     plain_ms = ctx.timing()[0] / args.plain_steps
+    plain_acc = accepted(ctx.stats()) - accepted(s0)
     log(f"plain form {plain_ms:.2f} ms/step")
 
     # launch schedule of the batched form chosen by the autotuner on these chains
@@ -286,11 +295,13 @@ def main():
 
     # batched + graph: the measured form
     ctx.run(args.warmup, capi.BATCHED_GRAPH)
+    s0 = ctx.stats()
     ctx.run(args.steps, capi.BATCHED_GRAPH)
     total_ms = ctx.timing()[0]
     ms_per_step = total_ms / args.steps
     value = chains * args.steps / (total_ms / 1000.0)
     st = ctx.stats()
+    batched_acc = accepted(st) - accepted(s0)
     log(f"batched+graph {ms_per_step:.3f} ms/step, {value:.4g} mutations/s; stats {st}")
 
     # roofline of the dominant stage kernel
@@ -338,8 +349,9 @@ def main():
         ncpu = min(args.cpu_threads, os.cpu_count() or 1)
         cchains, citers = (16384, 512) if args.config != 1 else (1024, 2048)
         log(f"cpu baseline: the oracle loop, {cchains} chains x {citers} iterations on {ncpu} threads")
-        cval, csec = cpu_baseline(sc, ncpu, cchains, citers)
+        cval, csec, cacc = cpu_baseline(sc, ncpu, cchains, citers)
         cpu = {"value": round(cval, 1), "unit": "mutations/s", "cores": ncpu, "kind": "port",
+               "accepted_mutations_per_s": round(cacc, 1),
                "sample": f"{cchains} chains x {citers} iterations of the same scene after a 20000-sample pre-pass "
                          f"({csec:.1f} s of chain loop)"}
         log(f"cpu baseline {cval:.4g} mutations/s on {ncpu} threads")
@@ -352,7 +364,9 @@ def main():
         "config": {"workload": f"config{args.config}: {len(sc['tri_p'])} tris, {chains} chains, maxdepth {dpt['maxdepth']}, "
                                f"{int(sc['cam'][10])}x{int(sc['cam'][11])} film, type={dpt['type']}",
                    "chains": chains, "form": "batched+graph", "schedule": sched},
-        "plain_form": {"ms_per_step": round(plain_ms, 3), "mutations_per_s": round(chains / (plain_ms / 1000.0), 1)},
+        "accepted_mutations_per_s": round(batched_acc / (total_ms / 1000.0), 1),
+        "plain_form": {"ms_per_step": round(plain_ms, 3), "mutations_per_s": round(chains / (plain_ms / 1000.0), 1),
+                       "accepted_mutations_per_s": round(plain_acc / (args.plain_steps * plain_ms / 1000.0), 1)},
         # the forms advance identical chains (identical acceptance statistics, tests/test_gpu_loop.py), so
         # equal RMSE is reached after equal iteration counts and the time ratio is the step-time ratio
         "equal_rmse_time_ratio_plain_over_batched": round(plain_ms / ms_per_step, 3),

@@ -12,6 +12,13 @@ namespace drmlt {
 #ifndef DRMLT_STAGE_WAVES
 #define DRMLT_STAGE_WAVES 2
 #endif
+// The same for the Gaussian and derivative kernels.
+#ifndef DRMLT_GAUSS_WAVES
+#define DRMLT_GAUSS_WAVES 1
+#endif
+#ifndef DRMLT_DERIV_WAVES
+#define DRMLT_DERIV_WAVES 1
+#endif
 
 #ifdef DRMLT_COUNT_WORK
 #define DRMLT_STAGE(name) name##_counted
@@ -98,7 +105,7 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)
 // kind 0 (which 0): every H2MC chain; the current state's Gaussian unless it
 // is cached, then the proposal offset. kind 2 (which 1): the proposal's
 // Gaussian, then (finishRatio) the H2MC ratio.
-__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which,
+__global__ void __launch_bounds__(64, DRMLT_GAUSS_WAVES) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which,
                                                              int finishRatio, const uint32_t *iterp) {
     __shared__ GWork w;
     const WaveExec x{int(threadIdx.x)};
@@ -178,7 +185,7 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
 
 // Derivative passes over list `kind` (slot: 0 current, 2 second), persistent
 // over (chain, pass) work items bin by bin.
-__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_deriv)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which) {
+__global__ void __launch_bounds__(64, DRMLT_DERIV_WAVES) DRMLT_STAGE(k_b_deriv)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which) {
     // bin b holds paths of length b (D = 2 max(b, 2)); its work items are
     // (chain, pass) pairs, all bins concatenated
     long total = 0;
