@@ -162,136 +162,168 @@ DEV int i4at(const int4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2
 #endif
 constexpr int kTriBatch = DRMLT_TRI_BATCH;
 
+// State of one ray's traversal between node visits.
+struct TraceState {
+    float o[3], d[3], inv[3];
+    float tmin, tmax, tbest;
+    int best, sp;
+    bool any, done;
+    Node4 N;  // the node the next step visits (its fetch already issued)
+    DRMLT_COUNT(unsigned nvis; unsigned ntri;)
+};
+DEV void trace_begin(const BvhRef &S, TraceState &t, float ox, float oy, float oz, float dx, float dy, float dz, float tmin, float tmax,
+                     bool any) {
+    t.o[0] = ox; t.o[1] = oy; t.o[2] = oz;
+    t.d[0] = dx; t.d[1] = dy; t.d[2] = dz;
+    t.inv[0] = 1.0f / dx; t.inv[1] = 1.0f / dy; t.inv[2] = 1.0f / dz;
+    t.tmin = tmin; t.tmax = tmax; t.tbest = tmax;
+    t.best = -1; t.sp = 0;
+    t.any = any; t.done = false;
+    DRMLT_COUNT(t.nvis = 0; t.ntri = 0;)
+    t.N = load_node4(S, 0);
+}
+
 // One traversal for both query kinds, so that lanes asking for a closest hit
 // and lanes asking for visibility share the loop.
 // any = false: closest hit in (tmin, tmax); ties in t go to the smaller
-// scene-order index. Returns the scene-order triangle index or -1 and the hit
-// distance in tbest.
-// any = true: returns a triangle with a hit in (tmin, tmax) (the first found)
-// or -1; the ray interval is not narrowed, so the answer is the same boolean
-// as a search of every triangle.
-// Per node: the four child boxes are tested, the next node to visit is chosen
-// (the nearest internal child in front of the best hit so far, the others
-// pushed farthest first, else the top of the stack) and its fetch issued, and
-// only then the node's leaves are tested, so the two memory latencies
-// overlap. A child chosen before the leaves shortened the ray may turn out to
-// lie behind the best hit: visiting it costs one fetch, its own box tests
-// cull it.
-DEV int trace_impl(const BvhRef &S, const float o[3], const float d[3], float tmin, float tmax, bool any, float &tbest) {
-    const float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+// scene-order index; the result is t.best (scene-order triangle index or -1)
+// with the hit distance t.tbest.
+// any = true: t.best is a triangle with a hit in (tmin, tmax) (the first
+// found) or -1; the ray interval is not narrowed, so the answer is the same
+// boolean as a search of every triangle.
+// One step visits one node: the four child boxes are tested, the next node to
+// visit is chosen (the nearest internal child in front of the best hit so
+// far, the others pushed farthest first, else the top of the stack) and its
+// fetch issued, and only then the node's leaves are tested, so the two memory
+// latencies overlap. A child chosen before the leaves shortened the ray may
+// turn out to lie behind the best hit: visiting it costs one fetch, its own
+// box tests cull it. t.done is set when the traversal has ended.
+DEV void trace_step(const BvhRef &S, TraceState &t) {
+    const float *o = t.o, *d = t.d, *inv = t.inv;
+    const float tmin = t.tmin, tmax = t.tmax;
+    const bool any = t.any;
     int *stack = g_trace_stack + threadIdx.x;  // entry k at stack[k * kTraceLanes]
-    int sp = 0;
-    int best = -1;
-    tbest = tmax;
-    DRMLT_COUNT(unsigned nvis = 0; unsigned ntri = 0;)
-    Node4 N = load_node4(S, 0);
-    for (;;) {
-        DRMLT_COUNT(nvis++;)
-        float tn[4];
-        bool hit[4];
+    int sp = t.sp, best = t.best;
+    float tbest = t.tbest;
+    const Node4 &N = t.N;
+    DRMLT_COUNT(t.nvis++;)
+    float tn[4];
+    bool hit[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            hit[k] = i4at(N.count, k) >= 0 &&
-                     slab(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tbest, tn[k]);
-        }
-        // internal children in front of the best hit, nearest first: a
-        // sorting network on (entry distance, child) in registers (an
-        // insertion sort over arrays would index them dynamically and put
-        // them in scratch memory); other slots get a key above every
-        // distance and sort last
-        const float kNone = __int_as_float(0x7f800000);
-        float k0 = (hit[0] && N.count.x == 0 && tn[0] <= tbest) ? fminf(tn[0], 3.4e38f) : kNone;
-        float k1 = (hit[1] && N.count.y == 0 && tn[1] <= tbest) ? fminf(tn[1], 3.4e38f) : kNone;
-        float k2 = (hit[2] && N.count.z == 0 && tn[2] <= tbest) ? fminf(tn[2], 3.4e38f) : kNone;
-        float k3 = (hit[3] && N.count.w == 0 && tn[3] <= tbest) ? fminf(tn[3], 3.4e38f) : kNone;
-        int c0 = N.child.x, c1 = N.child.y, c2 = N.child.z, c3 = N.child.w;
-        const int nh = int(k0 != kNone) + int(k1 != kNone) + int(k2 != kNone) + int(k3 != kNone);
+    for (int k = 0; k < 4; k++) {
+        hit[k] = i4at(N.count, k) >= 0 &&
+                 slab(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tbest, tn[k]);
+    }
+    // internal children in front of the best hit, nearest first: a
+    // sorting network on (entry distance, child) in registers (an
+    // insertion sort over arrays would index them dynamically and put
+    // them in scratch memory); other slots get a key above every
+    // distance and sort last
+    const float kNone = __int_as_float(0x7f800000);
+    float k0 = (hit[0] && N.count.x == 0 && tn[0] <= tbest) ? fminf(tn[0], 3.4e38f) : kNone;
+    float k1 = (hit[1] && N.count.y == 0 && tn[1] <= tbest) ? fminf(tn[1], 3.4e38f) : kNone;
+    float k2 = (hit[2] && N.count.z == 0 && tn[2] <= tbest) ? fminf(tn[2], 3.4e38f) : kNone;
+    float k3 = (hit[3] && N.count.w == 0 && tn[3] <= tbest) ? fminf(tn[3], 3.4e38f) : kNone;
+    int c0 = N.child.x, c1 = N.child.y, c2 = N.child.z, c3 = N.child.w;
+    const int nh = int(k0 != kNone) + int(k1 != kNone) + int(k2 != kNone) + int(k3 != kNone);
 #define DRMLT_CSWAP(ka, ca, kb, cb)                          \
-    {                                                        \
-        const bool sw = kb < ka;                             \
-        const float tk = sw ? kb : ka;                       \
-        const int tc = sw ? cb : ca;                         \
-        kb = sw ? ka : kb; cb = sw ? ca : cb;                \
-        ka = tk; ca = tc;                                    \
-    }
-        DRMLT_CSWAP(k0, c0, k1, c1)
-        DRMLT_CSWAP(k2, c2, k3, c3)
-        DRMLT_CSWAP(k0, c0, k2, c2)
-        DRMLT_CSWAP(k1, c1, k3, c3)
-        DRMLT_CSWAP(k1, c1, k2, c2)
+{                                                        \
+    const bool sw = kb < ka;                             \
+    const float tk = sw ? kb : ka;                       \
+    const int tc = sw ? cb : ca;                         \
+    kb = sw ? ka : kb; cb = sw ? ca : cb;                \
+    ka = tk; ca = tc;                                    \
+}
+    DRMLT_CSWAP(k0, c0, k1, c1)
+    DRMLT_CSWAP(k2, c2, k3, c3)
+    DRMLT_CSWAP(k0, c0, k2, c2)
+    DRMLT_CSWAP(k1, c1, k3, c3)
+    DRMLT_CSWAP(k1, c1, k2, c2)
 #undef DRMLT_CSWAP
-        int next = -1;
-        if (nh > 0) {
-            // farthest first onto the stack, visit the nearest
-            if (nh > 3) stack[kTraceLanes * sp++] = c3;
-            if (nh > 2) stack[kTraceLanes * sp++] = c2;
-            if (nh > 1) stack[kTraceLanes * sp++] = c1;
-            next = c0;
-        } else if (sp > 0) {
-            next = stack[kTraceLanes * --sp];
-        }
-        // the hit leaves of the node as a queue of up to four triangle
-        // ranges in registers (named, not an array, so nothing is indexed
-        // dynamically)
-        int qf0 = 0, qe0 = 0, qf1 = 0, qe1 = 0, qf2 = 0, qe2 = 0, qf3 = 0, qe3 = 0;
+    int next = -1;
+    if (nh > 0) {
+        // farthest first onto the stack, visit the nearest
+        if (nh > 3) stack[kTraceLanes * sp++] = c3;
+        if (nh > 2) stack[kTraceLanes * sp++] = c2;
+        if (nh > 1) stack[kTraceLanes * sp++] = c1;
+        next = c0;
+    } else if (sp > 0) {
+        next = stack[kTraceLanes * --sp];
+    }
+    // the hit leaves of the node as a queue of up to four triangle
+    // ranges in registers (named, not an array, so nothing is indexed
+    // dynamically)
+    int qf0 = 0, qe0 = 0, qf1 = 0, qe1 = 0, qf2 = 0, qe2 = 0, qf3 = 0, qe3 = 0;
 #define DRMLT_ENQ(k, cnt, first)                                                                        \
-    if (hit[k] && (cnt) > 0) {                                                                          \
-        DRMLT_COUNT(ntri += (cnt);)                                                                     \
-        if (qe0 == 0) { qf0 = (first); qe0 = (first) + (cnt); }                                         \
-        else if (qe1 == 0) { qf1 = (first); qe1 = (first) + (cnt); }                                    \
-        else if (qe2 == 0) { qf2 = (first); qe2 = (first) + (cnt); }                                    \
-        else { qf3 = (first); qe3 = (first) + (cnt); }                                                  \
-    }
-        DRMLT_ENQ(0, N.count.x, N.child.x)
-        DRMLT_ENQ(1, N.count.y, N.child.y)
-        DRMLT_ENQ(2, N.count.z, N.child.z)
-        DRMLT_ENQ(3, N.count.w, N.child.w)
+if (hit[k] && (cnt) > 0) {                                                                          \
+    DRMLT_COUNT(t.ntri += (cnt);)                                                                     \
+    if (qe0 == 0) { qf0 = (first); qe0 = (first) + (cnt); }                                         \
+    else if (qe1 == 0) { qf1 = (first); qe1 = (first) + (cnt); }                                    \
+    else if (qe2 == 0) { qf2 = (first); qe2 = (first) + (cnt); }                                    \
+    else { qf3 = (first); qe3 = (first) + (cnt); }                                                  \
+}
+    DRMLT_ENQ(0, N.count.x, N.child.x)
+    DRMLT_ENQ(1, N.count.y, N.child.y)
+    DRMLT_ENQ(2, N.count.z, N.child.z)
+    DRMLT_ENQ(3, N.count.w, N.child.w)
 #undef DRMLT_ENQ
-        // the next node's fetch and the first triangle batch go out together
-        Node4 NN;
-        if (next >= 0) NN = load_node4(S, next);
-        float4 A[kTriBatch], B[kTriBatch], Cc[kTriBatch];
-        int t0 = qf0, end = qe0;
-        bool have = end > 0;
+    // the next node's fetch and the first triangle batch go out together
+    Node4 NN;
+    if (next >= 0) NN = load_node4(S, next);
+    float4 A[kTriBatch], B[kTriBatch], Cc[kTriBatch];
+    int t0 = qf0, end = qe0;
+    bool have = end > 0;
 #define DRMLT_LOAD_BATCH()                                                                              \
-    _Pragma("unroll") for (int j = 0; j < kTriBatch; j++) {                                             \
-        const int t = min(t0 + j, end - 1); /* past the end: the last triangle again, not tested */     \
-        A[j] = gload(S.tri, 3 * size_t(t));                                                             \
-        B[j] = gload(S.tri, 3 * size_t(t) + 1);                                                         \
-        Cc[j] = gload(S.tri, 3 * size_t(t) + 2);                                                        \
-    }
-        if (have) { DRMLT_LOAD_BATCH() }
-        bool found = false;
-        while (have) {
+_Pragma("unroll") for (int j = 0; j < kTriBatch; j++) {                                             \
+    const int t = min(t0 + j, end - 1); /* past the end: the last triangle again, not tested */     \
+    A[j] = gload(S.tri, 3 * size_t(t));                                                             \
+    B[j] = gload(S.tri, 3 * size_t(t) + 1);                                                         \
+    Cc[j] = gload(S.tri, 3 * size_t(t) + 2);                                                        \
+}
+    if (have) { DRMLT_LOAD_BATCH() }
+    bool found = false;
+    while (have) {
 #pragma unroll
-            for (int j = 0; j < kTriBatch; j++) {
-                float th;
-                if (!found && t0 + j < end && tri_hit(A[j], B[j], Cc[j], o, d, tmin, (any || best < 0) ? tmax : next_up(tbest), th)) {
-                    const int id = __float_as_int(A[j].w);
-                    if (any) {
-                        best = id;
-                        found = true;
-                    } else if (th < tbest || (th == tbest && (best < 0 || id < best))) {
-                        tbest = th;
-                        best = id;
-                    }
+        for (int j = 0; j < kTriBatch; j++) {
+            float th;
+            if (!found && t0 + j < end && tri_hit(A[j], B[j], Cc[j], o, d, tmin, (any || best < 0) ? tmax : next_up(tbest), th)) {
+                const int id = __float_as_int(A[j].w);
+                if (any) {
+                    best = id;
+                    found = true;
+                } else if (th < tbest || (th == tbest && (best < 0 || id < best))) {
+                    tbest = th;
+                    best = id;
                 }
             }
-            // the next batch: further in this leaf, else the next leaf of the queue
-            t0 += kTriBatch;
-            if (t0 >= end) {
-                t0 = qf1; end = qe1;
-                qf1 = qf2; qe1 = qe2; qf2 = qf3; qe2 = qe3; qe3 = 0;
-            }
-            have = !found && end > 0;
-            if (have) { DRMLT_LOAD_BATCH() }
         }
-#undef DRMLT_LOAD_BATCH
-        if (found || next < 0) break;
-        N = NN;
+        // the next batch: further in this leaf, else the next leaf of the queue
+        t0 += kTriBatch;
+        if (t0 >= end) {
+            t0 = qf1; end = qe1;
+            qf1 = qf2; qe1 = qe2; qf2 = qf3; qe2 = qe3; qe3 = 0;
+        }
+        have = !found && end > 0;
+        if (have) { DRMLT_LOAD_BATCH() }
     }
-    DRMLT_COUNT(count_work(S, nvis, ntri);)
-    return best;
+#undef DRMLT_LOAD_BATCH
+    t.sp = sp;
+    t.best = best;
+    t.tbest = tbest;
+    if (found || next < 0) {
+        t.done = true;
+        DRMLT_COUNT(count_work(S, t.nvis, t.ntri);)
+        return;
+    }
+    t.N = NN;
+}
+
+DEV int trace_impl(const BvhRef &S, const float o[3], const float d[3], float tmin, float tmax, bool any, float &tbest) {
+    TraceState t;
+    trace_begin(S, t, o[0], o[1], o[2], d[0], d[1], d[2], tmin, tmax, any);
+    while (!t.done) trace_step(S, t);
+    tbest = t.tbest;
+    return t.best;
 }
 
 // The traversal is emitted once per kernel as a real call (not inlined at
