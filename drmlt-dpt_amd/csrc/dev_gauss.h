@@ -10,8 +10,9 @@
 //
 // Every routine is written once over an executor X that names the lanes
 // working on one Gaussian: SeqExec (one thread does everything, the plain
-// form) or WaveExec (the 64 lanes of a one-wave workgroup share the work, the
-// matrices in LDS). Work is split by index, every element is computed by the
+// form), WaveExec (the 64 lanes of a one-wave workgroup share the work, the
+// matrices in LDS) or GroupExec (a group of lanes of one wave, several
+// Gaussians per wave). Work is split by index, every element is computed by the
 // same expression in the same order in both, and reductions run in index
 // order on one lane, so both executors produce bit-identical results.
 // Floating-point contraction is off in these routines for the same reason.
@@ -26,7 +27,6 @@
 namespace drmlt {
 
 constexpr int kGMax = 24;          // largest D with derivatives (path length 12)
-constexpr int kGS = kGMax + 1;     // work-matrix row stride (odd, so rows fall on different LDS banks)
 constexpr int kSlotP = kGMax * kGMax, kSlotM = kSlotP + kGMax, kSlotLD = kSlotM + kGMax;
 constexpr int kGSlot = 640;        // doubles per Gaussian slot (V, p, mean, logDet, padding to 64 B)
 
@@ -63,15 +63,35 @@ struct WaveExec {
     static constexpr int n = 64;
     DEV void sync() const { __syncthreads(); }
 };
+// G consecutive lanes of a one-wave workgroup; the groups of a wave work on
+// different Gaussians and may diverge, so sync() is no barrier: the fences
+// complete this lane's memory operations and keep the compiler from moving
+// memory accesses across it, which is all lanes of one wave need to see
+// each other's writes.
+template <int G> struct GroupExec {
+    int lane;  // lane within the group
+    static constexpr int n = G;
+    DEV void sync() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+};
 
-// Scratch of one Gaussian computation (private for SeqExec, LDS for WaveExec).
-struct GWork {
-    double A[kGMax * kGS], V[kGMax * kGS];
-    double vec[kGMax], vec2[kGMax], part[kGMax];
-    double rc[kGMax / 2], rs[kGMax / 2];
-    int rp[kGMax / 2], rq[kGMax / 2];
+// Scratch of one Gaussian computation of dimension up to DM (private for
+// SeqExec, LDS otherwise). Work matrices have row stride kS = DM + 1 (odd for
+// even DM, so rows fall on different LDS banks); the stride changes no
+// arithmetic, so every size gives the same results.
+template <int DM> struct GWorkT {
+    static constexpr int kS = DM + 1;
+    static constexpr int kMaxD = DM;
+    double A[DM * kS], V[DM * kS];
+    double vec[DM], vec2[DM], part[DM];
+    double rc[DM / 2], rs[DM / 2];
+    int rp[DM / 2], rq[DM / 2];
     int flag;
 };
+using GWork = GWorkT<kGMax>;
 
 // Pair k of round r of the round-robin schedule over D (even) indices:
 // position 0 stays, positions 1..D-1 rotate.
@@ -97,11 +117,12 @@ template <class X> DEV void gauss_iso(const X &x, int D, double sigma, const GSl
     }
 }
 
-// Eigen-decomposition of w.A (D x D symmetric, stride kGS) into w.V; the
+// Eigen-decomposition of w.A (D x D symmetric, row stride W::kS) into w.V; the
 // eigenvalues end on the diagonal of w.A. Converged when the off-diagonal
 // mass is below 1e-30 of the diagonal mass.
-template <class X> DEV void gauss_eigen(const X &x, int D, GWork &w) {
+template <class X, class W> DEV void gauss_eigen(const X &x, int D, W &w) {
 #pragma clang fp contract(off)
+    constexpr int kGS = W::kS;
     for (int t = x.lane; t < D * D; t += x.n) w.V[(t / D) * kGS + t % D] = (t / D == t % D) ? 1.0 : 0.0;
     const int half = D / 2;
     for (int sweep = 0; sweep < 60; sweep++) {
@@ -163,9 +184,10 @@ template <class X> DEV void gauss_eigen(const X &x, int D, GWork &w) {
 // drsmallstep.cpp:219-266 + 18-93. grad(i) and hess(i, j) for i <= j (the
 // upper triangle of the row-major Hessian; Eigen reads the lower triangle of
 // the column-major view of the same buffer). Writes the Gaussian to g.
-template <class X, class GP, class HP>
-DEV void gauss_compute(const X &x, const H2Param &pr, double sc, int D, GP grad, HP hess, GWork &w, const GSlot &g) {
+template <class X, class GP, class HP, class W>
+DEV void gauss_compute(const X &x, const H2Param &pr, double sc, int D, GP grad, HP hess, W &w, const GSlot &g) {
 #pragma clang fp contract(off)
+    constexpr int kGS = W::kS;
     for (int t = x.lane; t < D * D; t += x.n) {
         const int i = t / D, j = t % D;
         w.A[i * kGS + j] = i <= j ? hess(i, j) : hess(j, i);
@@ -229,8 +251,8 @@ DEV void gauss_compute(const X &x, const H2Param &pr, double sc, int D, GP grad,
 
 // GenerateSample (gaussian.cpp:36-47): x = mean + covL n with
 // covL = V diag(p^-1/2); nrm(j) supplies the j-th standard normal.
-template <class X, class NP, class OW>
-DEV void gauss_sample(const X &x, int D, const GSlot &g, NP nrm, GWork &w, OW out) {
+template <class X, class NP, class W, class OW>
+DEV void gauss_sample(const X &x, int D, const GSlot &g, NP nrm, W &w, OW out) {
 #pragma clang fp contract(off)
     for (int j = x.lane; j < D; j += x.n) w.vec[j] = nrm(j) / sqrt(g.p(j));
     x.sync();
@@ -244,8 +266,8 @@ DEV void gauss_sample(const X &x, int D, const GSlot &g, NP nrm, GWork &w, OW ou
 
 // GaussianLogPdf (gaussian.cpp:18-26) of sign * xv(i):
 // -D/2 log 2pi + logDet/2 - 1/2 sum_k p_k (V^T (x - mean))_k^2.
-template <class X, class XP>
-DEV double gauss_logpdf(const X &x, int D, XP xv, double sign, const GSlot &g, GWork &w) {
+template <class X, class XP, class W>
+DEV double gauss_logpdf(const X &x, int D, XP xv, double sign, const GSlot &g, W &w) {
 #pragma clang fp contract(off)
     for (int i = x.lane; i < D; i += x.n) w.vec[i] = sign * xv(i) - g.m(i);
     x.sync();

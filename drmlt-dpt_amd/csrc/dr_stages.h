@@ -163,12 +163,12 @@ DEV void derivs_thread(const DevScene &Sc, const Chains &ch, const DrParams &P, 
 // buffers of `chain`: isotropic without derivatives (path too long, f <= 1e-15
 // or a non-finite gradient / Hessian, drsmallstep.cpp:135-163), else
 // ComputeGaussian. Marks the slot's Gaussian as initialised.
-template <class X> DEV void gaussian_from_derivs(const X &x, const Chains &ch, const DrParams &P, int chain, int p, GWork &w) {
+template <class X, class W> DEV void gaussian_from_derivs(const X &x, const Chains &ch, const DrParams &P, int chain, int p, W &w) {
     const int c = ch.c[p], l = ch.l[p], D = dim_of(c, l);
     const GSlot G = g_ref(ch, p);
     const double ssv = ch.ss[p];
     const float *gr = ch.grad + size_t(chain) * kGMax, *hs = ch.hess + size_t(chain) * kGMax * kGMax;
-    bool derivs = has_derivative(c, l, P.maxDervDepth) && D <= kGMax;
+    bool derivs = has_derivative(c, l, P.maxDervDepth) && D <= kGMax && D <= W::kMaxD;
     if (derivs) {
         if (x.lane == 0) w.flag = ssv > 1e-15 ? 1 : 0;
         x.sync();
@@ -196,7 +196,7 @@ template <class X> DEV void gaussian_from_derivs(const X &x, const Chains &ch, c
 // H2MC proposal offset, drawn from the current state's Gaussian with the
 // stage-2 stream's first D normals (drsmallstep.cpp:171-179,
 // gaussian.cpp:36-47); dr_stage2_propose skips those normals.
-template <class X> DEV void h2mc_offset(const X &x, const Chains &ch, const DrParams &P, int chain, uint32_t iter, GWork &w) {
+template <class X, class W> DEV void h2mc_offset(const X &x, const Chains &ch, const DrParams &P, int chain, uint32_t iter, W &w) {
     const int C = ch.C;
     const int pc = chain + C * perm_cur(ch.perm[chain]);
     const int D = dim_of(ch.c[pc], ch.l[pc]);
@@ -207,7 +207,7 @@ template <class X> DEV void h2mc_offset(const X &x, const Chains &ch, const DrPa
 
 // H2MC acceptance before the delayed-rejection correction
 // (drsmallstep.cpp:181-196): N_z(-off) / N_x(off) * f_z / f_x, unclamped.
-template <class X> DEV void h2mc_ratio(const X &x, const Chains &ch, int chain, GWork &w) {
+template <class X, class W> DEV void h2mc_ratio(const X &x, const Chains &ch, int chain, W &w) {
     const int C = ch.C;
     const int perm = ch.perm[chain];
     const int pc = chain + C * perm_cur(perm), ps = chain + C * perm_second(perm);

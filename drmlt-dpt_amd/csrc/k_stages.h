@@ -101,35 +101,64 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)
     }
 }
 
-// One wave per Gaussian (one-wave workgroups, the work matrices in LDS).
-// kind 0 (which 0): every H2MC chain; the current state's Gaussian unless it
-// is cached, then the proposal offset. kind 2 (which 1): the proposal's
-// Gaussian, then (finishRatio) the H2MC ratio.
+// Gaussian work of one list item (a chain). kind 0 (which 0): the current
+// state's Gaussian unless it is cached, then the proposal offset. kind 2
+// (which 1): the proposal's Gaussian, then (finishRatio) the H2MC ratio.
+template <class X, class W>
+DEV void gauss_item(const X &x, const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, int which, int finishRatio, uint32_t iter,
+                    W &w) {
+    const int perm = ch.perm[chain];
+    const int p = chain + ch.C * (which == 0 ? perm_cur(perm) : perm_second(perm));
+    const int D = dim_of(ch.c[p], ch.l[p]);
+    if (D > W::kMaxD) return;  // not reached: items are routed by their path-length bin
+    const bool cached = which == 0 && ch.gInit[p] != 0;
+    x.sync();
+    if (!cached) {
+        gaussian_from_derivs(x, ch, P, chain, p, w);
+        // gradient and upper Hessian read (float); V, p, mean, logDet written (double)
+        if (x.lane == 0) count_bytes(Sc, (unsigned long long)(4 * (D + D * (D + 1) / 2) + 8 * (D * D + 2 * D + 1)));
+    }
+    if (which == 0) h2mc_offset(x, ch, P, chain, iter, w);
+    if (finishRatio) h2mc_ratio(x, ch, chain, w);
+}
+
+// Bins 0..kGSmallBins-1 of a list hold paths of length <= 4 (D = 2 max(b, 2)
+// <= kGSmallD).
+constexpr int kGSmallD = 8;
+constexpr int kGSmallBins = kGSmallD / 2 + 1;
+DEV int list_prefix(const Lists &L, int kind, int nbins) {
+    int n = 0;
+    for (int b = 0; b < nbins; b++) n += L.count[kind * kMaxBins + b];
+    return n;
+}
+#ifndef DRMLT_GAUSS_GROUP
+#define DRMLT_GAUSS_GROUP 8  // lanes per small Gaussian (measured: 8 beats 16 and 32)
+#endif
+
+// One-wave workgroups over a Gaussian list: first the small Gaussians
+// (D <= 8), 64 / DRMLT_GAUSS_GROUP per wave, a group of lanes and an LDS work
+// area each (a whole wave would leave most lanes idle on such matrices);
+// then the larger ones, one wave each. Every executor gives the same results.
 __global__ void __launch_bounds__(64, DRMLT_GAUSS_WAVES) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which,
                                                              int finishRatio, const uint32_t *iterp) {
-    __shared__ GWork w;
-    const WaveExec x{int(threadIdx.x)};
+    constexpr int G = DRMLT_GAUSS_GROUP, NG = 64 / G;
+    union Work {
+        GWork big;
+        GWorkT<kGSmallD> small[NG];
+    };
+    __shared__ Work w;
     const uint32_t iter = *iterp;
-    const int n = list_length(L, kind);
+    const int n = list_length(L, kind), nSmall = list_prefix(L, kind, kGSmallBins);
     {
-        for (int it = blockIdx.x; it < n; it += gridDim.x) {
-            const int chain = list_item(L, kind, ch.C, it);
-            const int perm = ch.perm[chain];
-            const int p = chain + ch.C * (which == 0 ? perm_cur(perm) : perm_second(perm));
-            const bool cached = which == 0 && ch.gInit[p] != 0;
-            x.sync();
-            if (!cached) {
-                gaussian_from_derivs(x, ch, P, chain, p, w);
-                if (x.lane == 0) {
-                    // gradient and upper Hessian read (float); V, p, mean, logDet written (double)
-                    const int D = dim_of(ch.c[p], ch.l[p]);
-                    count_bytes(Sc, (unsigned long long)(4 * (D + D * (D + 1) / 2) + 8 * (D * D + 2 * D + 1)));
-                }
-            }
-            if (which == 0) h2mc_offset(x, ch, P, chain, iter, w);
-            if (finishRatio) h2mc_ratio(x, ch, chain, w);
-        }
+        const GroupExec<G> x{int(threadIdx.x) % G};
+        const int grp = threadIdx.x / G;
+        for (int it = blockIdx.x * NG + grp; it < nSmall; it += gridDim.x * NG)
+            gauss_item(x, Sc, ch, P, list_item(L, kind, ch.C, it), which, finishRatio, iter, w.small[grp]);
     }
+    __syncthreads();
+    const WaveExec x{int(threadIdx.x)};
+    for (int it = nSmall + blockIdx.x; it < n; it += gridDim.x)
+        gauss_item(x, Sc, ch, P, list_item(L, kind, ch.C, it), which, finishRatio, iter, w.big);
 }
 // Second-stage proposals: list 1 (binned), or every chain in chain order with
 // the lanes of chains without a second stage idle in the walk rounds.

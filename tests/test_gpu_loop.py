@@ -78,9 +78,12 @@ def test_forms_agree(cfg1):
 def test_batched_paths_equal_plain_paths(which, request):
     """One iteration from identical seeds: the batched stage kernels evaluate
     paths as walks that share one trace site per round, the plain form with
-    the sequential path code. Stage-1 moves (large and small steps) and the
-    isotropic second stage (with Green's y*) depend on nothing but those
-    evaluations and the random streams, so their counts must be identical."""
+    the sequential path code. Stage-1 moves (large and small steps), the
+    isotropic second stage (with Green's y*) and the H2MC second stage (the
+    same jet derivatives; Gaussians computed by one thread in the plain form,
+    by lane groups or whole waves in the batched one, which give bit-identical
+    results) depend on nothing but those evaluations and the random streams,
+    so their counts must be identical."""
     from drmlt import capi
 
     sc = request.getfixturevalue(which)
@@ -94,9 +97,11 @@ def test_batched_paths_equal_plain_paths(which, request):
         ctx.close()
     say("plain", res[capi.PLAIN])
     say("batched", res[capi.BATCHED])
-    for k in ("large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso", "stage2_iso_accepted", "stage2_h2mc"):
+    for k in ("large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso", "stage2_iso_accepted", "stage2_h2mc",
+              "stage2_h2mc_accepted"):
         assert res[capi.PLAIN][k] == res[capi.BATCHED][k], k
     assert res[capi.PLAIN]["stage1_accepted"] > 0 and res[capi.PLAIN]["stage2_iso_accepted"] > 0
+    assert res[capi.PLAIN]["stage2_h2mc_accepted"] > 0
 
 
 def test_schedules_agree_and_autotune(cfg2_small):
