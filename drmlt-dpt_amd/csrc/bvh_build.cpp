@@ -12,6 +12,13 @@
 #include <cmath>
 #include <cstring>
 
+#ifndef DRMLT_BVH_BINS
+#define DRMLT_BVH_BINS 32
+#endif
+#ifndef DRMLT_BVH_AXES
+#define DRMLT_BVH_AXES 3
+#endif
+
 namespace drmlt {
 
 namespace {
@@ -71,22 +78,24 @@ struct Builder {
         for (int k = 0; k < 3; k++)
             if (cb.hi[k] - cb.lo[k] > ext) { ext = cb.hi[k] - cb.lo[k]; axis = k; }
         int mid = first + n / 2;
-        if (ext > 0) {
-            const int NB = 16;
+        // binned SAH over the centroid extent of every axis (the longest
+        // axis alone with DRMLT_BVH_AXES=1)
+        constexpr int NB = DRMLT_BVH_BINS;
+        float best = 3e38f;
+        int bestAxis = -1, bestSplit = -1;
+        for (int k = 0; k < 3; k++) {
+            if (DRMLT_BVH_AXES == 1 && k != axis) continue;
+            const float ek = cb.hi[k] - cb.lo[k];
+            if (!(ek > 0)) continue;
             Box bb[NB];
             int bc[NB] = {0};
-            const float scale = NB / ext;
-            auto binOf = [&](int t) {
-                int b = int((cen[3 * t + axis] - cb.lo[axis]) * scale);
-                return std::min(std::max(b, 0), NB - 1);
-            };
+            const float scale = NB / ek;
             for (int i = first; i < first + n; i++) {
-                int b = binOf(idx[i]);
+                const int t = idx[i];
+                const int b = std::min(std::max(int((cen[3 * t + k] - cb.lo[k]) * scale), 0), NB - 1);
                 bc[b]++;
-                bb[b].grow(tb[idx[i]]);
+                bb[b].grow(tb[t]);
             }
-            float best = 3e38f;
-            int bestSplit = -1;
             Box lacc[NB];
             int lcnt[NB];
             Box acc;
@@ -103,14 +112,18 @@ struct Builder {
                 acc.grow(bb[b]);
                 cnt += bc[b];
                 if (lcnt[b - 1] == 0 || cnt == 0) continue;
-                float cost = lacc[b - 1].area() * lcnt[b - 1] + acc.area() * cnt;
-                if (cost < best) { best = cost; bestSplit = b; }
+                const float cost = lacc[b - 1].area() * lcnt[b - 1] + acc.area() * cnt;
+                if (cost < best) { best = cost; bestAxis = k; bestSplit = b; }
             }
-            if (bestSplit > 0) {
-                auto it = std::partition(idx.begin() + first, idx.begin() + first + n, [&](int t) { return binOf(t) < bestSplit; });
-                mid = int(it - idx.begin());
-                if (mid == first || mid == first + n) mid = first + n / 2;
-            }
+        }
+        if (bestSplit > 0) {
+            const int k = bestAxis;
+            const float scale = NB / (cb.hi[k] - cb.lo[k]);
+            auto it = std::partition(idx.begin() + first, idx.begin() + first + n, [&](int t) {
+                return std::min(std::max(int((cen[3 * t + k] - cb.lo[k]) * scale), 0), NB - 1) < bestSplit;
+            });
+            mid = int(it - idx.begin());
+            if (mid == first || mid == first + n) mid = first + n / 2;
         }
         if (mid == first + n / 2) {
             std::nth_element(idx.begin() + first, idx.begin() + mid, idx.begin() + first + n,

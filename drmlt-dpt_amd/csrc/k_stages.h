@@ -135,6 +135,18 @@ DEV int list_prefix(const Lists &L, int kind, int nbins) {
 #define DRMLT_GAUSS_GROUP 8  // lanes per small Gaussian (measured: 8 beats 16 and 32)
 #endif
 
+// Items [begin, end) of list `kind`, 64 / G per wave, a group of G lanes and
+// the work area w[group] each.
+template <int G, class W>
+DEV void gauss_groups(const DevScene &Sc, const Chains &ch, const DrParams &P, const Lists &L, int kind, int which, int finishRatio,
+                      uint32_t iter, int begin, int end, W *w) {
+    constexpr int NG = 64 / G;
+    const GroupExec<G> x{int(threadIdx.x) % G};
+    const int grp = threadIdx.x / G;
+    for (int it = begin + blockIdx.x * NG + grp; it < end; it += gridDim.x * NG)
+        gauss_item(x, Sc, ch, P, list_item(L, kind, ch.C, it), which, finishRatio, iter, w[grp]);
+}
+
 // One-wave workgroups over a Gaussian list: first the small Gaussians
 // (D <= 8), 64 / DRMLT_GAUSS_GROUP per wave, a group of lanes and an LDS work
 // area each (a whole wave would leave most lanes idle on such matrices);
@@ -149,12 +161,7 @@ __global__ void __launch_bounds__(64, DRMLT_GAUSS_WAVES) DRMLT_STAGE(k_b_gauss)(
     __shared__ Work w;
     const uint32_t iter = *iterp;
     const int n = list_length(L, kind), nSmall = list_prefix(L, kind, kGSmallBins);
-    {
-        const GroupExec<G> x{int(threadIdx.x) % G};
-        const int grp = threadIdx.x / G;
-        for (int it = blockIdx.x * NG + grp; it < nSmall; it += gridDim.x * NG)
-            gauss_item(x, Sc, ch, P, list_item(L, kind, ch.C, it), which, finishRatio, iter, w.small[grp]);
-    }
+    gauss_groups<G>(Sc, ch, P, L, kind, which, finishRatio, iter, 0, nSmall, w.small);
     __syncthreads();
     const WaveExec x{int(threadIdx.x)};
     for (int it = nSmall + blockIdx.x; it < n; it += gridDim.x)
