@@ -4,13 +4,16 @@
 // function-level parity surface.
 #include "../../include/drmlt_amd.h"
 #include "bvh_build.h"
+#include "host_io.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <random>
 #include <string>
+#include <sys/stat.h>
 #include <vector>
 
 #include "kernels_decl.h"
@@ -787,6 +790,95 @@ extern "C" int drmlt_read_stats(drmlt_ctx *c, drmlt_stats *st) {
     st->stage1 = v[2]; st->stage1_accepted = v[3];
     st->stage2_iso = v[4]; st->stage2_iso_accepted = v[5];
     st->stage2_h2mc = v[6]; st->stage2_h2mc_accepted = v[7];
+    return DRMLT_OK;
+}
+
+// ---------------------------------------------------------------- output files
+
+extern "C" int drmlt_write_exr(const char *path, const float *rgb, int32_t w, int32_t h, int32_t compression) {
+    std::string err;
+    if (write_exr(path, rgb, w, h, compression, err) != 0) return fail(DRMLT_ERR_ARG, err.c_str());
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_stats_text(const drmlt_stats *st, int32_t reference_labels, char *buf, int32_t cap) {
+    if (!st || cap < 0 || (cap > 0 && !buf)) return fail(DRMLT_ERR_ARG, "bad argument");
+    const uint64_t v[8] = {st->large_step, st->large_step_accepted, st->stage1, st->stage1_accepted,
+                           st->stage2_iso, st->stage2_iso_accepted, st->stage2_h2mc, st->stage2_h2mc_accepted};
+    const std::string t = stats_text(v, reference_labels != 0);
+    if (cap > 0) {
+        const size_t n = std::min(t.size(), size_t(cap - 1));
+        std::memcpy(buf, t.data(), n);
+        buf[n] = '\0';
+    }
+    return int(t.size());
+}
+
+static int write_text(const std::string &path, const std::string &text, bool append) {
+    FILE *fp = std::fopen(path.c_str(), append ? "ab" : "wb");
+    if (!fp) return fail(DRMLT_ERR_ARG, ("cannot open " + path).c_str());
+    const size_t wr = std::fwrite(text.data(), 1, text.size(), fp);
+    if (std::fclose(fp) != 0 || wr != text.size()) return fail(DRMLT_ERR_ARG, ("write failed: " + path).c_str());
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_write_stats(drmlt_ctx *c, const char *path, int32_t reference_labels) {
+    if (!c || !path) return fail(DRMLT_ERR_ARG, "bad argument");
+    drmlt_stats st;
+    const int rc = drmlt_read_stats(c, &st);
+    if (rc != DRMLT_OK) return rc;
+    const uint64_t v[8] = {st.large_step, st.large_step_accepted, st.stage1, st.stage1_accepted,
+                           st.stage2_iso, st.stage2_iso_accepted, st.stage2_h2mc, st.stage2_h2mc_accepted};
+    return write_text(path, stats_text(v, reference_labels != 0), false);
+}
+
+// The chain loop with the reference's intermediate outputs (delayed.cpp:
+// 234-242, 610-655): the partial directory is created if missing; whenever
+// the time since the call began reaches interval_seconds x k (k = 1, 2, ...),
+// the indirect buffer scaled by pixels / mutations so far is written to
+// <dir>/<base>_<k>.exr and the elapsed seconds to <dir>/<base>_time.csv
+// (truncated at k = 1, appended after); <dir>/<base>_stats.txt is rewritten
+// with every partial image and at the end. The loop runs in chunks of
+// chunk_iters iterations per chain, the reports fall between chunks.
+extern "C" int drmlt_render(drmlt_ctx *c, int64_t iters_per_chain, int32_t mode, const drmlt_report *rep) {
+    if (!c || iters_per_chain < 0 || !rep) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (mode < 0 || mode > 2) return fail(DRMLT_ERR_ARG, "unknown mode");
+    if (rep->compression != kExrNone && rep->compression != kExrZip) return fail(DRMLT_ERR_ARG, "unsupported EXR compression");
+    const bool files = rep->partial_dir && rep->partial_dir[0];
+    if (files && (!rep->base_name || !rep->base_name[0])) return fail(DRMLT_ERR_ARG, "partial outputs need a base name");
+    if (!c->haveChains || !c->lenCdf) return fail(DRMLT_ERR_STATE, "chains are not seeded (call drmlt_prepass or drmlt_seed_chains)");
+    std::string dir, base;
+    if (files) {
+        dir = rep->partial_dir;
+        base = dir + "/" + rep->base_name;
+        struct stat sb;
+        if (stat(dir.c_str(), &sb) != 0 && mkdir(dir.c_str(), 0755) != 0) return fail(DRMLT_ERR_ARG, ("cannot create " + dir).c_str());
+    }
+    const int64_t chunk = rep->chunk_iters > 0 ? rep->chunk_iters : 16;
+    const auto t0 = std::chrono::system_clock::now();
+    const size_t np = size_t(c->W) * c->H;
+    int imgId = 1;
+    for (int64_t done = 0; done < iters_per_chain;) {
+        const int64_t k = std::min(chunk, iters_per_chain - done);
+        int rc = drmlt_run(c, k, mode);
+        if (rc != DRMLT_OK) return rc;
+        done += k;
+        if (!files || !(rep->interval_seconds > 0)) continue;
+        const long ms = long(std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::system_clock::now() - t0).count());
+        const float elapsed = float(ms) / 1000.f;
+        if (!(elapsed >= float(rep->interval_seconds) * float(imgId))) continue;
+        std::vector<float> ind(np * 3);
+        if ((rc = drmlt_read_film(c, ind.data(), c->W, c->H, 1)) != DRMLT_OK) return rc;
+        const float scale = float(np) / float(double(c->itersDone) * double(c->ch.C));
+        for (float &v : ind) v *= scale;
+        std::string err;
+        if (write_exr((base + "_" + std::to_string(imgId) + ".exr").c_str(), ind.data(), c->W, c->H, rep->compression, err) != 0)
+            return fail(DRMLT_ERR_ARG, err.c_str());
+        if ((rc = write_text(base + "_time.csv", float_text(elapsed) + ",\n", imgId > 1)) != DRMLT_OK) return rc;
+        if ((rc = drmlt_write_stats(c, (base + "_stats.txt").c_str(), rep->reference_labels)) != DRMLT_OK) return rc;
+        imgId++;
+    }
+    if (files) return drmlt_write_stats(c, (base + "_stats.txt").c_str(), rep->reference_labels);
     return DRMLT_OK;
 }
 

@@ -15,7 +15,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result"]
-UNITS = ["k_plain.hip", "k_loop.hip", "k_count.hip", "k_deriv.hip", "k_deriv64.hip", "drmlt_capi.hip", "bvh_build.cpp"]
+UNITS = ["k_plain.hip", "k_loop.hip", "k_count.hip", "k_deriv.hip", "k_deriv64.hip", "drmlt_capi.hip", "bvh_build.cpp", "host_io.cpp"]
 
 
 def _compile(unit):
@@ -35,7 +35,7 @@ def build(jobs=None):
     jobs = jobs or min(len(UNITS), os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, UNITS))
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr[-6000:]}")

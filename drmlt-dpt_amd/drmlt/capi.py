@@ -19,9 +19,11 @@ EXPORTS = [
     "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
     "drmlt_grad_hess", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
     "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains",
+    "drmlt_write_exr", "drmlt_stats_text", "drmlt_write_stats", "drmlt_render",
 ]
 
 PLAIN, BATCHED, BATCHED_GRAPH = 0, 1, 2
+EXR_NONE, EXR_ZIP = 0, 3
 
 PATH_REC = np.dtype([
     ("c", np.int32), ("l", np.int32), ("light", np.int32), ("lprim", np.int32),
@@ -63,6 +65,11 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class Report(C.Structure):
+    _fields_ = [("interval_seconds", C.c_double), ("partial_dir", C.c_char_p), ("base_name", C.c_char_p),
+                ("compression", C.c_int32), ("reference_labels", C.c_int32), ("chunk_iters", C.c_int32)]
 
 
 _lib = None
@@ -176,6 +183,18 @@ class Context:
         s = Stats()
         check(lib().drmlt_read_stats(self.h, C.byref(s)))
         return s.as_dict()
+
+    def write_stats(self, path, reference_labels=True):
+        """The reference's statistics text (Stats::output) of the counters so far."""
+        check(lib().drmlt_write_stats(self.h, os.fsencode(path), int(reference_labels)))
+
+    def render(self, iters, mode=BATCHED_GRAPH, interval_seconds=0.0, partial_dir=None, base_name="drmlt",
+               compression=EXR_ZIP, reference_labels=True, chunk_iters=0):
+        """run() with the reference's intermediate outputs in partial_dir:
+        <base>_<k>.exr, <base>_time.csv and <base>_stats.txt."""
+        rep = Report(float(interval_seconds), os.fsencode(partial_dir) if partial_dir else None,
+                     base_name.encode() if base_name else None, int(compression), int(reference_labels), int(chunk_iters))
+        check(lib().drmlt_render(self.h, C.c_int64(iters), mode, C.byref(rep)))
 
     def count_work(self):
         """One batched step with work counters: per stage (nodes, triangles, rays, state bytes)."""
@@ -295,3 +314,22 @@ def dr_alpha2(rows):
     out = np.zeros(len(rows))
     check(lib().drmlt_dr_alpha2_f64(C.c_int64(len(rows)), P(rows), P(out)))
     return out
+
+
+def write_exr(path, rgb, compression=EXR_ZIP):
+    """An H x W x 3 float image as a HALF RGB OpenEXR file (the reference's WriteImage)."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    if rgb.ndim != 3 or rgb.shape[2] != 3:
+        raise ValueError("expected an H x W x 3 image")
+    check(lib().drmlt_write_exr(os.fsencode(path), P(rgb), rgb.shape[1], rgb.shape[0], int(compression)))
+
+
+def stats_text(stats: dict, reference_labels=True):
+    """Stats::output of a stats dict (Context.stats()) as the reference prints it."""
+    s = Stats(*[int(stats[n]) for n, _ in Stats._fields_])
+    n = lib().drmlt_stats_text(C.byref(s), int(reference_labels), None, 0)
+    if n < 0:
+        check(n)
+    buf = C.create_string_buffer(n + 1)
+    lib().drmlt_stats_text(C.byref(s), int(reference_labels), buf, n + 1)
+    return buf.value.decode()

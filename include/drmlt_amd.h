@@ -161,6 +161,41 @@ int drmlt_run(drmlt_ctx *ctx, int64_t iters_per_chain, int32_t mode);
  * buffer, 2 raw direct buffer. rgb holds w*h*3 floats. */
 int drmlt_read_film(drmlt_ctx *ctx, float *rgb, int32_t w, int32_t h, int32_t which);
 int drmlt_read_stats(drmlt_ctx *ctx, drmlt_stats *stats);
+/* ---------------- output files (host) ---------------- */
+
+/* OpenEXR compression codes accepted by the writers */
+#define DRMLT_EXR_NONE 0
+#define DRMLT_EXR_ZIP 3
+
+/* Replaces WriteImage (src/image.cpp:45-58, OpenImageIO): writes a w x h RGB
+ * float image (row-major, top row first) as a scanline OpenEXR file with
+ * three HALF channels. Host only; no context needed. */
+int drmlt_write_exr(const char *path, const float *rgb, int32_t w, int32_t h, int32_t compression);
+
+/* Stats::output (delayed.cpp:311-324) as text into buf (NUL-terminated,
+ * truncated to cap - 1 bytes); returns the full length. reference_labels = 1
+ * keeps the reference's swapped second-stage labels (delayed.cpp:320-321),
+ * 0 names each line's own component. */
+int drmlt_stats_text(const drmlt_stats *stats, int32_t reference_labels, char *buf, int32_t cap);
+/* The same text of the context's counters into a file (truncated). */
+int drmlt_write_stats(drmlt_ctx *ctx, const char *path, int32_t reference_labels);
+
+/* Intermediate outputs of drmlt_render (delayed.cpp:225-242, 610-655). */
+typedef struct {
+    double interval_seconds;   /* DptOptions reportIntervalSeconds; <= 0: no partial images */
+    const char *partial_dir;   /* <output dir>/<integrator or output name>_partial; NULL: no files */
+    const char *base_name;     /* the integrator name ("drmlt") or the output name */
+    int32_t compression;       /* DRMLT_EXR_NONE or DRMLT_EXR_ZIP */
+    int32_t reference_labels;  /* stats file labels, as drmlt_stats_text */
+    int32_t chunk_iters;       /* iterations per chain between report checks (0: 16) */
+} drmlt_report;
+
+/* drmlt_run with the reference's intermediate outputs: partial images of
+ * the indirect buffer (scaled by pixels / mutations so far) as
+ * <dir>/<base>_<k>.exr once interval_seconds x k have passed, their times in
+ * <dir>/<base>_time.csv, and the statistics in <dir>/<base>_stats.txt. */
+int drmlt_render(drmlt_ctx *ctx, int64_t iters_per_chain, int32_t mode, const drmlt_report *report);
+
 /* Device time of the last drmlt_run in milliseconds (after a function-level
  * call that says so: of its kernel, in out[0]), total and per stage
  * kernel (hipEvents on the context stream): out[0] total, out[1..7] stage 1,
