@@ -314,7 +314,7 @@ DEV int stage1_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int 
     count_if(stats + ST_LARGE_ACC, large && acc1);
     count_if(stats + ST_S1, !large);
     count_if(stats + ST_S1_ACC, !large && acc1);
-    const bool doSecond = !acc1 && !isLarge && curValid;
+    const bool doSecond = P.secondStage && !acc1 && !isLarge && curValid;
     bool isAniso = false;
     if (doSecond) isAniso = flip_coin(P.anisoPerturbProb, g.uni());
     ch.a1[chain] = a1;

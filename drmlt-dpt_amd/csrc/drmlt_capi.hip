@@ -25,6 +25,8 @@ static int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+// for the host-side loaders in other translation units
+int drmlt_set_error(int code, const char *msg) { return fail(code, msg); }
 #define HIPCHK(x)                                                                                          \
     do {                                                                                                   \
         hipError_t e_ = (x);                                                                               \
@@ -173,7 +175,8 @@ static DrParams make_params(const drmlt_ctx *c) {
     P.discreteStdDev = o.discrete_std_dev;
     P.type = o.type;
     P.maxDervDepth = o.max_derivatives_depth;
-    P.acceptanceMap = o.acceptance_map;
+    P.secondStage = o.integrator == 0;
+    P.acceptanceMap = P.secondStage ? o.acceptance_map : 0;  // a DR option (delayed.cpp:255-275)
     P.b = c->b;
     P.nLen = c->nLen;
     P.lenFunc = c->lenFunc;
@@ -215,6 +218,7 @@ extern "C" int drmlt_options_default(drmlt_options *o) {
     o->max_derivatives_depth = 9;   // main.cpp:76
     o->quirk_ignore_aniso_std_dev = 1;
     o->quirk_chad_mis = 1;
+    o->integrator = 0;
     o->seed = 1;
     return DRMLT_OK;
 }
@@ -247,6 +251,7 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     if (opt->max_depth > 12 || opt->max_depth < 3) return fail(DRMLT_ERR_ARG, "max_depth must be in [3, 12]");
     if (opt->max_derivatives_depth > 12) return fail(DRMLT_ERR_ARG, "max_derivatives_depth must be <= 12");
     if (!opt->large_step_multiplexed) return fail(DRMLT_ERR_ARG, "only multiplexed large steps are implemented");
+    if (opt->integrator != 0 && opt->integrator != 1) return fail(DRMLT_ERR_ARG, "integrator must be 0 (drmlt) or 1 (mcmc)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DRMLT_ERR_HIP, "no HIP device");
     drmlt_ctx *c = new drmlt_ctx();
@@ -590,7 +595,9 @@ static StepGrid step_grid(const drmlt_ctx *c) {
 }
 
 // One batched DR step: the stage-1 binning kernel, seven stage kernels and
-// the list reset (the binning kernel is timed with stage 1). With
+// the list reset (the binning kernel is timed with stage 1); without a
+// second stage (the mcmc integrator) only stage 1 and the reset run, the
+// events of the other stages still recorded. With
 // `counters` (7 x 4 words) the counting forms of the stage kernels run
 // instead, each accumulating its traversal work and state bytes into its own
 // row.
@@ -605,12 +612,14 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
         }
         if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin_counted, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
         hipLaunchKernelGGL(k_b_stage1_counted, dim3(G.stage), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
-        hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 0, 0);
-        hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0, c->iterDev);
-        hipLaunchKernelGGL(k_b_stage2_propose_counted, dim3(G.stage), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
-        hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
-        hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1, c->iterDev);
-        hipLaunchKernelGGL(k_b_stage2_accept_counted, dim3(G.stage), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+        if (P.secondStage) {
+            hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 0, 0);
+            hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0, c->iterDev);
+            hipLaunchKernelGGL(k_b_stage2_propose_counted, dim3(G.stage), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
+            hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
+            hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1, c->iterDev);
+            hipLaunchKernelGGL(k_b_stage2_accept_counted, dim3(G.stage), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+        }
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
         return;
     }
@@ -618,17 +627,17 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
     if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
     hipLaunchKernelGGL(k_b_stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[1], s);
-    hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
+    if (P.secondStage) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
     if (timed) (void)hipEventRecord(c->ev[2], s);
-    hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0, c->iterDev);
+    if (P.secondStage) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[3], s);
-    hipLaunchKernelGGL(k_b_stage2_propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
+    if (P.secondStage) hipLaunchKernelGGL(k_b_stage2_propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
     if (timed) (void)hipEventRecord(c->ev[4], s);
-    hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
+    if (P.secondStage) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
     if (timed) (void)hipEventRecord(c->ev[5], s);
-    hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1, c->iterDev);
+    if (P.secondStage) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[6], s);
-    hipLaunchKernelGGL(k_b_stage2_accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    if (P.secondStage) hipLaunchKernelGGL(k_b_stage2_accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[7], s);
     hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
     if (timed) (void)hipEventRecord(c->ev[8], s);
@@ -801,11 +810,11 @@ extern "C" int drmlt_write_exr(const char *path, const float *rgb, int32_t w, in
     return DRMLT_OK;
 }
 
-extern "C" int drmlt_stats_text(const drmlt_stats *st, int32_t reference_labels, char *buf, int32_t cap) {
-    if (!st || cap < 0 || (cap > 0 && !buf)) return fail(DRMLT_ERR_ARG, "bad argument");
+extern "C" int drmlt_stats_text(const drmlt_stats *st, int32_t format, char *buf, int32_t cap) {
+    if (!st || cap < 0 || (cap > 0 && !buf) || format < 0 || format > 2) return fail(DRMLT_ERR_ARG, "bad argument");
     const uint64_t v[8] = {st->large_step, st->large_step_accepted, st->stage1, st->stage1_accepted,
                            st->stage2_iso, st->stage2_iso_accepted, st->stage2_h2mc, st->stage2_h2mc_accepted};
-    const std::string t = stats_text(v, reference_labels != 0);
+    const std::string t = stats_text(v, format);
     if (cap > 0) {
         const size_t n = std::min(t.size(), size_t(cap - 1));
         std::memcpy(buf, t.data(), n);
@@ -829,7 +838,7 @@ extern "C" int drmlt_write_stats(drmlt_ctx *c, const char *path, int32_t referen
     if (rc != DRMLT_OK) return rc;
     const uint64_t v[8] = {st.large_step, st.large_step_accepted, st.stage1, st.stage1_accepted,
                            st.stage2_iso, st.stage2_iso_accepted, st.stage2_h2mc, st.stage2_h2mc_accepted};
-    return write_text(path, stats_text(v, reference_labels != 0), false);
+    return write_text(path, stats_text(v, c->opt.integrator == 1 ? 2 : (reference_labels != 0 ? 1 : 0)), false);
 }
 
 // The chain loop with the reference's intermediate outputs (delayed.cpp:

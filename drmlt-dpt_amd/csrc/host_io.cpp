@@ -156,16 +156,28 @@ int write_exr(const char *path, const float *rgb, int w, int h, int compression,
 }
 
 // Stats::output with the reference's Float (float) ratios and default
-// stream formatting. With referenceLabels the two second-stage lines carry
-// the reference's swapped labels (delayed.cpp:320-321: the "isotropic" line
-// prints the H2MC ratio and count and vice versa); without, each line names
-// its own component.
-std::string stats_text(const uint64_t s[8], bool referenceLabels) {
+// stream formatting. Format 1 gives the two second-stage lines the
+// reference's swapped labels (delayed.cpp:320-321: the "isotropic" line
+// prints the H2MC ratio and count and vice versa); format 0 names each
+// line's own component. Format 2 is the mcmc integrator's Stats::output
+// (mcmc.cpp:76-86): large and small (first-stage) steps, and lens steps,
+// which its SmallStep never makes (mutation.cpp:126-165), at 0 of 0.
+std::string stats_text(const uint64_t s[8], int format) {
     const float rLarge = float(s[1]) / float(s[0]), r1 = float(s[3]) / float(s[2]);
     const float rIso = float(s[5]) / float(s[4]), rH2 = float(s[7]) / float(s[6]);
     std::ostringstream o;
     o << "\n";
     o << "Rendering Statistics \n";
+    if (format == 2) {
+        volatile float zero = 0.0f;  // divided at run time, as the reference's 0 / 0 is (x86-64: the negative NaN)
+        const float rLens = zero / zero;
+        o << "    Large step mutations acceptance ratio       : " << rLarge << " (" << s[0] << ")\n";
+        o << "    Small step mutations acceptance ratio       : " << r1 << " (" << s[2] << ")\n";
+        o << "    Lens mutations acceptance ratio    :                    " << rLens << " (" << 0 << ")\n";
+        o << "\n";
+        return o.str();
+    }
+    const bool referenceLabels = format == 1;
     o << "    Large step mutations acceptance ratio                        : " << rLarge << " (" << s[0] << ")\n";
     o << "    First stage mutations acceptance ratio                       : " << r1 << " (" << s[2] << ")\n";
     if (referenceLabels) {

@@ -20,9 +20,11 @@ EXPORTS = [
     "drmlt_grad_hess", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
     "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains",
     "drmlt_write_exr", "drmlt_stats_text", "drmlt_write_stats", "drmlt_render",
+    "drmlt_scene_load", "drmlt_scene_get", "drmlt_scene_output", "drmlt_scene_free",
 ]
 
 PLAIN, BATCHED, BATCHED_GRAPH = 0, 1, 2
+INTEGRATORS = {"drmlt": 0, "mcmc": 1}  # the <dpt> integrator values the device path runs
 EXR_NONE, EXR_ZIP = 0, 3
 
 PATH_REC = np.dtype([
@@ -48,7 +50,7 @@ class Options(C.Structure):
                 ("discrete_std_dev", C.c_float), ("type", C.c_int32), ("aniso_perturb_prob", C.c_float),
                 ("aniso_perturb_std_dev", C.c_float), ("acceptance_map", C.c_int32),
                 ("max_derivatives_depth", C.c_int32), ("quirk_ignore_aniso_std_dev", C.c_int32),
-                ("quirk_chad_mis", C.c_int32), ("seed", C.c_uint64)]
+                ("quirk_chad_mis", C.c_int32), ("integrator", C.c_int32), ("seed", C.c_uint64)]
 
 
 class Schedule(C.Structure):
@@ -124,6 +126,8 @@ def options_from_dpt(dpt: dict, **overrides):
     for k, v in dpt.items():
         if k == "type":
             o.type = 1 if v == "mira" else 0
+        elif k == "integrator":
+            o.integrator = INTEGRATORS.get(v, -1)
         elif k in _KEYMAP:
             f, cast = _KEYMAP[k]
             setattr(o, f, cast(bool(v) if isinstance(v, bool) else v))
@@ -324,12 +328,51 @@ def write_exr(path, rgb, compression=EXR_ZIP):
     check(lib().drmlt_write_exr(os.fsencode(path), P(rgb), rgb.shape[1], rgb.shape[0], int(compression)))
 
 
-def stats_text(stats: dict, reference_labels=True):
-    """Stats::output of a stats dict (Context.stats()) as the reference prints it."""
+def stats_text(stats: dict, reference_labels=True, integrator="drmlt"):
+    """Stats::output of a stats dict (Context.stats()) as the reference prints
+    it: the DR integrator's text (with its swapped labels, or corrected ones),
+    or the mcmc integrator's."""
     s = Stats(*[int(stats[n]) for n, _ in Stats._fields_])
-    n = lib().drmlt_stats_text(C.byref(s), int(reference_labels), None, 0)
+    fmt = 2 if integrator == "mcmc" else int(bool(reference_labels))
+    n = lib().drmlt_stats_text(C.byref(s), fmt, None, 0)
     if n < 0:
         check(n)
     buf = C.create_string_buffer(n + 1)
-    lib().drmlt_stats_text(C.byref(s), int(reference_labels), buf, n + 1)
+    lib().drmlt_stats_text(C.byref(s), fmt, buf, n + 1)
     return buf.value.decode()
+
+
+def load_scene(xml_path):
+    """A scene file in the reference's XML format (drmlt_scene_load): the
+    scene dict Context takes (arrays copied out of the loaded scene), with its
+    <dpt> options as an Options struct under "opt", the output image name,
+    the integrator name and reportIntervalSeconds."""
+    L = lib()
+    h = C.c_void_p()
+    check(L.drmlt_scene_load(os.fsencode(xml_path), C.byref(h)))
+    try:
+        d, o = SceneDesc(), Options()
+        check(L.drmlt_scene_get(h, C.byref(d), C.byref(o)))
+        n = L.drmlt_scene_output(h, None, 0, None, 0, None)
+        if n < 0:
+            check(n)
+        out, integ, rep = C.create_string_buffer(n + 1), C.create_string_buffer(n + 1), C.c_int32()
+        check(min(0, L.drmlt_scene_output(h, out, n + 1, integ, n + 1, C.byref(rep))))
+
+        def arr(ptr, count, dtype, cols):
+            if count == 0:
+                return np.zeros((0, cols) if cols > 1 else 0, dtype)
+            buf = (C.c_char * (count * cols * np.dtype(dtype).itemsize)).from_address(ptr)
+            a = np.frombuffer(buf, dtype).copy()
+            return a.reshape(count, cols) if cols > 1 else a
+
+        nt, nm, nl = d.num_triangles, d.num_materials, d.num_lights
+        sc = {"tri_p": arr(d.tri_p, nt, np.float32, 9), "tri_n": arr(d.tri_n, nt, np.float32, 9),
+              "tri_mat": arr(d.tri_mat, nt, np.int32, 1), "tri_light": arr(d.tri_light, nt, np.int32, 1),
+              "mats": arr(d.materials, nm, np.float32, 16), "light_tris": arr(d.light_tris, nl, np.int32, 2),
+              "light_emit": arr(d.light_emit, nl, np.float32, 4), "cam": arr(d.camera, 14, np.float32, 1),
+              "opt": o, "output": out.value.decode(), "integrator": integ.value.decode(),
+              "report_interval_seconds": int(rep.value)}
+        return sc
+    finally:
+        L.drmlt_scene_free(h)

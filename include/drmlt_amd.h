@@ -86,6 +86,11 @@ typedef struct {
      * point light (src/path.cpp:2598-2604). 1 differentiates that function
      * (what the reference's H2MC sees), 0 differentiates the primal log f. */
     int32_t quirk_chad_mis;
+    /* The <dpt> integrator: 0 "drmlt" (DRIntegrator, delayed.cpp), 1 "mcmc"
+     * (MCMCIntegrator, src/integrators/mcmc.cpp:11-244: the loop's first
+     * stage alone -- a large or isotropic small step, splats weighted 1 - a
+     * and a, acceptance with probability a; no acceptance map). */
+    int32_t integrator;
     uint64_t seed;
 } drmlt_options;
 
@@ -161,6 +166,33 @@ int drmlt_run(drmlt_ctx *ctx, int64_t iters_per_chain, int32_t mode);
  * buffer, 2 raw direct buffer. rgb holds w*h*3 floats. */
 int drmlt_read_film(drmlt_ctx *ctx, float *rgb, int32_t w, int32_t h, int32_t which);
 int drmlt_read_stats(drmlt_ctx *ctx, drmlt_stats *stats);
+/* ---------------- scene files (host) ---------------- */
+
+/* A scene loaded from the reference's XML format; owns the arrays its
+ * drmlt_scene_desc points to. */
+typedef struct drmlt_scene drmlt_scene;
+
+/* Replaces ParseScene (src/parsescene.cpp:589-613) with its OBJ and
+ * .serialized loaders (src/parseobj.cpp, src/loadserialized.cpp) for what
+ * the device path renders: <sensor type="perspective"> with a look-at
+ * toWorld and <film> width/height/filename; <bsdf> diffuse, phong,
+ * roughdielectric (constant parameters) and twosided, by id or inline;
+ * <shape> obj and serialized (shapeIndex) with toWorld transforms
+ * (scale, translate, rotate, lookat, matrix) and <emitter type="area">
+ * radiance; <dpt> options. Relative file names resolve against the XML
+ * file's directory. Textures, point and environment emitters and moving
+ * shapes are rejected (DRMLT_ERR_ARG, drmlt_last_error names them). */
+int drmlt_scene_load(const char *xml_path, drmlt_scene **out);
+/* The scene's arrays (valid until drmlt_scene_free) and its <dpt> options
+ * applied over drmlt_options_default. Either output may be NULL. */
+int drmlt_scene_get(const drmlt_scene *scene, drmlt_scene_desc *desc, drmlt_options *opt);
+/* The output image name (the film's filename, else <xml dir>/image.exr),
+ * the integrator name and reportIntervalSeconds; strings are NUL-terminated
+ * and truncated to cap - 1; returns the longer string's full length. */
+int drmlt_scene_output(const drmlt_scene *scene, char *output_name, int32_t cap, char *integrator, int32_t icap,
+                       int32_t *report_interval_seconds);
+int drmlt_scene_free(drmlt_scene *scene);
+
 /* ---------------- output files (host) ---------------- */
 
 /* OpenEXR compression codes accepted by the writers */
@@ -173,11 +205,14 @@ int drmlt_read_stats(drmlt_ctx *ctx, drmlt_stats *stats);
 int drmlt_write_exr(const char *path, const float *rgb, int32_t w, int32_t h, int32_t compression);
 
 /* Stats::output (delayed.cpp:311-324) as text into buf (NUL-terminated,
- * truncated to cap - 1 bytes); returns the full length. reference_labels = 1
- * keeps the reference's swapped second-stage labels (delayed.cpp:320-321),
- * 0 names each line's own component. */
-int drmlt_stats_text(const drmlt_stats *stats, int32_t reference_labels, char *buf, int32_t cap);
-/* The same text of the context's counters into a file (truncated). */
+ * truncated to cap - 1 bytes); returns the full length. format 1 keeps the
+ * reference's swapped second-stage labels (delayed.cpp:320-321), 0 names
+ * each line's own component, 2 is the mcmc integrator's text
+ * (mcmc.cpp:76-86: large and small steps; lens steps, which it never
+ * makes, count 0). */
+int drmlt_stats_text(const drmlt_stats *stats, int32_t format, char *buf, int32_t cap);
+/* The same text of the context's counters into a file (truncated); a context
+ * of the mcmc integrator writes format 2 whatever reference_labels says. */
 int drmlt_write_stats(drmlt_ctx *ctx, const char *path, int32_t reference_labels);
 
 /* Intermediate outputs of drmlt_render (delayed.cpp:225-242, 610-655). */

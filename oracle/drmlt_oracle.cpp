@@ -377,6 +377,8 @@ extern "C" void orc_render(void *h, const double *opts, const PathRec *seeds, co
     dro.acceptanceMap = opts[8] != 0;
     dro.chadMisQuirk = opts[9] != 0;
     dro.fdStep = opts[10];
+    dro.secondStage = opts[11] != 0;
+    if (!dro.secondStage) dro.acceptanceMap = false;
     Dist1D<float> ld;
     ld.build(lengthFunc, nLen);
     const int W = s->f.cam.W, H = s->f.cam.H;

@@ -76,6 +76,10 @@ struct DrOptions {
     // acceptance visibly; from 3e-6 down the acceptance no longer moves and
     // equals the one exact derivatives give.
     double fdStep = 2e-6;
+    // false: integrator=mcmc, the first stage alone (src/integrators/mcmc.cpp:
+    // 125-160: large or isotropic small step, splats (1 - a) and a, accept with
+    // probability a); the acceptance map is a DR option and stays off there
+    bool secondStage = true;
 };
 
 struct DrStats {
@@ -295,7 +299,7 @@ template <class F> inline void DrIteration(DrChainCtx<F> &cx, MState<F> &cur, MS
         if (IsInvalid(a1)) { a1 = 0; acc1 = false; }
         else { acc1 = FlipCoin(a1, F(rng.uni())); if (acc1) st.stage1Acc++; }
     }
-    const bool doSecond = !acc1 && !isLarge && cur.valid;
+    const bool doSecond = cx.dro.secondStage && !acc1 && !isLarge && cur.valid;
     if (doSecond) {
         isAniso = FlipCoin(F(cx.dro.anisoPerturbProb), F(rng.uni()));
         if (isAniso) { a2 = H2MCMutate(cx, cur, second, rng); st.h2mc++; }

@@ -184,3 +184,56 @@ def test_count_work_and_timing(cfg1):
     t = ctx.timing()
     assert t[0] > 0 and (t[1:8] >= 0).all()
     ctx.close()
+
+
+@pytest.mark.parametrize("which", ["cfg1", "cfg2_small"])
+def test_mcmc_integrator(which, request):
+    """integrator=mcmc (MCMCIntegrator, mcmc.cpp:125-160: the first stage
+    alone) on the same kernels. The plain, batched and graph forms advance
+    identically; one iteration from the same seeds counts exactly what the
+    DR loop's first stage counts; no second-stage moves are made; and against
+    the oracle's mcmc loop from the oracle's seeds the acceptance rates and
+    the film mean agree within Monte Carlo error."""
+    from drmlt import capi
+
+    sc = request.getfixturevalue(which)
+    res = {}
+    for integ, mode in (("mcmc", capi.PLAIN), ("mcmc", capi.BATCHED), ("mcmc", capi.BATCHED_GRAPH), ("drmlt", capi.BATCHED)):
+        ctx = _ctx(sc, num_init_samples=8192, num_chains=4096, seed=5, integrator=capi.INTEGRATORS[integ])
+        say(f"{which} {integ} form {mode}: prepass + 1 iteration")
+        ctx.prepass()
+        ctx.run(1, mode)
+        res[(integ, mode)] = ctx.stats()
+        ctx.close()
+    m = res[("mcmc", capi.PLAIN)]
+    say("mcmc", m, "\ndrmlt", res[("drmlt", capi.BATCHED)])
+    assert m == res[("mcmc", capi.BATCHED)] == res[("mcmc", capi.BATCHED_GRAPH)]
+    d = res[("drmlt", capi.BATCHED)]
+    for k in ("large_step", "large_step_accepted", "stage1", "stage1_accepted"):
+        assert m[k] == d[k], k
+    assert m["stage2_iso"] == m["stage2_h2mc"] == 0 and d["stage2_iso"] + d["stage2_h2mc"] > 0
+
+    O = orc.Scene(sc)
+    maxd = int(sc["options"]["maxdepth"])
+    chains, iters = 2048, 32
+    b, lf, seeds, c8, _ = O.prepass(20000, maxd, chains, seed=13)
+    opt = capi.options_from_dpt(dict(sc["options"], integrator="mcmc"), num_chains=chains, seed=17)
+    ctx = capi.Context(sc, opt)
+    say(f"mcmc loop {which} {chains} chains x {iters} iterations vs the oracle")
+    ctx.seed_chains(seeds, c8, b, lf)
+    ctx.run(iters, capi.BATCHED_GRAPH)
+    gfilm, gs = ctx.film(1), ctx.stats()
+    ctx.close()
+    ofilm, ost, _ = O.render(seeds, c8, lf, b, iters, seed=19, opts={"secondstage": 0})
+    names = ["large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso", "stage2_iso_accepted",
+             "stage2_h2mc", "stage2_h2mc_accepted"]
+    os_ = dict(zip(names, [int(v) for v in ost]))
+    say("gpu", gs, "\noracle", os_)
+    assert os_["stage2_iso"] == os_["stage2_h2mc"] == gs["stage2_iso"] == gs["stage2_h2mc"] == 0
+    for a, t in (("large_step_accepted", "large_step"), ("stage1_accepted", "stage1")):
+        rg, ro = gs[a] / max(gs[t], 1), os_[a] / max(os_[t], 1)
+        tol = 4 * np.sqrt(ro * (1 - ro) / max(os_[t], 1)) + 0.01
+        assert abs(rg - ro) < tol, (a, rg, ro, tol)
+    gm, om = gfilm.mean(axis=(0, 1)), ofilm.mean(axis=(0, 1))
+    say("film mean gpu", gm, "oracle", om)
+    assert np.allclose(gm, om, rtol=0.05)

@@ -63,3 +63,46 @@ def test_render_without_reports_equals_run(cfg1):
         ctx.close()
     assert res[0][0] == res[1][0]
     np.testing.assert_allclose(res[0][1], res[1][1], rtol=1e-4, atol=1e-6)
+
+
+def test_scene_file_renders_like_the_procedural_scene(cfg1, tmp_path):
+    """The config-1 room written as XML + OBJ files and read back by
+    drmlt_scene_load renders as the procedural scene does: the arrays agree
+    up to the rounding of normalised normals and the recovered look-at, so
+    the chains' statistics and the film agree closely."""
+    import sys
+    import os
+    from drmlt import capi
+
+    sys.path.insert(0, os.path.dirname(__file__))
+    from scene_files import export_scene
+
+    ld = capi.load_scene(export_scene(cfg1, tmp_path))
+    res = []
+    for sc in (cfg1, ld):
+        opt = capi.options_from_dpt(cfg1["options"], num_init_samples=8192, num_chains=2048, seed=23)
+        ctx = capi.Context(sc, opt)
+        ctx.prepass()
+        ctx.direct_lighting()
+        ctx.run(16, capi.BATCHED_GRAPH)
+        res.append((ctx.stats(), ctx.film(0)))
+        ctx.close()
+    (s0, f0), (s1, f1) = res
+    print("procedural", s0, "\nfrom files", s1, flush=True)
+    assert s0["large_step"] + s0["stage1"] == s1["large_step"] + s1["stage1"]
+    for k in ("stage1_accepted", "stage2_iso", "stage2_h2mc"):
+        assert abs(s0[k] - s1[k]) <= 0.02 * max(s0[k], 1), k
+    assert abs(f0.mean() - f1.mean()) <= 0.02 * f0.mean()
+
+
+def test_mcmc_render_writes_the_mcmc_stats_text(cfg1, tmp_path):
+    from drmlt import capi
+
+    opt = capi.options_from_dpt(dict(cfg1["options"], integrator="mcmc"), num_init_samples=8192, num_chains=1024, seed=24)
+    ctx = capi.Context(cfg1, opt)
+    ctx.prepass()
+    ctx.render(8, capi.BATCHED_GRAPH, partial_dir=tmp_path / "mcmc_partial", base_name="mcmc")
+    st = ctx.stats()
+    assert st["stage2_iso"] == st["stage2_h2mc"] == 0 and st["large_step"] + st["stage1"] == 8 * 1024
+    assert (tmp_path / "mcmc_partial" / "mcmc_stats.txt").read_text() == capi.stats_text(st, integrator="mcmc")
+    ctx.close()

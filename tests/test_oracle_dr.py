@@ -57,6 +57,34 @@ def test_drmlt_converges_to_bdpt(small):
     assert ib.sum() == pytest.approx(rb.sum(), rel=0.05)
 
 
+def test_mcmc_converges_to_bdpt(small):
+    """The oracle's mcmc integrator (the first stage alone, mcmc.cpp:125-160)
+    converges to the same independent bidirectional estimate, and makes no
+    second-stage moves."""
+    sc, s = small
+    W = H = 64
+    rng = np.random.default_rng(1)
+    N = 300000
+    uni = rng.uniform(0, 1, (N, 160)).astype(np.float32)
+    cs, nc = s.generate_bidir(uni, 3, 5, dbl=False)
+    c = cs[np.arange(cs.shape[1])[None] < nc[:, None]]
+    ix = np.clip((c[:, 7] * W).astype(int), 0, W - 1)
+    iy = np.clip((c[:, 8] * H).astype(int), 0, H - 1)
+    ref = np.zeros((H, W, 3))
+    np.add.at(ref, (iy, ix), c[:, 2:5])
+    ref *= (W * H) / N
+    b, lf, seeds, c8, _ = s.prepass(50000, 5, 256, seed=3)
+    iters = 4000
+    film, stats, _ = s.render(seeds, c8, lf, b, iters, threads=8, seed=6, opts={"secondstage": 0})
+    assert stats[4:].sum() == 0 and stats[0] + stats[2] == iters * 256
+    img = film / (iters * 256 / (W * H))
+    rb, ib = _blocks(ref), _blocks(img)
+    mask = rb > 0.05 * rb.max()
+    rel = np.abs(ib[mask] - rb[mask]) / rb[mask]
+    assert np.median(rel) < 0.08, np.median(rel)
+    assert ib.sum() == pytest.approx(rb.sum(), rel=0.05)
+
+
 def test_stats_structure(small):
     sc, s = small
     b, lf, seeds, c8, _ = s.prepass(20000, 5, 64, seed=4)

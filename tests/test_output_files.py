@@ -126,3 +126,18 @@ def test_stats_text_is_the_references(swapped):
     # the reference's swap: its "isotropic" line carries the H2MC count
     t = capi.stats_text(cases[0], reference_labels=True).splitlines()
     assert t[4].endswith("(201864)") and "isotropic" in t[4]
+
+
+def test_mcmc_stats_text():
+    """The mcmc integrator's Stats::output (mcmc.cpp:76-86): large and small
+    steps; lens steps are never made by its SmallStep (mutation.cpp:126-165),
+    so their line reads 0 of 0."""
+    s = dict(large_step=300, large_step_accepted=20, stage1=700, stage1_accepted=350, stage2_iso=0, stage2_iso_accepted=0,
+             stage2_h2mc=0, stage2_h2mc_accepted=0)
+    want = ("\n"
+            "Rendering Statistics \n"
+            f"    Large step mutations acceptance ratio       : {float(np.float32(20) / np.float32(300)):g} (300)\n"
+            "    Small step mutations acceptance ratio       : 0.5 (700)\n"
+            "    Lens mutations acceptance ratio    :                    -nan (0)\n"
+            "\n")
+    assert capi.stats_text(s, integrator="mcmc") == want
