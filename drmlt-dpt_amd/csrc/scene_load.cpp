@@ -491,6 +491,8 @@ Mesh load_serialized(const std::string &file, int shapeIndex, const M4 &toWorld)
         uint32_t count;
         std::memcpy(&count, &all[all.size() - 4], 4);
         if (uint32_t(shapeIndex) >= count) throw Err("shapeIndex beyond the shapes of " + file);
+        const size_t table = size_t(count) * (version == 4 ? 8 : 4) + 4;
+        if (table > all.size()) throw Err("offset table larger than the file " + file);
         if (version == 4) {
             uint64_t o;
             std::memcpy(&o, &all[all.size() - 4 - 8 * size_t(count - shapeIndex)], 8);

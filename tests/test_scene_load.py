@@ -232,3 +232,19 @@ def test_integrator_option(tmp_path, name, code):
     ld = capi.load_scene(export_scene(scenes.config1(), tmp_path, dpt={"integrator": name}))
     assert ld["integrator"] == name and ld["opt"].integrator == code
     assert capi.options_from_dpt({"integrator": name}).integrator == code
+
+
+def test_corrupt_serialized_is_rejected(tmp_path):
+    write_serialized(tmp_path / "m.serialized", [{"v": np.eye(3), "f": [[0, 1, 2]]}] * 2)
+    raw = bytearray((tmp_path / "m.serialized").read_bytes())
+    raw[-4:] = (10 ** 9).to_bytes(4, "little")  # a mesh count far beyond the file
+    (tmp_path / "bad.serialized").write_bytes(bytes(raw))
+    xml = ('<shape type="serialized"><string name="filename" value="bad.serialized"/>'
+           '<integer name="shapeIndex" value="1"/><ref id="white"/></shape>')
+    with pytest.raises(capi.DrmltError, match="offset table"):
+        capi.load_scene(_one_shape_scene(tmp_path, xml))
+    raw = bytearray((tmp_path / "m.serialized").read_bytes())
+    raw[10:14] = b"\xff\xff\xff\xff"  # inside the first zlib stream
+    (tmp_path / "bad2.serialized").write_bytes(bytes(raw))
+    with pytest.raises(capi.DrmltError):
+        capi.load_scene(_one_shape_scene(tmp_path, xml.replace("bad.serialized", "bad2.serialized").replace('value="1"', 'value="0"')))
