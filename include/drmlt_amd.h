@@ -11,13 +11,17 @@
  *     the chains, runs iterations and reads back the film and statistics.
  *  2. Function level: batched versions of the functions the loop is made of,
  *     for parity tests and for callers that drive their own loop.
+ * Around them, on the host: scene files in the reference's XML format
+ * (drmlt_scene_*) and the output files of a render (drmlt_write_exr,
+ * drmlt_render, drmlt_write_stats).
  *
  * Conventions: every function returns an int status (DRMLT_OK = 0, negative
  * on error; drmlt_last_error() describes the last failure). Buffers are
  * caller-owned host memory unless a name says otherwise; functions copy in and
  * out. A context owns one HIP stream on the device current at creation and is
  * not thread-safe (one host thread drives it). No function allocates on behalf
- * of the caller beyond the context's own device state.
+ * of the caller beyond the context's own device state and the host object of
+ * a loaded scene (released by drmlt_scene_free).
  */
 #ifndef DRMLT_AMD_H
 #define DRMLT_AMD_H
