@@ -162,7 +162,7 @@ struct drmlt_ctx {
     hipEvent_t ev[9];
     bool evInit = false;
     // launch schedule of the batched step (drmlt_set_schedule / drmlt_autotune)
-    drmlt_schedule sched = {0, 0, 0, 0};
+    drmlt_schedule sched = {0, 0, 0, 0, 0};
 };
 
 static DrParams make_params(const drmlt_ctx *c) {
@@ -190,6 +190,7 @@ static DrParams make_params(const drmlt_ctx *c) {
     P.oDeriv = P.o;
     P.oDeriv.chadMisQuirk = o.quirk_chad_mis;
     P.binStages = c->sched.bin_stages;
+    P.stageLanes = c->sched.stage_lanes > 0 ? c->sched.stage_lanes : 64;
     return P;
 }
 
@@ -587,7 +588,8 @@ static StepGrid step_grid(const drmlt_ctx *c) {
     const int C = c->ch.C;
     const int pb = persistent_blocks(c);
     StepGrid g;
-    const int perChain = (C + 63) / 64;
+    const int lanes = c->sched.stage_lanes > 0 ? c->sched.stage_lanes : 64;
+    const int perChain = (C + lanes - 1) / lanes;
     g.stage = c->sched.stage_blocks > 0 ? std::min(c->sched.stage_blocks, perChain) : std::min(perChain, pb);
     g.deriv = c->sched.deriv_blocks > 0 ? c->sched.deriv_blocks : pb;
     g.gauss = c->sched.gauss_blocks > 0 ? c->sched.gauss_blocks : pb;
@@ -712,7 +714,8 @@ extern "C" int drmlt_get_schedule(drmlt_ctx *c, drmlt_schedule *out) {
 
 extern "C" int drmlt_set_schedule(drmlt_ctx *c, const drmlt_schedule *sc) {
     if (!c || !sc) return fail(DRMLT_ERR_ARG, "bad argument");
-    if (sc->bin_stages < 0 || sc->bin_stages > 1 || sc->stage_blocks < 0 || sc->deriv_blocks < 0 || sc->gauss_blocks < 0)
+    if (sc->bin_stages < 0 || sc->bin_stages > 1 || sc->stage_blocks < 0 || sc->deriv_blocks < 0 || sc->gauss_blocks < 0 ||
+        (sc->stage_lanes != 0 && sc->stage_lanes != 16 && sc->stage_lanes != 32 && sc->stage_lanes != 64))
         return fail(DRMLT_ERR_ARG, "schedule out of range");
     c->sched = *sc;
     drop_graph(c);  // the captured step bakes in the launch geometry
@@ -720,15 +723,15 @@ extern "C" int drmlt_set_schedule(drmlt_ctx *c, const drmlt_schedule *sc) {
 }
 
 // Coordinate search over the schedule: first the path-stage scheduling
-// (binned or chain order, with the stage kernels' workgroup count) by the
-// whole step's time, then the derivative and Gaussian kernels' workgroup
+// (chains per workgroup, binned or chain order, the stage kernels'
+// workgroup count) by the whole step's time, then the derivative and Gaussian kernels' workgroup
 // counts by their own kernel times. Every candidate runs one untimed and
 // `steps` timed batched steps of the context's own chains; those are ordinary
 // DR iterations (they advance the chains and splat into the film).
 extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best, double *bestMs) {
     if (!c || steps <= 0) return fail(DRMLT_ERR_ARG, "bad argument");
     if (!c->haveChains || !c->lenCdf) return fail(DRMLT_ERR_STATE, "chains are not seeded (call drmlt_prepass or drmlt_seed_chains)");
-    const int C = c->ch.C, cus = c->cus, perChain = (C + 63) / 64;
+    const int C = c->ch.C, cus = c->cus;
     // per-step milliseconds of stage kernels k0..k1 (timing slots) under schedule sc
     auto measure = [&](const drmlt_schedule &sc, int k0, int k1, double &ms) -> int {
         c->sched = sc;
@@ -744,14 +747,17 @@ extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best,
     double curMs = 0, ms = 0;
     int r = measure(cur, 1, 7, curMs);
     if (r) return r;
-    std::vector<int> stageOpts = {cus * 4, cus * 8, cus * 16, perChain};
-    for (int bin = 0; bin <= 1; bin++)
-        for (int sb : stageOpts) {
-            cand = cur;
-            cand.bin_stages = bin;
-            cand.stage_blocks = std::min(sb, perChain);
-            if ((r = measure(cand, 1, 7, ms))) return r;
-            if (ms < curMs) { cur = cand; curMs = ms; }
+    for (int lanes : {64, 32})
+        for (int bin = 0; bin <= 1; bin++) {
+            const int perChain = (C + lanes - 1) / lanes;
+            for (int sb : {cus * 4, cus * 8, cus * 16, perChain}) {
+                cand = cur;
+                cand.stage_lanes = lanes;
+                cand.bin_stages = bin;
+                cand.stage_blocks = std::min(sb, perChain);
+                if ((r = measure(cand, 1, 7, ms))) return r;
+                if (ms < curMs) { cur = cand; curMs = ms; }
+            }
         }
     const int kernelSlots[2][2] = {{2, 5}, {3, 6}};  // derivative and Gaussian kernels (current + proposal)
     for (int which = 0; which < 2; which++) {

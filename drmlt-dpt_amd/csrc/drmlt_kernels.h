@@ -58,6 +58,7 @@ struct DrParams {
     Opts oDeriv;  // options for the derivative evaluation (chad MIS quirk)
     int binStages;  // path-stage kernels take their chains from the length bins (1) or in chain order (0)
     int secondStage;  // 1: delayed rejection; 0: the first stage alone (the mcmc integrator)
+    int stageLanes;   // chains per 64-lane workgroup of the path-stage kernels (64, 32 or 16)
 };
 
 // Work lists of the batched pipeline, each binned by path length:

@@ -79,12 +79,21 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage1_bin)(Chains ch, DrP
     }
 }
 
+// First chain (or list item) of this lane in the path-stage kernels, which
+// take P.stageLanes chains per 64-lane workgroup (lanes at and above it take
+// none): with fewer chains than the chip has SIMDs x 64, half-filled waves
+// give each SIMD two waves whose memory waits overlap, and each walk round
+// waits for the slowest of fewer lanes.
+DEV int stage_first(const DrParams &P, int n) {
+    return int(threadIdx.x) < P.stageLanes ? int(blockIdx.x) * P.stageLanes + int(threadIdx.x) : n;
+}
+
 // Stage 1 over every chain: from list 3 (binned) or in chain order.
 __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
                                                  unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
     const int n = P.binStages ? list_length(L, 3) : ch.C;
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
+    for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
         const int chain = P.binStages ? list_item(L, 3, ch.C, it) : it;
         Philox g(P.seed, iter, chain, TAG_S1);
         const Stage1 a = stage1_begin(ch, P, chain, g, !P.binStages);
@@ -172,7 +181,7 @@ __global__ void __launch_bounds__(64, DRMLT_GAUSS_WAVES) DRMLT_STAGE(k_b_gauss)(
 __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_propose)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
     const uint32_t iter = *iterp;
     const int n = P.binStages ? list_length(L, 1) : ch.C;
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
+    for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
         const int chain = P.binStages ? list_item(L, 1, ch.C, it) : it;
         const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
         Philox g(P.seed, iter, chain, TAG_S2);
@@ -201,7 +210,7 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
                                                         unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
     const int n = P.binStages ? list_length(L, 1) : ch.C;
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
+    for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
         const int chain = P.binStages ? list_item(L, 1, ch.C, it) : it;
         const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
         Philox g(P.seed, iter, chain, TAG_S3);
