@@ -715,7 +715,7 @@ extern "C" int drmlt_get_schedule(drmlt_ctx *c, drmlt_schedule *out) {
 extern "C" int drmlt_set_schedule(drmlt_ctx *c, const drmlt_schedule *sc) {
     if (!c || !sc) return fail(DRMLT_ERR_ARG, "bad argument");
     if (sc->bin_stages < 0 || sc->bin_stages > 1 || sc->stage_blocks < 0 || sc->deriv_blocks < 0 || sc->gauss_blocks < 0 ||
-        (sc->stage_lanes != 0 && sc->stage_lanes != 16 && sc->stage_lanes != 32 && sc->stage_lanes != 64))
+        sc->stage_lanes < 0 || sc->stage_lanes > 64)
         return fail(DRMLT_ERR_ARG, "schedule out of range");
     c->sched = *sc;
     drop_graph(c);  // the captured step bakes in the launch geometry
@@ -747,7 +747,13 @@ extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best,
     double curMs = 0, ms = 0;
     int r = measure(cur, 1, 7, curMs);
     if (r) return r;
-    for (int lanes : {64, 32})
+    // chains per workgroup: full waves, half waves, and the count that fills
+    // exactly two waves per SIMD (the path-stage kernels' occupancy), which
+    // measured best when the chains fill no more than that
+    const int fill = std::min(64, std::max(1, (C + 2 * 4 * cus - 1) / (2 * 4 * cus)));
+    std::vector<int> laneOpts = {64, 32};
+    if (fill != 64 && fill != 32) laneOpts.push_back(fill);
+    for (int lanes : laneOpts)
         for (int bin = 0; bin <= 1; bin++) {
             const int perChain = (C + lanes - 1) / lanes;
             for (int sb : {cus * 4, cus * 8, cus * 16, perChain}) {

@@ -131,8 +131,8 @@ typedef struct {
     int32_t stage_blocks;  /* 64-lane workgroups of the path-stage kernels (default: one per 64 chains, at most 16 per CU) */
     int32_t deriv_blocks;  /* workgroups of the derivative kernel (default 16 per CU) */
     int32_t gauss_blocks;  /* one-wave workgroups of the Gaussian kernel (default 16 per CU) */
-    int32_t stage_lanes;   /* chains per 64-lane workgroup of the path-stage kernels: 64 (default), 32
-                              or 16; fewer than 64 helps when the chains fill few waves per SIMD */
+    int32_t stage_lanes;   /* chains per 64-lane workgroup of the path-stage kernels, 1..64 (default
+                              64); fewer helps when the chains fill few waves per SIMD */
 } drmlt_schedule;
 
 /* Execution forms of drmlt_run */

@@ -71,7 +71,7 @@ def test_argument_errors(deep):
     with pytest.raises(capi.DrmltError):
         ctx.set_schedule(stage_blocks=-1)
     with pytest.raises(capi.DrmltError):
-        ctx.set_schedule(stage_lanes=48)
+        ctx.set_schedule(stage_lanes=65)
     with pytest.raises(capi.DrmltError):
         ctx.read_chains(10 ** 7)
     with pytest.raises(capi.DrmltError):
