@@ -244,6 +244,9 @@ def main():
     ap.add_argument("--rmse-ref-steps", type=int, default=2000, help="batched steps of the RMSE study's reference render")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch schedule (profiled runs: no candidate schedules among the launches)")
+    ap.add_argument("--schedule", type=str, default="",
+                    help='a fixed launch schedule as JSON (e.g. \'{"stage_lanes": 32}\') instead of the autotuner\'s; '
+                         "profiled runs use the schedule the autotuner picks for the workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device to device (hipMemcpy), the known-size kernel that scales "
                          "rocprof FETCH_SIZE / WRITE_SIZE readings")
@@ -278,7 +281,11 @@ def main():
     log(f"plain form {plain_ms:.2f} ms/step")
 
     # launch schedule of the batched form chosen by the autotuner on these chains
-    if args.no_tune:
+    if args.schedule:
+        ctx.set_schedule(**json.loads(args.schedule))
+        sched = ctx.schedule()
+        log(f"fixed schedule {sched}")
+    elif args.no_tune:
         sched = ctx.schedule()
         log(f"default schedule {sched}")
     else:
@@ -340,6 +347,7 @@ def main():
     mcmc = None
     if not args.no_mcmc:
         mctx = capi.Context(sc, capi.options_from_dpt(dict(dpt, integrator="mcmc"), num_chains=chains, seed=1234))
+        mctx.set_schedule(**sched)  # the path-stage scheduling tuned for these chains
         mctx.prepass()
         mctx.run(args.warmup, capi.BATCHED_GRAPH)
         m0 = mctx.stats()
