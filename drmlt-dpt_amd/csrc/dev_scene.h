@@ -134,6 +134,20 @@ DEV bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz, 
     return t0 <= t1;
 }
 
+// The same test for a ray whose inverse direction is finite in every
+// component: no slab bound can be NaN then, so the guards are left out (the
+// box decisions are identical).
+DEV bool slab_finite(float lox, float loy, float loz, float hix, float hiy, float hiz, const float o[3], const float inv[3], float tmin,
+                     float tmax, float &tnear) {
+    const float ax = (lox - o[0]) * inv[0], bx = (hix - o[0]) * inv[0];
+    const float ay = (loy - o[1]) * inv[1], by = (hiy - o[1]) * inv[1];
+    const float az = (loz - o[2]) * inv[2], bz = (hiz - o[2]) * inv[2];
+    const float t0 = fmaxf(fmaxf(fmaxf(tmin, fminf(ax, bx)), fminf(ay, by)), fminf(az, bz));
+    const float t1 = fminf(fminf(fminf(tmax, fmaxf(ax, bx)), fmaxf(ay, by)), fmaxf(az, bz));
+    tnear = t0;
+    return t0 <= t1;
+}
+
 // next representable float above a positive finite x
 DEV float next_up(float x) { return __int_as_float(__float_as_int(x) + 1); }
 
@@ -168,6 +182,7 @@ struct TraceState {
     float tmin, tmax, tbest;
     int best, sp;
     bool any, done;
+    bool finite;  // every inverse direction component finite: slab_finite applies
     Node4 N;  // the node the next step visits (its fetch already issued)
     DRMLT_COUNT(unsigned nvis; unsigned ntri;)
 };
@@ -179,6 +194,7 @@ DEV void trace_begin(const BvhRef &S, TraceState &t, float ox, float oy, float o
     t.tmin = tmin; t.tmax = tmax; t.tbest = tmax;
     t.best = -1; t.sp = 0;
     t.any = any; t.done = false;
+    t.finite = isfinite(t.inv[0]) && isfinite(t.inv[1]) && isfinite(t.inv[2]);
     DRMLT_COUNT(t.nvis = 0; t.ntri = 0;)
     t.N = load_node4(S, 0);
 }
@@ -209,10 +225,16 @@ DEV void trace_step(const BvhRef &S, TraceState &t) {
     DRMLT_COUNT(t.nvis++;)
     float tn[4];
     bool hit[4];
+    if (t.finite) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        hit[k] = i4at(N.count, k) >= 0 &&
-                 slab(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tbest, tn[k]);
+        for (int k = 0; k < 4; k++)
+            hit[k] = i4at(N.count, k) >= 0 &&
+                     slab_finite(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tbest, tn[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            hit[k] = i4at(N.count, k) >= 0 &&
+                     slab(f4at(N.lx, k), f4at(N.ly, k), f4at(N.lz, k), f4at(N.hx, k), f4at(N.hy, k), f4at(N.hz, k), o, inv, tmin, tbest, tn[k]);
     }
     // internal children in front of the best hit, nearest first: a
     // sorting network on (entry distance, child) in registers (an
