@@ -231,6 +231,7 @@ template <class X, class W> DEV void h2mc_ratio(const X &x, const Chains &ch, in
 struct Stage1 {
     int perm, pc, pf;
     bool isLarge, curValid;
+    bool h2mcMove;  // h2mc integrator: this small step is an H2MC move, made by the second-stage kernels
     int cam, lgt;  // large step: the sampled strategy
 };
 // copyState: copy the current state into the proposal slot for a small step
@@ -247,12 +248,16 @@ DEV Stage1 stage1_begin(const Chains &ch, const DrParams &P, int chain, Philox &
     const bool wellFormed = cc >= 1 && cl >= 0 && cc <= kVMax + 1 && cl <= kVMax + 1 && cc + cl - 1 >= 2 && 2 * (cc + cl - 1) <= kDMax;
     a.curValid = ch.valid[a.pc] != 0 && wellFormed;
     a.cam = a.lgt = 0;
+    a.h2mcMove = false;
     if (!a.curValid || a.isLarge) {
         const int len = sample_discrete(P.lenCdf, P.nLen, g.uni());
         a.lgt = min(max(int(g.uni() * float(len + 1)), 0), len);
         a.cam = len - a.lgt + 1;
-    } else if (copyState) {
-        copy_state(slot_ref(ch, a.pc), slot_ref(ch, a.pf));
+    } else {
+        // h2mc (H2MCSmallStep::Mutate, h2mc.cpp:448-460): isotropic with
+        // probability uniformMixingProbability, else an H2MC move
+        if (P.integrator == 2) a.h2mcMove = !(g.uni() < P.uniformMixing);
+        if (copyState && !a.h2mcMove) copy_state(slot_ref(ch, a.pc), slot_ref(ch, a.pf));
     }
     return a;
 }
@@ -280,6 +285,13 @@ DEV void stage1_walk(const Chains &ch, const DrParams &P, const Stage1 &a, Philo
 // second stage follows.
 DEV int stage1_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, const Stage1 &a, const POut<float> &out, Philox &g,
                    float *film, unsigned long long *stats) {
+    if (a.h2mcMove) {
+        // no first-stage proposal: the second-stage kernels make the H2MC
+        // move with a1 = 0 and accept it with min(1, ratio)
+        ch.a1[chain] = 0.0f;
+        ch.flags[chain] = 2 | 4;
+        return 2 | 4;
+    }
     const int perm = a.perm, pc = a.pc, pf = a.pf;
     const bool isLarge = a.isLarge, curValid = a.curValid;
     float a1 = 0.0f;
@@ -314,7 +326,7 @@ DEV int stage1_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int 
     count_if(stats + ST_LARGE_ACC, large && acc1);
     count_if(stats + ST_S1, !large);
     count_if(stats + ST_S1_ACC, !large && acc1);
-    const bool doSecond = P.secondStage && !acc1 && !isLarge && curValid;
+    const bool doSecond = P.integrator == 0 && !acc1 && !isLarge && curValid;
     bool isAniso = false;
     if (doSecond) isAniso = flip_coin(P.anisoPerturbProb, g.uni());
     ch.a1[chain] = a1;
@@ -341,7 +353,10 @@ DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int c
     const Stage1 a = stage1_begin(ch, P, chain, g);
     const SRef rf = slot_ref(ch, a.pf);
     POut<float> out;
-    if (!a.curValid || a.isLarge) {
+    out.valid = false;
+    if (a.h2mcMove) {
+        // made by the second stage
+    } else if (!a.curValid || a.isLarge) {
         generate_subpath(Sc, P.o, a.cam, a.lgt, rf, g, out);
     } else {
         PerturbU<OffNormal> up{rf.u, rf.s, 0, OffNormal{&g, P.perturbStdDev}};
@@ -442,7 +457,7 @@ DEV Stage3 stage3_begin(const Chains &ch, const DrParams &P, int chain) {
     const bool zValid = (a.flags & 16) != 0;
     a.a1 = ch.a1[chain];
     a.a2 = zValid ? ch.a2d[chain] : 0.0;
-    a.traceStar = !is_invalid(a.a2) && P.type == 0 && a.a2 != 0.0 && ch.valid[a.ps];
+    a.traceStar = !is_invalid(a.a2) && P.type == 0 && P.integrator == 0 && a.a2 != 0.0 && ch.valid[a.ps];
     if (a.traceStar) copy_state(slot_ref(ch, a.ps), slot_ref(ch, a.pstar));
     return a;
 }
@@ -466,6 +481,10 @@ DEV void stage3_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int
     bool acc2 = false;
     if (is_invalid(a2)) {
         a2 = 0.0;
+    } else if (P.integrator == 2) {
+        // h2mc: the H2MC move alone, min(1, ratio) (h2mc.cpp:584-587)
+        a2 = fmin(1.0, a2);
+        acc2 = flip_coin(a2, g.uni());
     } else if (P.type == 0) {
         // Green & Mira
         float aStar = 0.0f;

@@ -175,8 +175,9 @@ static DrParams make_params(const drmlt_ctx *c) {
     P.discreteStdDev = o.discrete_std_dev;
     P.type = o.type;
     P.maxDervDepth = o.max_derivatives_depth;
-    P.secondStage = o.integrator == 0;
-    P.acceptanceMap = P.secondStage ? o.acceptance_map : 0;  // a DR option (delayed.cpp:255-275)
+    P.integrator = o.integrator;
+    P.uniformMixing = o.uniform_mixing_prob;
+    P.acceptanceMap = o.integrator == 0 ? o.acceptance_map : 0;  // a DR option (delayed.cpp:255-275)
     P.b = c->b;
     P.nLen = c->nLen;
     P.lenFunc = c->lenFunc;
@@ -220,6 +221,8 @@ extern "C" int drmlt_options_default(drmlt_options *o) {
     o->quirk_ignore_aniso_std_dev = 1;
     o->quirk_chad_mis = 1;
     o->integrator = 0;
+    o->uniform_mixing_prob = 0.2f;  // dptoptions.h:29
+    o->lens_perturb_prob = 0.3f;    // dptoptions.h:20
     o->seed = 1;
     return DRMLT_OK;
 }
@@ -252,7 +255,9 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     if (opt->max_depth > 12 || opt->max_depth < 3) return fail(DRMLT_ERR_ARG, "max_depth must be in [3, 12]");
     if (opt->max_derivatives_depth > 12) return fail(DRMLT_ERR_ARG, "max_derivatives_depth must be <= 12");
     if (!opt->large_step_multiplexed) return fail(DRMLT_ERR_ARG, "only multiplexed large steps are implemented");
-    if (opt->integrator != 0 && opt->integrator != 1) return fail(DRMLT_ERR_ARG, "integrator must be 0 (drmlt) or 1 (mcmc)");
+    if (opt->integrator < 0 || opt->integrator > 2) return fail(DRMLT_ERR_ARG, "integrator must be 0 (drmlt), 1 (mcmc) or 2 (h2mc)");
+    if (opt->integrator == 2 && opt->lens_perturb_prob != 0.0f)
+        return fail(DRMLT_ERR_ARG, "the h2mc integrator runs without lens perturbations here: set lensperturbprob to 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DRMLT_ERR_HIP, "no HIP device");
     drmlt_ctx *c = new drmlt_ctx();
@@ -614,7 +619,7 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
         }
         if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin_counted, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
         hipLaunchKernelGGL(k_b_stage1_counted, dim3(G.stage), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
-        if (P.secondStage) {
+        if (P.integrator != 1) {
             hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 0, 0);
             hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0, c->iterDev);
             hipLaunchKernelGGL(k_b_stage2_propose_counted, dim3(G.stage), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
@@ -629,17 +634,17 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
     if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
     hipLaunchKernelGGL(k_b_stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[1], s);
-    if (P.secondStage) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
+    if (P.integrator != 1) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
     if (timed) (void)hipEventRecord(c->ev[2], s);
-    if (P.secondStage) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0, c->iterDev);
+    if (P.integrator != 1) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[3], s);
-    if (P.secondStage) hipLaunchKernelGGL(k_b_stage2_propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
+    if (P.integrator != 1) hipLaunchKernelGGL(k_b_stage2_propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
     if (timed) (void)hipEventRecord(c->ev[4], s);
-    if (P.secondStage) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
+    if (P.integrator != 1) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
     if (timed) (void)hipEventRecord(c->ev[5], s);
-    if (P.secondStage) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1, c->iterDev);
+    if (P.integrator != 1) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[6], s);
-    if (P.secondStage) hipLaunchKernelGGL(k_b_stage2_accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    if (P.integrator != 1) hipLaunchKernelGGL(k_b_stage2_accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[7], s);
     hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
     if (timed) (void)hipEventRecord(c->ev[8], s);
@@ -850,7 +855,7 @@ extern "C" int drmlt_write_stats(drmlt_ctx *c, const char *path, int32_t referen
     if (rc != DRMLT_OK) return rc;
     const uint64_t v[8] = {st.large_step, st.large_step_accepted, st.stage1, st.stage1_accepted,
                            st.stage2_iso, st.stage2_iso_accepted, st.stage2_h2mc, st.stage2_h2mc_accepted};
-    return write_text(path, stats_text(v, c->opt.integrator == 1 ? 2 : (reference_labels != 0 ? 1 : 0)), false);
+    return write_text(path, stats_text(v, c->opt.integrator != 0 ? 2 : (reference_labels != 0 ? 1 : 0)), false);
 }
 
 // The chain loop with the reference's intermediate outputs (delayed.cpp:

@@ -57,7 +57,8 @@ struct DrParams {
     Opts o;
     Opts oDeriv;  // options for the derivative evaluation (chad MIS quirk)
     int binStages;  // path-stage kernels take their chains from the length bins (1) or in chain order (0)
-    int secondStage;  // 1: delayed rejection; 0: the first stage alone (the mcmc integrator)
+    int integrator;   // 0 drmlt (delayed rejection), 1 mcmc (the first stage alone), 2 h2mc (no lens moves)
+    float uniformMixing;  // h2mc: probability of an isotropic small step
     int stageLanes;   // chains per 64-lane workgroup of the path-stage kernels (64, 32 or 16)
 };
 

@@ -159,9 +159,11 @@ int write_exr(const char *path, const float *rgb, int w, int h, int compression,
 // stream formatting. Format 1 gives the two second-stage lines the
 // reference's swapped labels (delayed.cpp:320-321: the "isotropic" line
 // prints the H2MC ratio and count and vice versa); format 0 names each
-// line's own component. Format 2 is the mcmc integrator's Stats::output
-// (mcmc.cpp:76-86): large and small (first-stage) steps, and lens steps,
-// which its SmallStep never makes (mutation.cpp:126-165), at 0 of 0.
+// line's own component. Format 2 is the Stats::output of the mcmc and h2mc
+// integrators (mcmc.cpp:76-86, h2mc.cpp:140-150): large steps, small steps
+// (isotropic first-stage moves plus H2MC moves), and lens steps, which
+// neither makes here (mcmc's SmallStep never does, mutation.cpp:126-165; h2mc
+// runs with lensperturbprob 0), at 0 of 0.
 std::string stats_text(const uint64_t s[8], int format) {
     const float rLarge = float(s[1]) / float(s[0]), r1 = float(s[3]) / float(s[2]);
     const float rIso = float(s[5]) / float(s[4]), rH2 = float(s[7]) / float(s[6]);
@@ -172,7 +174,8 @@ std::string stats_text(const uint64_t s[8], int format) {
         volatile float zero = 0.0f;  // divided at run time, as the reference's 0 / 0 is (x86-64: the negative NaN)
         const float rLens = zero / zero;
         o << "    Large step mutations acceptance ratio       : " << rLarge << " (" << s[0] << ")\n";
-        o << "    Small step mutations acceptance ratio       : " << r1 << " (" << s[2] << ")\n";
+        const float rSmall = float(s[3] + s[7]) / float(s[2] + s[6]);
+        o << "    Small step mutations acceptance ratio       : " << rSmall << " (" << (s[2] + s[6]) << ")\n";
         o << "    Lens mutations acceptance ratio    :                    " << rLens << " (" << 0 << ")\n";
         o << "\n";
         return o.str();

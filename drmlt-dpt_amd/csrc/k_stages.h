@@ -100,7 +100,8 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)
         Walk w;
         stage1_walk(ch, P, a, g, w);
         POut<float> out;
-        walk_run(Sc, P.o, w, true, out);
+        out.valid = false;
+        walk_run(Sc, P.o, w, !a.h2mcMove, out);
         const int flags = stage1_end(Sc, ch, P, chain, a, out, g, film, stats);
         // second stage: list 1 (every chain with one), list 0 (H2MC)
         const bool second = (flags & 2) != 0;

@@ -743,7 +743,7 @@ struct Loader {
             const bool t = v == "true";
             if (nm == "integrator") {
                 S.integrator = v;
-                o.integrator = v == "drmlt" ? 0 : v == "mcmc" ? 1 : -1;  // others: drmlt_ctx_create refuses them
+                o.integrator = v == "drmlt" ? 0 : v == "mcmc" ? 1 : v == "h2mc" ? 2 : -1;  // others: drmlt_ctx_create refuses them
             }
             else if (nm == "spp") o.spp = to_int(v, "spp");
             else if (nm == "numinitsamples") o.num_init_samples = to_int(v, "numinitsamples");
@@ -762,8 +762,9 @@ struct Loader {
             else if (nm == "anisoperturbstddev") o.aniso_perturb_std_dev = to_float(v, "anisoperturbstddev");
             else if (nm == "anisoperturbprob") o.aniso_perturb_prob = to_float(v, "anisoperturbprob");
             else if (nm == "acceptanceMap") o.acceptance_map = t;
-            else if (nm == "bidirectional" || nm == "lensperturbstddev" || nm == "lensperturbprob" ||
-                     nm == "uniformmixingprobability" || nm == "reportintervalspp") {
+            else if (nm == "uniformmixingprobability") o.uniform_mixing_prob = to_float(v, "uniformmixingprobability");
+            else if (nm == "lensperturbprob") o.lens_perturb_prob = to_float(v, "lensperturbprob");
+            else if (nm == "bidirectional" || nm == "lensperturbstddev" || nm == "reportintervalspp") {
                 // parsed by the reference, used by its other integrators only
             } else std::fprintf(stderr, "Unknown dpt option:%s\n", nm.c_str());
         }

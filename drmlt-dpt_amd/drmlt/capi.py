@@ -24,7 +24,7 @@ EXPORTS = [
 ]
 
 PLAIN, BATCHED, BATCHED_GRAPH = 0, 1, 2
-INTEGRATORS = {"drmlt": 0, "mcmc": 1}  # the <dpt> integrator values the device path runs
+INTEGRATORS = {"drmlt": 0, "mcmc": 1, "h2mc": 2}  # the <dpt> integrator values the device path runs
 EXR_NONE, EXR_ZIP = 0, 3
 
 PATH_REC = np.dtype([
@@ -50,7 +50,8 @@ class Options(C.Structure):
                 ("discrete_std_dev", C.c_float), ("type", C.c_int32), ("aniso_perturb_prob", C.c_float),
                 ("aniso_perturb_std_dev", C.c_float), ("acceptance_map", C.c_int32),
                 ("max_derivatives_depth", C.c_int32), ("quirk_ignore_aniso_std_dev", C.c_int32),
-                ("quirk_chad_mis", C.c_int32), ("integrator", C.c_int32), ("seed", C.c_uint64)]
+                ("quirk_chad_mis", C.c_int32), ("integrator", C.c_int32), ("uniform_mixing_prob", C.c_float),
+                ("lens_perturb_prob", C.c_float), ("seed", C.c_uint64)]
 
 
 class Schedule(C.Structure):
@@ -116,6 +117,7 @@ _KEYMAP = {
     "uselightcoordinatesampling": ("use_light_coordinate_sampling", int),
     "largestepmultiplexed": ("large_step_multiplexed", int), "anisoperturbprob": ("aniso_perturb_prob", float),
     "anisoperturbstddev": ("aniso_perturb_std_dev", float), "acceptanceMap": ("acceptance_map", int),
+    "uniformmixingprobability": ("uniform_mixing_prob", float), "lensperturbprob": ("lens_perturb_prob", float),
 }
 
 
@@ -331,9 +333,9 @@ def write_exr(path, rgb, compression=EXR_ZIP):
 def stats_text(stats: dict, reference_labels=True, integrator="drmlt"):
     """Stats::output of a stats dict (Context.stats()) as the reference prints
     it: the DR integrator's text (with its swapped labels, or corrected ones),
-    or the mcmc integrator's."""
+    or the mcmc / h2mc integrators' (small steps: first-stage plus H2MC moves)."""
     s = Stats(*[int(stats[n]) for n, _ in Stats._fields_])
-    fmt = 2 if integrator == "mcmc" else int(bool(reference_labels))
+    fmt = 2 if integrator in ("mcmc", "h2mc") else int(bool(reference_labels))
     n = lib().drmlt_stats_text(C.byref(s), fmt, None, 0)
     if n < 0:
         check(n)

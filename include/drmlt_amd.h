@@ -93,8 +93,14 @@ typedef struct {
     /* The <dpt> integrator: 0 "drmlt" (DRIntegrator, delayed.cpp), 1 "mcmc"
      * (MCMCIntegrator, src/integrators/mcmc.cpp:11-244: the loop's first
      * stage alone -- a large or isotropic small step, splats weighted 1 - a
-     * and a, acceptance with probability a; no acceptance map). */
+     * and a, acceptance with probability a; no acceptance map), 2 "h2mc"
+     * (H2MCIntegrator, src/integrators/h2mc.cpp:196-600, without lens
+     * perturbations: lens_perturb_prob must be 0 -- a small step is
+     * isotropic with probability uniform_mixing_prob, else one H2MC move
+     * accepted with min(1, ratio)). */
     int32_t integrator;
+    float uniform_mixing_prob;  /* uniformmixingprobability (dptoptions.h:29, h2mc only) */
+    float lens_perturb_prob;    /* lensperturbprob (dptoptions.h:20); h2mc needs 0 here */
     uint64_t seed;
 } drmlt_options;
 
@@ -213,12 +219,12 @@ int drmlt_write_exr(const char *path, const float *rgb, int32_t w, int32_t h, in
 /* Stats::output (delayed.cpp:311-324) as text into buf (NUL-terminated,
  * truncated to cap - 1 bytes); returns the full length. format 1 keeps the
  * reference's swapped second-stage labels (delayed.cpp:320-321), 0 names
- * each line's own component, 2 is the mcmc integrator's text
- * (mcmc.cpp:76-86: large and small steps; lens steps, which it never
- * makes, count 0). */
+ * each line's own component, 2 is the mcmc and h2mc integrators' text
+ * (mcmc.cpp:76-86, h2mc.cpp:140-150: large steps, small steps -- first-stage
+ * plus H2MC moves -- and lens steps, which neither makes here, at 0). */
 int drmlt_stats_text(const drmlt_stats *stats, int32_t format, char *buf, int32_t cap);
 /* The same text of the context's counters into a file (truncated); a context
- * of the mcmc integrator writes format 2 whatever reference_labels says. */
+ * of the mcmc or h2mc integrator writes format 2 whatever reference_labels says. */
 int drmlt_write_stats(drmlt_ctx *ctx, const char *path, int32_t reference_labels);
 
 /* Intermediate outputs of drmlt_render (delayed.cpp:225-242, 610-655). */

@@ -378,6 +378,9 @@ extern "C" void orc_render(void *h, const double *opts, const PathRec *seeds, co
     dro.chadMisQuirk = opts[9] != 0;
     dro.fdStep = opts[10];
     dro.secondStage = opts[11] != 0;
+    dro.h2mc = opts[12] != 0;
+    dro.uniformMixing = opts[13];
+    if (dro.h2mc) dro.secondStage = false;
     if (!dro.secondStage) dro.acceptanceMap = false;
     Dist1D<float> ld;
     ld.build(lengthFunc, nLen);

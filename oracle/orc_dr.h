@@ -80,6 +80,12 @@ struct DrOptions {
     // 125-160: large or isotropic small step, splats (1 - a) and a, accept with
     // probability a); the acceptance map is a DR option and stays off there
     bool secondStage = true;
+    // integrator=h2mc with lensPerturbProb = 0 (src/integrators/h2mc.cpp:
+    // 196-250 with H2MCSmallStep::Mutate 448-600 minus its lens branch): a
+    // small step is isotropic with probability uniformMixing, else one H2MC
+    // move accepted with min(1, ratio); no second stage follows
+    bool h2mc = false;
+    double uniformMixing = 0.2;
 };
 
 struct DrStats {
@@ -293,6 +299,15 @@ template <class F> inline void DrIteration(DrChainCtx<F> &cx, MState<F> &cur, MS
         a1 = LargeStepMutate(cx, cur, first, rng);
         if (IsInvalid(a1)) { a1 = 0; acc1 = false; }
         else { acc1 = FlipCoin(a1, F(rng.uni())); if (acc1) st.largeStepAcc++; }
+    } else if (cx.dro.h2mc && !(double(rng.uni()) < cx.dro.uniformMixing)) {
+        // the H2MC move as the only stage: a1 = 0 (so w1 = 0 below), a2 = min(1, ratio)
+        isAniso = true;
+        st.h2mc++;
+        a2 = H2MCMutate(cx, cur, second, rng);
+        if (IsInvalid(a2)) a2 = 0;
+        a2 = std::min(1.0, a2);
+        acc2 = FlipCoin(F(a2), F(rng.uni()));
+        if (acc2) st.h2mcAcc++;
     } else {
         st.stage1++;
         a1 = SmallStepMutate(cx, cur, first, cx.dro.perturbStdDev, rng);

@@ -133,11 +133,11 @@ class Scene:
     def render(self, seeds, c8, length_func, b, iters, opts=None, threads=8, seed=7):
         o = {"type": 0, "largestepprob": 0.3, "perturbstddev": 0.01, "anisoperturbprob": 0.5, "anisostddev": 0.0025,
              "discretestddev": 0.01, "roughthr": 0.03, "maxdervdepth": 9, "acceptancemap": 0, "chadquirk": 1, "fdstep": 2e-6,
-             "secondstage": 1}
+             "secondstage": 1, "h2mc": 0, "uniformmixing": 0.2}
         o.update(opts or {})
         ov = np.array([o["type"], o["largestepprob"], o["perturbstddev"], o["anisoperturbprob"], o["anisostddev"],
                        o["discretestddev"], o["roughthr"], o["maxdervdepth"], o["acceptancemap"], o["chadquirk"],
-                       o["fdstep"], o["secondstage"]],
+                       o["fdstep"], o["secondstage"], o["h2mc"], o["uniformmixing"]],
                       np.float64)
         seeds = np.ascontiguousarray(seeds, PATH_REC); c8 = np.ascontiguousarray(c8, np.float32)
         lf = np.ascontiguousarray(length_func, np.float32)

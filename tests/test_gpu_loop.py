@@ -239,3 +239,58 @@ def test_mcmc_integrator(which, request):
     gm, om = gfilm.mean(axis=(0, 1)), ofilm.mean(axis=(0, 1))
     say("film mean gpu", gm, "oracle", om)
     assert np.allclose(gm, om, rtol=0.05)
+
+
+@pytest.mark.parametrize("which", ["cfg1", "cfg2_small"])
+def test_h2mc_integrator(which, request):
+    """integrator=h2mc without lens moves (H2MCIntegrator, h2mc.cpp:196-600
+    with lensperturbprob 0) on the same kernels: an isotropic small step with
+    probability 0.2, else one H2MC move accepted with min(1, ratio). The
+    plain, batched and graph forms advance identically; large steps count as
+    in the DR loop from the same seeds; and against the oracle's h2mc loop
+    from the oracle's seeds the acceptance rates of large steps, isotropic
+    moves and H2MC moves and the film mean agree within Monte Carlo error."""
+    from drmlt import capi
+
+    sc = request.getfixturevalue(which)
+    dpt = dict(sc["options"], integrator="h2mc", lensperturbprob=0.0)
+    with pytest.raises(capi.DrmltError, match="lensperturbprob"):
+        capi.Context(sc, capi.options_from_dpt(dict(dpt, lensperturbprob=0.3), num_chains=1024))
+    res = {}
+    for mode in (capi.PLAIN, capi.BATCHED, capi.BATCHED_GRAPH):
+        ctx = capi.Context(sc, capi.options_from_dpt(dpt, num_init_samples=8192, num_chains=4096, seed=5))
+        say(f"{which} h2mc form {mode}: prepass + 2 iterations")
+        ctx.prepass()
+        ctx.run(2, mode)
+        res[mode] = ctx.stats()
+        ctx.close()
+    say("h2mc", res[capi.PLAIN])
+    assert res[capi.PLAIN] == res[capi.BATCHED] == res[capi.BATCHED_GRAPH]
+    st = res[capi.PLAIN]
+    assert st["stage2_iso"] == 0 and st["stage2_h2mc"] > 0 and st["stage1"] > 0
+    assert st["large_step"] + st["stage1"] + st["stage2_h2mc"] == 2 * 4096
+
+    O = orc.Scene(sc)
+    maxd = int(sc["options"]["maxdepth"])
+    chains, iters = 2048, 32
+    b, lf, seeds, c8, _ = O.prepass(20000, maxd, chains, seed=13)
+    ctx = capi.Context(sc, capi.options_from_dpt(dpt, num_chains=chains, seed=17))
+    say(f"h2mc loop {which} {chains} chains x {iters} iterations vs the oracle")
+    ctx.seed_chains(seeds, c8, b, lf)
+    ctx.run(iters, capi.BATCHED_GRAPH)
+    gfilm, gs = ctx.film(1), ctx.stats()
+    ctx.close()
+    ofilm, ost, _ = O.render(seeds, c8, lf, b, iters, seed=19, opts={"h2mc": 1})
+    names = ["large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso", "stage2_iso_accepted",
+             "stage2_h2mc", "stage2_h2mc_accepted"]
+    os_ = dict(zip(names, [int(v) for v in ost]))
+    say("gpu", gs, "\noracle", os_)
+    for a, t in (("large_step_accepted", "large_step"), ("stage1_accepted", "stage1"), ("stage2_h2mc_accepted", "stage2_h2mc")):
+        rg, ro = gs[a] / max(gs[t], 1), os_[a] / max(os_[t], 1)
+        tol = 4 * np.sqrt(ro * (1 - ro) / max(os_[t], 1)) + 0.01
+        assert abs(rg - ro) < tol, (a, rg, ro, tol)
+    gsmall, osmall = gs["stage1"] + gs["stage2_h2mc"], os_["stage1"] + os_["stage2_h2mc"]
+    assert abs(gs["stage1"] / gsmall - 0.2) < 0.02 and abs(os_["stage1"] / osmall - 0.2) < 0.02
+    gm, om = gfilm.mean(axis=(0, 1)), ofilm.mean(axis=(0, 1))
+    say("film mean gpu", gm, "oracle", om)
+    assert np.allclose(gm, om, rtol=0.05)

@@ -106,3 +106,17 @@ def test_mcmc_render_writes_the_mcmc_stats_text(cfg1, tmp_path):
     assert st["stage2_iso"] == st["stage2_h2mc"] == 0 and st["large_step"] + st["stage1"] == 8 * 1024
     assert (tmp_path / "mcmc_partial" / "mcmc_stats.txt").read_text() == capi.stats_text(st, integrator="mcmc")
     ctx.close()
+
+
+def test_h2mc_render_writes_its_stats_text(cfg1, tmp_path):
+    from drmlt import capi
+
+    opt = capi.options_from_dpt(dict(cfg1["options"], integrator="h2mc", lensperturbprob=0.0), num_init_samples=8192,
+                                num_chains=1024, seed=25)
+    ctx = capi.Context(cfg1, opt)
+    ctx.prepass()
+    ctx.render(8, capi.BATCHED_GRAPH, partial_dir=tmp_path / "h2mc_partial", base_name="h2mc")
+    st = ctx.stats()
+    assert st["large_step"] + st["stage1"] + st["stage2_h2mc"] == 8 * 1024
+    assert (tmp_path / "h2mc_partial" / "h2mc_stats.txt").read_text() == capi.stats_text(st, integrator="h2mc")
+    ctx.close()

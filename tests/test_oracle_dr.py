@@ -85,6 +85,37 @@ def test_mcmc_converges_to_bdpt(small):
     assert ib.sum() == pytest.approx(rb.sum(), rel=0.05)
 
 
+def test_h2mc_converges_to_bdpt(small):
+    """The oracle's h2mc integrator without lens moves (h2mc.cpp:196-600:
+    isotropic small steps with probability 0.2, else single-stage H2MC
+    moves accepted with min(1, ratio)) converges to the same independent
+    bidirectional estimate."""
+    sc, s = small
+    W = H = 64
+    rng = np.random.default_rng(2)
+    N = 300000
+    uni = rng.uniform(0, 1, (N, 160)).astype(np.float32)
+    cs, nc = s.generate_bidir(uni, 3, 5, dbl=False)
+    c = cs[np.arange(cs.shape[1])[None] < nc[:, None]]
+    ix = np.clip((c[:, 7] * W).astype(int), 0, W - 1)
+    iy = np.clip((c[:, 8] * H).astype(int), 0, H - 1)
+    ref = np.zeros((H, W, 3))
+    np.add.at(ref, (iy, ix), c[:, 2:5])
+    ref *= (W * H) / N
+    b, lf, seeds, c8, _ = s.prepass(50000, 5, 256, seed=3)
+    iters = 2000
+    film, stats, _ = s.render(seeds, c8, lf, b, iters, threads=8, seed=7, opts={"h2mc": 1})
+    assert stats[4] == stats[5] == 0 and stats[0] + stats[2] + stats[6] == iters * 256
+    small_moves = stats[2] + stats[6]
+    assert abs(stats[2] / small_moves - 0.2) < 0.01  # the uniform-mixing share
+    img = film / (iters * 256 / (W * H))
+    rb, ib = _blocks(ref), _blocks(img)
+    mask = rb > 0.05 * rb.max()
+    rel = np.abs(ib[mask] - rb[mask]) / rb[mask]
+    assert np.median(rel) < 0.08, np.median(rel)
+    assert ib.sum() == pytest.approx(rb.sum(), rel=0.05)
+
+
 def test_stats_structure(small):
     sc, s = small
     b, lf, seeds, c8, _ = s.prepass(20000, 5, 64, seed=4)

@@ -141,3 +141,12 @@ def test_mcmc_stats_text():
             "    Lens mutations acceptance ratio    :                    -nan (0)\n"
             "\n")
     assert capi.stats_text(s, integrator="mcmc") == want
+
+
+def test_h2mc_stats_text():
+    """h2mc's Stats::output (h2mc.cpp:140-150) has the mcmc layout; its small
+    steps are the isotropic mixing moves plus the H2MC moves."""
+    s = dict(large_step=10, large_step_accepted=1, stage1=20, stage1_accepted=10, stage2_iso=0, stage2_iso_accepted=0,
+             stage2_h2mc=80, stage2_h2mc_accepted=30)
+    t = capi.stats_text(s, integrator="h2mc").splitlines()
+    assert t[3] == "    Small step mutations acceptance ratio       : 0.4 (100)"

@@ -187,7 +187,7 @@ def test_serialized_meshes(tmp_path, version, double):
 
 
 def test_dpt_options(tmp_path):
-    dpt = {"integrator": "drmlt", "spp": 64, "numinitsamples": 5000, "largestepprob": 0.25, "mindepth": 2, "maxdepth": 7,
+    dpt = {"integrator": "drmlt", "uniformmixingprobability": 0.35, "lensperturbprob": 0.0, "spp": 64, "numinitsamples": 5000, "largestepprob": 0.25, "mindepth": 2, "maxdepth": 7,
            "directspp": 4, "perturbstddev": 0.02, "roughnessthreshold": 0.05, "numchains": 2048, "seedoffset": 3,
            "uselightcoordinatesampling": False, "largestepmultiplexed": True, "type": "mira", "anisoperturbprob": 0.7,
            "anisoperturbstddev": 0.004, "acceptanceMap": True, "reportintervalseconds": 30}
@@ -227,7 +227,7 @@ def test_malformed_files_are_rejected(tmp_path):
         capi.load_scene(tmp_path / "flip.xml")
 
 
-@pytest.mark.parametrize("name,code", [("drmlt", 0), ("mcmc", 1), ("h2mc", -1)])
+@pytest.mark.parametrize("name,code", [("drmlt", 0), ("mcmc", 1), ("h2mc", 2), ("mc", -1)])
 def test_integrator_option(tmp_path, name, code):
     ld = capi.load_scene(export_scene(scenes.config1(), tmp_path, dpt={"integrator": name}))
     assert ld["integrator"] == name and ld["opt"].integrator == code
