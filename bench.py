@@ -238,7 +238,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--tune-steps", type=int, default=3, help="timed steps per candidate schedule of the autotuner")
     ap.add_argument("--no-deriv", action="store_true", help="skip the derivative-kernel workload (config 3)")
-    ap.add_argument("--no-mcmc", action="store_true", help="skip the mcmc integrator's step time")
+    ap.add_argument("--no-mcmc", action="store_true", help="skip the step times of the mcmc and h2mc integrators")
     ap.add_argument("--rmse-config", type=int, default=2, help="scene configuration of the equal-time RMSE study (0: none)")
     ap.add_argument("--rmse-budget", type=float, default=2.0, help="seconds each form renders in the RMSE study")
     ap.add_argument("--rmse-ref-steps", type=int, default=2000, help="batched steps of the RMSE study's reference render")
@@ -343,21 +343,23 @@ def main():
 
     ctx.close()
 
-    # the mcmc integrator (the first stage alone) on the same scene and chains
-    mcmc = None
+    # the mcmc integrator (the first stage alone) and the h2mc integrator
+    # (lens moves off) on the same scene and chains
+    others = {}
     if not args.no_mcmc:
-        mctx = capi.Context(sc, capi.options_from_dpt(dict(dpt, integrator="mcmc"), num_chains=chains, seed=1234))
-        mctx.set_schedule(**sched)  # the path-stage scheduling tuned for these chains
-        mctx.prepass()
-        mctx.run(args.warmup, capi.BATCHED_GRAPH)
-        m0 = mctx.stats()
-        mctx.run(args.steps, capi.BATCHED_GRAPH)
-        mms = mctx.timing()[0] / args.steps
-        macc = accepted(mctx.stats()) - accepted(m0)
-        mctx.close()
-        mcmc = {"ms_per_step": round(mms, 4), "mutations_per_s": round(chains / (mms / 1000.0), 1),
-                "accepted_mutations_per_s": round(macc / (args.steps * mms / 1000.0), 1)}
-        log(f"mcmc integrator {mms:.3f} ms/step")
+        for name, extra in (("mcmc", {}), ("h2mc", {"lensperturbprob": 0.0})):
+            mctx = capi.Context(sc, capi.options_from_dpt(dict(dpt, integrator=name, **extra), num_chains=chains, seed=1234))
+            mctx.set_schedule(**sched)  # the path-stage scheduling tuned for these chains
+            mctx.prepass()
+            mctx.run(args.warmup, capi.BATCHED_GRAPH)
+            m0 = mctx.stats()
+            mctx.run(args.steps, capi.BATCHED_GRAPH)
+            mms = mctx.timing()[0] / args.steps
+            macc = accepted(mctx.stats()) - accepted(m0)
+            mctx.close()
+            others[name] = {"ms_per_step": round(mms, 4), "mutations_per_s": round(chains / (mms / 1000.0), 1),
+                            "accepted_mutations_per_s": round(macc / (args.steps * mms / 1000.0), 1)}
+            log(f"{name} integrator {mms:.3f} ms/step")
     study = None
     if args.rmse_config:
         study = rmse_study(args.rmse_config, args.rmse_budget, args.rmse_ref_steps, sched,
@@ -394,7 +396,8 @@ def main():
         # the forms advance identical chains (identical acceptance statistics, tests/test_gpu_loop.py), so
         # equal RMSE is reached after equal iteration counts and the time ratio is the step-time ratio
         "equal_rmse_time_ratio_plain_over_batched": round(plain_ms / ms_per_step, 3),
-        "mcmc_integrator": mcmc,
+        "mcmc_integrator": others.get("mcmc"),
+        "h2mc_integrator": others.get("h2mc"),
         "stage_ms": {n: round(float(v), 4) for n, v in zip(STAGES, stage_ms)},
         "acceptance": st,
         "roofline": roof,

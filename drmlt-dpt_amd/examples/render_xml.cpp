@@ -1,5 +1,5 @@
-// Renders a scene file of the reference's XML format with integrator drmlt
-// or mcmc through the C ABI of libdrmlt_hip.so, as the reference's main
+// Renders a scene file of the reference's XML format with integrator drmlt,
+// mcmc or h2mc (lensperturbprob 0) through the C ABI of libdrmlt_hip.so, as the reference's main
 // does with ParseScene and the integrator's Render (src/main.cpp:101-134):
 // pre-pass, direct lighting, the chain loop with the partial outputs of
 // <dpt> reportintervalseconds in <output dir>/<integrator>_partial, and the
