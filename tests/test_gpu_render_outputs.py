@@ -120,3 +120,40 @@ def test_h2mc_render_writes_its_stats_text(cfg1, tmp_path):
     assert st["large_step"] + st["stage1"] + st["stage2_h2mc"] == 8 * 1024
     assert (tmp_path / "h2mc_partial" / "h2mc_stats.txt").read_text() == capi.stats_text(st, integrator="h2mc")
     ctx.close()
+
+
+def test_acceptance_map(cfg1, tmp_path):
+    """acceptanceMap (delayed.cpp:246-275): radiance splats are replaced by a
+    (1, 0, 0) splat per accepted first-stage small step and (0, 1, 0) per
+    accepted second stage, the image the reference's tools/heatmap.py reads.
+    The film holds integer counts, so the plain and graph-replayed batched
+    forms (identical chains) give identical maps. The splat goes to the state
+    being left; seed states carry no splat list, and a chain leaves its seed at
+    most once, so red and green fall short of the accepted counts by at most
+    one per chain in total. The map survives the EXR round trip."""
+    from drmlt import capi
+    from exr_read import read_exr
+
+    films = {}
+    for mode in (capi.PLAIN, capi.BATCHED_GRAPH):
+        opt = capi.options_from_dpt(dict(cfg1["options"], acceptanceMap=True), num_init_samples=8192, num_chains=2048, seed=26)
+        ctx = capi.Context(cfg1, opt)
+        ctx.prepass()
+        print(f"acceptance map, form {mode}: 12 iterations", flush=True)
+        ctx.run(12, mode)
+        films[mode] = (ctx.film(1), ctx.stats())
+        ctx.close()
+    (f, st), (fb, stb) = films[capi.PLAIN], films[capi.BATCHED_GRAPH]
+    assert st == stb and np.array_equal(f, fb)
+    assert np.array_equal(f, np.round(f)) and f[..., 2].sum() == 0
+    red, green = f[..., 0].sum(), f[..., 1].sum()
+    acc2 = st["stage2_iso_accepted"] + st["stage2_h2mc_accepted"]
+    print("red", red, "stage-1 small accepted", st["stage1_accepted"], "green", green, "stage-2 accepted", acc2, flush=True)
+    assert red <= st["stage1_accepted"] and green <= acc2
+    assert (st["stage1_accepted"] - red) + (acc2 - green) <= 2048
+    capi.write_exr(tmp_path / "map.exr", f, capi.EXR_NONE)
+    ch, _ = read_exr(tmp_path / "map.exr")
+    assert np.array_equal(ch["R"].astype(np.float32), f[..., 0]) and np.array_equal(ch["G"].astype(np.float32), f[..., 1])
+    # heatmap.py's stage map, green / (red + green + eps), lies in [0, 1)
+    ratio = f[..., 1] / (f[..., 0] + f[..., 1] + 1e-2)
+    assert ratio.min() >= 0 and ratio.max() < 1
