@@ -44,6 +44,11 @@ template <class T> static hipError_t upload(T **p, const T *src, size_t n, hipSt
     if (n) e = hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, s);
     return e;
 }
+// device buffer freed when it goes out of scope (also on an early error return)
+template <class T> struct DBuf {
+    T *p = nullptr;
+    ~DBuf() { if (p) (void)hipFree(p); }
+};
 
 // PiecewiseConstant1D cdf (distribution.h:8-33) in float
 static void build_cdf(const float *f, int n, std::vector<float> &cdf, float &funcInt) {
@@ -494,11 +499,14 @@ extern "C" int drmlt_prepass(drmlt_ctx *c, double *bOut, float *lfOut, int32_t n
     const int N = c->opt.num_init_samples, C = c->opt.num_chains, maxDepth = c->opt.max_depth;
     if (N <= 0 || C <= 0) return fail(DRMLT_ERR_ARG, "num_init_samples and num_chains must be positive");
     hipStream_t s = c->stream;
-    int *cnt, *cl;
-    float *ls;
-    double *lenSum, *total;
-    HIPCHK(dalloc(&cnt, N)); HIPCHK(dalloc(&cl, size_t(N) * kPreMaxC)); HIPCHK(dalloc(&ls, size_t(N) * kPreMaxC));
-    HIPCHK(dalloc(&lenSum, kMaxBins)); HIPCHK(dalloc(&total, 1));
+    DBuf<int> bCnt, bCl;
+    DBuf<float> bLs;
+    DBuf<double> bLenSum, bTotal;
+    HIPCHK(dalloc(&bCnt.p, N)); HIPCHK(dalloc(&bCl.p, size_t(N) * kPreMaxC)); HIPCHK(dalloc(&bLs.p, size_t(N) * kPreMaxC));
+    HIPCHK(dalloc(&bLenSum.p, kMaxBins)); HIPCHK(dalloc(&bTotal.p, 1));
+    int *cnt = bCnt.p, *cl = bCl.p;
+    float *ls = bLs.p;
+    double *lenSum = bLenSum.p, *total = bTotal.p;
     HIPCHK(hipMemsetAsync(lenSum, 0, kMaxBins * 8, s));
     HIPCHK(hipMemsetAsync(total, 0, 8, s));
     DrParams P = make_params(c);
@@ -515,7 +523,6 @@ extern "C" int drmlt_prepass(drmlt_ctx *c, double *bOut, float *lfOut, int32_t n
     HIPCHK(hipMemcpyAsync(hlen.data(), lenSum, kMaxBins * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&htot, total, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    (void)hipFree(cnt); (void)hipFree(cl); (void)hipFree(ls); (void)hipFree(lenSum); (void)hipFree(total);
     // candidates in sample order, equal-spaced seeding (Veach's thesis p. 340, delayed.cpp:123-167)
     std::vector<int> candSample, candCl;
     std::vector<double> cdf(1, 0.0);
@@ -547,14 +554,12 @@ extern "C" int drmlt_prepass(drmlt_ctx *c, double *bOut, float *lfOut, int32_t n
     }
     int r = alloc_chains(c, C);
     if (r) return r;
-    int *dS, *dCl;
-    HIPCHK(upload(&dS, selS.data(), C, s));
-    HIPCHK(upload(&dCl, selCl.data(), C, s));
-    hipLaunchKernelGGL(k_prepass_seed, dim3((C + 63) / 64), dim3(64), 0, s, c->S, P.o, c->ch, maxDepth, P.seed ^ 0x5151515151ull, dS, dCl);
+    DBuf<int> dS, dCl;
+    HIPCHK(upload(&dS.p, selS.data(), C, s));
+    HIPCHK(upload(&dCl.p, selCl.data(), C, s));
+    hipLaunchKernelGGL(k_prepass_seed, dim3((C + 63) / 64), dim3(64), 0, s, c->S, P.o, c->ch, maxDepth, P.seed ^ 0x5151515151ull, dS.p, dCl.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
-    (void)hipFree(dS);
-    (void)hipFree(dCl);
     c->b = float(htot / double(N));
     r = set_length_dist(c, lf.data(), nLen);
     if (r) return r;
@@ -912,8 +917,9 @@ extern "C" int drmlt_count_work(drmlt_ctx *c, uint64_t *out) {
     if (!c || !out) return fail(DRMLT_ERR_ARG, "bad argument");
     if (!c->haveChains || !c->lenCdf) return fail(DRMLT_ERR_STATE, "chains are not seeded");
     hipStream_t s = c->stream;
-    unsigned long long *d;
-    HIPCHK(dalloc(&d, 28));
+    DBuf<unsigned long long> buf;
+    HIPCHK(dalloc(&buf.p, 28));
+    unsigned long long *d = buf.p;
     HIPCHK(hipMemsetAsync(d, 0, 28 * 8, s));
     HIPCHK(hipMemcpyAsync(c->iterDev, &c->iterHost, 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemsetAsync(c->L.count, 0, kListKinds * kMaxBins * 4, s));
@@ -922,17 +928,12 @@ extern "C" int drmlt_count_work(drmlt_ctx *c, uint64_t *out) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, d, 28 * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    (void)hipFree(d);
     c->iterHost += 1;
     c->itersDone += 1;
     return DRMLT_OK;
 }
 
 // ---------------------------------------------------------------- tier 2
-template <class T> struct DBuf {
-    T *p = nullptr;
-    ~DBuf() { if (p) (void)hipFree(p); }
-};
 #define DUP(buf, src, n) HIPCHK(upload(&buf.p, src, size_t(n), s))
 #define DNEW(buf, n) HIPCHK(dalloc(&buf.p, size_t(n)))
 #define DGET(dst, buf, n) HIPCHK(hipMemcpyAsync(dst, buf.p, size_t(n) * sizeof(*buf.p), hipMemcpyDeviceToHost, s))

@@ -510,13 +510,19 @@ Mesh load_serialized(const std::string &file, int shapeIndex, const M4 &toWorld)
     std::unique_ptr<z_stream, int (*)(z_stream *)> guard(&z, inflateEnd);
     z.next_in = &all[off];
     z.avail_in = uInt(all.size() - off);
-    auto read = [&](void *p, size_t n) {
-        z.next_out = static_cast<Bytef *>(p);
-        z.avail_out = uInt(n);
-        while (z.avail_out > 0) {
-            const int r = inflate(&z, Z_NO_FLUSH);
-            if (r == Z_STREAM_END && z.avail_out > 0) throw Err("serialized stream ends early in " + file);
-            if (r != Z_OK && r != Z_STREAM_END) throw Err("serialized stream is corrupt in " + file);
+    auto read = [&](void *p, size_t n) {  // in pieces: zlib counts bytes in 32 bits
+        Bytef *dst = static_cast<Bytef *>(p);
+        while (n > 0) {
+            const size_t piece = std::min(n, size_t(1) << 30);
+            z.next_out = dst;
+            z.avail_out = uInt(piece);
+            while (z.avail_out > 0) {
+                const int r = inflate(&z, Z_NO_FLUSH);
+                if (r == Z_STREAM_END && z.avail_out > 0) throw Err("serialized stream ends early in " + file);
+                if (r != Z_OK && r != Z_STREAM_END) throw Err("serialized stream is corrupt in " + file);
+            }
+            dst += piece;
+            n -= piece;
         }
     };
     uint32_t flags;
@@ -530,6 +536,9 @@ Mesh load_serialized(const std::string &file, int shapeIndex, const M4 &toWorld)
     read(&nv, 8);
     read(&nt, 8);
     if (nv > (1ull << 31) || nt > (1ull << 31)) throw Err("serialized mesh too large: " + file);
+    // deflate expands at most 1032:1, so larger arrays than the stream can hold mean a corrupt header
+    // (checked before anything is allocated)
+    if ((nv * 12 + nt * 12) / 1032 > all.size() - off) throw Err("serialized mesh header exceeds the stream in " + file);
     const bool dbl = (flags & 0x2000) != 0;
     auto vecs = [&](size_t count, int dim, std::vector<float> &out) {
         out.resize(count * dim);

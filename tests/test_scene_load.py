@@ -248,3 +248,16 @@ def test_corrupt_serialized_is_rejected(tmp_path):
     (tmp_path / "bad2.serialized").write_bytes(bytes(raw))
     with pytest.raises(capi.DrmltError):
         capi.load_scene(_one_shape_scene(tmp_path, xml.replace("bad.serialized", "bad2.serialized").replace('value="1"', 'value="0"')))
+
+
+def test_serialized_header_beyond_the_stream_is_rejected(tmp_path):
+    """Vertex and triangle counts that a stream of this length cannot hold
+    (deflate expands at most 1032:1) are refused before any allocation."""
+    import struct
+    import zlib
+    body = struct.pack("<I", 0x1000) + b"mesh\0" + struct.pack("<QQ", 1 << 30, 1 << 30) + bytes(64)
+    (tmp_path / "huge.serialized").write_bytes(struct.pack("<HH", 0x041C, 4) + zlib.compress(body) + struct.pack("<QI", 0, 1))
+    xml = ('<shape type="serialized"><string name="filename" value="huge.serialized"/>'
+           '<integer name="shapeIndex" value="0"/><ref id="white"/></shape>')
+    with pytest.raises(capi.DrmltError, match="exceeds the stream"):
+        capi.load_scene(_one_shape_scene(tmp_path, xml))
