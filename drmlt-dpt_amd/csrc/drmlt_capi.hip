@@ -625,7 +625,7 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
         if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin_counted, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
         hipLaunchKernelGGL(k_b_stage1_counted, dim3(G.stage), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
         if (P.integrator != 1) {
-            hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 0, 0);
+            hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 4, 0);
             hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0, c->iterDev);
             hipLaunchKernelGGL(k_b_stage2_propose_counted, dim3(G.stage), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
             hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
@@ -639,7 +639,7 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
     if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
     hipLaunchKernelGGL(k_b_stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[1], s);
-    if (P.integrator != 1) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0);
+    if (P.integrator != 1) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 4, 0);
     if (timed) (void)hipEventRecord(c->ev[2], s);
     if (P.integrator != 1) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[3], s);

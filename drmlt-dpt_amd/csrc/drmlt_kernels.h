@@ -68,8 +68,11 @@ struct DrParams {
 // kind 2: H2MC chains whose proposal is valid (its Gaussian and the ratio);
 // kind 3: every chain, binned by the length of the path its first stage
 //         evaluates (the current length for a small step, the sampled one
-//         for a large step).
-constexpr int kListKinds = 4;
+//         for a large step);
+// kind 4: the H2MC chains of kind 0 whose current state needs derivatives
+//         (no cached Gaussian, a path within the derivative depth, a
+//         positive score).
+constexpr int kListKinds = 5;
 struct Lists {
     int *count;  // [kMaxBins] per list kind
     int *items;  // [kMaxBins][C]

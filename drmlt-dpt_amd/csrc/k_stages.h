@@ -103,11 +103,14 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)
         out.valid = false;
         walk_run(Sc, P.o, w, !a.h2mcMove, out);
         const int flags = stage1_end(Sc, ch, P, chain, a, out, g, film, stats);
-        // second stage: list 1 (every chain with one), list 0 (H2MC)
-        const bool second = (flags & 2) != 0;
-        const int bin = length_bin(ch.c[a.pc] + ch.l[a.pc] - 1);
+        // second stage: list 1 (every chain with one), list 0 (H2MC), list 4
+        // (H2MC needing the current state's derivatives)
+        const bool second = (flags & 2) != 0, h2mc = second && (flags & 4);
+        const int pc = a.pc, bin = length_bin(ch.c[pc] + ch.l[pc] - 1);
         list_push_wave(L, 1, ch.C, bin, chain, second);
-        list_push_wave(L, 0, ch.C, bin, chain, second && (flags & 4));
+        list_push_wave(L, 0, ch.C, bin, chain, h2mc);
+        list_push_wave(L, 4, ch.C, bin, chain,
+                       h2mc && !ch.gInit[pc] && has_derivative(ch.c[pc], ch.l[pc], P.maxDervDepth) && double(ch.ss[pc]) > 1e-15);
     }
 }
 
@@ -229,8 +232,9 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
     }
 }
 
-// Derivative passes over list `kind` (slot: 0 current, 2 second), persistent
-// over (chain, pass) work items bin by bin.
+// Derivative passes over list `kind` (4 with which 0: the current states that
+// need them; 2 with which 1: the proposals), persistent over (chain, pass)
+// work items bin by bin.
 __global__ void __launch_bounds__(64, DRMLT_DERIV_WAVES) DRMLT_STAGE(k_b_deriv)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which) {
     // bin b holds paths of length b (D = 2 max(b, 2)); its work items are
     // (chain, pass) pairs, all bins concatenated
