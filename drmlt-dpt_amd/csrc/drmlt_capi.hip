@@ -166,8 +166,13 @@ struct drmlt_ctx {
     double timing[8] = {0};
     hipEvent_t ev[9];
     bool evInit = false;
+    // parts 1.. of the split step: their streams (default priority), join
+    // events and work lists (part 0 runs on `stream` with `L`)
+    hipStream_t streamX[kMaxParts - 1] = {};
+    hipEvent_t evFork = nullptr, evJoin[kMaxParts - 1] = {};
+    Lists LX[kMaxParts - 1];
     // launch schedule of the batched step (drmlt_set_schedule / drmlt_autotune)
-    drmlt_schedule sched = {0, 0, 0, 0, 0};
+    drmlt_schedule sched = {0, 0, 0, 0, 0, 0};
 };
 
 static DrParams make_params(const drmlt_ctx *c) {
@@ -197,6 +202,8 @@ static DrParams make_params(const drmlt_ctx *c) {
     P.oDeriv.chadMisQuirk = o.quirk_chad_mis;
     P.binStages = c->sched.bin_stages;
     P.stageLanes = c->sched.stage_lanes > 0 ? c->sched.stage_lanes : 64;
+    P.chainBase = 0;
+    P.chainCount = c->ch.C;
     return P;
 }
 
@@ -247,8 +254,15 @@ extern "C" int drmlt_ctx_destroy(drmlt_ctx *c) {
     free_list(c->chainAllocs);
     for (void *p : {(void *)c->lenFunc, (void *)c->lenCdf, (void *)c->filmInd, (void *)c->filmDir, (void *)c->stats, (void *)c->iterDev})
         if (p) (void)hipFree(p);
+    for (hipStream_t x : c->streamX)
+        if (x) (void)hipStreamSynchronize(x);
     if (c->evInit)
         for (auto &e : c->ev) (void)hipEventDestroy(e);
+    if (c->evFork) (void)hipEventDestroy(c->evFork);
+    for (hipEvent_t e : c->evJoin)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t x : c->streamX)
+        if (x) (void)hipStreamDestroy(x);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return DRMLT_OK;
@@ -271,7 +285,13 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     auto bail = [&](int code) { drmlt_ctx_destroy(c); return code; };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(DRMLT_ERR_HIP, "hipGetDevice"));
     (void)hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(DRMLT_ERR_HIP, "stream"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(DRMLT_ERR_HIP, "stream"));
+    for (int k = 0; k < kMaxParts - 1; k++)
+        if (hipStreamCreateWithFlags(&c->streamX[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->evJoin[k], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(DRMLT_ERR_HIP, "stream"));
     const int nt = sc->num_triangles;
     c->nt = nt;
     // BVH over the triangles (host build, leaf-order copy for traversal)
@@ -403,14 +423,15 @@ static int alloc_chains(drmlt_ctx *c, int C) {
               A(&ch.ls, NS) && A(&ch.ss, NS) && A(&ch.perm, size_t(C)) && A(&ch.gauss, size_t(kGSlot) * GS) &&
               A(&ch.off, 32 * size_t(C)) && A(&ch.grad, size_t(kGMax) * C) && A(&ch.hess, size_t(kGMax * kGMax) * C) &&
               A(&ch.a1, size_t(C)) && A(&ch.a2, size_t(C)) && A(&ch.a2d, size_t(C)) && A(&ch.flags, size_t(C)) &&
-              A(&c->L.count, kListKinds * kMaxBins) &&
-              A(&c->L.items, size_t(kListKinds) * kMaxBins * C);
+              A(&c->L.count, kListKinds * kMaxBins) && A(&c->L.items, size_t(kListKinds) * kMaxBins * C);
+    for (Lists &lx : c->LX) ok = ok && A(&lx.count, kListKinds * kMaxBins) && A(&lx.items, size_t(kListKinds) * kMaxBins * C);
     if (!ok) return fail(DRMLT_ERR_HIP, "chain state allocation");
     hipStream_t s = c->stream;
     // every state field starts zeroed: c = l = 0 is an empty path
     for (size_t k = 0; k < c->chainAllocs.size(); k++)
         if (c->chainAllocs[k]) HIPCHK(hipMemsetAsync(c->chainAllocs[k], 0, c->chainBytes[k], s));
     (void)hipMemsetAsync(c->L.count, 0, kListKinds * kMaxBins * 4, s);
+    for (Lists &lx : c->LX) (void)hipMemsetAsync(lx.count, 0, kListKinds * kMaxBins * 4, s);
     c->haveChains = true;
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -594,8 +615,8 @@ static int persistent_blocks(const drmlt_ctx *c) { return c->cus * 16; }
 struct StepGrid {
     int stage, deriv, gauss;
 };
-static StepGrid step_grid(const drmlt_ctx *c) {
-    const int C = c->ch.C;
+// (for `C` chains)
+static StepGrid step_grid(const drmlt_ctx *c, int C) {
     const int pb = persistent_blocks(c);
     StepGrid g;
     const int lanes = c->sched.stage_lanes > 0 ? c->sched.stage_lanes : 64;
@@ -613,9 +634,23 @@ static StepGrid step_grid(const drmlt_ctx *c) {
 // `counters` (7 x 4 words) the counting forms of the stage kernels run
 // instead, each accumulating its traversal work and state bytes into its own
 // row.
+// The step's kernels for the chains of P on stream s with lists L (the
+// split step's halves).
+static void launch_step_kernels(drmlt_ctx *c, const DrParams &P, hipStream_t s, const Lists &L, const StepGrid &G) {
+    if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, L);
+    hipLaunchKernelGGL(k_b_stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
+    if (P.integrator == 1) return;
+    hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, L, 4, 0);
+    hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, L, 0, 0, 0, c->iterDev);
+    hipLaunchKernelGGL(k_b_stage2_propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, L);
+    hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, L, 2, 1);
+    hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, L, 2, 1, 1, c->iterDev);
+    hipLaunchKernelGGL(k_b_stage2_accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
+}
+
 static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, unsigned long long *counters = nullptr) {
     hipStream_t s = c->stream;
-    const StepGrid G = step_grid(c);
+    const StepGrid G = step_grid(c, c->ch.C);
     if (counters) {
         DevSceneCounted Sk[7];
         for (int k = 0; k < 7; k++) {
@@ -632,6 +667,30 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
             hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1, c->iterDev);
             hipLaunchKernelGGL(k_b_stage2_accept_counted, dim3(G.stage), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
         }
+        hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
+        return;
+    }
+    const int parts = std::min(c->sched.stage_split, c->ch.C);
+    if (!timed && parts > 1) {
+        // the chains in `parts` contiguous ranges, each with its own lists,
+        // on streams that fork and join around the step; the iteration
+        // counter advances after the join
+        const int C = c->ch.C;
+        (void)hipEventRecord(c->evFork, s);
+        for (int k = 0; k < parts; k++) {
+            DrParams Pk = P;
+            Pk.chainBase = int(int64_t(C) * k / parts);
+            Pk.chainCount = int(int64_t(C) * (k + 1) / parts) - Pk.chainBase;
+            hipStream_t sk = k == 0 ? s : c->streamX[k - 1];
+            const Lists &Lk = k == 0 ? c->L : c->LX[k - 1];
+            if (k > 0) (void)hipStreamWaitEvent(sk, c->evFork, 0);
+            launch_step_kernels(c, Pk, sk, Lk, step_grid(c, Pk.chainCount));
+            if (k > 0) {
+                hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, sk, Lk, c->iterDev, 0);
+                (void)hipEventRecord(c->evJoin[k - 1], sk);
+            }
+        }
+        for (int k = 1; k < parts; k++) (void)hipStreamWaitEvent(s, c->evJoin[k - 1], 0);
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
         return;
     }
@@ -669,6 +728,7 @@ extern "C" int drmlt_run(drmlt_ctx *c, int64_t iters, int32_t mode) {
     // the device iteration counter follows the host one (plain form reads the host one)
     HIPCHK(hipMemcpyAsync(c->iterDev, &c->iterHost, 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemsetAsync(c->L.count, 0, kListKinds * kMaxBins * 4, s));
+    for (Lists &lx : c->LX) HIPCHK(hipMemsetAsync(lx.count, 0, kListKinds * kMaxBins * 4, s));
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipEventRecord(e0, s));
     if (mode == DRMLT_PLAIN) {
@@ -725,7 +785,7 @@ extern "C" int drmlt_get_schedule(drmlt_ctx *c, drmlt_schedule *out) {
 extern "C" int drmlt_set_schedule(drmlt_ctx *c, const drmlt_schedule *sc) {
     if (!c || !sc) return fail(DRMLT_ERR_ARG, "bad argument");
     if (sc->bin_stages < 0 || sc->bin_stages > 1 || sc->stage_blocks < 0 || sc->deriv_blocks < 0 || sc->gauss_blocks < 0 ||
-        sc->stage_lanes < 0 || sc->stage_lanes > 64)
+        sc->stage_lanes < 0 || sc->stage_lanes > 64 || sc->stage_split < 0 || sc->stage_split > kMaxParts)
         return fail(DRMLT_ERR_ARG, "schedule out of range");
     c->sched = *sc;
     drop_graph(c);  // the captured step bakes in the launch geometry
@@ -787,6 +847,26 @@ extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best,
             ms += c->timing[kernelSlots[which][1]] / steps;
             if (ms < kMs) { cur = cand; kMs = ms; }
         }
+    }
+    // the split step (graph-replayed form only), by the captured step's whole time
+    auto graphMs = [&](const drmlt_schedule &sc, double &ms) -> int {
+        c->sched = sc;
+        drop_graph(c);
+        int rr = drmlt_run(c, 1, DRMLT_BATCHED_GRAPH);
+        if (rr) return rr;
+        rr = drmlt_run(c, steps, DRMLT_BATCHED_GRAPH);
+        if (rr) return rr;
+        ms = c->timing[0] / steps;
+        return DRMLT_OK;
+    };
+    double splitBest = 0;
+    cur.stage_split = 0;
+    if ((r = graphMs(cur, splitBest))) return r;
+    for (int parts = 2; parts <= kMaxParts; parts++) {
+        cand = cur;
+        cand.stage_split = parts;
+        if ((r = graphMs(cand, ms))) return r;
+        if (ms < splitBest) { cur.stage_split = parts; splitBest = ms; }
     }
     if ((r = measure(cur, 1, 7, curMs))) return r;
     c->sched = cur;

@@ -60,6 +60,7 @@ struct DrParams {
     int integrator;   // 0 drmlt (delayed rejection), 1 mcmc (the first stage alone), 2 h2mc (no lens moves)
     float uniformMixing;  // h2mc: probability of an isotropic small step
     int stageLanes;   // chains per 64-lane workgroup of the path-stage kernels (64, 32 or 16)
+    int chainBase, chainCount;  // the chains [chainBase, chainBase + chainCount) the batched stage kernels take
 };
 
 // Work lists of the batched pipeline, each binned by path length:
@@ -73,6 +74,8 @@ struct DrParams {
 //         (no cached Gaussian, a path within the derivative depth, a
 //         positive score).
 constexpr int kListKinds = 5;
+// most chain ranges of the graph-replayed step on concurrent streams
+constexpr int kMaxParts = 4;
 struct Lists {
     int *count;  // [kMaxBins] per list kind
     int *items;  // [kMaxBins][C]

@@ -72,7 +72,8 @@ DEV int list_item(const Lists &L, int kind, int C, int w) {
 // small step's state is copied into its proposal slot here.
 __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage1_bin)(Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
     const uint32_t iter = *iterp;
-    for (int chain = blockIdx.x * blockDim.x + threadIdx.x; chain < ch.C; chain += gridDim.x * blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.chainCount; i += gridDim.x * blockDim.x) {
+        const int chain = P.chainBase + i;
         Philox g(P.seed, iter, chain, TAG_S1);
         const Stage1 a = stage1_begin(ch, P, chain, g);
         list_push_wave(L, 3, ch.C, length_bin(stage1_length(ch, a)), chain, true);
@@ -92,9 +93,9 @@ DEV int stage_first(const DrParams &P, int n) {
 __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
                                                  unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
-    const int n = P.binStages ? list_length(L, 3) : ch.C;
+    const int n = P.binStages ? list_length(L, 3) : P.chainCount;
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
-        const int chain = P.binStages ? list_item(L, 3, ch.C, it) : it;
+        const int chain = P.binStages ? list_item(L, 3, ch.C, it) : P.chainBase + it;
         Philox g(P.seed, iter, chain, TAG_S1);
         const Stage1 a = stage1_begin(ch, P, chain, g, !P.binStages);
         Walk w;
@@ -184,9 +185,9 @@ __global__ void __launch_bounds__(64, DRMLT_GAUSS_WAVES) DRMLT_STAGE(k_b_gauss)(
 // the lanes of chains without a second stage idle in the walk rounds.
 __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_propose)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
     const uint32_t iter = *iterp;
-    const int n = P.binStages ? list_length(L, 1) : ch.C;
+    const int n = P.binStages ? list_length(L, 1) : P.chainCount;
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
-        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : it;
+        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : P.chainBase + it;
         const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
         Philox g(P.seed, iter, chain, TAG_S2);
         Stage2 a;
@@ -213,9 +214,9 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
 __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_accept)(DevScene Sc, Chains ch, DrParams P, uint32_t *iterp, float *film,
                                                         unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
-    const int n = P.binStages ? list_length(L, 1) : ch.C;
+    const int n = P.binStages ? list_length(L, 1) : P.chainCount;
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
-        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : it;
+        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : P.chainBase + it;
         const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
         Philox g(P.seed, iter, chain, TAG_S3);
         Stage3 a;

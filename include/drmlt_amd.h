@@ -139,6 +139,10 @@ typedef struct {
     int32_t gauss_blocks;  /* one-wave workgroups of the Gaussian kernel (default 16 per CU) */
     int32_t stage_lanes;   /* chains per 64-lane workgroup of the path-stage kernels, 1..64 (default
                               64); fewer helps when the chains fill few waves per SIMD */
+    int32_t stage_split;   /* 2..4: the graph-replayed step runs that many contiguous ranges of the
+                              chains on as many streams that fork and join around it (one range's
+                              kernel tails overlap the others' work); 0 or 1 (default 0): one
+                              stream. The other forms ignore it */
 } drmlt_schedule;
 
 /* Execution forms of drmlt_run */
@@ -261,7 +265,8 @@ int drmlt_get_schedule(drmlt_ctx *ctx, drmlt_schedule *out);
 int drmlt_set_schedule(drmlt_ctx *ctx, const drmlt_schedule *sched);
 /* Autotuner: times candidate schedules on the context's own chains (each
  * candidate one untimed and `steps` timed batched steps, hipEvents on the
- * context stream) and keeps the fastest. The steps are ordinary DR iterations:
+ * context stream; stage_split by `steps` graph-replayed steps with and without
+ * it) and keeps the fastest. The steps are ordinary DR iterations:
  * they advance the chains, count in the statistics and splat into the film.
  * best and best_ms (both optional) receive the schedule and its time per step. */
 int drmlt_autotune(drmlt_ctx *ctx, int32_t steps, drmlt_schedule *best, double *best_ms);

@@ -106,24 +106,34 @@ def test_batched_paths_equal_plain_paths(which, request):
 
 def test_schedules_agree_and_autotune(cfg2_small):
     """Binned and chain-order scheduling of the path-stage kernels, 64, 32 or
-    16 chains per workgroup, and other workgroup counts, advance the chains
-    identically (one iteration from the
-    same seeds gives the same counts); the autotuner returns a schedule the
-    context then runs under, graph-replayed."""
+    16 chains per workgroup, other workgroup counts, and the graph-replayed
+    step split into 2-4 ranges of the chains on as many streams (also for a
+    chain count that none of them divides) advance the chains identically (iterations from the same
+    seeds give the same counts, films within float summation order); the
+    autotuner returns a schedule the context then runs under,
+    graph-replayed."""
     from drmlt import capi
 
-    res = []
-    for sched in ({}, {"bin_stages": 1}, {"bin_stages": 1, "stage_blocks": 16, "deriv_blocks": 64, "gauss_blocks": 32},
-                  {"stage_lanes": 32}, {"bin_stages": 1, "stage_lanes": 16, "stage_blocks": 7}):
-        ctx = _ctx(cfg2_small, num_init_samples=8192, num_chains=4096, seed=5)
-        ctx.prepass()
-        ctx.set_schedule(**sched)
-        say(f"schedule {ctx.schedule()}: 2 iterations")
-        ctx.run(2, capi.BATCHED)
-        res.append(ctx.stats())
-        ctx.close()
-    say(res)
-    assert all(r == res[0] for r in res)
+    for chains in (4096, 4097):
+        res, films = [], []
+        for sched, mode in (({}, capi.BATCHED), ({"bin_stages": 1}, capi.BATCHED),
+                            ({"bin_stages": 1, "stage_blocks": 16, "deriv_blocks": 64, "gauss_blocks": 32}, capi.BATCHED),
+                            ({"stage_lanes": 32}, capi.BATCHED), ({"bin_stages": 1, "stage_lanes": 16, "stage_blocks": 7}, capi.BATCHED),
+                            ({"stage_split": 2}, capi.BATCHED_GRAPH), ({"stage_split": 3, "bin_stages": 1, "stage_lanes": 32}, capi.BATCHED_GRAPH),
+                            ({"stage_split": 4}, capi.BATCHED_GRAPH)):
+            if chains != 4096 and not sched.get("stage_split") and sched:
+                continue
+            ctx = _ctx(cfg2_small, num_init_samples=8192, num_chains=chains, seed=5)
+            ctx.prepass()
+            ctx.set_schedule(**sched)
+            say(f"{chains} chains, schedule {ctx.schedule()}, form {mode}: 2 iterations")
+            ctx.run(2, mode)
+            res.append(ctx.stats())
+            films.append(ctx.film(1).sum())
+            ctx.close()
+        say(res)
+        assert all(r == res[0] for r in res)
+        assert np.allclose(films, films[0], rtol=1e-4)
     ctx = _ctx(cfg2_small, num_init_samples=8192, num_chains=4096, seed=5)
     ctx.prepass()
     say("autotune cfg2_small 4096 chains")
