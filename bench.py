@@ -236,7 +236,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=5)
-    ap.add_argument("--tune-steps", type=int, default=3, help="timed steps per candidate schedule of the autotuner")
+    ap.add_argument("--tune-steps", type=int, default=8, help="timed steps per candidate schedule of the autotuner")
     ap.add_argument("--no-deriv", action="store_true", help="skip the derivative-kernel workload (config 3)")
     ap.add_argument("--no-mcmc", action="store_true", help="skip the step times of the mcmc and h2mc integrators")
     ap.add_argument("--rmse-config", type=int, default=2, help="scene configuration of the equal-time RMSE study (0: none)")

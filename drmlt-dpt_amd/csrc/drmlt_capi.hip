@@ -848,15 +848,17 @@ extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best,
             if (ms < kMs) { cur = cand; kMs = ms; }
         }
     }
-    // the split step (graph-replayed form only), by the captured step's whole time
+    // the split step (graph-replayed form only), by the captured step's whole
+    // time over at least 10 steps (the ranges differ by a few percent)
+    const int gsteps = std::max(steps, 10);
     auto graphMs = [&](const drmlt_schedule &sc, double &ms) -> int {
         c->sched = sc;
         drop_graph(c);
         int rr = drmlt_run(c, 1, DRMLT_BATCHED_GRAPH);
         if (rr) return rr;
-        rr = drmlt_run(c, steps, DRMLT_BATCHED_GRAPH);
+        rr = drmlt_run(c, gsteps, DRMLT_BATCHED_GRAPH);
         if (rr) return rr;
-        ms = c->timing[0] / steps;
+        ms = c->timing[0] / gsteps;
         return DRMLT_OK;
     };
     double splitBest = 0;
