@@ -861,15 +861,20 @@ extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best,
         ms = c->timing[0] / gsteps;
         return DRMLT_OK;
     };
-    double splitBest = 0;
-    cur.stage_split = 0;
-    if ((r = graphMs(cur, splitBest))) return r;
-    for (int parts = 2; parts <= kMaxParts; parts++) {
-        cand = cur;
-        cand.stage_split = parts;
-        if ((r = graphMs(cand, ms))) return r;
-        if (ms < splitBest) { cur.stage_split = parts; splitBest = ms; }
-    }
+    // (the best of two interleaved rounds per choice: one stream, 2..kMaxParts ranges)
+    double splitMs[kMaxParts + 1];
+    for (double &m : splitMs) m = 1e30;
+    for (int round = 0; round < 2; round++)
+        for (int parts = 1; parts <= kMaxParts; parts++) {
+            cand = cur;
+            cand.stage_split = parts == 1 ? 0 : parts;
+            if ((r = graphMs(cand, ms))) return r;
+            splitMs[parts] = std::min(splitMs[parts], ms);
+        }
+    int bestParts = 1;
+    for (int parts = 2; parts <= kMaxParts; parts++)
+        if (splitMs[parts] < splitMs[bestParts]) bestParts = parts;
+    cur.stage_split = bestParts == 1 ? 0 : bestParts;
     if ((r = measure(cur, 1, 7, curMs))) return r;
     c->sched = cur;
     drop_graph(c);
