@@ -203,23 +203,27 @@ def test_mcmc_integrator(which, request):
     """integrator=mcmc (MCMCIntegrator, mcmc.cpp:125-160: the first stage
     alone) on the same kernels. The plain, batched and graph forms advance
     identically; one iteration from the same seeds counts exactly what the
-    DR loop's first stage counts; no second-stage moves are made; and against
+    DR loop's first stage counts (also with the graph step split into chain
+    ranges); no second-stage moves are made; and against
     the oracle's mcmc loop from the oracle's seeds the acceptance rates and
     the film mean agree within Monte Carlo error."""
     from drmlt import capi
 
     sc = request.getfixturevalue(which)
     res = {}
-    for integ, mode in (("mcmc", capi.PLAIN), ("mcmc", capi.BATCHED), ("mcmc", capi.BATCHED_GRAPH), ("drmlt", capi.BATCHED)):
+    for integ, mode in (("mcmc", capi.PLAIN), ("mcmc", capi.BATCHED), ("mcmc", capi.BATCHED_GRAPH), ("mcmc", "split"),
+                        ("drmlt", capi.BATCHED)):
         ctx = _ctx(sc, num_init_samples=8192, num_chains=4096, seed=5, integrator=capi.INTEGRATORS[integ])
         say(f"{which} {integ} form {mode}: prepass + 1 iteration")
         ctx.prepass()
-        ctx.run(1, mode)
+        if mode == "split":  # the graph step in three chain ranges on concurrent streams
+            ctx.set_schedule(stage_split=3)
+        ctx.run(1, capi.BATCHED_GRAPH if mode == "split" else mode)
         res[(integ, mode)] = ctx.stats()
         ctx.close()
     m = res[("mcmc", capi.PLAIN)]
     say("mcmc", m, "\ndrmlt", res[("drmlt", capi.BATCHED)])
-    assert m == res[("mcmc", capi.BATCHED)] == res[("mcmc", capi.BATCHED_GRAPH)]
+    assert m == res[("mcmc", capi.BATCHED)] == res[("mcmc", capi.BATCHED_GRAPH)] == res[("mcmc", "split")]
     d = res[("drmlt", capi.BATCHED)]
     for k in ("large_step", "large_step_accepted", "stage1", "stage1_accepted"):
         assert m[k] == d[k], k
@@ -256,7 +260,8 @@ def test_h2mc_integrator(which, request):
     """integrator=h2mc without lens moves (H2MCIntegrator, h2mc.cpp:196-600
     with lensperturbprob 0) on the same kernels: an isotropic small step with
     probability 0.2, else one H2MC move accepted with min(1, ratio). The
-    plain, batched and graph forms advance identically; large steps count as
+    plain, batched and graph forms (also the graph step split into chain
+    ranges) advance identically; large steps count as
     in the DR loop from the same seeds; and against the oracle's h2mc loop
     from the oracle's seeds the acceptance rates of large steps, isotropic
     moves and H2MC moves and the film mean agree within Monte Carlo error."""
@@ -267,15 +272,17 @@ def test_h2mc_integrator(which, request):
     with pytest.raises(capi.DrmltError, match="lensperturbprob"):
         capi.Context(sc, capi.options_from_dpt(dict(dpt, lensperturbprob=0.3), num_chains=1024))
     res = {}
-    for mode in (capi.PLAIN, capi.BATCHED, capi.BATCHED_GRAPH):
+    for mode in (capi.PLAIN, capi.BATCHED, capi.BATCHED_GRAPH, "split"):
         ctx = capi.Context(sc, capi.options_from_dpt(dpt, num_init_samples=8192, num_chains=4096, seed=5))
         say(f"{which} h2mc form {mode}: prepass + 2 iterations")
         ctx.prepass()
-        ctx.run(2, mode)
+        if mode == "split":  # the graph step in three chain ranges on concurrent streams
+            ctx.set_schedule(stage_split=3)
+        ctx.run(2, capi.BATCHED_GRAPH if mode == "split" else mode)
         res[mode] = ctx.stats()
         ctx.close()
     say("h2mc", res[capi.PLAIN])
-    assert res[capi.PLAIN] == res[capi.BATCHED] == res[capi.BATCHED_GRAPH]
+    assert res[capi.PLAIN] == res[capi.BATCHED] == res[capi.BATCHED_GRAPH] == res["split"]
     st = res[capi.PLAIN]
     assert st["stage2_iso"] == 0 and st["stage2_h2mc"] > 0 and st["stage1"] > 0
     assert st["large_step"] + st["stage1"] + st["stage2_h2mc"] == 2 * 4096
