@@ -637,7 +637,7 @@ static StepGrid step_grid(const drmlt_ctx *c, int C) {
 // The step's kernels for the chains of P on stream s with lists L (the
 // split step's halves).
 static void launch_step_kernels(drmlt_ctx *c, const DrParams &P, hipStream_t s, const Lists &L, const StepGrid &G) {
-    if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, L);
+    if (P.binStages == 1) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, L);
     hipLaunchKernelGGL(k_b_stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
     if (P.integrator == 1) return;
     hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, L, 4, 0);
@@ -657,7 +657,7 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
             static_cast<DevSceneData &>(Sk[k]) = c->S;
             Sk[k].counters = counters + 4 * k;
         }
-        if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin_counted, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
+        if (P.binStages == 1) hipLaunchKernelGGL(k_b_stage1_bin_counted, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
         hipLaunchKernelGGL(k_b_stage1_counted, dim3(G.stage), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
         if (P.integrator != 1) {
             hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 4, 0);
@@ -695,7 +695,7 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
         return;
     }
     if (timed) (void)hipEventRecord(c->ev[0], s);
-    if (P.binStages) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
+    if (P.binStages == 1) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
     hipLaunchKernelGGL(k_b_stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[1], s);
     if (P.integrator != 1) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 4, 0);
@@ -784,7 +784,7 @@ extern "C" int drmlt_get_schedule(drmlt_ctx *c, drmlt_schedule *out) {
 
 extern "C" int drmlt_set_schedule(drmlt_ctx *c, const drmlt_schedule *sc) {
     if (!c || !sc) return fail(DRMLT_ERR_ARG, "bad argument");
-    if (sc->bin_stages < 0 || sc->bin_stages > 1 || sc->stage_blocks < 0 || sc->deriv_blocks < 0 || sc->gauss_blocks < 0 ||
+    if (sc->bin_stages < 0 || sc->bin_stages > 2 || sc->stage_blocks < 0 || sc->deriv_blocks < 0 || sc->gauss_blocks < 0 ||
         sc->stage_lanes < 0 || sc->stage_lanes > 64 || sc->stage_split < 0 || sc->stage_split > kMaxParts)
         return fail(DRMLT_ERR_ARG, "schedule out of range");
     c->sched = *sc;
@@ -824,7 +824,7 @@ extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best,
     std::vector<int> laneOpts = {64, 32};
     if (fill != 64 && fill != 32) laneOpts.push_back(fill);
     for (int lanes : laneOpts)
-        for (int bin = 0; bin <= 1; bin++) {
+        for (int bin = 0; bin <= 2; bin++) {
             const int perChain = (C + lanes - 1) / lanes;
             for (int sb : {cus * 4, cus * 8, cus * 16, perChain}) {
                 cand = cur;

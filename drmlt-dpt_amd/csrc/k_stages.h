@@ -93,11 +93,11 @@ DEV int stage_first(const DrParams &P, int n) {
 __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
                                                  unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
-    const int n = P.binStages ? list_length(L, 3) : P.chainCount;
+    const int n = P.binStages == 1 ? list_length(L, 3) : P.chainCount;
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
-        const int chain = P.binStages ? list_item(L, 3, ch.C, it) : P.chainBase + it;
+        const int chain = P.binStages == 1 ? list_item(L, 3, ch.C, it) : P.chainBase + it;
         Philox g(P.seed, iter, chain, TAG_S1);
-        const Stage1 a = stage1_begin(ch, P, chain, g, !P.binStages);
+        const Stage1 a = stage1_begin(ch, P, chain, g, P.binStages != 1);
         Walk w;
         stage1_walk(ch, P, a, g, w);
         POut<float> out;

@@ -133,7 +133,9 @@ typedef struct {
  * CPU threads (src/parallel.cpp). Zero fields take the defaults. */
 typedef struct {
     int32_t bin_stages;    /* 1: the path-stage kernels take their chains from path-length bins
-                              (waves of equal-length paths); 0 (default): in chain order */
+                              (waves of equal-length paths); 2: only the second-stage kernels do
+                              (compacted lists of the chains with a second stage), stage 1 in
+                              chain order; 0 (default): all in chain order */
     int32_t stage_blocks;  /* 64-lane workgroups of the path-stage kernels (default: one per 64 chains, at most 16 per CU) */
     int32_t deriv_blocks;  /* workgroups of the derivative kernel (default 16 per CU) */
     int32_t gauss_blocks;  /* one-wave workgroups of the Gaussian kernel (default 16 per CU) */

@@ -105,7 +105,8 @@ def test_batched_paths_equal_plain_paths(which, request):
 
 
 def test_schedules_agree_and_autotune(cfg2_small):
-    """Binned and chain-order scheduling of the path-stage kernels, 64, 32 or
+    """Binned (all path-stage kernels, or only the second-stage ones) and
+    chain-order scheduling of the path-stage kernels, 64, 32 or
     16 chains per workgroup, other workgroup counts, and the graph-replayed
     step split into 2-4 ranges of the chains on as many streams (also for a
     chain count that none of them divides) advance the chains identically (iterations from the same
@@ -119,6 +120,7 @@ def test_schedules_agree_and_autotune(cfg2_small):
         for sched, mode in (({}, capi.BATCHED), ({"bin_stages": 1}, capi.BATCHED),
                             ({"bin_stages": 1, "stage_blocks": 16, "deriv_blocks": 64, "gauss_blocks": 32}, capi.BATCHED),
                             ({"stage_lanes": 32}, capi.BATCHED), ({"bin_stages": 1, "stage_lanes": 16, "stage_blocks": 7}, capi.BATCHED),
+                            ({"bin_stages": 2}, capi.BATCHED), ({"bin_stages": 2, "stage_split": 2}, capi.BATCHED_GRAPH),
                             ({"stage_split": 2}, capi.BATCHED_GRAPH), ({"stage_split": 3, "bin_stages": 1, "stage_lanes": 32}, capi.BATCHED_GRAPH),
                             ({"stage_split": 4}, capi.BATCHED_GRAPH)):
             if chains != 4096 and not sched.get("stage_split") and sched:
@@ -139,7 +141,7 @@ def test_schedules_agree_and_autotune(cfg2_small):
     say("autotune cfg2_small 4096 chains")
     best, ms = ctx.autotune(steps=1)
     say("autotune", best, ms)
-    assert ms > 0 and best == ctx.schedule() and best["bin_stages"] in (0, 1)
+    assert ms > 0 and best == ctx.schedule() and best["bin_stages"] in (0, 1, 2)
     before = ctx.stats()
     ctx.run(3, capi.BATCHED_GRAPH)
     after = ctx.stats()
