@@ -875,6 +875,20 @@ extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best,
     for (int parts = 2; parts <= kMaxParts; parts++)
         if (splitMs[parts] < splitMs[bestParts]) bestParts = parts;
     cur.stage_split = bestParts == 1 ? 0 : bestParts;
+    // the derivative and Gaussian workgroup counts were timed on one stream:
+    // against their defaults under the chosen split, by the captured step's time
+    if (cur.deriv_blocks != 0 || cur.gauss_blocks != 0) {
+        cand = cur;
+        cand.deriv_blocks = cand.gauss_blocks = 0;
+        double tunedMs = 1e30, defMs = 1e30;
+        for (int round = 0; round < 2; round++) {
+            if ((r = graphMs(cur, ms))) return r;
+            tunedMs = std::min(tunedMs, ms);
+            if ((r = graphMs(cand, ms))) return r;
+            defMs = std::min(defMs, ms);
+        }
+        if (defMs < tunedMs) cur = cand;
+    }
     if ((r = measure(cur, 1, 7, curMs))) return r;
     c->sched = cur;
     drop_graph(c);
