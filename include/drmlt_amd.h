@@ -268,7 +268,8 @@ int drmlt_set_schedule(drmlt_ctx *ctx, const drmlt_schedule *sched);
 /* Autotuner: times candidate schedules on the context's own chains (each
  * candidate one untimed and `steps` timed batched steps, hipEvents on the
  * context stream; stage_split by the best of two rounds of max(steps, 10)
- * graph-replayed steps per choice) and keeps the fastest. The steps are ordinary DR iterations:
+ * graph-replayed steps per choice, then the derivative and Gaussian
+ * workgroup counts once more against their defaults under that split) and keeps the fastest. The steps are ordinary DR iterations:
  * they advance the chains, count in the statistics and splat into the film.
  * best and best_ms (both optional) receive the schedule and its time per step. */
 int drmlt_autotune(drmlt_ctx *ctx, int32_t steps, drmlt_schedule *best, double *best_ms);
