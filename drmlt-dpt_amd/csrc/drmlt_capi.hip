@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <sys/stat.h>
@@ -133,9 +134,74 @@ static DevCamera make_camera(const float *cam) {
     return c;
 }
 
+// ---------------------------------------------------------------- no-alloc scratch
+// Device memory of drmlt_grad_hess_noalloc: one array per GPU, held from the
+// library's load until the process ends, shared by every call on that GPU.
+// On the host the array's name stands for a host-side placeholder, not the
+// device copy, so host code never hands it to a kernel or a copy: it uses the
+// device address hipGetSymbolAddress gives, looked up once per device when
+// the first context is created on it.
+constexpr int64_t kNoAllocStates = 4096;  // most states per drmlt_grad_hess_noalloc call
+constexpr size_t na_align(size_t x) { return (x + 255) / 256 * 256; }
+// layout: records | pass offsets | ok flags | FP32 gradient | FP32 Hessian | FP64 gradient | FP64 Hessian
+constexpr size_t kNaRec = 0;
+constexpr size_t kNaPass = na_align(kNaRec + size_t(kNoAllocStates) * sizeof(drmlt_path_rec));
+constexpr size_t kNaOk = na_align(kNaPass + size_t(kNoAllocStates + 1) * 4);
+constexpr size_t kNaG32 = na_align(kNaOk + size_t(kNoAllocStates) * 4);
+constexpr size_t kNaH32 = na_align(kNaG32 + size_t(kNoAllocStates) * 32 * 4);
+constexpr size_t kNaG64 = na_align(kNaH32 + size_t(kNoAllocStates) * 1024 * 4);
+constexpr size_t kNaH64 = na_align(kNaG64 + size_t(kNoAllocStates) * 32 * 8);
+constexpr size_t kScratchBytes = na_align(kNaH64 + size_t(kNoAllocStates) * 1024 * 8);  // 51.6 MiB
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[kMaxDevices];
+// The device address of `scratch` on `device`, which must be current.
+static int scratch_address(int device, char **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return fail(DRMLT_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(DRMLT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<char *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return DRMLT_OK;
+}
+
+// Fills n words with v (a kernel, not a memset).
+__global__ void __launch_bounds__(256) k_na_fill(uint32_t *p, size_t n, uint32_t v) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+// The derivative kernels' output (gradient, upper Hessian triangle) as FP64
+// gradient rows and full, mirrored Hessians: FP32 results are converted,
+// FP64 results get their lower triangle filled in place.
+__global__ void __launch_bounds__(256) k_na_finish(int n, int fp64, const float *g32, const float *h32, double *g64, double *h64) {
+    const size_t total = size_t(n) * (32 + 1024);
+    for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += size_t(gridDim.x) * blockDim.x) {
+        const size_t r = t / (32 + 1024);
+        const int e = int(t % (32 + 1024));
+        if (e < 32) {
+            if (!fp64) g64[r * 32 + e] = g32[r * 32 + e];
+            continue;
+        }
+        const int a = (e - 32) / 32, b = (e - 32) % 32;
+        if (fp64) {
+            if (b < a) h64[(r * 32 + a) * 32 + b] = h64[(r * 32 + b) * 32 + a];
+        } else {
+            h64[(r * 32 + a) * 32 + b] = h32[(r * 32 + (a < b ? a : b)) * 32 + (a < b ? b : a)];
+        }
+    }
+}
+static unsigned na_grid(size_t n) { return unsigned(std::max<size_t>(1, std::min<size_t>(2048, (n + 255) / 256))); }
+
 struct drmlt_ctx {
     int device = 0;
     int cus = 256;
+    char *scratch = nullptr;  // this device's address of `scratch`
     hipStream_t stream = nullptr;
     drmlt_options opt;
     DevScene S;
@@ -284,6 +350,7 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     *out = nullptr;
     auto bail = [&](int code) { drmlt_ctx_destroy(c); return code; };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(DRMLT_ERR_HIP, "hipGetDevice"));
+    if (int e = scratch_address(c->device, &c->scratch)) return bail(e);
     (void)hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess)
@@ -1254,6 +1321,62 @@ extern "C" int drmlt_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st
         for (int a = 0; a < 32; a++)
             for (int b = 0; b < a; b++) H[a * 32 + b] = H[b * 32 + a];
     }
+    return DRMLT_OK;
+}
+
+// drmlt_grad_hess without allocating: the records, the pass table and the
+// results go through this device's `scratch` array (at most kNoAllocStates
+// states per call), cleared by fill kernels. Overlapping calls on one GPU
+// share the array and race.
+extern "C" int drmlt_grad_hess_noalloc(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st, int32_t fp64, double *grad, double *hess,
+                                       int32_t *okOut) {
+    if (!c || n < 0 || !st || !grad || !hess || !okOut) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (n > kNoAllocStates) return fail(DRMLT_ERR_ARG, "drmlt_grad_hess_noalloc takes at most 4096 states per call");
+    if (!c->scratch) return fail(DRMLT_ERR_STATE, "no scratch address for this context's device");
+    if (!recs_ok(c, st, n) || !recs_fixed_ok(st, n)) return fail(DRMLT_ERR_ARG, "path record out of range");
+    if (n == 0) return DRMLT_OK;
+    int passOff[kNoAllocStates + 1];
+    passOff[0] = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const int D = 2 * std::max(st[i].c + st[i].l - 1, 2);
+        if (D > kGMax) return fail(DRMLT_ERR_ARG, "derivatives are provided up to D = 24");
+        passOff[i + 1] = passOff[i] + jet_passes_host(D);
+    }
+    const int total = passOff[n];
+    hipStream_t s = c->stream;
+    char *base = c->scratch;
+    char *dRec = base + kNaRec;
+    int *dPass = reinterpret_cast<int *>(base + kNaPass), *dOk = reinterpret_cast<int *>(base + kNaOk);
+    float *g32 = reinterpret_cast<float *>(base + kNaG32), *h32 = reinterpret_cast<float *>(base + kNaH32);
+    double *g64 = reinterpret_cast<double *>(base + kNaG64), *h64 = reinterpret_cast<double *>(base + kNaH64);
+    HIPCHK(hipMemcpyAsync(dRec, st, size_t(n) * sizeof(drmlt_path_rec), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dPass, passOff, size_t(n + 1) * 4, hipMemcpyHostToDevice, s));
+    // results start at 0 (entries beyond D stay 0), the ok flags at 1
+    const size_t wOk = size_t(n), wG = size_t(n) * 32 * (fp64 ? 2 : 1), wH = size_t(n) * 1024 * (fp64 ? 2 : 1);
+    uint32_t *zg = reinterpret_cast<uint32_t *>(fp64 ? static_cast<void *>(g64) : static_cast<void *>(g32));
+    uint32_t *zh = reinterpret_cast<uint32_t *>(fp64 ? static_cast<void *>(h64) : static_cast<void *>(h32));
+    hipLaunchKernelGGL(k_na_fill, dim3(na_grid(wOk)), dim3(256), 0, s, reinterpret_cast<uint32_t *>(dOk), wOk, 1u);
+    hipLaunchKernelGGL(k_na_fill, dim3(na_grid(wG)), dim3(256), 0, s, zg, wG, 0u);
+    hipLaunchKernelGGL(k_na_fill, dim3(na_grid(wH)), dim3(256), 0, s, zh, wH, 0u);
+    HIPCHK(hipGetLastError());
+    const DrParams P = make_params(c);
+    HIPCHK(hipEventRecord(c->ev[0], s));
+    if (fp64)
+        hipLaunchKernelGGL(k_gradhess_rec_f64, dim3(unsigned((total + 63) / 64)), dim3(64), 0, s, c->S, P.oDeriv, int(n), dRec, dPass, total,
+                           g64, h64, dOk);
+    else
+        hipLaunchKernelGGL(k_gradhess_rec_f32, dim3(unsigned((total + 63) / 64)), dim3(64), 0, s, c->S, P.oDeriv, int(n), dRec, dPass, total,
+                           g32, h32, dOk);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[1], s));
+    const size_t wFin = size_t(n) * (32 + 1024);
+    hipLaunchKernelGGL(k_na_finish, dim3(na_grid(wFin)), dim3(256), 0, s, int(n), int(fp64 != 0), g32, h32, g64, h64);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(grad, g64, size_t(n) * 32 * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hess, h64, size_t(n) * 1024 * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(okOut, dOk, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    kernel_time(c);
     return DRMLT_OK;
 }
 

@@ -17,7 +17,7 @@ EXPORTS = [
     "drmlt_last_error", "drmlt_options_default", "drmlt_ctx_create", "drmlt_ctx_destroy", "drmlt_prepass",
     "drmlt_seed_chains", "drmlt_direct_lighting", "drmlt_run", "drmlt_read_film", "drmlt_read_stats", "drmlt_read_timing",
     "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
-    "drmlt_grad_hess", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
+    "drmlt_grad_hess", "drmlt_grad_hess_noalloc", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
     "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains",
     "drmlt_write_exr", "drmlt_stats_text", "drmlt_write_stats", "drmlt_render",
     "drmlt_scene_load", "drmlt_scene_get", "drmlt_scene_output", "drmlt_scene_free",
@@ -290,6 +290,15 @@ class Context:
         n = len(states)
         g = np.zeros((n, 32)); h = np.zeros((n, 32, 32)); ok = np.zeros(n, np.int32)
         check(lib().drmlt_grad_hess(self.h, C.c_int64(n), P(states), int(fp64), P(g), P(h), P(ok)))
+        return g, h, ok
+
+    def grad_hess_noalloc(self, states, fp64=False):
+        """grad_hess through the entry point that allocates no device memory
+        (at most 4096 states per call)."""
+        states = np.ascontiguousarray(states, PATH_REC)
+        n = len(states)
+        g = np.zeros((n, 32)); h = np.zeros((n, 32, 32)); ok = np.zeros(n, np.int32)
+        check(lib().drmlt_grad_hess_noalloc(self.h, C.c_int64(n), P(states), int(fp64), P(g), P(h), P(ok)))
         return g, h, ok
 
 

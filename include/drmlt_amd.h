@@ -323,6 +323,16 @@ int drmlt_eval_logf(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int
  * device time of the derivative kernel. */
 int drmlt_grad_hess(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int32_t fp64, double *grad, double *hess, int32_t *ok);
 
+/* drmlt_grad_hess for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It has no counterpart in the reference. Same arguments and
+ * results as drmlt_grad_hess, at most 4096 states per call (more:
+ * DRMLT_ERR_ARG); it calls no hipMalloc or hipFree and stages the records,
+ * the pass table and the results in one fixed device array of 51.6 MiB per
+ * GPU, held from the library's load until the process ends. Calls that
+ * overlap on one GPU share that array and race. */
+int drmlt_grad_hess_noalloc(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int32_t fp64, double *grad, double *hess, int32_t *ok);
+
 /* PH2MCSmallStep::ComputeGaussian (drsmallstep.cpp:18-93, 219-266) in FP64
  * on the current device: D[i] dimensions, sc[i] the state's ssScore, sigma the
  * prior, grad n x 32, hess n x 32 x 32. Outputs mean n x 32, covL and invCov
