@@ -128,6 +128,20 @@ extern "C" int orc_bsdf_sample(const float *mat, int adjoint, const double *wi, 
     return ok ? 1 : 0;
 }
 extern "C" double orc_roughness(const float *mat, double disc) { return Roughness(MatD(mat), disc); }
+// orc_bsdf_eval over n direction pairs (wi, wo: n x 3; one normal) into n x 6.
+extern "C" void orc_bsdf_eval_batch(const float *mat, int adjoint, int n, const double *wi, const double *nn, const double *wo, double *out6) {
+    const Material<double> M = MatD(mat);
+    const V3<double> N(nn[0], nn[1], nn[2]);
+    for (int i = 0; i < n; i++) {
+        BsdfOut<double> o;
+        BsdfEval(M, adjoint != 0, V3<double>(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), N, V3<double>(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), o);
+        double *q = out6 + 6 * size_t(i);
+        q[0] = o.contrib.x; q[1] = o.contrib.y; q[2] = o.contrib.z;
+        q[3] = o.cosWo; q[4] = o.pdf; q[5] = o.revPdf;
+    }
+}
+// Plants mistake `which` in the BSDF code (orc_core.h, g_plant); 0 removes it.
+extern "C" void orc_plant(int which) { g_plant = which; }
 
 template <class F>
 static void PerturbT(const SceneT<F> &s, int n, const PathRec *in, const float *off, const float *disc, int occl, int fixed, int chadQuirk,

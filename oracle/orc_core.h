@@ -129,6 +129,17 @@ template <class F> struct BsdfOut {
     F cosWo = 0, pdf = 0, revPdf = 0;
 };
 
+// Planted mistakes, for the tests that show each BSDF check fails on the
+// mistake it is meant to catch (orc_plant; 0, the default, plants nothing):
+// 1 Phong lobe normalised by (n + 1) / 2pi instead of (n + 2) / 2pi,
+// 2 Phong lobe pdf normalised by (n + 2) / 2pi instead of (n + 1) / 2pi,
+// 3 rough dielectric reflection without the Fresnel factor,
+// 4 rough dielectric without the shadowing-masking term G,
+// 5 rough dielectric transmission without eta'^2 in the numerator,
+// 6 rough dielectric radiance transmission without the 1 / eta'^2 scaling,
+// 7 rough dielectric transmission weighted by F instead of 1 - F.
+inline int g_plant = 0;
+
 // lambertian.cpp:15-38
 template <class F> inline void LambertEval(const Material<F> &m, const V3<F> &wi, const V3<F> &n, const V3<F> &wo, BsdfOut<F> &o) {
     F cosWi = Dot(n, wi);
@@ -174,8 +185,8 @@ template <class F> inline void PhongEval(const Material<F> &m, const V3<F> &wi, 
         const F alpha = std::max(Dot(Reflect(wi, n_), wo), F(0));
         const F w = std::pow(alpha, m.expo) * C<F>::INVTWOPI;
         if (w > F(1e-10)) {
-            o.contrib = m.b * ((m.expo + F(2)) * w);
-            o.pdf = m.ksWeight * (m.expo + F(1)) * w;
+            o.contrib = m.b * ((m.expo + F(g_plant == 1 ? 1 : 2)) * w);
+            o.pdf = m.ksWeight * (m.expo + F(g_plant == 2 ? 2 : 1)) * w;
             o.revPdf = o.pdf;
         }
     }
@@ -292,22 +303,24 @@ template <class F> inline void DielEval(const Material<F> &m, bool adjoint, cons
     F dummy;
     F Fr = Fresnel(cHi, dummy, m.eta, m.invEta);
     F aCi = std::fabs(cosWi), aCo = std::fabs(cosWo);
-    F G = BeckmannG1(alp, aCi) * BeckmannG1(alp, aCo);
+    F G = g_plant == 4 ? F(1) : BeckmannG1(alp, aCi) * BeckmannG1(alp, aCo);
     F sa = alp * (F(1.2) - F(0.2) * std::sqrt(aCi));
     F prob = lh.z * BeckmannD(lh, sa);
     if (prob < F(1e-20)) { o.contrib = V3<F>(); return; }
     F rsa = alp * (F(1.2) - F(0.2) * std::sqrt(aCo));
     F revProb = lh.z * BeckmannD(lh, rsa);
     if (reflect) {
-        F s = std::fabs(Fr * D * G / (F(4) * cosWi));
+        F s = std::fabs((g_plant == 3 ? F(1) : Fr) * D * G / (F(4) * cosWi));
         o.contrib = m.a * s;
         o.pdf = std::fabs(prob * Fr / (F(4) * cHo));
         o.revPdf = std::fabs(revProb * Fr / (F(4) * revCHo));
     } else {
         F sd = cHi + eta_ * cHo;
         F rsd = revCHi + revEta_ * revCHo;
-        F factor = adjoint ? F(1) : sq(F(1) / eta_);
-        F s = std::fabs(factor * ((F(1) - Fr) * D * G * sq(eta_) * cHi * cHo) / (cosWi * sq(sd)));
+        F factor = (adjoint || g_plant == 6) ? F(1) : sq(F(1) / eta_);
+        F tw = g_plant == 7 ? Fr : F(1) - Fr;
+        F e2 = g_plant == 5 ? F(1) : sq(eta_);
+        F s = std::fabs(factor * (tw * D * G * e2 * cHi * cHo) / (cosWi * sq(sd)));
         o.contrib = m.b * s;
         o.pdf = std::fabs(prob * (F(1) - Fr) * (sq(eta_) * cHo) / sq(sd));
         o.revPdf = std::fabs(revProb * (F(1) - Fr) * (sq(revEta_) * revCHo) / sq(rsd));

@@ -155,6 +155,32 @@ def bsdf_eval(mat, wi, n, wo, adjoint=False):
     return out
 
 
+def bsdf_eval_batch(mat, wi, n, wo, adjoint=False):
+    """bsdf_eval over rows of wi and wo (m x 3) with one normal: m x 6."""
+    wi = np.ascontiguousarray(np.broadcast_to(wi, np.shape(wo)), np.float64)
+    wo = np.ascontiguousarray(wo, np.float64)
+    out = np.zeros((len(wo), 6))
+    lib().orc_bsdf_eval_batch(P(np.ascontiguousarray(mat, np.float32)), int(adjoint), len(wo), P(wi),
+                              P(np.ascontiguousarray(n, np.float64)), P(wo), P(out))
+    return out
+
+
+class planted:
+    """Context manager: the oracle's BSDF code with mistake `which` planted
+    (orc_core.h, g_plant), removed on exit."""
+
+    def __init__(self, which):
+        self.which = which
+
+    def __enter__(self):
+        lib().orc_plant(int(self.which))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_plant(0)
+        return False
+
+
 def bsdf_sample(mat, wi, n, r0, r1, disc, adjoint=False):
     out = np.zeros(6); wo = np.zeros(3)
     mat = np.ascontiguousarray(mat, np.float32)
