@@ -330,7 +330,7 @@ DEV int stage1_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int 
     bool isAniso = false;
     if (doSecond) isAniso = flip_coin(P.anisoPerturbProb, g.uni());
     ch.a1[chain] = a1;
-    const int flags = (acc1 ? 1 : 0) | (doSecond ? 2 : 0) | (isAniso ? 4 : 0) | (isLarge ? 8 : 0);
+    const int flags = (acc1 ? 1 : 0) | (doSecond ? 2 : 0) | (isAniso ? 4 : 0) | (isLarge ? 8 : 0) | (out.valid ? 32 : 0);
     ch.flags[chain] = flags;
     if (!doSecond) {
         const int W = Sc.cam.W, H = Sc.cam.H;
@@ -472,57 +472,71 @@ DEV void stage3_walk(const Chains &ch, const DrParams &P, const Stage3 &a, Philo
     w.c = ch.c[a.ps];
     w.l = ch.l[a.ps];
 }
+// Second-stage acceptance probability from a1, the second stage's own ratio
+// a2 (before the delayed-rejection correction), aStar and the Tierney-Mira
+// kernel ratio tr: Green & Mira (delayed.cpp:466-500; aStar is y*'s
+// acceptance from z, 0 when y* was not traced), Tierney & Mira
+// (delayed.cpp:502-541; aStar = min(1, f_y / f_z), tr = Q1(z, y) / Q1(x, y)),
+// or the h2mc integrator's min(1, a2) (h2mc.cpp:584-587). draw tells whether
+// the reference draws the acceptance coin.
+DEV double dr_alpha2(int integrator, int type, float a1, double a2, float aStar, double tr, bool &draw) {
+    draw = false;
+    if (is_invalid(a2)) return 0.0;
+    if (integrator == 2) {
+        draw = true;
+        return fmin(1.0, a2);
+    }
+    if (type == 0) {
+        if (is_invalid(aStar)) aStar = 0.0f;
+        if (aStar == 1.0f) return 0.0;
+        draw = true;
+        return fmin(1.0, a2 * double(1.0f - aStar) / double(1.0f - a1));
+    }
+    if (is_invalid(aStar)) aStar = 1.0f;
+    if (aStar == 1.0f) return 0.0;
+    if (is_invalid(tr)) return 0.0;
+    draw = true;
+    return fmin(1.0, a2 * tr * double(1.0f - aStar) / double(1.0f - a1));
+}
+// Q1(z, y) / Q1(x, y) of the isotropic first-stage kernel (delayed.cpp:514-527).
+DEV double mira_kernel_ratio(const Chains &ch, const DrParams &P, int pc, int pf, int ps) {
+    const int D = dim_of(ch.c[pc], ch.l[pc]);
+    double qzy = 0, qxy = 0;
+    for (int i = 0; i < D; i++) {
+        const double y = ch.u[i * ch.NS + pf];
+        const double dz = double(ch.u[i * ch.NS + ps]) - y, dx = double(ch.u[i * ch.NS + pc]) - y;
+        qzy += dz * dz;
+        qxy += dx * dx;
+    }
+    const double s2 = double(P.perturbStdDev) * double(P.perturbStdDev);
+    return exp(-0.5 * qzy / s2 + 0.5 * qxy / s2);
+}
+
 DEV void stage3_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, const Stage3 &a, const POut<float> &starOut,
                     Philox &g, float *film, unsigned long long *stats) {
     const int perm = a.perm, pc = a.pc, pf = a.pf, ps = a.ps;
     const bool isAniso = a.isAniso;
     const float a1 = a.a1;
-    double a2 = a.a2;
-    bool acc2 = false;
-    if (is_invalid(a2)) {
-        a2 = 0.0;
-    } else if (P.integrator == 2) {
-        // h2mc: the H2MC move alone, min(1, ratio) (h2mc.cpp:584-587)
-        a2 = fmin(1.0, a2);
-        acc2 = flip_coin(a2, g.uni());
-    } else if (P.type == 0) {
-        // Green & Mira
-        float aStar = 0.0f;
-        if (a.traceStar) {
-            count_state(Sc, ch.c[ps], ch.l[ps]);
-            if (starOut.valid) aStar = clamp_ref(starOut.ssScore / ch.ss[ps]);
-        }
-        if (is_invalid(aStar)) aStar = 0.0f;
-        if (aStar == 1.0f) {
-            a2 = 0.0;
-        } else {
-            a2 = fmin(1.0, a2 * double(1.0f - aStar) / double(1.0f - a1));
-            acc2 = flip_coin(a2, g.uni());
-        }
-    } else {
-        // Tierney & Mira (delayed.cpp:502-541)
-        float aStar = fminf(1.0f, ch.ss[pf] / ch.ss[ps]);
-        if (is_invalid(aStar)) aStar = 1.0f;
-        if (aStar == 1.0f) {
-            a2 = 0.0;
-        } else {
-            const int D = dim_of(ch.c[pc], ch.l[pc]);
-            double qzy = 0, qxy = 0;
-            for (int i = 0; i < D; i++) {
-                const double y = ch.u[i * ch.NS + pf];
-                const double dz = double(ch.u[i * ch.NS + ps]) - y, dx = double(ch.u[i * ch.NS + pc]) - y;
-                qzy += dz * dz;
-                qxy += dx * dx;
+    float aStar = 0.0f;
+    double tr = 1.0;
+    if (!is_invalid(a.a2) && P.integrator == 0) {
+        if (P.type == 0) {
+            if (a.traceStar) {
+                count_state(Sc, ch.c[ps], ch.l[ps]);
+                if (starOut.valid) aStar = clamp_ref(starOut.ssScore / ch.ss[ps]);
             }
-            const double s2 = double(P.perturbStdDev) * double(P.perturbStdDev);
-            const double tr = exp(-0.5 * qzy / s2 + 0.5 * qxy / s2);
-            if (is_invalid(tr)) {
-                a2 = 0.0;
-            } else {
-                a2 = fmin(1.0, a2 * tr * double(1.0f - aStar) / double(1.0f - a1));
-                acc2 = flip_coin(a2, g.uni());
-            }
+        } else {
+            aStar = fminf(1.0f, ch.ss[pf] / ch.ss[ps]);
+            if (!is_invalid(aStar) && aStar != 1.0f) tr = mira_kernel_ratio(ch, P, pc, pf, ps);
         }
+    }
+    bool draw;
+    const double a2 = dr_alpha2(P.integrator, P.type, a1, a.a2, aStar, tr, draw);
+    const bool acc2 = draw && flip_coin(a2, g.uni());
+    ch.a2d[chain] = a2;  // the iteration's alpha2, y*'s validity and score (read back by drmlt_read_step)
+    if (a.traceStar && starOut.valid) {
+        ch.flags[chain] = a.flags | 64;
+        ch.ss[a.pstar] = starOut.ssScore;
     }
     count_if(stats + ST_ISO, !isAniso);
     count_if(stats + ST_ISO_ACC, !isAniso && acc2);

@@ -1226,11 +1226,14 @@ extern "C" int drmlt_eval_logf(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st
     return DRMLT_OK;
 }
 
-extern "C" int drmlt_read_chains(drmlt_ctx *c, int64_t n, drmlt_path_rec *st, float *c8) {
-    if (!c || n < 0 || !st || !c8) return fail(DRMLT_ERR_ARG, "bad argument");
-    if (!c->haveChains) return fail(DRMLT_ERR_STATE, "chains are not seeded (call drmlt_prepass or drmlt_seed_chains)");
-    if (n > c->ch.C) return fail(DRMLT_ERR_ARG, "more states asked for than there are chains");
-    if (n == 0) return DRMLT_OK;
+// The chain state in host memory: every slot's fields and the per-chain
+// scratch of the last iteration.
+struct HostChains {
+    std::vector<int> perm, ci, li, lig, lpr, valid, cp, lp, flags;
+    std::vector<float> scr, u, cd, ld, crr, lrr, rgb, ls, ss, a1, off;
+    std::vector<double> a2;
+};
+static int fetch_chains(drmlt_ctx *c, HostChains &h, bool scratch) {
     const Chains &ch = c->ch;
     const size_t C = size_t(ch.C), NS = size_t(ch.NS);
     hipStream_t s = c->stream;
@@ -1238,29 +1241,60 @@ extern "C" int drmlt_read_chains(drmlt_ctx *c, int64_t n, drmlt_path_rec *st, fl
         host.resize(count);
         return hipMemcpyAsync(host.data(), dev, count * sizeof(host[0]), hipMemcpyDeviceToHost, s);
     };
-    std::vector<int> perm, ci, li, lig, lpr, valid, cp, lp;
-    std::vector<float> scr, u, cd, ld, crr, lrr, rgb, ls, ss;
-    HIPCHK(get(ch.perm, C, perm)); HIPCHK(get(ch.c, NS, ci)); HIPCHK(get(ch.l, NS, li)); HIPCHK(get(ch.light, NS, lig));
-    HIPCHK(get(ch.lprim, NS, lpr)); HIPCHK(get(ch.valid, NS, valid)); HIPCHK(get(ch.cprim, 16 * NS, cp));
-    HIPCHK(get(ch.lgprim, 16 * NS, lp)); HIPCHK(get(ch.screen, 2 * NS, scr)); HIPCHK(get(ch.u, 32 * NS, u));
-    HIPCHK(get(ch.cdisc, 16 * NS, cd)); HIPCHK(get(ch.ldisc, 16 * NS, ld)); HIPCHK(get(ch.crr, 16 * NS, crr));
-    HIPCHK(get(ch.lrr, 16 * NS, lrr)); HIPCHK(get(ch.rgb, 3 * NS, rgb)); HIPCHK(get(ch.ls, NS, ls)); HIPCHK(get(ch.ss, NS, ss));
+    HIPCHK(get(ch.perm, C, h.perm)); HIPCHK(get(ch.c, NS, h.ci)); HIPCHK(get(ch.l, NS, h.li)); HIPCHK(get(ch.light, NS, h.lig));
+    HIPCHK(get(ch.lprim, NS, h.lpr)); HIPCHK(get(ch.valid, NS, h.valid)); HIPCHK(get(ch.cprim, 16 * NS, h.cp));
+    HIPCHK(get(ch.lgprim, 16 * NS, h.lp)); HIPCHK(get(ch.screen, 2 * NS, h.scr)); HIPCHK(get(ch.u, 32 * NS, h.u));
+    HIPCHK(get(ch.cdisc, 16 * NS, h.cd)); HIPCHK(get(ch.ldisc, 16 * NS, h.ld)); HIPCHK(get(ch.crr, 16 * NS, h.crr));
+    HIPCHK(get(ch.lrr, 16 * NS, h.lrr)); HIPCHK(get(ch.rgb, 3 * NS, h.rgb)); HIPCHK(get(ch.ls, NS, h.ls)); HIPCHK(get(ch.ss, NS, h.ss));
+    if (scratch) {
+        HIPCHK(get(ch.flags, C, h.flags)); HIPCHK(get(ch.a1, C, h.a1)); HIPCHK(get(ch.a2d, C, h.a2)); HIPCHK(get(ch.off, 32 * C, h.off));
+    }
     HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}
+// Physical slot p (p = chain + C * k) as a record and its contrib8 row.
+static void slot_record(const HostChains &h, size_t NS, size_t p, drmlt_path_rec &R, float *o) {
+    R.c = h.ci[p]; R.l = h.li[p]; R.light = h.lig[p]; R.lprim = h.lpr[p];
+    R.screen[0] = h.scr[p]; R.screen[1] = h.scr[p + NS];
+    for (int k = 0; k < 32; k++) R.u[k] = h.u[k * NS + p];
+    for (int k = 0; k < 16; k++) {
+        R.cprim[k] = h.cp[k * NS + p]; R.lgprim[k] = h.lp[k * NS + p];
+        R.cdisc[k] = h.cd[k * NS + p]; R.ldisc[k] = h.ld[k * NS + p];
+        R.crr[k] = h.crr[k * NS + p]; R.lrr[k] = h.lrr[k * NS + p];
+    }
+    o[0] = h.valid[p] ? 1.0f : 0.0f;
+    o[1] = h.rgb[p]; o[2] = h.rgb[p + NS]; o[3] = h.rgb[p + 2 * NS];
+    o[4] = h.ls[p]; o[5] = h.ss[p]; o[6] = h.scr[p]; o[7] = h.scr[p + NS];
+}
+
+extern "C" int drmlt_read_chains(drmlt_ctx *c, int64_t n, drmlt_path_rec *st, float *c8) {
+    if (!c || n < 0 || !st || !c8) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!c->haveChains) return fail(DRMLT_ERR_STATE, "chains are not seeded (call drmlt_prepass or drmlt_seed_chains)");
+    if (n > c->ch.C) return fail(DRMLT_ERR_ARG, "more states asked for than there are chains");
+    if (n == 0) return DRMLT_OK;
+    HostChains h;
+    if (int r = fetch_chains(c, h, false)) return r;
+    const size_t C = size_t(c->ch.C), NS = size_t(c->ch.NS);
+    for (int64_t i = 0; i < n; i++) slot_record(h, NS, size_t(i) + C * size_t(h.perm[i] & 3), st[i], c8 + 8 * size_t(i));
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_read_step(drmlt_ctx *c, int64_t n, drmlt_path_rec *slots, float *c8, int32_t *perm, int32_t *flags, float *a1,
+                               double *alpha2, float *off) {
+    if (!c || n < 0 || !slots || !c8 || !perm || !flags || !a1 || !alpha2 || !off) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!c->haveChains) return fail(DRMLT_ERR_STATE, "chains are not seeded (call drmlt_prepass or drmlt_seed_chains)");
+    if (n > c->ch.C) return fail(DRMLT_ERR_ARG, "more chains asked for than there are");
+    if (n == 0) return DRMLT_OK;
+    HostChains h;
+    if (int r = fetch_chains(c, h, true)) return r;
+    const size_t C = size_t(c->ch.C), NS = size_t(c->ch.NS);
     for (int64_t i = 0; i < n; i++) {
-        const size_t p = size_t(i) + C * size_t(perm[i] & 3);
-        drmlt_path_rec &R = st[i];
-        R.c = ci[p]; R.l = li[p]; R.light = lig[p]; R.lprim = lpr[p];
-        R.screen[0] = scr[p]; R.screen[1] = scr[p + NS];
-        for (int k = 0; k < 32; k++) R.u[k] = u[k * NS + p];
-        for (int k = 0; k < 16; k++) {
-            R.cprim[k] = cp[k * NS + p]; R.lgprim[k] = lp[k * NS + p];
-            R.cdisc[k] = cd[k * NS + p]; R.ldisc[k] = ld[k * NS + p];
-            R.crr[k] = crr[k * NS + p]; R.lrr[k] = lrr[k * NS + p];
-        }
-        float *o = c8 + 8 * size_t(i);
-        o[0] = valid[p] ? 1.0f : 0.0f;
-        o[1] = rgb[p]; o[2] = rgb[p + NS]; o[3] = rgb[p + 2 * NS];
-        o[4] = ls[p]; o[5] = ss[p]; o[6] = scr[p]; o[7] = scr[p + NS];
+        for (int k = 0; k < 4; k++) slot_record(h, NS, size_t(i) + C * size_t(k), slots[4 * i + k], c8 + 8 * (4 * size_t(i) + k));
+        perm[i] = h.perm[i];
+        flags[i] = h.flags[i];
+        a1[i] = h.a1[i];
+        alpha2[i] = (h.flags[i] & 2) ? h.a2[i] : 0.0;
+        for (int k = 0; k < 32; k++) off[32 * i + k] = h.off[size_t(k) * C + size_t(i)];
     }
     return DRMLT_OK;
 }
@@ -1416,6 +1450,36 @@ extern "C" int drmlt_gaussian_logpdf_f64(int64_t n, const int32_t *D, const doub
     hipLaunchKernelGGL(k_logpdf_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, int(n), d.p, xx.p, sign, dm.p, dQ.p, dl.p, o.p);
     HIPCHK(hipGetLastError());
     DGET(out, o, n);
+    HIPCHK(hipStreamSynchronize(s));
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_gaussian_eval_f64(int64_t n, const int32_t *D, const double *sc, double sigma, const double *grad, const double *hess,
+                                       const double *x, double sign, const double *nrm, double *logpdf, double *sample) {
+    if (n < 0 || !D || !sc || !grad || !hess || !x || !nrm || !logpdf || !sample) return fail(DRMLT_ERR_ARG, "bad argument");
+    for (int64_t i = 0; i < n; i++)
+        if (D[i] < 4 || D[i] > kGMax || D[i] % 2) return fail(DRMLT_ERR_ARG, "D must be even, 4 <= D <= 24");
+    if (n == 0) return DRMLT_OK;
+    std::vector<int> small, big;
+    for (int64_t i = 0; i < n; i++) (D[i] <= kGSmallD ? small : big).push_back(int(i));
+    hipStream_t s = 0;
+    DBuf<int> d, is, ib;
+    DBuf<double> a, g, h, xx, nn, sl, lp, sm;
+    DUP(d, D, n); DUP(a, sc, n); DUP(g, grad, 32 * n); DUP(h, hess, 1024 * n); DUP(xx, x, 32 * n); DUP(nn, nrm, 32 * n);
+    DUP(is, small.data(), small.size()); DUP(ib, big.data(), big.size());
+    DNEW(sl, size_t(kGSlot) * n); DNEW(lp, n); DNEW(sm, 32 * size_t(n));
+    HIPCHK(hipMemsetAsync(sm.p, 0, 32 * size_t(n) * 8, s));
+    const H2Param h2 = make_h2param(sigma);
+    constexpr int NG = 64 / DRMLT_GAUSS_GROUP;
+    if (!small.empty())
+        hipLaunchKernelGGL(k_gauss_eval_rec, dim3(unsigned(std::min<size_t>((small.size() + NG - 1) / NG, 65535))), dim3(64), 0, s,
+                           int(small.size()), is.p, 1, h2, d.p, a.p, g.p, h.p, xx.p, sign, nn.p, sl.p, lp.p, sm.p);
+    if (!big.empty())
+        hipLaunchKernelGGL(k_gauss_eval_rec, dim3(unsigned(std::min<size_t>(big.size(), 65535))), dim3(64), 0, s, int(big.size()), ib.p, 0,
+                           h2, d.p, a.p, g.p, h.p, xx.p, sign, nn.p, sl.p, lp.p, sm.p);
+    HIPCHK(hipGetLastError());
+    DGET(logpdf, lp, n);
+    DGET(sample, sm, 32 * n);
     HIPCHK(hipStreamSynchronize(s));
     return DRMLT_OK;
 }

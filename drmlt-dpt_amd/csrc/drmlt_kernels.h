@@ -8,6 +8,12 @@ namespace drmlt {
 
 constexpr int kJetK = 2;     // Hessian columns per derivative pass
 constexpr int kMaxBins = 16; // path-length bins (path length 0..15)
+// Gaussians of dimension up to kGSmallD are computed by groups of
+// DRMLT_GAUSS_GROUP lanes, several per wave; larger ones by a whole wave.
+constexpr int kGSmallD = 8;
+#ifndef DRMLT_GAUSS_GROUP
+#define DRMLT_GAUSS_GROUP 8  // lanes per small Gaussian (measured: 8 beats 16 and 32)
+#endif
 
 // Chain state: 4 physical slots per chain (current, first- and second-stage
 // proposal, Green's y*), all fields structure-of-arrays with stride NS = 4C.
@@ -28,7 +34,7 @@ struct Chains {
     float *grad, *hess;    // [C][24], [C][24*24] derivative output (Hessian: upper triangle)
     float *a1, *a2;        // stage acceptances
     double *a2d;
-    int *flags;            // bit0 acc1, bit1 doSecond, bit2 isAniso, bit3 large, bit4 z valid
+    int *flags;            // bit0 acc1, bit1 doSecond, bit2 isAniso, bit3 large, bit4 z valid, bit5 y valid, bit6 y* valid
 };
 
 DEV SRef slot_ref(const Chains &ch, int p) {

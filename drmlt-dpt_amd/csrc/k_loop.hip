@@ -145,6 +145,45 @@ __global__ void __launch_bounds__(64) k_gauss_rec(int n, H2Param h2, const int *
                       [&](int a, int b) { return hess[(size_t(i) * 32 + a) * 32 + b]; }, w, GSlot{slots + size_t(i) * kGSlot});
     }
 }
+// ComputeGaussian, GaussianLogPdf of sign * x and GenerateSample from the
+// standard normals nrm with the loop's own device functions (gauss_compute,
+// gauss_logpdf, gauss_sample: what k_b_gauss, h2mc_offset and h2mc_ratio run)
+// for the records idx[0..nidx): small = 1 takes them in groups of
+// DRMLT_GAUSS_GROUP lanes with a work area each, as k_b_gauss takes the
+// Gaussians of dimension <= kGSmallD; small = 0 one wave each.
+template <class X, class W>
+DEV void gauss_eval_item(const X &x, const H2Param &h2, int i, const int *Ds, const double *sc, const double *grad, const double *hess,
+                         const double *xv, double sign, const double *nrm, double *slots, double *logpdf, double *sample, W &w) {
+    const int D = Ds[i];
+    const GSlot G{slots + size_t(i) * kGSlot};
+    x.sync();
+    gauss_compute(x, h2, sc[i], D, [&](int k) { return grad[size_t(i) * 32 + k]; },
+                  [&](int a, int b) { return hess[(size_t(i) * 32 + a) * 32 + b]; }, w, G);
+    const double lp = gauss_logpdf(x, D, [&](int k) { return xv[size_t(i) * 32 + k]; }, sign, G, w);
+    if (x.lane == 0) logpdf[i] = lp;
+    gauss_sample(x, D, G, [&](int j) { return nrm[size_t(i) * 32 + j]; }, w, [&](int k, double v) { sample[size_t(i) * 32 + k] = v; });
+}
+__global__ void __launch_bounds__(64) k_gauss_eval_rec(int nidx, const int *idx, int small, H2Param h2, const int *Ds, const double *sc,
+                                                       const double *grad, const double *hess, const double *xv, double sign,
+                                                       const double *nrm, double *slots, double *logpdf, double *sample) {
+    constexpr int G = DRMLT_GAUSS_GROUP, NG = 64 / G;
+    union Work {
+        GWork big;
+        GWorkT<kGSmallD> small[NG];
+    };
+    __shared__ Work w;
+    if (small) {
+        const GroupExec<G> x{int(threadIdx.x) % G};
+        const int grp = threadIdx.x / G;
+        for (int it = blockIdx.x * NG + grp; it < nidx; it += gridDim.x * NG)
+            gauss_eval_item(x, h2, idx[it], Ds, sc, grad, hess, xv, sign, nrm, slots, logpdf, sample, w.small[grp]);
+    } else {
+        const WaveExec x{int(threadIdx.x)};
+        for (int it = blockIdx.x; it < nidx; it += gridDim.x)
+            gauss_eval_item(x, h2, idx[it], Ds, sc, grad, hess, xv, sign, nrm, slots, logpdf, sample, w.big);
+    }
+}
+
 // The reference's representation of a factored Gaussian: mean (n x 32),
 // covL = V diag(p^-1/2) and invCov = V diag(p) V^T (n x 24 x 24), logDet.
 __global__ void __launch_bounds__(64) k_gauss_expand(int n, const int *Ds, const double *slots, double *mean, double *covL, double *invCov,
@@ -186,27 +225,18 @@ __global__ void __launch_bounds__(64) k_logpdf_rec(int n, const int *Ds, const d
     out[r] = double(D) * (-0.5 * log(2.0 * 3.14159265358979323846)) + 0.5 * logDet[r] - 0.5 * q;
 }
 
-// Second-stage acceptance arithmetic alone (delayed.cpp:466-541), batched:
-// in[i] = (type, a1, a2, aStar, transRatio, secondValid); out = alpha2.
+// The loop's second-stage acceptance (dr_alpha2, the function stage3_end
+// calls), batched: in[i] = (type, a1, a2, aStar, transRatio, secondValid);
+// out = alpha2. For Green & Mira secondValid = 0 means y* was not traced
+// (aStar = 0), as in stage3_end.
 __global__ void __launch_bounds__(64) k_alpha2(int n, const double *in, double *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double *q = in + 6 * size_t(i);
     const int type = int(q[0]);
-    const float a1 = float(q[1]);
-    double a2 = q[2];
-    if (is_invalid(a2)) { out[i] = 0.0; return; }
-    if (type == 0) {
-        float aStar = q[5] != 0.0 ? float(q[3]) : 0.0f;
-        if (is_invalid(aStar)) aStar = 0.0f;
-        out[i] = aStar == 1.0f ? 0.0 : fmin(1.0, a2 * double(1.0f - aStar) / double(1.0f - a1));
-    } else {
-        float aStar = float(q[3]);
-        if (is_invalid(aStar)) aStar = 1.0f;
-        if (aStar == 1.0f) { out[i] = 0.0; return; }
-        const double tr = q[4];
-        out[i] = is_invalid(tr) ? 0.0 : fmin(1.0, a2 * tr * double(1.0f - aStar) / double(1.0f - a1));
-    }
+    const float aStar = (type == 0 && q[5] == 0.0) ? 0.0f : float(q[3]);
+    bool draw;
+    out[i] = dr_alpha2(0, type, float(q[1]), q[2], aStar, q[4], draw);
 }
 
 }  // namespace drmlt

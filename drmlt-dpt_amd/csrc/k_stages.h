@@ -138,16 +138,12 @@ DEV void gauss_item(const X &x, const DevScene &Sc, const Chains &ch, const DrPa
 
 // Bins 0..kGSmallBins-1 of a list hold paths of length <= 4 (D = 2 max(b, 2)
 // <= kGSmallD).
-constexpr int kGSmallD = 8;
 constexpr int kGSmallBins = kGSmallD / 2 + 1;
 DEV int list_prefix(const Lists &L, int kind, int nbins) {
     int n = 0;
     for (int b = 0; b < nbins; b++) n += L.count[kind * kMaxBins + b];
     return n;
 }
-#ifndef DRMLT_GAUSS_GROUP
-#define DRMLT_GAUSS_GROUP 8  // lanes per small Gaussian (measured: 8 beats 16 and 32)
-#endif
 
 // Items [begin, end) of list `kind`, 64 / G per wave, a group of G lanes and
 // the work area w[group] each.

@@ -32,4 +32,6 @@ __global__ void k_gauss_rec(int, H2Param, const int *, const double *, const dou
 __global__ void k_gauss_expand(int, const int *, const double *, double *, double *, double *, double *);
 __global__ void k_logpdf_rec(int, const int *, const double *, double, const double *, const double *, const double *, double *);
 __global__ void k_alpha2(int, const double *, double *);
+__global__ void k_gauss_eval_rec(int, const int *, int, H2Param, const int *, const double *, const double *, const double *, const double *,
+                                 double, const double *, double *, double *, double *);
 }  // namespace drmlt

@@ -18,7 +18,7 @@ EXPORTS = [
     "drmlt_seed_chains", "drmlt_direct_lighting", "drmlt_run", "drmlt_read_film", "drmlt_read_stats", "drmlt_read_timing",
     "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
     "drmlt_grad_hess", "drmlt_grad_hess_noalloc", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
-    "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains",
+    "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains", "drmlt_read_step", "drmlt_gaussian_eval_f64",
     "drmlt_write_exr", "drmlt_stats_text", "drmlt_write_stats", "drmlt_render",
     "drmlt_scene_load", "drmlt_scene_get", "drmlt_scene_output", "drmlt_scene_free",
 ]
@@ -275,6 +275,18 @@ class Context:
         check(lib().drmlt_read_chains(self.h, C.c_int64(n), P(st), P(c8)))
         return st, c8
 
+    def read_step(self, n=None):
+        """What the last iteration left in chains 0..n-1: a dict with the four
+        physical slots (n x 4 records and n x 4 x 8 contrib8 rows), perm,
+        flags, a1, alpha2 and the H2MC offset (n x 32)."""
+        n = int(self.opt.num_chains if n is None else n)
+        slots = np.zeros((n, 4), PATH_REC)
+        c8 = np.zeros((n, 4, 8), np.float32)
+        perm = np.zeros(n, np.int32); flags = np.zeros(n, np.int32); a1 = np.zeros(n, np.float32)
+        a2 = np.zeros(n); off = np.zeros((n, 32), np.float32)
+        check(lib().drmlt_read_step(self.h, C.c_int64(n), P(slots), P(c8), P(perm), P(flags), P(a1), P(a2), P(off)))
+        return {"slots": slots, "contrib8": c8, "perm": perm, "flags": flags, "a1": a1, "alpha2": a2, "off": off}
+
     def eval_logf(self, states, fp64=False):
         """log f at each state with its triangles fixed (the function
         grad_hess differentiates); returns (logf, ok)."""
@@ -323,6 +335,20 @@ def gaussian_logpdf(D, x, sign, mean, covL, inv, ld):
                                           P(np.ascontiguousarray(covL)), P(np.ascontiguousarray(inv)),
                                           P(np.ascontiguousarray(ld)), P(out)))
     return out
+
+
+def gaussian_eval(D, sc, sigma, grad, hess, x, sign, nrm):
+    """The loop's own ComputeGaussian -> GaussianLogPdf(sign x) ->
+    GenerateSample(nrm) chain on the device: (logpdf n, sample n x 32)."""
+    D = np.ascontiguousarray(D, np.int32)
+    n = len(D)
+    arr = lambda a, shape: np.ascontiguousarray(a, np.float64).reshape(shape)
+    sc, grad, hess = arr(sc, n), arr(grad, (n, 32)), arr(hess, (n, 32, 32))
+    x, nrm = arr(x, (n, 32)), arr(nrm, (n, 32))
+    lp = np.zeros(n); sm = np.zeros((n, 32))
+    check(lib().drmlt_gaussian_eval_f64(C.c_int64(n), P(D), P(sc), C.c_double(sigma), P(grad), P(hess), P(x), C.c_double(sign), P(nrm),
+                                        P(lp), P(sm)))
+    return lp, sm
 
 
 def dr_alpha2(rows):

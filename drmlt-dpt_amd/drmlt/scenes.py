@@ -268,24 +268,27 @@ def config2(seed=2, sphere_level=4, grid_res=256):
     return b.build(cam, opts)
 
 
-def config4(seed=4, sphere_level=4, grid_res=512):
+def config4(seed=4, sphere_level=4, grid_res=512, instances=512, phong_n=(100.0, 1000.0)):
     """512 seeded meshes (icospheres subdivision 4 and extruded boxes) on a
     displaced floor, ~2M triangles; Phong with n log-uniform in [100,1000] on 60%
     of surfaces, Lambertian elsewhere; 0.2x0.2 m light (radiance 50); 1024x1024;
-    max depth 10; 1,048,576 chains."""
+    max depth 10; 1,048,576 chains. The tessellation, the number of meshes
+    and the exponent range can be reduced for parity tests (n > 555 gives
+    roughness below 0.03: the relative parameterisation, path.cpp:616-660)."""
     rng = np.random.default_rng(seed)
     b = _Builder()
     b.add_light_quad((0.0, 3.0, 0.0), 0.2, (50.0, 50.0, 50.0))
+    lo, hi = np.log(phong_n[0]), np.log(phong_n[1])
 
     def mat():
         if rng.uniform() < 0.6:
-            return phong(rng.uniform(0.05, 0.3, 3), rng.uniform(0.3, 0.6, 3), float(np.exp(rng.uniform(np.log(100), np.log(1000)))))
+            return phong(rng.uniform(0.05, 0.3, 3), rng.uniform(0.3, 0.6, 3), float(np.exp(rng.uniform(lo, hi))))
         return lambert(rng.uniform(0.2, 0.8, 3))
 
     V, F, VN, Hh, xs = displaced_grid(grid_res, 4.0, 0.05, rng)
     b.add_mesh(V, F, VN, b.add_material(mat()))
     SV, SF = icosphere(sphere_level)
-    for k in range(512):
+    for k in range(instances):
         x, z = rng.uniform(-3.6, 3.6, 2)
         if k % 2 == 0:
             r = rng.uniform(0.08, 0.3)
@@ -301,20 +304,22 @@ def config4(seed=4, sphere_level=4, grid_res=512):
     return b.build(cam, opts)
 
 
-def config5(seed=5, sphere_level=3, grid_res=128):
+def config5(seed=5, sphere_level=3, grid_res=128, spheres=128, alpha=(0.02, 0.1)):
     """128 rough dielectric spheres (alpha ~ U[0.02,0.1], IOR 1.5, r ~ U[0.05,0.2])
     over a Lambertian floor (albedo 0.5); 0.1x0.1 m light (radiance 500);
-    acceptanceMap=true; max depth 12; 1,048,576 chains; 512x512."""
+    acceptanceMap=true; max depth 12; 1,048,576 chains; 512x512. The
+    tessellation, sphere count and roughness range can be reduced for parity
+    tests (alpha < 0.03: the relative parameterisation, path.cpp:616-660)."""
     rng = np.random.default_rng(seed)
     b = _Builder()
     b.add_light_quad((0.0, 2.0, 0.0), 0.1, (500.0, 500.0, 500.0))
     V, F, VN, Hh, xs = displaced_grid(grid_res, 2.0, 0.0, rng)
     b.add_mesh(V, F, VN, b.add_material(lambert((0.5, 0.5, 0.5))))
     SV, SF = icosphere(sphere_level)
-    for _ in range(128):
+    for _ in range(spheres):
         r = rng.uniform(0.05, 0.2)
         x, z = rng.normal(0.0, 0.5, 2)
-        b.add_mesh(SV * r + np.array([x, r + 1e-3, z]), SF, SV, b.add_material(rough_dielectric(rng.uniform(0.02, 0.1))))
+        b.add_mesh(SV * r + np.array([x, r + 1e-3, z]), SF, SV, b.add_material(rough_dielectric(rng.uniform(*alpha))))
     cam = [0, 1.5, 3.5, 0, 0.2, 0, 0, 1, 0, 40.0, 512, 512, 1e-2, 1000.0]
     opts = dict(DEFAULT_OPTIONS, maxdepth=12, numchains=1048576, spp=16, acceptanceMap=True, type="green")
     return b.build(cam, opts)

@@ -279,6 +279,20 @@ int drmlt_autotune(drmlt_ctx *ctx, int32_t steps, drmlt_schedule *best, double *
  * drmlt_seed_chains (valid flag, rgb, lsScore, ssScore, screen x, y). */
 int drmlt_read_chains(drmlt_ctx *ctx, int64_t n, drmlt_path_rec *states, float *contrib8);
 
+/* What the last iteration left in chains 0..n-1, for per-state checks of the
+ * loop (the reference's per-chain MarkovStates and DR variables,
+ * delayed.cpp:343-371): the four physical state slots of each chain as
+ * records (slots n x 4; slot 3 holds Green's y*) with their contrib8 rows
+ * (n x 4 x 8, [0] the slot's valid flag); perm (2 bits each: the slot of the
+ * current state, of the first- and of the second-stage proposal); flags
+ * (bit 0 first stage accepted, 1 second stage run, 2 H2MC second stage,
+ * 3 large step, 4 second-stage proposal valid, 5 first-stage proposal
+ * valid, 6 y* traced and valid); a1; alpha2, the second-stage
+ * acceptance (0 where no second stage ran); off, the H2MC offset (n x 32).
+ * Read before an iteration, perm names the roles the slots take in it. */
+int drmlt_read_step(drmlt_ctx *ctx, int64_t n, drmlt_path_rec *slots, float *contrib8, int32_t *perm, int32_t *flags, float *a1,
+                    double *alpha2, float *off);
+
 /* ---------------- tier 2: batched functions ---------------- */
 
 /* Scene::Intersect / Occluded (src/scene.cpp:80-111). any = 0: closest hit,
@@ -345,8 +359,19 @@ int drmlt_compute_gaussian_f64(int64_t n, const int32_t *D, const double *sc, do
 int drmlt_gaussian_logpdf_f64(int64_t n, const int32_t *D, const double *x, double sign, const double *mean, const double *covL,
                               const double *invCov, const double *logDet, double *out);
 
-/* Second-stage acceptance of delayed.cpp:466-541: per row (type, a1, a2,
- * aStar, transRatio, secondValid) -> alpha2. */
+/* The loop's own Gaussian code on n inputs (the device functions its H2MC
+ * kernels run; dimensions <= 8 in groups of 8 lanes, as there):
+ * ComputeGaussian (drsmallstep.cpp:18-93, 219-266) from D, sc, sigma, grad
+ * n x 32 and hess n x 32 x 32 as drmlt_compute_gaussian_f64 takes them, then
+ * GaussianLogPdf (gaussian.cpp:18-26) of sign * x[i] (x n x 32) into logpdf
+ * and GenerateSample (gaussian.cpp:36-47) from the standard normals nrm
+ * (n x 32) into sample (n x 32). */
+int drmlt_gaussian_eval_f64(int64_t n, const int32_t *D, const double *sc, double sigma, const double *grad, const double *hess,
+                            const double *x, double sign, const double *nrm, double *logpdf, double *sample);
+
+/* Second-stage acceptance of delayed.cpp:466-541 with the loop's own
+ * function: per row (type, a1, a2, aStar, transRatio, secondValid) ->
+ * alpha2. */
 int drmlt_dr_alpha2_f64(int64_t n, const double *in6, double *alpha2);
 
 #ifdef __cplusplus

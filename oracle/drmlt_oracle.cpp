@@ -271,6 +271,16 @@ extern "C" double orc_gaussian_logpdf(int D, const double *x, const double *mean
     G.logDet = logDet;
     return GaussianLogPdf(x, G);
 }
+// GenerateSample (gaussian.cpp:36-47) with the caller's standard normals.
+extern "C" void orc_gaussian_sample(int D, const double *nrm, const double *mean, const double *covL32, double *x) {
+    Gauss G;
+    G.D = D;
+    G.mean.assign(mean, mean + D);
+    G.covL.resize(D * D);
+    for (int a = 0; a < D; a++)
+        for (int b = 0; b < D; b++) G.covL[a * D + b] = covL32[a * 32 + b];
+    GenerateSample(G, nrm, x);
+}
 extern "C" void orc_jacobi_eigen(int D, const double *A, double *lam, double *V) {
     std::vector<double> a(A, A + D * D), v(D * D);
     JacobiEigen(D, a.data(), v.data());

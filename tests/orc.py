@@ -204,6 +204,14 @@ def gaussian_logpdf(D, x, mean, invCov32, logDet):
     return lib().orc_gaussian_logpdf(D, P(x), P(mean), P(invCov32), C.c_double(logDet))
 
 
+def gaussian_sample(D, nrm, mean, covL32):
+    nrm = np.ascontiguousarray(nrm, np.float64); mean = np.ascontiguousarray(mean, np.float64)
+    covL32 = np.ascontiguousarray(covL32, np.float64)
+    x = np.zeros(D)
+    lib().orc_gaussian_sample(D, P(nrm), P(mean), P(covL32), P(x))
+    return x
+
+
 def jacobi_eigen(A):
     A = np.ascontiguousarray(A, np.float64); D = len(A)
     lam = np.zeros(D); V = np.zeros((D, D))

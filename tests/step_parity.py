@@ -1,0 +1,269 @@
+"""Per-state parity of one DR iteration (delayed.cpp:396-606) against the
+oracle: the helper behind tests/test_gpu_step_parity.py.
+
+One iteration runs on the device from states that are read back first
+(drmlt_read_step). For every chain, the proposals the loop made -- the
+first-stage proposal y, the second-stage proposal z, Green's y* -- are
+re-evaluated by the oracle from their exact primary samples
+(PerturbPathBidir in double, path.cpp:1480-1628). Two things are checked:
+
+(A) each proposal's f (ssScore) as the loop evaluated it (FP32) against the
+    oracle's FP64 evaluation of the same primary samples, with the
+    triangles the two traced;
+(B) the loop's acceptances from the f values it evaluated, against the
+    reference's formulas computed here: a1 (mutation.cpp:86-99 for a large
+    step, mutation.cpp:197 for a small one), the stage-2 ratio
+    (mutation.cpp:197 isotropic; drsmallstep.cpp:181-196 for H2MC, with the
+    oracle's FP64 ComputeGaussian / GaussianLogPdf over the device's
+    derivative values from drmlt_grad_hess) and alpha2
+    (delayed.cpp:466-541, isInvalid included).
+
+The same acceptances computed from the oracle's own f values (C) are
+reported; they differ from (B) by the FP32 rounding of f that (A) measures."""
+import numpy as np
+
+import orc
+
+
+def _iso_gaussian(D, sigma):
+    # gaussian.cpp:4-16
+    inv = np.zeros((32, 32))
+    inv[np.arange(D), np.arange(D)] = 1.0 / sigma ** 2
+    return np.zeros(32), inv, D * np.log(1.0 / sigma ** 2)
+
+
+def has_derivative(c, l, max_derv):
+    # path.cpp:3186-3206 with the derivative library's depth
+    return c >= 1 and l >= 0 and c + l >= 3 and c + l - 1 <= max_derv
+
+
+def run_one_iteration(ctx, warm, mode):
+    """warm iterations, then the read-back before and after one more."""
+    ctx.run(warm, mode)
+    before = ctx.read_step()
+    ctx.run(1, mode)
+    after = ctx.read_step()
+    return before, after
+
+
+def well_formed(recs, nlights, ntris):
+    """Records the loop could have evaluated (the checks of drmlt_perturb's
+    argument validation): vertex counts within the fixed slots, light and
+    triangle indices within the scene."""
+    c, l = recs["c"], recs["l"]
+    P = c + l - 1
+    ok = (c >= 1) & (l >= 0) & (c <= 17) & (l <= 17) & (P >= 2) & (2 * np.maximum(P, 2) <= 32) & ~((c == 1) & (l < 2))
+    ok &= (l < 1) | ((recs["light"] >= 0) & (recs["light"] < nlights) & (recs["lprim"] >= 0) & (recs["lprim"] < ntris))
+    k = np.arange(16)[None]
+    ok &= ~(((recs["cprim"] < -1) | (recs["cprim"] >= ntris)) & (k < (c - 1)[:, None])).any(1)
+    ok &= ~(((recs["lgprim"] < -1) | (recs["lgprim"] >= ntris)) & (k < (l - 1)[:, None])).any(1)
+    return ok
+
+
+def oracle_eval(O, recs, nlights, ntris):
+    """The oracle's PerturbPathBidir (double) of each record at its own
+    primary samples; records that are not well formed evaluate as invalid."""
+    ok = well_formed(recs, nlights, ntris)
+    out = recs.copy()
+    c8 = np.zeros((len(recs), 8), np.float32)
+    if ok.any():
+        z = np.zeros((int(ok.sum()), 32), np.float32)
+        out[ok], c8[ok] = O.perturb(recs[ok], z, z, dbl=True)
+    return out, c8
+
+
+def _same_triangles(a, b):
+    ok = np.ones(len(a), bool)
+    for i in range(len(a)):
+        nc, nl = a["c"][i] - 1, a["l"][i] - 1
+        ok[i] = np.array_equal(a["cprim"][i, :nc], b["cprim"][i, :nc]) and np.array_equal(a["lgprim"][i, :nl], b["lgprim"][i, :nl])
+    return ok
+
+
+def _invalid(x):
+    return ~np.isfinite(x) | (x < 0)  # delayed.cpp:252
+
+
+def _clamp(x):
+    return np.where(np.isnan(x), x, np.clip(x, 0.0, 1.0))  # utils.h:245-248 (NaN passes)
+
+
+def _stage1(gen, cur_valid, y_valid, lx, ly, ls_x, ss_x, ls_y, ss_y, lf):
+    lfd = np.asarray(lf, np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        large = ((ly + 1) * ls_y * lfd[np.clip(lx, 0, len(lfd) - 1)]) / ((lx + 1) * ls_x * lfd[np.clip(ly, 0, len(lfd) - 1)])
+        small = ss_y / ss_x
+    a = np.where(gen, np.where(cur_valid, large, 0.0), small)
+    a = np.where(y_valid, _clamp(a), 0.0)
+    return np.where(_invalid(a), 0.0, a)  # delayed.cpp:417-419, 430-432
+
+
+def _alpha2(dr_type, a2pre, astar, tr, a1):
+    one_m_a1 = (np.float32(1) - a1.astype(np.float32)).astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        if dr_type == 0:
+            astar = np.where(_invalid(astar), 0.0, astar)
+            one_m_as = (np.float32(1) - astar.astype(np.float32)).astype(np.float64)
+            a = np.where(astar.astype(np.float32) == 1, 0.0, np.minimum(1.0, a2pre * one_m_as / one_m_a1))
+        else:
+            astar = np.where(_invalid(astar), 1.0, astar)
+            one_m_as = (np.float32(1) - astar.astype(np.float32)).astype(np.float64)
+            a = np.where((astar.astype(np.float32) == 1) | _invalid(tr), 0.0, np.minimum(1.0, a2pre * tr * one_m_as / one_m_a1))
+    return np.where(_invalid(a2pre), 0.0, a)
+
+
+def analyse(O, ctx, lf, before, after, dr_type=0, sigma=0.01, max_derv=9):
+    """Per-chain comparisons (A), (B), (C); returns a dict of arrays and the
+    masks of the chains each comparison covers."""
+    n = len(after["perm"])
+    idx = np.arange(n)
+    pb = before["perm"]
+    pc, pf, ps = pb & 3, (pb >> 2) & 3, (pb >> 4) & 3
+    X, x8 = before["slots"][idx, pc], before["contrib8"][idx, pc]
+    Y, Z, S = after["slots"][idx, pf], after["slots"][idx, ps], after["slots"][idx, 3]
+    y8, z8, s8 = after["contrib8"][idx, pf], after["contrib8"][idx, ps], after["contrib8"][idx, 3]
+    ps_valid_before = before["contrib8"][idx, ps, 0] > 0
+    flags, a1g, a2g = after["flags"], after["a1"].astype(np.float64), after["alpha2"]
+    off = after["off"].astype(np.float64)
+    cur_valid = x8[:, 0] > 0
+    gen = ((flags & 8) != 0) | ~cur_valid
+    second = (flags & 2) != 0
+    aniso = second & ((flags & 4) != 0)
+    y_gpu, z_gpu, s_gpu = (flags & 32) != 0, (flags & 16) != 0, (flags & 64) != 0
+
+    nl, nt = len(O.sc["light_tris"]), len(O.sc["tri_p"])
+    Yo, y8o = oracle_eval(O, Y, nl, nt)
+    Zo, z8o = oracle_eval(O, Z, nl, nt)
+    So, s8o = oracle_eval(O, S, nl, nt)
+    y_o, z_o, s_o = y8o[:, 0] > 0, z8o[:, 0] > 0, s8o[:, 0] > 0
+
+    # (A) f of the proposals, loop (FP32) against oracle (FP64)
+    def frel(g8, o8, both):
+        both = both & (o8[:, 5] > 0)
+        return np.abs(g8[both, 5].astype(np.float64) - o8[both, 5]) / o8[both, 5]
+
+    y_both, z_both = y_gpu & y_o, second & z_gpu & z_o
+    lx, ly = X["c"] + X["l"] - 1, Y["c"] + Y["l"] - 1
+    ss_x, ls_x = x8[:, 5].astype(np.float64), x8[:, 4].astype(np.float64)
+
+    # (B) and (C) first stage
+    a1B = _stage1(gen, cur_valid, y_gpu, lx, ly, ls_x, ss_x, y8[:, 4].astype(np.float64), y8[:, 5].astype(np.float64), lf)
+    a1C = _stage1(gen, cur_valid, y_o, lx, ly, ls_x, ss_x, y8o[:, 4].astype(np.float64), y8o[:, 5].astype(np.float64), lf)
+
+    # second stage: the stage-2 ratio before the delayed-rejection correction
+    def ratio(ss_z, z_valid, log_gauss):
+        with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+            iso = _clamp(ss_z / ss_x)
+            h2 = np.exp(log_gauss) * ss_z / ss_x
+        return np.where(z_valid, np.where(aniso, h2, iso), 0.0)
+
+    log_gauss = np.zeros(n)
+    h = np.nonzero(aniso & (z_gpu | z_o) & cur_valid)[0]
+    if len(h):
+        D = 2 * np.maximum(X["c"][h] + X["l"][h] - 1, 2)
+        gx, hx, okx = ctx.grad_hess(X[h], fp64=False)
+        gz, hz, okz = ctx.grad_hess(np.where(well_formed(Z[h], nl, nt), Z[h], X[h]), fp64=False)
+        for k, i in enumerate(h):
+            d = int(D[k])
+            lps = []
+            for g_, h_, ok_, sc_, sign in ((gx[k], hx[k], okx[k], ss_x[i], 1.0), (gz[k], hz[k], okz[k], float(z8[i, 5]), -1.0)):
+                use = (has_derivative(int(X["c"][i]), int(X["l"][i]), max_derv) and d <= 24 and sc_ > 1e-15 and ok_
+                       and np.isfinite(g_[:d]).all() and np.isfinite(h_[:d, :d]).all())
+                if use:
+                    m, _, inv, ld = orc.compute_gaussian(np.array([d]), np.array([sc_]), sigma, g_[None], h_[None])
+                    m, inv, ld = m[0], inv[0], float(ld[0])
+                else:
+                    m, inv, ld = _iso_gaussian(d, sigma)
+                lps.append(orc.gaussian_logpdf(d, sign * off[i, :d], m[:d], inv[:32, :32], ld))
+            log_gauss[i] = lps[1] - lps[0]
+    a2preB = ratio(z8[:, 5].astype(np.float64), z_gpu, log_gauss)
+    a2preC = ratio(z8o[:, 5].astype(np.float64), z_o, log_gauss)
+    # y* is traced where the reference traces it: Green & Mira, a valid
+    # ratio that is not zero, a second proposal object marked valid
+    traceB = second & ~_invalid(a2preB) & (a2preB != 0) & ps_valid_before & (dr_type == 0)
+    traceC = second & ~_invalid(a2preC) & (a2preC != 0) & ps_valid_before & (dr_type == 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if dr_type == 0:
+            astarB = np.where(traceB & s_gpu, _clamp(s8[:, 5].astype(np.float64) / z8[:, 5]), 0.0)
+            astarC = np.where(traceC & s_o, _clamp(s8o[:, 5].astype(np.float64) / z8o[:, 5]), 0.0)
+            tr = np.ones(n)
+        else:
+            # Tierney & Mira: the stored scores of the two proposal objects (a
+            # stale score where a proposal was invalid, as in the reference)
+            astarB = np.minimum(1.0, y8[:, 5].astype(np.float64) / z8[:, 5])
+            astarC = np.minimum(1.0, np.where(y_o, y8o[:, 5], y8[:, 5]).astype(np.float64) / np.where(z_o, z8o[:, 5], z8[:, 5]))
+            dzy = Z["u"].astype(np.float64) - Y["u"]
+            dxy = X["u"].astype(np.float64) - Y["u"]
+            D = 2 * np.maximum(X["c"] + X["l"] - 1, 2)
+            mask = np.arange(32)[None] < D[:, None]
+            s2 = np.float64(np.float32(sigma)) ** 2
+            with np.errstate(over="ignore"):
+                tr = np.exp(-0.5 * (dzy ** 2 * mask).sum(1) / s2 + 0.5 * (dxy ** 2 * mask).sum(1) / s2)
+    a2B = np.where(second, _alpha2(dr_type, a2preB, astarB, tr, after["a1"]), 0.0)
+    a2C = np.where(second, _alpha2(dr_type, a2preC, astarC, tr, after["a1"]), 0.0)
+    s_both = traceB & s_gpu & s_o
+    return {"n": n, "second": second, "aniso": aniso, "large": gen, "flags": flags,
+            "y_valid_agree": y_gpu == y_o, "z_valid_agree": ~second | (z_gpu == z_o), "s_valid_agree": ~traceB | (s_gpu == s_o),
+            "y_tri": _same_triangles(Y, Yo)[y_both], "z_tri": _same_triangles(Z, Zo)[z_both], "s_tri": _same_triangles(S, So)[s_both],
+            "f_rel_y": frel(y8, y8o, y_both), "f_rel_z": frel(z8, z8o, z_both), "f_rel_s": frel(s8, s8o, s_both),
+            "a1_gpu": a1g, "a1_B": a1B, "a1_C": a1C, "alpha2_gpu": a2g, "alpha2_B": a2B, "alpha2_C": a2C,
+            "alpha2_pre_B": a2preB, "astar_B": astarB, "star_traced": traceB, "Y": Y, "Z": Z, "S": S, "Yo": Yo, "Zo": Zo, "So": So,
+            "y8o": y8o, "z8o": z8o, "s8o": s8o, "y_both": y_both, "z_both": z_both, "s_both": s_both}
+
+
+def rounding_baselines(O, ctx64, r):
+    """The f error the loop's FP32 evaluation has against the oracle's FP64
+    set beside two baselines on the same valid states with the same
+    triangles: the oracle's float build (the reference as shipped computes
+    in float) against FP64, and the device's FP64 fixed-topology evaluation
+    (drmlt_eval_logf; ctx64 must differentiate the primal, quirk_chad_mis 0)
+    against the oracle's FP64. Returns the three relative-error arrays."""
+    nl, nt = len(O.sc["light_tris"]), len(O.sc["tri_p"])
+    rec, o64, g32 = [], [], []
+    for key, okey, both in (("Y", "y8o", "y_both"), ("Z", "z8o", "z_both"), ("S", "s8o", "s_both")):
+        m = r[both] & (r[okey][:, 5] > 0) & _same_triangles(r[key], r[key + "o"])
+        rec.append(r[key][m])
+        o64.append(r[okey][m, 5].astype(np.float64))
+    recs, o64 = np.concatenate(rec), np.concatenate(o64)
+    _, o32 = oracle_eval_float(O, recs, nl, nt)
+    lf64, ok = ctx64.eval_logf(recs, fp64=True)
+    keep = (o32[:, 0] > 0) & (ok == 1)
+    e_orc32 = np.abs(o32[keep, 5].astype(np.float64) - o64[keep]) / o64[keep]
+    e_dev64 = np.abs(np.exp(lf64[keep]) - o64[keep]) / o64[keep]
+    return e_orc32, e_dev64
+
+
+def oracle_eval_float(O, recs, nlights, ntris):
+    ok = well_formed(recs, nlights, ntris)
+    out = recs.copy()
+    c8 = np.zeros((len(recs), 8), np.float32)
+    if ok.any():
+        z = np.zeros((int(ok.sum()), 32), np.float32)
+        out[ok], c8[ok] = O.perturb(recs[ok], z, z, dbl=False)
+    return out, c8
+
+
+def summary(r):
+    d1B, d2B = np.abs(r["a1_gpu"] - r["a1_B"]), np.abs(r["alpha2_gpu"] - r["alpha2_B"])[r["second"]]
+    d1C, d2C = np.abs(r["a1_gpu"] - r["a1_C"]), np.abs(r["alpha2_gpu"] - r["alpha2_C"])[r["second"]]
+    f = np.concatenate([r["f_rel_y"], r["f_rel_z"], r["f_rel_s"]])
+    pct = lambda a, q: float(np.percentile(a, q)) if len(a) else 0.0
+    return {"chains": r["n"], "large": int(r["large"].sum()), "second": int(r["second"].sum()), "aniso": int(r["aniso"].sum()),
+            "star_traced": int(r["star_traced"].sum()),
+            "valid_agree": float(np.mean(r["y_valid_agree"] & r["z_valid_agree"] & r["s_valid_agree"])),
+            "tri_equal": float(np.mean(np.concatenate([r["y_tri"], r["z_tri"], r["s_tri"]]))),
+            "f_compared": int(len(f)), "f_rel_median": pct(f, 50), "f_rel_p99": pct(f, 99), "f_within_1e-4": float(np.mean(f <= 1e-4)),
+            "a1_B_within_1e-5": float(np.mean(d1B <= 1e-5)), "a1_B_max": float(d1B.max()),
+            "alpha2_B_within_1e-5": float(np.mean(d2B <= 1e-5)) if len(d2B) else 1.0, "alpha2_B_max": float(d2B.max()) if len(d2B) else 0.0,
+            "a1_C_within_1e-5": float(np.mean(d1C <= 1e-5)), "alpha2_C_within_1e-5": float(np.mean(d2C <= 1e-5)) if len(d2C) else 1.0}
+
+
+def worst(r, k=5):
+    """The chains with the largest (B) disagreement, for the log."""
+    out = []
+    for key, mask in (("a1", np.ones(r["n"], bool)), ("alpha2", r["second"])):
+        d = np.where(mask, np.abs(r[key + "_gpu"] - r[key + "_B"]), -1.0)
+        for i in np.argsort(-d)[:k]:
+            out.append({"what": key, "chain": int(i), "err": float(d[i]), "flags": int(r["flags"][i]), "gpu": float(r[key + "_gpu"][i]),
+                        "B": float(r[key + "_B"][i]), "a2pre": float(r["alpha2_pre_B"][i]), "astar": float(r["astar_B"][i])})
+    return out

@@ -209,6 +209,50 @@ def test_gaussian_and_logpdf():
         assert abs(lp[i] - ref) <= 1e-8 * max(1.0, abs(ref))
 
 
+def test_loop_gaussian_code_vs_oracle():
+    """The loop's own Gaussian code (drmlt_gaussian_eval_f64: gauss_compute,
+    gauss_logpdf and gauss_sample, the functions k_b_gauss, h2mc_offset and
+    h2mc_ratio run, dimensions <= 8 in 8-lane groups) against the oracle's
+    ComputeGaussian / GaussianLogPdf / GenerateSample: log-densities within
+    1e-9 relative; a sample from zero normals is the mean; any sample's
+    Mahalanobis norm under the oracle's inverse covariance is the normals'
+    squared norm (the covariance factor itself is fixed only up to
+    eigenvector signs, so the sample is compared through that invariant)."""
+    from drmlt import capi
+
+    rng = np.random.default_rng(17)
+    n = 800
+    D = rng.integers(2, 13, n).astype(np.int32) * 2
+    grad = rng.normal(0, 50, (n, 32))
+    A = rng.normal(0, 3e3, (n, 32, 32))
+    hess = -(A + A.transpose(0, 2, 1)) / 2
+    sc = np.where(rng.uniform(size=n) < 0.05, 1e-16, 1.0)
+    x = rng.normal(0, 0.01, (n, 32))
+    nrm = rng.normal(size=(n, 32))
+    nrm[::2] = 0.0
+    for i in range(n):
+        grad[i, D[i]:] = 0
+        hess[i, D[i]:, :] = 0
+        hess[i, :, D[i]:] = 0
+    om, _, oQ, old = orc.compute_gaussian(D, sc, 0.01, grad, hess)
+    for sign in (1.0, -1.0):
+        say(f"gaussian_eval n={n} sign={sign}")
+        lp, smp = capi.gaussian_eval(D, sc, 0.01, grad, hess, x, sign, nrm)
+        assert (D <= 8).sum() > 100 and (D > 8).sum() > 100  # both executors
+        for i in range(n):
+            d = int(D[i])
+            ref = orc.gaussian_logpdf(d, sign * x[i, :d], om[i, :d], oQ[i, :32, :32], float(old[i]))
+            assert abs(lp[i] - ref) <= 1e-9 * max(1.0, abs(ref)), (i, d, lp[i], ref)
+            dl = smp[i, :d] - om[i, :d]
+            if i % 2 == 0:
+                assert np.allclose(smp[i, :d], om[i, :d], rtol=1e-9, atol=1e-12 * max(1.0, np.abs(om[i, :d]).max()))
+            else:
+                maha = dl @ oQ[i, :d, :d] @ dl
+                ref_n = nrm[i, :d] @ nrm[i, :d]
+                assert abs(maha - ref_n) <= 1e-7 * ref_n, (i, d, maha, ref_n)
+            assert not smp[i, d:].any()
+
+
 def test_alpha2_closed_form():
     """Second-stage acceptance (delayed.cpp:466-541) against its closed form."""
     from drmlt import capi
