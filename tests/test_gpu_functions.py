@@ -33,6 +33,26 @@ def c2(cfg2_small):
     ctx.close()
 
 
+@pytest.fixture(scope="module")
+def c4(cfg4_small):
+    from drmlt import capi
+
+    opt = capi.options_from_dpt(cfg4_small["options"], num_init_samples=4096, num_chains=4096, seed=3)
+    ctx = capi.Context(cfg4_small, opt)
+    yield cfg4_small, ctx, orc.Scene(cfg4_small)
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def c5(cfg5_small):
+    from drmlt import capi
+
+    opt = capi.options_from_dpt(cfg5_small["options"], num_init_samples=4096, num_chains=4096, seed=3)
+    ctx = capi.Context(cfg5_small, opt)
+    yield cfg5_small, ctx, orc.Scene(cfg5_small)
+    ctx.close()
+
+
 def _rays(rng, n, center, half):
     org = rng.uniform(-half, half, (n, 3)) + np.asarray(center)
     d = rng.normal(size=(n, 3))
@@ -107,29 +127,45 @@ def test_generate_subpath_parity(c1):
     pytest.skip("drmlt_generate_subpath is switched off: its kernel made a GPU memory fault whose cause is not established")
 
 
-@pytest.mark.parametrize("which", ["c1", "c2"])
+@pytest.mark.parametrize("which", ["c1", "c2", "c4", "c5"])
 def test_generate_bidir_parity(which, request):
     """GeneratePathBidir (all strategies, MIS weights): the same contribution
-    list as the oracle for nearly every sample."""
+    list as the oracle (FP64) for nearly every sample. On the smooth-lobe
+    scenes at depths 10 and 12 (c4, c5) float arithmetic itself errs more:
+    there the device's FP32 errors are held to those of the oracle's float
+    build (the reference as shipped) on the same samples."""
     sc, ctx, O = request.getfixturevalue(which)
     rng = np.random.default_rng(5)
-    n = 2000
+    n = 8000 if which in ("c4", "c5") else 2000  # small lights: fewer random paths contribute
     maxd = int(sc["options"]["maxdepth"])
     uni = rng.uniform(0, 1, (n, 160)).astype(np.float32)
     say(f"generate_bidir {which} n={n} maxdepth={maxd}")
     gcs, gnc = ctx.generate_bidir(uni, 3, maxd)
     ocs, onc = O.generate_bidir(uni, 3, maxd, dbl=True)
-    assert onc.sum() > n // 10
-    same_n = gnc == onc
-    assert same_n.mean() > 0.99
-    idx = np.nonzero(same_n)[0]
-    g = np.concatenate([gcs[i, :gnc[i]] for i in idx])
-    o = np.concatenate([ocs[i, :onc[i]] for i in idx])
-    assert np.array_equal(g[:, :2], o[:, :2])  # strategies (c, l)
+    fcs, fnc = O.generate_bidir(uni, 3, maxd, dbl=False)
+    assert onc.sum() > n // 20
+
+    def errors(cs, nc):
+        same = nc == onc
+        idx = np.nonzero(same)[0]
+        a = np.concatenate([cs[i, :nc[i]] for i in idx])
+        o = np.concatenate([ocs[i, :onc[i]] for i in idx])
+        assert np.array_equal(a[:, :2], o[:, :2])  # strategies (c, l)
+        return same.mean(), {k: np.abs(a[:, k] - o[:, k]) / np.maximum(np.abs(o[:, k]), 1e-20) for k in (2, 3, 4, 6, 9)}
+
+    same_g, eg = errors(gcs, gnc)
+    same_f, ef = errors(fcs, fnc)
+    say(f"generate_bidir {which}: same count gpu {same_g:.4f} oracle-float {same_f:.4f}; ssScore median rel gpu "
+        f"{np.median(eg[6]):.3g} oracle-float {np.median(ef[6]):.3g}")
+    smooth = which in ("c4", "c5")
+    assert same_g > (min(0.99, same_f - 0.005) if smooth else 0.99)
     for k in (2, 3, 4, 6, 9):  # rgb, ssScore, MIS weight
-        rel = np.abs(g[:, k] - o[:, k]) / np.maximum(np.abs(o[:, k]), 1e-20)
-        assert np.median(rel) < 1e-5, (k, np.median(rel))
-        assert np.mean(rel < 1e-3) > 0.99, (k, np.mean(rel < 1e-3))
+        if smooth:
+            assert np.median(eg[k]) <= max(1e-5, 1.5 * np.median(ef[k])), (k, np.median(eg[k]), np.median(ef[k]))
+            assert np.mean(eg[k] < 1e-3) >= min(0.99, np.mean(ef[k] < 1e-3) - 0.01), (k, np.mean(eg[k] < 1e-3), np.mean(ef[k] < 1e-3))
+        else:
+            assert np.median(eg[k]) < 1e-5, (k, np.median(eg[k]))
+            assert np.mean(eg[k] < 1e-3) > 0.99, (k, np.mean(eg[k] < 1e-3))
 
 
 @pytest.mark.parametrize("which", ["c1", "c2"])
@@ -174,6 +210,77 @@ def test_grad_hess_vs_finite_differences(which, request):
         d = D[i]
         H = h64[i, :d, :d]
         assert np.allclose(H, H.T, rtol=1e-9, atol=1e-9 * max(np.abs(H).max(), 1.0))
+
+
+def _roughness(sc, prim, disc):
+    # Roughness per vertex (phong.cpp:148-155, roughdielectric.h:61-63, lambertian.h:39-41)
+    m = sc["mats"][sc["tri_mat"][prim]]
+    return float(orc.lib().orc_roughness(orc.P(np.ascontiguousarray(m)), orc.C.c_double(float(disc))))
+
+
+@pytest.mark.parametrize("which", ["cfg4_small", "cfg5_small"])
+def test_logf_and_derivatives_on_smooth_lobes(which, request):
+    """log f (drmlt_eval_logf, FP64) and its gradient and Hessian
+    (drmlt_grad_hess, FP64 and FP32) at path states the loop reached on the
+    reduced config-4 and config-5 scenes -- dimensions up to 20 and 24,
+    vertices on Phong n in [600, 1000] and dielectric alpha in [0.02, 0.03)
+    in the relative parameterisation (path.cpp:616-660) -- against the
+    oracle's FP64 log f and its central differences at two steps (where the
+    two agree)."""
+    from drmlt import capi
+
+    sc = request.getfixturevalue(which)
+    O = orc.Scene(sc)
+    ctx = capi.Context(sc, capi.options_from_dpt(sc["options"], num_init_samples=20000, num_chains=8192, seed=21))
+    say(f"smooth-lobe states {which}: pre-pass + 3 iterations")
+    ctx.prepass()
+    ctx.run(3, capi.BATCHED_GRAPH)
+    recs, c8 = ctx.read_chains()
+    P = recs["c"] + recs["l"] - 1
+    recs = recs[(c8[:, 0] > 0) & (c8[:, 5] > 1e-15) & (recs["c"] + recs["l"] >= 3) & (P <= 12)]
+    D = 2 * np.maximum(recs["c"] + recs["l"] - 1, 2)
+    smooth = np.array([any(_roughness(sc, r["cprim"][k], r["cdisc"][k]) < 0.03 for k in range(r["c"] - 1)) or
+                       any(_roughness(sc, r["lgprim"][k], r["ldisc"][k]) < 0.03 for k in range(r["l"] - 1)) for r in recs])
+    say(f"{which}: {len(recs)} states, D histogram {np.bincount(D).tolist()}, {int(smooth.sum())} with a smooth vertex")
+    long_d = 12 if which == "cfg4_small" else 16  # the loop's config-4 paths are shorter
+    assert smooth.sum() > len(recs) // 10 and (D >= long_d).sum() > 50
+    say(f"eval_logf fp64 {which} n={len(recs)}")
+    l64, ok64 = ctx.eval_logf(recs, fp64=True)
+    lo = O.logf_fixed(recs, np.zeros((len(recs), 32), np.float32), chad_quirk=True)
+    both = (ok64 == 1) & np.isfinite(lo)
+    assert both.mean() > 0.99
+    d64 = np.abs(l64[both] - lo[both])
+    say(f"{which} log f fp64 vs oracle: median {np.median(d64):.3g} p95 {np.percentile(d64, 95):.3g}")
+    assert np.median(d64) < 1e-5 and np.percentile(d64, 95) < 1e-4
+    # derivatives: states with a smooth vertex first, long ones among them
+    order = np.lexsort((-D, ~smooth))
+    sub = recs[order[:300]]
+    Ds = D[order[:300]]
+    say(f"grad_hess fp64/fp32 {which} n={len(sub)}")
+    g64, h64, okg = ctx.grad_hess(sub, fp64=True)
+    g32, h32, ok32 = ctx.grad_hess(sub, fp64=False)
+    lf, gfd, hfd = O.grad_hess_fd(sub, 2e-5, chad_quirk=True)
+    _, gfd2, hfd2 = O.grad_hess_fd(sub, 1e-5, chad_quirk=True)
+    rel = lambda a, b: np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-9)
+    eg, eh, e32, big = [], [], [], 0
+    for i in range(len(sub)):
+        d = Ds[i]
+        if not (okg[i] and np.isfinite(lf[i]) and np.isfinite(hfd[i]).all() and np.isfinite(hfd2[i]).all()):
+            continue
+        if rel(gfd[i, :d], gfd2[i, :d]) > 1e-5 or rel(hfd[i, :d, :d], hfd2[i, :d, :d]) > 1e-3:
+            continue
+        eg.append(rel(g64[i, :d], gfd2[i, :d]))
+        eh.append(rel(h64[i, :d, :d], hfd2[i, :d, :d]))
+        big += d >= long_d
+        if ok32[i]:
+            e32.append(rel(h32[i, :d, :d], h64[i, :d, :d]))
+    say(f"{which}: {len(eg)} derivative states compared ({big} with D >= {long_d}): median rel grad {np.median(eg):.3g}, "
+        f"hess {np.median(eh):.3g}, fp32 vs fp64 hess {np.median(e32):.3g}")
+    assert len(eg) >= 60 and big >= 10
+    assert np.median(eg) < 1e-5 and np.percentile(eg, 90) < 1e-3
+    assert np.median(eh) < 1e-4 and np.percentile(eh, 90) < 1e-2
+    assert np.median(e32) < 1e-3
+    ctx.close()
 
 
 def test_gaussian_and_logpdf():

@@ -149,16 +149,20 @@ def test_schedules_agree_and_autotune(cfg2_small):
     ctx.close()
 
 
-@pytest.mark.parametrize("which,kind", [("cfg1", "green"), ("cfg2_small", "green"), ("cfg1", "mira"), ("cfg2_small", "mira")])
+@pytest.mark.parametrize("which,kind", [("cfg1", "green"), ("cfg2_small", "green"), ("cfg1", "mira"), ("cfg2_small", "mira"),
+                                        ("cfg4_small", "green"), ("cfg5_small", "green")])
 def test_loop_vs_oracle(which, kind, request):
     """The DR loop (H2MC second stage; Green-Mira or Tierney-Mira acceptance)
     against the oracle's loop from the oracle's own seeds: acceptance rates per
     stage and the film mean agree within Monte Carlo error (cfg1: Lambertian
-    room; cfg2_small: Phong and rough dielectric surfaces)."""
+    room; cfg2_small: Phong and rough dielectric surfaces; cfg4_small and
+    cfg5_small: Phong n in [600, 1000] and dielectric alpha in [0.02, 0.03)
+    at maximum depths 10 and 12, the relative parameterisation of smooth
+    vertices)."""
     cfg1 = request.getfixturevalue(which)
     O = orc.Scene(cfg1)
     maxd = int(cfg1["options"]["maxdepth"])
-    chains, iters = 2048, 32
+    chains, iters = (2048, 32) if which in ("cfg1", "cfg2_small") else (8192, 32)
     b, lf, seeds, c8, _ = O.prepass(20000, maxd, chains, seed=13)
     from drmlt import capi
 
@@ -168,7 +172,9 @@ def test_loop_vs_oracle(which, kind, request):
     ctx.seed_chains(seeds, c8, b, lf)
     ctx.run(iters, 2)
     gfilm, gs = ctx.film(1), ctx.stats()
-    ofilm, ost, _ = O.render(seeds, c8, lf, b, iters, seed=19, opts={"type": 1 if kind == "mira" else 0})
+    # config 5 renders the acceptance map (counts of accepted moves) in both
+    amap = 1 if cfg1["options"].get("acceptanceMap") else 0
+    ofilm, ost, _ = O.render(seeds, c8, lf, b, iters, seed=19, opts={"type": 1 if kind == "mira" else 0, "acceptancemap": amap})
     names = ["large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso", "stage2_iso_accepted",
              "stage2_h2mc", "stage2_h2mc_accepted"]
     os_ = dict(zip(names, [int(v) for v in ost]))
