@@ -1,17 +1,22 @@
 """DRMLT mutation-loop benchmark on one MI355X.
 
-A step is one DR iteration (delayed.cpp:396-656) of every chain. The workload
-is the batched configuration: the config-2 scene (displaced 256x256 grid +
-64 subdivision-4 icospheres, 50% Lambertian / 30% Phong / 20% rough
-dielectric, 512x512 film), 65,536 chains, path lengths up to 8, FP32 path
-evaluation, FP64 Gaussians and acceptance ratios. Chain states, scene and
-BVH are resident in HBM before the timed region; the pre-pass and the
-direct-lighting pass run before it.
+A step is one DR iteration (delayed.cpp:396-656) of every chain. The
+workload is config 4, the configuration the equal-RMSE target is quoted on:
+a seeded procedural scene of 512 icospheres and extruded boxes on a
+displaced floor (1,836,036 triangles), Phong n log-uniform in [100, 1000] on
+60% of the surfaces, a 0.2 x 0.2 m light of radiance 50, 1024 x 1024 film,
+1,048,576 chains, maximum depth 10; FP32 path evaluation, FP64 Gaussians and
+acceptance ratios. Chain states, scene and BVH are resident in HBM before
+the timed region; the pre-pass, the direct-lighting pass and the autotuner
+run before it.
 
 Prints one JSON line: mutations/s of the batched + adjoint form (hipGraph
 replay of the per-stage persistent kernels), the plain one-kernel-per-step
-form beside it, the roofline of the dominant kernel and the CPU baseline
-(the oracle's C++ restatement of the reference loop on the host cores).
+form beside it, the roofline of the dominant kernel, the CPU baseline (the
+oracle's C++ restatement of the reference loop on the host cores), the
+equal-RMSE study on the same configuration (the time the batched form needs
+to reach the plain form's 60 s RMSE), config 2 at its maximum depth 9, the
+mcmc and h2mc integrators and the derivative kernels alone (config 3).
 """
 import argparse
 import json
@@ -91,7 +96,7 @@ def derivative_workload(n=262144, seed=4321):
             "ok_fraction": float(both.mean())}
 
 
-def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
+def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234, plain_chunk=4, batched_chunk=4):
     """Equal-time image RMSE against a long reference render.
 
     The reference is a batched render from independent seeds with 256 direct
@@ -132,7 +137,7 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
     relmse = lambda a, b: float(np.mean((a.astype(np.float64) - b) ** 2 / (b.astype(np.float64) ** 2 + 1e-2)))
     t0 = time.time()
     ref, b_ref = make(seed + 7777, direct_spp=256)
-    if config != 2 or not sched:
+    if not sched:
         # a schedule for this workload, tuned on the reference's chains (its
         # tuning steps only add to the reference render)
         sched, _ = ref.autotune(steps=2)
@@ -146,9 +151,12 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
     rfull, rind = images(ref)
     ref.close()
 
-    def curve(mode, step_chunk):
+    def curve(mode, step_chunk, target=None):
+        """RMSE points every step_chunk steps until budget_s of device time;
+        with a target, stop once the curve has stayed at or below it for
+        30% (at least 1 s) beyond its first crossing."""
         ctx, _ = make(seed, b_common=b_ref)
-        t, steps, pts = 0.0, 0, []
+        t, steps, pts, crossed = 0.0, 0, [], None
         t1 = time.time()
         while t < budget_s * 1000.0:
             ctx.run(step_chunk, mode)
@@ -157,14 +165,21 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
             full, ind = images(ctx)
             pts.append((t / 1000.0, steps, rmse(full, rfull), rmse(ind, rind), relmse(ind, rind),
                         float(ind.mean() / max(rind.mean(), 1e-30))))
+            if target is not None:
+                if pts[-1][3] > target:
+                    crossed = None
+                elif crossed is None:
+                    crossed = pts[-1][0]
+                if crossed is not None and pts[-1][0] >= max(1.3 * crossed, crossed + 1.0):
+                    break
             if time.time() - t1 > 10:
                 log(f"rmse study: mode {mode} {t / 1000.0:.1f}/{budget_s} s")
                 t1 = time.time()
         ctx.close()
         return pts
 
-    plain = curve(capi.PLAIN, 1)
-    batched = curve(capi.BATCHED_GRAPH, 2)
+    plain = curve(capi.PLAIN, plain_chunk)
+    batched = curve(capi.BATCHED_GRAPH, batched_chunk, target=plain[-1][3])
     # the batched form's time to the plain form's indirect RMSE at the budget:
     # the first point from which its curve stays at or below it (an almost
     # empty early film can lie below it before the estimate has settled)
@@ -179,6 +194,8 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
                         "curve_s_rmse_indirect": [(round(p[0], 3), p[3]) for p in pts[:: max(1, len(pts) // 12)]]}
     out = {"workload": f"config{config}: {len(sc['tri_p'])} tris, {chains} chains, {W}x{H}",
            "budget_s": budget_s, "schedule": sched,
+           "criterion": "indirect (DR) RMSE: the direct pass is the same 16-spp pass in every form",
+           "batched_curve": "stops once it has stayed at or below the plain form's final RMSE for 30% (>= 1 s) past its crossing",
            "reference": {"steps": ref_steps, "direct_spp": 256, "seed": "independent", "b": b_ref,
                          "note": "test renders keep their own seed states and take the reference's normalisation b"},
            "plain": pick(plain), "batched": pick(batched),
@@ -202,6 +219,47 @@ def rmse_study(config, budget_s, ref_steps, sched, cpu_threads=0, seed=1234):
                       "rmse": rmse(ind + direct, rfull), "rmse_indirect": rmse(ind, rind), "relmse_indirect": relmse(ind, rind),
                       "mean_ratio_indirect": round(float(ind.mean() / max(rind.mean(), 1e-30)), 4)}
     return out
+
+
+def cpu_info():
+    """The host CPU's model name and count as the OS reports them."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def config2_depth9(steps, warmup, tune_steps):
+    """Config 2 (65,536 chains, 457,732 triangles) at maximum depth 9, the
+    path lengths 3..9 (D 6..18) its definition names: graph-replayed step time
+    under its own autotuned schedule, the plain form beside it."""
+    from drmlt import capi, scenes
+
+    sc = scenes.config2()
+    dpt = dict(sc["options"], maxdepth=9)
+    chains = int(dpt["numchains"])
+    ctx = capi.Context(sc, capi.options_from_dpt(dpt, num_chains=chains, seed=1234))
+    ctx.prepass()
+    ctx.direct_lighting()
+    ctx.run(2, capi.PLAIN)
+    plain_ms = ctx.timing()[0] / 2
+    sched, _ = ctx.autotune(steps=tune_steps)
+    ctx.run(warmup, capi.BATCHED_GRAPH)
+    s0 = ctx.stats()
+    ctx.run(steps, capi.BATCHED_GRAPH)
+    ms = ctx.timing()[0] / steps
+    acc = accepted(ctx.stats()) - accepted(s0)
+    ctx.close()
+    return {"workload": f"config2: {len(sc['tri_p'])} tris, {chains} chains, maxdepth 9, 512x512", "schedule": sched,
+            "ms_per_step": round(ms, 4), "mutations_per_s": round(chains / (ms / 1000.0), 1),
+            "accepted_mutations_per_s": round(acc / (steps * ms / 1000.0), 1),
+            "plain_ms_per_step": round(plain_ms, 3), "plain_mutations_per_s": round(chains / (plain_ms / 1000.0), 1)}
 
 
 def accepted(st):
@@ -231,17 +289,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=4)
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the CPU baseline (the CPU share of one GPU on the box)")
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--tune-steps", type=int, default=8, help="timed steps per candidate schedule of the autotuner")
     ap.add_argument("--no-deriv", action="store_true", help="skip the derivative-kernel workload (config 3)")
     ap.add_argument("--no-mcmc", action="store_true", help="skip the step times of the mcmc and h2mc integrators")
-    ap.add_argument("--rmse-config", type=int, default=2, help="scene configuration of the equal-time RMSE study (0: none)")
-    ap.add_argument("--rmse-budget", type=float, default=2.0, help="seconds each form renders in the RMSE study")
+    ap.add_argument("--no-config2", action="store_true", help="skip config 2 at its maximum depth 9")
+    ap.add_argument("--rmse-config", type=int, default=-1,
+                    help="scene configuration of the equal-RMSE study (-1: the bench configuration, 0: none)")
+    ap.add_argument("--rmse-budget", type=float, default=60.0, help="seconds the plain form renders in the RMSE study")
     ap.add_argument("--rmse-ref-steps", type=int, default=2000, help="batched steps of the RMSE study's reference render")
+    ap.add_argument("--rmse-cpu", action="store_true", help="also render with the CPU loop in the RMSE study")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch schedule (profiled runs: no candidate schedules among the launches)")
     ap.add_argument("--schedule", type=str, default="",
@@ -360,10 +421,15 @@ def main():
             others[name] = {"ms_per_step": round(mms, 4), "mutations_per_s": round(chains / (mms / 1000.0), 1),
                             "accepted_mutations_per_s": round(macc / (args.steps * mms / 1000.0), 1)}
             log(f"{name} integrator {mms:.3f} ms/step")
+    cfg2 = None
+    if not args.no_config2:
+        cfg2 = config2_depth9(args.steps, args.warmup, args.tune_steps)
+        log(f"config 2 at maxdepth 9: {cfg2}")
     study = None
-    if args.rmse_config:
-        study = rmse_study(args.rmse_config, args.rmse_budget, args.rmse_ref_steps, sched,
-                           cpu_threads=0 if args.no_cpu else min(args.cpu_threads, os.cpu_count() or 1))
+    rmse_cfg = args.config if args.rmse_config < 0 else args.rmse_config
+    if rmse_cfg:
+        study = rmse_study(rmse_cfg, args.rmse_budget, args.rmse_ref_steps, sched if rmse_cfg == args.config else None,
+                           cpu_threads=min(args.cpu_threads, os.cpu_count() or 1) if args.rmse_cpu else 0)
         log(f"rmse study: {study}")
     deriv = None
     if not args.no_deriv:
@@ -373,18 +439,24 @@ def main():
     cpu = None
     if not args.no_cpu:
         ncpu = min(args.cpu_threads, os.cpu_count() or 1)
-        cchains, citers = (16384, 512) if args.config != 1 else (1024, 2048)
-        log(f"cpu baseline: the oracle loop, {cchains} chains x {citers} iterations on {ncpu} threads")
+        cchains, citers = {1: (1024, 2048), 2: (16384, 512)}.get(args.config, (16384, 256))
+        model, host_cpus = cpu_info()
+        log(f"cpu baseline: the oracle loop, {cchains} chains x {citers} iterations on {ncpu} threads ({model}, {host_cpus} CPUs)")
         cval, csec, cacc = cpu_baseline(sc, ncpu, cchains, citers)
         cpu = {"value": round(cval, 1), "unit": "mutations/s", "cores": ncpu, "kind": "port",
                "accepted_mutations_per_s": round(cacc, 1),
                "sample": f"{cchains} chains x {citers} iterations of the same scene after a 20000-sample pre-pass "
-                         f"({csec:.1f} s of chain loop)"}
+                         f"({csec:.1f} s of chain loop)",
+               "host_cpu_model": model, "host_cpu_count": host_cpus,
+               "threads_note": "one std::thread per core over chains; the GPU box grants one GPU's share of 16 CPU threads"}
         log(f"cpu baseline {cval:.4g} mutations/s on {ncpu} threads")
 
     out = {
-        "metric": "DRMLT mutations/s (batched+adjoint GPU form; plain GPU form and CPU restatement alongside)",
+        "metric": "DRMLT mutations/s + sec to equal RMSE: batched+adjoint vs plain GPU vs CPU",
         "value": round(value, 1), "unit": "mutations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "seconds_to_equal_rmse": None if not study else {
+            "batched_s": study["batched_time_to_plain_rmse_s"], "plain_s": study["plain"]["time_s"],
+            "ratio_plain_over_batched": study["time_ratio_plain_over_batched_at_equal_rmse"], "target": "plain form's RMSE at its budget"},
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 path / f64 gaussian+ratio", "data": "synthetic (seeded procedural scene)",
         "config": {"workload": f"config{args.config}: {len(sc['tri_p'])} tris, {chains} chains, maxdepth {dpt['maxdepth']}, "
@@ -404,6 +476,7 @@ def main():
         "cpu_baseline": cpu,
         "derivatives": deriv,
         "equal_time_rmse": study,
+        "config2_maxdepth9": cfg2,
     }
     print(json.dumps(out), flush=True)
 

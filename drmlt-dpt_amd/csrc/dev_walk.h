@@ -21,81 +21,78 @@
 //    the next vertex needs: the segment (eps, (1 - eps) d) is occluded exactly
 //    when that closest hit lies before its end.
 #pragma once
-#include "dev_path.h"
+#include "drmlt_kernels.h"
 
 namespace drmlt {
 
 // Where a walk's primary samples come from: generated from uniforms
 // (GenerateSubpath), or the state's own u plus an offset -- N(0, sigma)
-// draws, an offset array, or the difference of two states (Green's y*,
-// mutation.cpp:210-237).
+// draws, the chain's offset array, or the difference of two states (Green's
+// y*, mutation.cpp:210-237).
 enum : int { WK_GEN = 0, WK_OFF_NORMAL = 1, WK_OFF_ARRAY = 2, WK_OFF_DIFF = 3 };
 // What a walk waits for: a light-subpath hit, a camera-subpath hit, or the
 // visibility of a finished contribution.
 enum : int { WP_LHIT = 1, WP_CHIT = 2, WP_VIS = 3 };
 
+// The per-lane state of a walk. Chain state is named by slot index (p, and
+// px / py for the y* offset) against the chain arrays, which every lane of a
+// kernel shares, so no per-lane pointers are held; the pending query is the
+// walk's own ray r (for a visibility test, the shadow segment), and a
+// contribution awaiting its visibility sits in the walk's output already.
 struct Walk {
-    SRef ref;                  // the state evaluated (and, for WK_GEN, written) in place
+    int p;                     // the state slot evaluated (and, for WK_GEN, written) in place
     int mode, c, l;
     Philox *g;                 // uniforms (WK_GEN); offset and discrete-dimension normals (perturbation)
-    float sigma, dsigma;       // WK_OFF_NORMAL sigma; discrete-dimension sigma
-    const float *oa;           // WK_OFF_ARRAY: offset k at oa[k * os]
-    int os;
-    const float *ux, *uy;      // WK_OFF_DIFF: offset k = ux[k * us] - uy[k * us]
-    int us;
+    float sigma;               // WK_OFF_NORMAL sigma
+    int offChain;              // WK_OFF_ARRAY: offset k of chain offChain in the chains' offset array
+    int px, py;                // WK_OFF_DIFF: offset k = u_k of slot px - u_k of slot py
     int k;                     // next primary-sample slot
     int phase, d;              // what the walk waits for; vertex index of the subpath being built
-    RayT<float> r;
+    RayT<float> r;             // the ray of the pending query
+    bool qany;                 // the pending query is a visibility test
     BState<float> ls, cs;
     int lastLightPrim;
     float sx, sy;
     bool lcp;                  // light-coordinate sampling of the last camera edge
     float lcpEnd;              // end of the light-coordinate segment awaiting its answer (< 0: none)
-    bool visScreen;            // the deferred contribution also sets the state's screen position
-    POut<float> pend;          // contribution awaiting its visibility
-    V3<float> qo, qd;          // the pending query
-    float qmin, qmax;
-    int qany;
+    bool visScreen;            // the contribution awaiting visibility also sets the state's screen position
 };
 
-DEV float walk_next_u(Walk &w) {  // PerturbU::next with the walk's offset
+DEV float walk_next_u(const Chains &ch, Walk &w, const SRef &ref) {  // PerturbU::next with the walk's offset
     float off;
     if (w.mode == WK_OFF_NORMAL) off = float(w.sigma * w.g->normal());
-    else if (w.mode == WK_OFF_ARRAY) off = w.oa[w.k * w.os];
-    else off = w.ux[w.k * w.us] - w.uy[w.k * w.us];
-    float *u = w.ref.u + w.k * w.ref.s;
+    else if (w.mode == WK_OFF_ARRAY) off = ch.off[w.k * ch.C + w.offChain];
+    else off = ch.u[w.k * ch.NS + w.px] - ch.u[w.k * ch.NS + w.py];
+    float *u = ref.u + w.k * ref.s;
     const float v = modulo1(*u + off);
     *u = v;
     w.k++;
     return v;
 }
-DEV float walk_disc(Walk &w) { return float(w.dsigma * w.g->normal()); }  // DiscNormal
-DEV void walk_put(Walk &w, float v) { w.ref.u[w.k * w.ref.s] = v; w.k++; }  // UWriter::put
+DEV float walk_disc(Walk &w, float dsigma) { return float(dsigma * w.g->normal()); }  // DiscNormal
+DEV void walk_put(Walk &w, const SRef &ref, float v) { ref.u[w.k * ref.s] = v; w.k++; }  // UWriter::put
 
-DEV void walk_closest(Walk &w) {
-    w.qo = w.r.org; w.qd = w.r.dir;
-    w.qmin = w.r.minT; w.qmax = w.r.maxT;
-    w.qany = 0;
-}
-// occluded() of dev_scene.h as a query
-DEV void walk_visibility(Walk &w, const ShadowQ &q, const POut<float> &pend, bool screen) {
-    w.qo = q.o; w.qd = q.d;
-    w.qmin = kIsectEps; w.qmax = (1.0f - kShadowEps) * q.dist;
-    w.qany = 1;
-    w.pend = pend;
+// The next query is the closest hit along w.r.
+DEV void walk_closest(Walk &w) { w.qany = false; }
+// occluded() of dev_scene.h as a query: the shadow segment replaces the ray
+// (the walk ends with the answer); the contribution is already in out.
+DEV void walk_visibility(Walk &w, const ShadowQ &q, bool screen) {
+    w.r.org = q.o; w.r.dir = q.d;
+    w.r.minT = kIsectEps; w.r.maxT = (1.0f - kShadowEps) * q.dist;
+    w.qany = true;
     w.visScreen = screen;
     w.phase = WP_VIS;
 }
 
-DEV bool walk_camera_start(const DevScene &S, Walk &w) {
+DEV bool walk_camera_start(const DevScene &S, const Chains &ch, Walk &w, const SRef &ref) {
     if (w.mode == WK_GEN) {
         const float sx = w.g->uni(), sy = w.g->uni();
-        walk_put(w, sx); walk_put(w, sy);
-        w.ref.screen[0] = sx; w.ref.screen[w.ref.s] = sy;
+        walk_put(w, ref, sx); walk_put(w, ref, sy);
+        ref.screen[0] = sx; ref.screen[ref.s] = sy;
         w.sx = sx; w.sy = sy;
     } else {
-        w.sx = walk_next_u(w);
-        w.sy = walk_next_u(w);
+        w.sx = walk_next_u(ch, w, ref);
+        w.sy = walk_next_u(ch, w, ref);
     }
     emit_from_camera(S, w.sx, w.sy, w.r, w.cs);
     w.lcp = false;
@@ -109,9 +106,9 @@ DEV bool walk_camera_start(const DevScene &S, Walk &w) {
 
 // Begins the walk; true when a query is pending, false when it has ended
 // (out holds its contribution).
-DEV bool walk_start(const DevScene &S, const Opts &o, Walk &w, POut<float> &out) {
+DEV bool walk_start(const DevScene &S, const Opts &o, const Chains &ch, Walk &w, POut<float> &out) {
     out.valid = false;
-    const SRef &ref = w.ref;
+    const SRef ref = slot_ref(ch, w.p);
     const int s = ref.s;
     w.k = 0;
     w.lastLightPrim = -1;
@@ -136,10 +133,10 @@ DEV bool walk_start(const DevScene &S, const Opts &o, Walk &w, POut<float> &out)
             li = pick_light(S, w.g->uni());
             lp = pick_light_tri(S, li, w.g->uni());
             *ref.light = li; *ref.lprim = lp;
-            walk_put(w, p0); walk_put(w, p1); walk_put(w, d0); walk_put(w, d1);
+            walk_put(w, ref, p0); walk_put(w, ref, p1); walk_put(w, ref, d0); walk_put(w, ref, d1);
         } else {
             li = *ref.light; lp = *ref.lprim;
-            p0 = walk_next_u(w); p1 = walk_next_u(w); d0 = walk_next_u(w); d1 = walk_next_u(w);
+            p0 = walk_next_u(ch, w, ref); p1 = walk_next_u(ch, w, ref); d0 = walk_next_u(ch, w, ref); d1 = walk_next_u(ch, w, ref);
         }
         emit_from_light(S, o, li, lp, p0, p1, d0, d1, w.r, w.ls);
         w.r.minT = kIsectEps;
@@ -149,20 +146,18 @@ DEV bool walk_start(const DevScene &S, const Opts &o, Walk &w, POut<float> &out)
         walk_closest(w);
         return true;
     }
-    return walk_camera_start(S, w);
+    return walk_camera_start(S, ch, w, ref);
 }
 
 // Resumes the walk with the answer h to its query; true when the next query
 // is pending, false when the walk has ended.
-DEV bool walk_resume(const DevScene &S, const Opts &o, Walk &w, const TraceHit &h, POut<float> &out) {
-    const SRef &ref = w.ref;
+DEV bool walk_resume(const DevScene &S, const Opts &o, const Chains &ch, float dsigma, Walk &w, const TraceHit &h, POut<float> &out) {
+    const SRef ref = slot_ref(ch, w.p);
     const int s = ref.s;
     const bool gen = w.mode == WK_GEN;
     if (w.phase == WP_VIS) {
-        if (h.prim < 0) {
-            out = w.pend;
-            if (w.visScreen) { ref.screen[0] = out.screen[0]; ref.screen[s] = out.screen[1]; }
-        }
+        if (h.prim >= 0) out.valid = false;
+        else if (w.visScreen) { ref.screen[0] = out.screen[0]; ref.screen[s] = out.screen[1]; }
         return false;
     }
     if (w.phase == WP_LHIT) {
@@ -177,7 +172,7 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, Walk &w, const TraceHit &
             w.ls.wi = -w.r.dir;
         } else {
             w.ls.wi = -w.r.dir;
-            dv = modulo1(ref.ldisc[d * s] + walk_disc(w));
+            dv = modulo1(ref.ldisc[d * s] + walk_disc(w, dsigma));
             ref.ldisc[d * s] = dv;
         }
         convert_mis(w.r, w.ls);
@@ -185,21 +180,19 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, Walk &w, const TraceHit &
         if (d == w.l - 2) {
             if (w.c == 1) {
                 ShadowQ q;
-                POut<float> pend;
-                pend.valid = false;
-                connect_to_camera<float, false>(S, o, d, w.ls, prim, dv, w.r.org, pend, &q);
-                if (!pend.valid) return false;
-                walk_visibility(w, q, pend, true);
+                connect_to_camera<float, false>(S, o, d, w.ls, prim, dv, w.r.org, out, &q);
+                if (!out.valid) return false;
+                walk_visibility(w, q, true);
                 return true;
             }
-            return walk_camera_start(S, w);
+            return walk_camera_start(S, ch, w, ref);
         }
         float r0, r1;
         if (gen) { r0 = w.g->uni(); r1 = w.g->uni(); }
-        else { r0 = walk_next_u(w); r1 = walk_next_u(w); }
+        else { r0 = walk_next_u(ch, w, ref); r1 = walk_next_u(ch, w, ref); }
         V3<float> dir, bc;
         if (!bsdf_sampling(S, o, true, !gen, w.ls, prim, dv, r0, r1, w.ls, dir, bc)) return false;
-        if (gen) { walk_put(w, r0); walk_put(w, r1); }
+        if (gen) { walk_put(w, ref, r0); walk_put(w, ref, r1); }
         else w.ls.thr = w.ls.thr * ref.lrr[d * s];
         w.r.org = w.ls.isect.pos;
         w.r.dir = dir;
@@ -243,26 +236,24 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, Walk &w, const TraceHit &
         ref.cdisc[d * s] = dv;
         if (d + 2 == w.c) {
             ShadowQ q;
-            POut<float> pend;
-            pend.valid = false;
             if (w.l == 1) {
                 const int li = pick_light(S, w.g->uni());
                 const float r0 = w.g->uni(), r1 = w.g->uni();
                 const int lp = pick_light_tri(S, li, w.g->uni());
                 *ref.light = li; *ref.lprim = lp;
-                walk_put(w, r0); walk_put(w, r1);
-                direct_lighting<float, false>(S, o, d, w.cs, w.sx, w.sy, li, lp, r0, r1, prim, pend, &q);
+                walk_put(w, ref, r0); walk_put(w, ref, r1);
+                direct_lighting<float, false>(S, o, d, w.cs, w.sx, w.sy, li, lp, r0, r1, prim, out, &q);
             } else {
-                connect_vertex<float, false>(S, d, w.l - 2, w.ls, w.lastLightPrim, w.cs, prim, w.sx, w.sy, pend, &q);
+                connect_vertex<float, false>(S, d, w.l - 2, w.ls, w.lastLightPrim, w.cs, prim, w.sx, w.sy, out, &q);
             }
-            if (!pend.valid) return false;
-            walk_visibility(w, q, pend, false);
+            if (!out.valid) return false;
+            walk_visibility(w, q, false);
             return true;
         }
         float r0 = w.g->uni(), r1 = w.g->uni();
         V3<float> dir, bc;
         if (!bsdf_sampling(S, o, false, false, w.cs, prim, dv, r0, r1, w.cs, dir, bc)) return false;
-        walk_put(w, r0); walk_put(w, r1);
+        walk_put(w, ref, r0); walk_put(w, ref, r1);
         w.r.org = w.cs.isect.pos;
         w.r.dir = dir;
         w.r.minT = kIsectEps;
@@ -280,24 +271,22 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, Walk &w, const TraceHit &
         return false;
     }
     if (!hit) return false;
-    const float dv = modulo1(ref.cdisc[d * s] + walk_disc(w));
+    const float dv = modulo1(ref.cdisc[d * s] + walk_disc(w, dsigma));
     ref.cdisc[d * s] = dv;
     if (d == 1 && length2(w.cs.isect.pos - w.r.org) <= 0.0f) return false;
     if (d == nc - 1) {
         ShadowQ q;
-        POut<float> pend;
-        pend.valid = false;
         if (w.l == 1) {
-            const float r0 = walk_next_u(w), r1 = walk_next_u(w);
-            direct_lighting<float, false>(S, o, d, w.cs, w.sx, w.sy, *ref.light, *ref.lprim, r0, r1, prim, pend, &q);
+            const float r0 = walk_next_u(ch, w, ref), r1 = walk_next_u(ch, w, ref);
+            direct_lighting<float, false>(S, o, d, w.cs, w.sx, w.sy, *ref.light, *ref.lprim, r0, r1, prim, out, &q);
         } else {
-            connect_vertex<float, false>(S, d, nl - 1, w.ls, w.lastLightPrim, w.cs, prim, w.sx, w.sy, pend, &q);
+            connect_vertex<float, false>(S, d, nl - 1, w.ls, w.lastLightPrim, w.cs, prim, w.sx, w.sy, out, &q);
         }
-        if (!pend.valid) return false;
-        walk_visibility(w, q, pend, false);
+        if (!out.valid) return false;
+        walk_visibility(w, q, false);
         return true;
     }
-    const float r0 = walk_next_u(w), r1 = walk_next_u(w);
+    const float r0 = walk_next_u(ch, w, ref), r1 = walk_next_u(ch, w, ref);
     if (o.lightCoord && d == nc - 2 && w.l == 0) {
         const int np = ref.cprim[(nc - 1) * s];
         if (np >= 0 && S.tlight[np] >= 0) w.lcp = true;
@@ -322,14 +311,17 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, Walk &w, const TraceHit &
 }
 
 // Runs the walks of the lanes with `active` set to their end, tracing every
-// round's queries at one call site.
-DEV void walk_run(const DevScene &S, const Opts &o, Walk &w, bool active, POut<float> &out) {
+// round's queries at one site (the traversal inlined there: the stage
+// kernels trace nowhere else). dsigma: the discrete-dimension sigma.
+DEV void walk_run(const DevScene &S, const Opts &o, const Chains &ch, float dsigma, Walk &w, bool active, POut<float> &out) {
     const BvhRef B = bvh_of(S);
-    bool q = active && walk_start(S, o, w, out);
+    bool q = active && walk_start(S, o, ch, w, out);
     while (__any(q)) {
         if (q) {
-            const TraceHit h = trace_call(B, w.qo.x, w.qo.y, w.qo.z, w.qd.x, w.qd.y, w.qd.z, w.qmin, w.qmax, w.qany);
-            q = walk_resume(S, o, w, h, out);
+            const float org[3] = {w.r.org.x, w.r.org.y, w.r.org.z}, dir[3] = {w.r.dir.x, w.r.dir.y, w.r.dir.z};
+            TraceHit h;
+            h.prim = trace_impl(B, org, dir, w.r.minT, w.r.maxT, w.qany, h.t);
+            q = walk_resume(S, o, ch, dsigma, w, h, out);
         }
     }
 }

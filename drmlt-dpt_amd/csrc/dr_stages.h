@@ -267,9 +267,8 @@ DEV int stage1_length(const Chains &ch, const Stage1 &a) {
 }
 // The walk of stage 1 on slot pf (the small step's state was copied there).
 DEV void stage1_walk(const Chains &ch, const DrParams &P, const Stage1 &a, Philox &g, Walk &w) {
-    w.ref = slot_ref(ch, a.pf);
+    w.p = a.pf;
     w.g = &g;
-    w.dsigma = P.discreteStdDev;
     if (!a.curValid || a.isLarge) {
         w.mode = WK_GEN;
         w.c = a.cam;
@@ -388,15 +387,13 @@ DEV Stage2 stage2_begin(const Chains &ch, int chain, Philox &g) {
     return a;
 }
 DEV void stage2_walk(const Chains &ch, const DrParams &P, int chain, const Stage2 &a, Philox &g, Walk &w) {
-    w.ref = slot_ref(ch, a.ps);
+    w.p = a.ps;
     w.g = &g;
-    w.dsigma = P.discreteStdDev;
     w.c = ch.c[a.pc];
     w.l = ch.l[a.pc];
     if (a.isAniso) {
         w.mode = WK_OFF_ARRAY;
-        w.oa = ch.off + chain;
-        w.os = ch.C;
+        w.offChain = chain;
     } else {
         w.mode = WK_OFF_NORMAL;
         w.sigma = P.anisoStdDev;
@@ -462,13 +459,11 @@ DEV Stage3 stage3_begin(const Chains &ch, const DrParams &P, int chain) {
     return a;
 }
 DEV void stage3_walk(const Chains &ch, const DrParams &P, const Stage3 &a, Philox &g, Walk &w) {
-    w.ref = slot_ref(ch, a.pstar);
+    w.p = a.pstar;
     w.g = &g;
-    w.dsigma = P.discreteStdDev;
     w.mode = WK_OFF_DIFF;
-    w.ux = ch.u + a.pc;
-    w.uy = ch.u + a.pf;
-    w.us = ch.NS;
+    w.px = a.pc;
+    w.py = a.pf;
     w.c = ch.c[a.ps];
     w.l = ch.l[a.ps];
 }

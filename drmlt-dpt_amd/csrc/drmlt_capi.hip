@@ -238,7 +238,7 @@ struct drmlt_ctx {
     hipEvent_t evFork = nullptr, evJoin[kMaxParts - 1] = {};
     Lists LX[kMaxParts - 1];
     // launch schedule of the batched step (drmlt_set_schedule / drmlt_autotune)
-    drmlt_schedule sched = {0, 0, 0, 0, 0, 0};
+    drmlt_schedule sched = {0, 0, 0, 0, 0, 0, 0};
 };
 
 static DrParams make_params(const drmlt_ctx *c) {
@@ -703,16 +703,28 @@ static StepGrid step_grid(const drmlt_ctx *c, int C) {
 // row.
 // The step's kernels for the chains of P on stream s with lists L (the
 // split step's halves).
+// The path-stage kernels built for the schedule's resident waves per SIMD.
+struct PathStageKernels {
+    decltype(&k_b_stage1) stage1;
+    decltype(&k_b_stage2_propose) propose;
+    decltype(&k_b_stage2_accept) accept;
+};
+static PathStageKernels path_stage_kernels(const drmlt_ctx *c) {
+    if (c->sched.stage_waves == 3) return {k_b_stage1_w3, k_b_stage2_propose_w3, k_b_stage2_accept_w3};
+    return {k_b_stage1, k_b_stage2_propose, k_b_stage2_accept};
+}
+
 static void launch_step_kernels(drmlt_ctx *c, const DrParams &P, hipStream_t s, const Lists &L, const StepGrid &G) {
+    const PathStageKernels K = path_stage_kernels(c);
     if (P.binStages == 1) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, L);
-    hipLaunchKernelGGL(k_b_stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
+    hipLaunchKernelGGL(K.stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
     if (P.integrator == 1) return;
     hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, L, 4, 0);
     hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, L, 0, 0, 0, c->iterDev);
-    hipLaunchKernelGGL(k_b_stage2_propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, L);
+    hipLaunchKernelGGL(K.propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, L);
     hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, L, 2, 1);
     hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, L, 2, 1, 1, c->iterDev);
-    hipLaunchKernelGGL(k_b_stage2_accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
+    hipLaunchKernelGGL(K.accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
 }
 
 static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, unsigned long long *counters = nullptr) {
@@ -761,21 +773,22 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
         return;
     }
+    const PathStageKernels K = path_stage_kernels(c);
     if (timed) (void)hipEventRecord(c->ev[0], s);
     if (P.binStages == 1) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
-    hipLaunchKernelGGL(k_b_stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    hipLaunchKernelGGL(K.stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[1], s);
     if (P.integrator != 1) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 4, 0);
     if (timed) (void)hipEventRecord(c->ev[2], s);
     if (P.integrator != 1) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 0, 0, 0, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[3], s);
-    if (P.integrator != 1) hipLaunchKernelGGL(k_b_stage2_propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
+    if (P.integrator != 1) hipLaunchKernelGGL(K.propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->L);
     if (timed) (void)hipEventRecord(c->ev[4], s);
     if (P.integrator != 1) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1);
     if (timed) (void)hipEventRecord(c->ev[5], s);
     if (P.integrator != 1) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[6], s);
-    if (P.integrator != 1) hipLaunchKernelGGL(k_b_stage2_accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    if (P.integrator != 1) hipLaunchKernelGGL(K.accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[7], s);
     hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
     if (timed) (void)hipEventRecord(c->ev[8], s);
@@ -852,7 +865,8 @@ extern "C" int drmlt_get_schedule(drmlt_ctx *c, drmlt_schedule *out) {
 extern "C" int drmlt_set_schedule(drmlt_ctx *c, const drmlt_schedule *sc) {
     if (!c || !sc) return fail(DRMLT_ERR_ARG, "bad argument");
     if (sc->bin_stages < 0 || sc->bin_stages > 2 || sc->stage_blocks < 0 || sc->deriv_blocks < 0 || sc->gauss_blocks < 0 ||
-        sc->stage_lanes < 0 || sc->stage_lanes > 64 || sc->stage_split < 0 || sc->stage_split > kMaxParts)
+        sc->stage_lanes < 0 || sc->stage_lanes > 64 || sc->stage_split < 0 || sc->stage_split > kMaxParts ||
+        !(sc->stage_waves == 0 || sc->stage_waves == 2 || sc->stage_waves == 3))
         return fail(DRMLT_ERR_ARG, "schedule out of range");
     c->sched = *sc;
     drop_graph(c);  // the captured step bakes in the launch geometry
@@ -902,6 +916,11 @@ extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best,
                 if (ms < curMs) { cur = cand; curMs = ms; }
             }
         }
+    // the path-stage kernels' register bound: three waves per SIMD against two
+    cand = cur;
+    cand.stage_waves = cur.stage_waves == 3 ? 2 : 3;
+    if ((r = measure(cand, 1, 7, ms))) return r;
+    if (ms < curMs) { cur = cand; curMs = ms; }
     const int kernelSlots[2][2] = {{2, 5}, {3, 6}};  // derivative and Gaussian kernels (current + proposal)
     for (int which = 0; which < 2; which++) {
         double kMs = 0;

@@ -7,11 +7,10 @@
 
 namespace drmlt {
 
-// Minimum waves per SIMD the path-stage kernels are compiled for (bounds
-// their registers); more resident waves hide traversal latency.
-#ifndef DRMLT_STAGE_WAVES
-#define DRMLT_STAGE_WAVES 2
-#endif
+// The path-stage kernels are compiled twice, bounded to two and to three
+// resident waves per SIMD (256 and 168 VGPRs): more waves hide more
+// traversal latency, at the price of spills; which pays depends on how many
+// waves of chains there are (the schedule's stage_waves picks).
 // The same for the Gaussian and derivative kernels.
 #ifndef DRMLT_GAUSS_WAVES
 #define DRMLT_GAUSS_WAVES 1
@@ -90,8 +89,8 @@ DEV int stage_first(const DrParams &P, int n) {
 }
 
 // Stage 1 over every chain: from list 3 (binned) or in chain order.
-__global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
-                                                 unsigned long long *stats, Lists L) {
+DEV void stage1_kernel(const DevScene &Sc, const Chains &ch, const DrParams &P, const uint32_t *iterp, float *film, unsigned long long *stats,
+                       const Lists &L) {
     const uint32_t iter = *iterp;
     const int n = P.binStages == 1 ? list_length(L, 3) : P.chainCount;
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
@@ -102,7 +101,7 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage1)
         stage1_walk(ch, P, a, g, w);
         POut<float> out;
         out.valid = false;
-        walk_run(Sc, P.o, w, !a.h2mcMove, out);
+        walk_run(Sc, P.o, ch, P.discreteStdDev, w, !a.h2mcMove, out);
         const int flags = stage1_end(Sc, ch, P, chain, a, out, g, film, stats);
         // second stage: list 1 (every chain with one), list 0 (H2MC), list 4
         // (H2MC needing the current state's derivatives)
@@ -179,7 +178,7 @@ __global__ void __launch_bounds__(64, DRMLT_GAUSS_WAVES) DRMLT_STAGE(k_b_gauss)(
 }
 // Second-stage proposals: list 1 (binned), or every chain in chain order with
 // the lanes of chains without a second stage idle in the walk rounds.
-__global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_propose)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, Lists L) {
+DEV void stage2_propose_kernel(const DevScene &Sc, const Chains &ch, const DrParams &P, const uint32_t *iterp, const Lists &L) {
     const uint32_t iter = *iterp;
     const int n = P.binStages ? list_length(L, 1) : P.chainCount;
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
@@ -194,7 +193,7 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
         }
         POut<float> out;
         out.valid = false;
-        walk_run(Sc, P.o, w, act, out);
+        walk_run(Sc, P.o, ch, P.discreteStdDev, w, act, out);
         int bin = 0, flags = 0;
         if (act) {
             stage2_end(Sc, ch, chain, a, out);
@@ -207,8 +206,8 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
 
 // Second-stage acceptance over the same chains; Green's y* walks where they
 // are needed.
-__global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_accept)(DevScene Sc, Chains ch, DrParams P, uint32_t *iterp, float *film,
-                                                        unsigned long long *stats, Lists L) {
+DEV void stage2_accept_kernel(const DevScene &Sc, const Chains &ch, const DrParams &P, const uint32_t *iterp, float *film,
+                              unsigned long long *stats, const Lists &L) {
     const uint32_t iter = *iterp;
     const int n = P.binStages ? list_length(L, 1) : P.chainCount;
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
@@ -224,10 +223,31 @@ __global__ void __launch_bounds__(64, DRMLT_STAGE_WAVES) DRMLT_STAGE(k_b_stage2_
         }
         POut<float> out;
         out.valid = false;
-        walk_run(Sc, P.o, w, a.traceStar, out);
+        walk_run(Sc, P.o, ch, P.discreteStdDev, w, a.traceStar, out);
         if (act) stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
     }
 }
+
+// The path-stage kernels at 2 (the names without suffix) and 3 resident
+// waves per SIMD (suffix _w3); the counting build has the first only.
+#define DRMLT_PATH_STAGE_KERNELS(WAVES, SFX)                                                                                          \
+    __global__ void __launch_bounds__(64, WAVES) DRMLT_STAGE(k_b_stage1##SFX)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, \
+                                                                              float *film, unsigned long long *stats, Lists L) {      \
+        stage1_kernel(Sc, ch, P, iterp, film, stats, L);                                                                             \
+    }                                                                                                                                 \
+    __global__ void __launch_bounds__(64, WAVES) DRMLT_STAGE(k_b_stage2_propose##SFX)(DevScene Sc, Chains ch, DrParams P,               \
+                                                                                      const uint32_t *iterp, Lists L) {              \
+        stage2_propose_kernel(Sc, ch, P, iterp, L);                                                                                  \
+    }                                                                                                                                 \
+    __global__ void __launch_bounds__(64, WAVES) DRMLT_STAGE(k_b_stage2_accept##SFX)(DevScene Sc, Chains ch, DrParams P, uint32_t *iterp, \
+                                                                                     float *film, unsigned long long *stats, Lists L) { \
+        stage2_accept_kernel(Sc, ch, P, iterp, film, stats, L);                                                                      \
+    }
+DRMLT_PATH_STAGE_KERNELS(2, )
+#ifndef DRMLT_COUNT_WORK
+DRMLT_PATH_STAGE_KERNELS(3, _w3)
+#endif
+#undef DRMLT_PATH_STAGE_KERNELS
 
 // Derivative passes over list `kind` (4 with which 0: the current states that
 // need them; 2 with which 1: the proposals), persistent over (chain, pass)

@@ -11,6 +11,10 @@ __global__ void k_b_deriv(DevScene, Chains, DrParams, Lists, int, int);
 __global__ void k_b_gauss(DevScene, Chains, DrParams, Lists, int, int, int, const uint32_t *);
 __global__ void k_b_stage2_propose(DevScene, Chains, DrParams, const uint32_t *, Lists);
 __global__ void k_b_stage2_accept(DevScene, Chains, DrParams, uint32_t *, float *, unsigned long long *, Lists);
+// the same path-stage kernels bounded to three waves per SIMD
+__global__ void k_b_stage1_w3(DevScene, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
+__global__ void k_b_stage2_propose_w3(DevScene, Chains, DrParams, const uint32_t *, Lists);
+__global__ void k_b_stage2_accept_w3(DevScene, Chains, DrParams, uint32_t *, float *, unsigned long long *, Lists);
 __global__ void k_b_reset(Lists, uint32_t *, int);
 // counting forms of the stage kernels (k_count.hip)
 __global__ void k_b_stage1_bin_counted(Chains, DrParams, const uint32_t *, Lists);

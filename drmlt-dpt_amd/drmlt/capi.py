@@ -57,7 +57,7 @@ class Options(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("bin_stages", C.c_int32), ("stage_blocks", C.c_int32), ("deriv_blocks", C.c_int32),
                 ("gauss_blocks", C.c_int32), ("stage_lanes", C.c_int32),
-                ("stage_split", C.c_int32)]
+                ("stage_split", C.c_int32), ("stage_waves", C.c_int32)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -145,6 +145,9 @@ typedef struct {
                               chains on as many streams that fork and join around it (one range's
                               kernel tails overlap the others' work); 0 or 1 (default 0): one
                               stream. The other forms ignore it */
+    int32_t stage_waves;   /* resident waves per SIMD the path-stage kernels are built for: 3 (168
+                              VGPRs, some spills, more traversal latency hidden) or 2 (0, default:
+                              256 VGPRs); 3 pays when the chains fill more than two waves per SIMD */
 } drmlt_schedule;
 
 /* Execution forms of drmlt_run */

@@ -77,6 +77,8 @@ def test_argument_errors(deep):
     with pytest.raises(capi.DrmltError):
         ctx.set_schedule(bin_stages=3)
     with pytest.raises(capi.DrmltError):
+        ctx.set_schedule(stage_waves=4)
+    with pytest.raises(capi.DrmltError):
         ctx.read_chains(10 ** 7)
     with pytest.raises(capi.DrmltError):
         out = np.zeros((8, 8, 3), np.float32)

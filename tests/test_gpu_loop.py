@@ -107,7 +107,8 @@ def test_batched_paths_equal_plain_paths(which, request):
 def test_schedules_agree_and_autotune(cfg2_small):
     """Binned (all path-stage kernels, or only the second-stage ones) and
     chain-order scheduling of the path-stage kernels, 64, 32 or
-    16 chains per workgroup, other workgroup counts, and the graph-replayed
+    16 chains per workgroup, other workgroup counts, the path-stage kernels
+    built for three resident waves per SIMD, and the graph-replayed
     step split into 2-4 ranges of the chains on as many streams (also for a
     chain count that none of them divides) advance the chains identically (iterations from the same
     seeds give the same counts, films within float summation order); the
@@ -122,7 +123,8 @@ def test_schedules_agree_and_autotune(cfg2_small):
                             ({"stage_lanes": 32}, capi.BATCHED), ({"bin_stages": 1, "stage_lanes": 16, "stage_blocks": 7}, capi.BATCHED),
                             ({"bin_stages": 2}, capi.BATCHED), ({"bin_stages": 2, "stage_split": 2}, capi.BATCHED_GRAPH),
                             ({"stage_split": 2}, capi.BATCHED_GRAPH), ({"stage_split": 3, "bin_stages": 1, "stage_lanes": 32}, capi.BATCHED_GRAPH),
-                            ({"stage_split": 4}, capi.BATCHED_GRAPH)):
+                            ({"stage_split": 4}, capi.BATCHED_GRAPH), ({"stage_waves": 3}, capi.BATCHED),
+                            ({"stage_waves": 3, "bin_stages": 1, "stage_split": 2}, capi.BATCHED_GRAPH)):
             if chains != 4096 and not sched.get("stage_split") and sched:
                 continue
             ctx = _ctx(cfg2_small, num_init_samples=8192, num_chains=chains, seed=5)
