@@ -5,9 +5,10 @@
 // direct-lighting path (GeneratePath, path.cpp:343-431), with the MIS
 // bookkeeping of path.cpp:433-976.
 //
-// A path state lives in memory behind an SRef (pointers + element stride), so
-// the same code reads structure-of-arrays chain state in HBM (stride = number
-// of chains) and packed records (stride = 1). The evaluator is templated on
+// A path state lives in memory behind an SRef (pointers + strides), so the
+// same code reads the chain-state slot records in HBM (vertex fields
+// interleaved, vertex stride 3) and the packed records of the C ABI (one
+// array per field, stride 1). The evaluator is templated on
 // the scalar type: float for the primal, jets for derivatives, in which case
 // the triangles of the state are re-intersected instead of traced and no
 // occlusion tests are made (the function the reference's AD twin
@@ -27,7 +28,8 @@ struct SRef {
     float *u;
     int *cprim, *lgprim;
     float *cdisc, *ldisc, *crr, *lrr;
-    int s;  // element stride
+    int s;   // stride of u and screen
+    int vs;  // stride of the vertex fields (cprim .. lrr)
 };
 
 struct Opts {
@@ -394,7 +396,7 @@ struct DiscNone {
 template <class T, bool FIXED, class U, class DISC>
 DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &uprov, const DISC &disc, POut<T> &out) {
     out.valid = false;
-    const int c = *ref.c, l = *ref.l, s = ref.s;
+    const int c = *ref.c, l = *ref.l, s = ref.s, vs = ref.vs;
     const int nl = l - 1, nc = c - 1;
     BState<T> ls;
     int lastLightPrim = -1;
@@ -408,16 +410,16 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
         for (int d = 0; d < nl; d++) {
             int prim;
             if (FIXED) {
-                prim = ref.lgprim[d * s];
+                prim = ref.lgprim[d * vs];
                 if (!re_intersect(S, prim, r, ls.isect)) return;
             } else {
                 if (!intersect(S, r, ls.isect)) return;
                 prim = ls.isect.prim;
-                ref.lgprim[d * s] = prim;
+                ref.lgprim[d * vs] = prim;
             }
             ls.wi = -r.dir;
-            float dv = ref.ldisc[d * s];
-            if (!FIXED) { dv = modulo1(dv + disc(kVMax + d)); ref.ldisc[d * s] = dv; }
+            float dv = ref.ldisc[d * vs];
+            if (!FIXED) { dv = modulo1(dv + disc(kVMax + d)); ref.ldisc[d * vs] = dv; }
             convert_mis(r, ls);
             lastLightPrim = prim;
             if (d == nl - 1 && c == 1) {
@@ -429,7 +431,7 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
             T r0 = uprov.next(), r1 = uprov.next();
             V3<T> dir, bc;
             if (!bsdf_sampling(S, o, true, true, ls, prim, dv, r0, r1, ls, dir, bc)) return;
-            ls.thr = ls.thr * T(ref.lrr[d * s]);
+            ls.thr = ls.thr * T(ref.lrr[d * vs]);
             r.org = ls.isect.pos;
             r.dir = dir;
         }
@@ -444,11 +446,11 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
         int prim = -1;
         bool hit;
         if (FIXED) {
-            prim = ref.cprim[d * s];
+            prim = ref.cprim[d * vs];
             hit = re_intersect(S, prim, r, cs.isect);
         } else {
             hit = intersect(S, r, cs.isect);
-            if (hit) { prim = cs.isect.prim; ref.cprim[d * s] = prim; }
+            if (hit) { prim = cs.isect.prim; ref.cprim[d * vs] = prim; }
         }
         cs.wi = -r.dir;
         if (hit) convert_mis(r, cs);
@@ -460,8 +462,8 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
             return;
         }
         if (!hit) return;
-        float dv = ref.cdisc[d * s];
-        if (!FIXED) { dv = modulo1(dv + disc(d)); ref.cdisc[d * s] = dv; }
+        float dv = ref.cdisc[d * vs];
+        if (!FIXED) { dv = modulo1(dv + disc(d)); ref.cdisc[d * vs] = dv; }
         if (d == 1 && val(length2(cs.isect.pos - r.org)) <= 0) { out.valid = false; return; }
         if (d == nc - 1) {
             if (l == 1) {
@@ -474,16 +476,16 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
         }
         T r0 = uprov.next(), r1 = uprov.next();
         if (o.lightCoord && d == nc - 2 && l == 0) {
-            const int np = ref.cprim[(nc - 1) * s];
+            const int np = ref.cprim[(nc - 1) * vs];
             if (np >= 0 && S.tlight[np] >= 0) lcp = true;
         }
         V3<T> dir, bc;
         if (lcp) {
-            if (!light_coordinate_sampling<T, FIXED>(S, prim, r0, r1, ref.cprim[(nc - 1) * s], cs, dir)) return;
+            if (!light_coordinate_sampling<T, FIXED>(S, prim, r0, r1, ref.cprim[(nc - 1) * vs], cs, dir)) return;
         } else {
             if (!bsdf_sampling(S, o, false, true, cs, prim, dv, r0, r1, cs, dir, bc)) return;
         }
-        cs.thr = cs.thr * T(ref.crr[d * s]);
+        cs.thr = cs.thr * T(ref.crr[d * vs]);
         r.org = cs.isect.pos;
         r.dir = dir;
         r.minT = kIsectEps;
@@ -498,8 +500,8 @@ DEV void copy_state(const SRef &a, const SRef &b) {
     const int c = *a.c, l = *a.l;
     const int P = c + l - 1, D = (P > 2 ? P : 2) * 2;
     for (int k = 0; k < D; k++) b.u[k * b.s] = a.u[k * a.s];
-    for (int k = 0; k < c - 1; k++) { b.cprim[k * b.s] = a.cprim[k * a.s]; b.cdisc[k * b.s] = a.cdisc[k * a.s]; b.crr[k * b.s] = a.crr[k * a.s]; }
-    for (int k = 0; k < l - 1; k++) { b.lgprim[k * b.s] = a.lgprim[k * a.s]; b.ldisc[k * b.s] = a.ldisc[k * a.s]; b.lrr[k * b.s] = a.lrr[k * a.s]; }
+    for (int k = 0; k < c - 1; k++) { b.cprim[k * b.vs] = a.cprim[k * a.vs]; b.cdisc[k * b.vs] = a.cdisc[k * a.vs]; b.crr[k * b.vs] = a.crr[k * a.vs]; }
+    for (int k = 0; k < l - 1; k++) { b.lgprim[k * b.vs] = a.lgprim[k * a.vs]; b.ldisc[k * b.vs] = a.ldisc[k * a.vs]; b.lrr[k * b.vs] = a.lrr[k * a.vs]; }
 }
 DEV int dim_of(int c, int l) { const int P = c + l - 1; return (P > 2 ? P : 2) * 2; }  // path.h:132-139, path.cpp:2286-2289
 
@@ -516,12 +518,12 @@ struct UWriter {
 // per camera vertex disc, then [light pick, rnd(2), triangle pick] or rnd(2).
 template <class G> DEV void generate_subpath(const DevScene &S, const Opts &o, int camLen, int lgtLen, const SRef &ref, G &g, POut<float> &out) {
     out.valid = false;
-    const int s = ref.s;
+    const int s = ref.s, vs = ref.vs;
     *ref.c = camLen; *ref.l = lgtLen;
     *ref.light = -1; *ref.lprim = -1;
     for (int k = 0; k < kVMax; k++) {
-        ref.cprim[k * s] = -1; ref.lgprim[k * s] = -1; ref.crr[k * s] = 1.0f; ref.lrr[k * s] = 1.0f;
-        ref.cdisc[k * s] = 0.0f; ref.ldisc[k * s] = 0.0f;
+        ref.cprim[k * vs] = -1; ref.lgprim[k * vs] = -1; ref.crr[k * vs] = 1.0f; ref.lrr[k * vs] = 1.0f;
+        ref.cdisc[k * vs] = 0.0f; ref.ldisc[k * vs] = 0.0f;
     }
     for (int k = 0; k < kDMax; k++) ref.u[k * s] = 0.0f;
     // the camera subpath needs two vertices unless a light subpath of two or
@@ -544,9 +546,9 @@ template <class G> DEV void generate_subpath(const DevScene &S, const Opts &o, i
         for (int d = 0;; d++) {
             if (!intersect(S, r, ls.isect)) return;
             const int prim = ls.isect.prim;
-            ref.lgprim[d * s] = prim;
+            ref.lgprim[d * vs] = prim;
             const float dv = g.uni();
-            ref.ldisc[d * s] = dv;
+            ref.ldisc[d * vs] = dv;
             ls.wi = -r.dir;
             convert_mis(r, ls);
             lastLightPrim = prim;
@@ -574,7 +576,7 @@ template <class G> DEV void generate_subpath(const DevScene &S, const Opts &o, i
     for (int d = 0;; d++) {
         const bool hit = intersect(S, r, cs.isect);
         const int prim = hit ? cs.isect.prim : -1;
-        if (hit) ref.cprim[d * s] = prim;
+        if (hit) ref.cprim[d * vs] = prim;
         cs.wi = -r.dir;
         if (hit) convert_mis(r, cs);
         if (d + 2 >= camLen && lgtLen == 0) {
@@ -597,7 +599,7 @@ template <class G> DEV void generate_subpath(const DevScene &S, const Opts &o, i
         if (!hit) return;
         if (d == 1 && length2(cs.isect.pos - r.org) <= 0.0f) return;
         const float dv = g.uni();
-        ref.cdisc[d * s] = dv;
+        ref.cdisc[d * vs] = dv;
         if (d + 2 == camLen) {
             if (lgtLen == 1) {
                 const int li = pick_light(S, g.uni());
@@ -764,17 +766,17 @@ DEV void generate_path_bidir(const DevScene &S, const Opts &o, int minDepth, int
     }
     if (ref) *ref->c = 0;  // marks "strategy not found" unless written below
     if (ref && keepDone) {
-        const int s = ref->s;
+        const int s = ref->s, vs = ref->vs;
         const int c = keepC, l = keepL;
         *ref->c = c; *ref->l = l;
         *ref->light = -1; *ref->lprim = -1;
         for (int k = 0; k < kVMax; k++) {
-            ref->cprim[k * s] = k < c - 1 ? cprims[k] : -1;
-            ref->cdisc[k * s] = k < c - 1 ? cdiscs[k] : 0.0f;
-            ref->crr[k * s] = k < c - 1 ? crrs[k] : 1.0f;
-            ref->lgprim[k * s] = k < l - 1 ? lprims[k] : -1;
-            ref->ldisc[k * s] = k < l - 1 ? ldiscs[k] : 0.0f;
-            ref->lrr[k * s] = k < l - 1 ? lrrs[k] : 1.0f;
+            ref->cprim[k * vs] = k < c - 1 ? cprims[k] : -1;
+            ref->cdisc[k * vs] = k < c - 1 ? cdiscs[k] : 0.0f;
+            ref->crr[k * vs] = k < c - 1 ? crrs[k] : 1.0f;
+            ref->lgprim[k * vs] = k < l - 1 ? lprims[k] : -1;
+            ref->ldisc[k * vs] = k < l - 1 ? ldiscs[k] : 0.0f;
+            ref->lrr[k * vs] = k < l - 1 ? lrrs[k] : 1.0f;
         }
         for (int k = 0; k < kDMax; k++) ref->u[k * s] = 0.0f;
         UWriter uw{ref->u, s, 0};

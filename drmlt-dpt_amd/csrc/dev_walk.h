@@ -61,8 +61,8 @@ struct Walk {
 DEV float walk_next_u(const Chains &ch, Walk &w, const SRef &ref) {  // PerturbU::next with the walk's offset
     float off;
     if (w.mode == WK_OFF_NORMAL) off = float(w.sigma * w.g->normal());
-    else if (w.mode == WK_OFF_ARRAY) off = ch.off[w.k * ch.C + w.offChain];
-    else off = ch.u[w.k * ch.NS + w.px] - ch.u[w.k * ch.NS + w.py];
+    else if (w.mode == WK_OFF_ARRAY) off = chain_off(ch, w.offChain)[w.k];
+    else off = slot_u(ch, w.px)[w.k] - slot_u(ch, w.py)[w.k];
     float *u = ref.u + w.k * ref.s;
     const float v = modulo1(*u + off);
     *u = v;
@@ -109,7 +109,7 @@ DEV bool walk_camera_start(const DevScene &S, const Chains &ch, Walk &w, const S
 DEV bool walk_start(const DevScene &S, const Opts &o, const Chains &ch, Walk &w, POut<float> &out) {
     out.valid = false;
     const SRef ref = slot_ref(ch, w.p);
-    const int s = ref.s;
+    const int s = ref.s, vs = ref.vs;
     w.k = 0;
     w.lastLightPrim = -1;
     w.lcpEnd = -1.0f;
@@ -117,8 +117,8 @@ DEV bool walk_start(const DevScene &S, const Opts &o, const Chains &ch, Walk &w,
         *ref.c = w.c; *ref.l = w.l;
         *ref.light = -1; *ref.lprim = -1;
         for (int k = 0; k < kVMax; k++) {
-            ref.cprim[k * s] = -1; ref.lgprim[k * s] = -1; ref.crr[k * s] = 1.0f; ref.lrr[k * s] = 1.0f;
-            ref.cdisc[k * s] = 0.0f; ref.ldisc[k * s] = 0.0f;
+            ref.cprim[k * vs] = -1; ref.lgprim[k * vs] = -1; ref.crr[k * vs] = 1.0f; ref.lrr[k * vs] = 1.0f;
+            ref.cdisc[k * vs] = 0.0f; ref.ldisc[k * vs] = 0.0f;
         }
         for (int k = 0; k < kDMax; k++) ref.u[k * s] = 0.0f;
         // as generate_subpath: no path without two camera vertices or a light
@@ -153,7 +153,7 @@ DEV bool walk_start(const DevScene &S, const Opts &o, const Chains &ch, Walk &w,
 // is pending, false when the walk has ended.
 DEV bool walk_resume(const DevScene &S, const Opts &o, const Chains &ch, float dsigma, Walk &w, const TraceHit &h, POut<float> &out) {
     const SRef ref = slot_ref(ch, w.p);
-    const int s = ref.s;
+    const int s = ref.s, vs = ref.vs;
     const bool gen = w.mode == WK_GEN;
     if (w.phase == WP_VIS) {
         if (h.prim >= 0) out.valid = false;
@@ -164,16 +164,16 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, const Chains &ch, float d
         if (h.prim < 0 || !re_intersect(S, h.prim, w.r, w.ls.isect)) return false;
         const int prim = w.ls.isect.prim;
         const int d = w.d;
-        ref.lgprim[d * s] = prim;
+        ref.lgprim[d * vs] = prim;
         float dv;
         if (gen) {
             dv = w.g->uni();
-            ref.ldisc[d * s] = dv;
+            ref.ldisc[d * vs] = dv;
             w.ls.wi = -w.r.dir;
         } else {
             w.ls.wi = -w.r.dir;
-            dv = modulo1(ref.ldisc[d * s] + walk_disc(w, dsigma));
-            ref.ldisc[d * s] = dv;
+            dv = modulo1(ref.ldisc[d * vs] + walk_disc(w, dsigma));
+            ref.ldisc[d * vs] = dv;
         }
         convert_mis(w.r, w.ls);
         w.lastLightPrim = prim;
@@ -193,7 +193,7 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, const Chains &ch, float d
         V3<float> dir, bc;
         if (!bsdf_sampling(S, o, true, !gen, w.ls, prim, dv, r0, r1, w.ls, dir, bc)) return false;
         if (gen) { walk_put(w, ref, r0); walk_put(w, ref, r1); }
-        else w.ls.thr = w.ls.thr * ref.lrr[d * s];
+        else w.ls.thr = w.ls.thr * ref.lrr[d * vs];
         w.r.org = w.ls.isect.pos;
         w.r.dir = dir;
         w.d = d + 1;
@@ -209,7 +209,7 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, const Chains &ch, float d
     }
     const bool hit = h.prim >= 0 && re_intersect(S, h.prim, w.r, w.cs.isect);
     const int prim = hit ? w.cs.isect.prim : -1;
-    if (hit) ref.cprim[d * s] = prim;
+    if (hit) ref.cprim[d * vs] = prim;
     w.cs.wi = -w.r.dir;
     if (hit) convert_mis(w.r, w.cs);
     if (gen) {
@@ -233,7 +233,7 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, const Chains &ch, float d
         if (!hit) return false;
         if (d == 1 && length2(w.cs.isect.pos - w.r.org) <= 0.0f) return false;
         const float dv = w.g->uni();
-        ref.cdisc[d * s] = dv;
+        ref.cdisc[d * vs] = dv;
         if (d + 2 == w.c) {
             ShadowQ q;
             if (w.l == 1) {
@@ -271,8 +271,8 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, const Chains &ch, float d
         return false;
     }
     if (!hit) return false;
-    const float dv = modulo1(ref.cdisc[d * s] + walk_disc(w, dsigma));
-    ref.cdisc[d * s] = dv;
+    const float dv = modulo1(ref.cdisc[d * vs] + walk_disc(w, dsigma));
+    ref.cdisc[d * vs] = dv;
     if (d == 1 && length2(w.cs.isect.pos - w.r.org) <= 0.0f) return false;
     if (d == nc - 1) {
         ShadowQ q;
@@ -288,19 +288,19 @@ DEV bool walk_resume(const DevScene &S, const Opts &o, const Chains &ch, float d
     }
     const float r0 = walk_next_u(ch, w, ref), r1 = walk_next_u(ch, w, ref);
     if (o.lightCoord && d == nc - 2 && w.l == 0) {
-        const int np = ref.cprim[(nc - 1) * s];
+        const int np = ref.cprim[(nc - 1) * vs];
         if (np >= 0 && S.tlight[np] >= 0) w.lcp = true;
     }
     V3<float> dir, bc;
     if (w.lcp) {
         ShadowQ q;
-        if (!light_coordinate_sampling<float, false>(S, prim, r0, r1, ref.cprim[(nc - 1) * s], w.cs, dir, &q)) return false;
+        if (!light_coordinate_sampling<float, false>(S, prim, r0, r1, ref.cprim[(nc - 1) * vs], w.cs, dir, &q)) return false;
         w.lcpEnd = (1.0f - kShadowEps) * q.dist;
     } else {
         float a0 = r0, a1 = r1;
         if (!bsdf_sampling(S, o, false, true, w.cs, prim, dv, a0, a1, w.cs, dir, bc)) return false;
     }
-    w.cs.thr = w.cs.thr * ref.crr[d * s];
+    w.cs.thr = w.cs.thr * ref.crr[d * vs];
     w.r.org = w.cs.isect.pos;
     w.r.dir = dir;
     w.r.minT = kIsectEps;

@@ -39,22 +39,24 @@ DEV bool flip_coin(double x, double d) { return (x == 1.0) || (d < x); }        
 DEV float clamp_ref(float v) { float m = (v < 0.0f) ? 0.0f : v; return (1.0f < m) ? 1.0f : m; }  // utils.h:245-248 (NaN passes)
 
 DEV void store_contrib(const Chains &ch, int p, const POut<float> &o) {
-    ch.rgb[p] = o.contrib.x;
-    ch.rgb[p + ch.NS] = o.contrib.y;
-    ch.rgb[p + 2 * ch.NS] = o.contrib.z;
+    float *rgb = &ch.rgb[p], *scr = &ch.screen[p];
+    rgb[0] = o.contrib.x;
+    rgb[1] = o.contrib.y;
+    rgb[2] = o.contrib.z;
     ch.ls[p] = o.lsScore;
     ch.ss[p] = o.ssScore;
-    ch.screen[p] = o.screen[0];
-    ch.screen[p + ch.NS] = o.screen[1];
+    scr[0] = o.screen[0];
+    scr[1] = o.screen[1];
 }
 
 DEV void splat(const Chains &ch, const DrParams &P, int W, int H, float *film, int p, float w) {
     if (P.acceptanceMap || !ch.hasSplat[p]) return;
+    const float *rgb = &ch.rgb[p], *scr = &ch.screen[p];
     const float k = w * (P.b / ch.ls[p]);
-    const float r = ch.rgb[p] * k, g = ch.rgb[p + ch.NS] * k, b = ch.rgb[p + 2 * ch.NS] * k;
+    const float r = rgb[0] * k, g = rgb[1] * k, b = rgb[2] * k;
     if (r == 0.0f && g == 0.0f && b == 0.0f) return;
     if (!(isfinite(r) && isfinite(g) && isfinite(b))) return;
-    const int ix = min(max(int(ch.screen[p] * W), 0), W - 1), iy = min(max(int(ch.screen[p + ch.NS] * H), 0), H - 1);
+    const int ix = min(max(int(scr[0] * W), 0), W - 1), iy = min(max(int(scr[1] * H), 0), H - 1);
     float *px = film + 3 * (size_t(iy) * W + ix);
     atomicAdd(px, r);
     atomicAdd(px + 1, g);
@@ -62,7 +64,8 @@ DEV void splat(const Chains &ch, const DrParams &P, int W, int H, float *film, i
 }
 DEV void splat_const(const Chains &ch, int W, int H, float *film, int p, float r, float g, float b) {
     if (!ch.hasSplat[p]) return;
-    const int ix = min(max(int(ch.screen[p] * W), 0), W - 1), iy = min(max(int(ch.screen[p + ch.NS] * H), 0), H - 1);
+    const float *scr = &ch.screen[p];
+    const int ix = min(max(int(scr[0] * W), 0), W - 1), iy = min(max(int(scr[1] * H), 0), H - 1);
     float *px = film + 3 * (size_t(iy) * W + ix);
     atomicAdd(px, r);
     atomicAdd(px + 1, g);
@@ -202,7 +205,7 @@ template <class X, class W> DEV void h2mc_offset(const X &x, const Chains &ch, c
     const int D = dim_of(ch.c[pc], ch.l[pc]);
     const Philox g(P.seed, iter, chain, TAG_S2);
     gauss_sample(x, D, g_ref(ch, pc), [&](int j) { return g.normal_at(j); }, w,
-                 [&](int i, double v) { ch.off[i * C + chain] = float(v); });
+                 [&](int i, double v) { chain_off(ch, chain)[i] = float(v); });
 }
 
 // H2MC acceptance before the delayed-rejection correction
@@ -212,7 +215,8 @@ template <class X, class W> DEV void h2mc_ratio(const X &x, const Chains &ch, in
     const int perm = ch.perm[chain];
     const int pc = chain + C * perm_cur(perm), ps = chain + C * perm_second(perm);
     const int D = dim_of(ch.c[pc], ch.l[pc]);
-    auto off = [&](int i) { return double(ch.off[i * C + chain]); };
+    const float *o = chain_off(ch, chain);
+    auto off = [&](int i) { return double(o[i]); };
     const double py = gauss_logpdf(x, D, off, 1.0, g_ref(ch, pc), w);
     const double px = gauss_logpdf(x, D, off, -1.0, g_ref(ch, ps), w);
     if (x.lane == 0) ch.a2d[chain] = exp(px - py) * double(ch.ss[ps]) / double(ch.ss[pc]);
@@ -423,7 +427,7 @@ DEV void dr_stage2_propose(const DevScene &Sc, const Chains &ch, const DrParams 
     const SRef rs = slot_ref(ch, a.ps);
     POut<float> out;
     if (a.isAniso) {
-        PerturbU<OffArray> up{rs.u, rs.s, 0, OffArray{ch.off + chain, ch.C}};
+        PerturbU<OffArray> up{rs.u, rs.s, 0, OffArray{chain_off(ch, chain), 1}};
         perturb_eval<float, false>(Sc, P.o, rs, up, DiscNormal{&g, P.discreteStdDev}, out);
     } else {
         PerturbU<OffNormal> up{rs.u, rs.s, 0, OffNormal{&g, P.anisoStdDev}};
@@ -496,10 +500,11 @@ DEV double dr_alpha2(int integrator, int type, float a1, double a2, float aStar,
 // Q1(z, y) / Q1(x, y) of the isotropic first-stage kernel (delayed.cpp:514-527).
 DEV double mira_kernel_ratio(const Chains &ch, const DrParams &P, int pc, int pf, int ps) {
     const int D = dim_of(ch.c[pc], ch.l[pc]);
+    const float *uc = slot_u(ch, pc), *uf = slot_u(ch, pf), *us = slot_u(ch, ps);
     double qzy = 0, qxy = 0;
     for (int i = 0; i < D; i++) {
-        const double y = ch.u[i * ch.NS + pf];
-        const double dz = double(ch.u[i * ch.NS + ps]) - y, dx = double(ch.u[i * ch.NS + pc]) - y;
+        const double y = uf[i];
+        const double dz = double(us[i]) - y, dx = double(uc[i]) - y;
         qzy += dz * dz;
         qxy += dx * dx;
     }
@@ -558,7 +563,7 @@ DEV void dr_stage2_accept(const DevScene &Sc, const Chains &ch, const DrParams &
     out.valid = false;
     if (a.traceStar) {
         const SRef rt = slot_ref(ch, a.pstar);
-        PerturbU<OffDiff> up{rt.u, rt.s, 0, OffDiff{ch.u + a.pc, ch.u + a.pf, ch.NS, ch.NS}};
+        PerturbU<OffDiff> up{rt.u, rt.s, 0, OffDiff{slot_u(ch, a.pc), slot_u(ch, a.pf), 1, 1}};
         perturb_eval<float, false>(Sc, P.o, rt, up, DiscNormal{&g, P.discreteStdDev}, out);
     }
     stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
@@ -577,6 +582,7 @@ DEV SRef rec_ref(char *base) {
     r.cprim = ip + 38; r.lgprim = ip + 54;
     r.cdisc = fp + 70; r.ldisc = fp + 86; r.crr = fp + 102; r.lrr = fp + 118;
     r.s = 1;
+    r.vs = 1;
     return r;
 }
 constexpr int kRecBytes = 536;

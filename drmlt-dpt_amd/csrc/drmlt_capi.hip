@@ -484,15 +484,13 @@ static int alloc_chains(drmlt_ctx *c, int C) {
         c->chainBytes.push_back(std::max<size_t>(n, 1) * sizeof(**p));
         return e == hipSuccess;
     };
-    bool ok = A(&ch.c, NS) && A(&ch.l, NS) && A(&ch.light, NS) && A(&ch.lprim, NS) && A(&ch.screen, 2 * NS) && A(&ch.u, 32 * NS) &&
-              A(&ch.cprim, 16 * NS) && A(&ch.lgprim, 16 * NS) && A(&ch.cdisc, 16 * NS) && A(&ch.ldisc, 16 * NS) && A(&ch.crr, 16 * NS) &&
-              A(&ch.lrr, 16 * NS) && A(&ch.valid, NS) && A(&ch.hasSplat, NS) && A(&ch.gInit, NS) && A(&ch.rgb, 3 * NS) &&
-              A(&ch.ls, NS) && A(&ch.ss, NS) && A(&ch.perm, size_t(C)) && A(&ch.gauss, size_t(kGSlot) * GS) &&
+    bool ok = A(&ch.slots, size_t(kSlotWords) * NS) && A(&ch.perm, size_t(C)) && A(&ch.gauss, size_t(kGSlot) * GS) &&
               A(&ch.off, 32 * size_t(C)) && A(&ch.grad, size_t(kGMax) * C) && A(&ch.hess, size_t(kGMax * kGMax) * C) &&
               A(&ch.a1, size_t(C)) && A(&ch.a2, size_t(C)) && A(&ch.a2d, size_t(C)) && A(&ch.flags, size_t(C)) &&
               A(&c->L.count, kListKinds * kMaxBins) && A(&c->L.items, size_t(kListKinds) * kMaxBins * C);
     for (Lists &lx : c->LX) ok = ok && A(&lx.count, kListKinds * kMaxBins) && A(&lx.items, size_t(kListKinds) * kMaxBins * C);
     if (!ok) return fail(DRMLT_ERR_HIP, "chain state allocation");
+    bind_slot_fields(ch);
     hipStream_t s = c->stream;
     // every state field starts zeroed: c = l = 0 is an empty path
     for (size_t k = 0; k < c->chainAllocs.size(); k++)
@@ -528,7 +526,61 @@ static int set_length_dist(drmlt_ctx *c, const float *lf, int n) {
     return DRMLT_OK;
 }
 
-// Host-side mirror of the seed records into chain slot 0 (structure of arrays).
+// Host images of one slot record (the layout of Chains::slots).
+template <class T> static void wput(uint32_t *w, int i, T v) {
+    static_assert(sizeof(T) == 4, "32-bit field");
+    std::memcpy(w + i, &v, 4);
+}
+template <class T> static T wget(const uint32_t *w, int i) {
+    static_assert(sizeof(T) == 4, "32-bit field");
+    T v;
+    std::memcpy(&v, w + i, 4);
+    return v;
+}
+// An empty state: c = l = 0, no light, every vertex triangle -1 and
+// Russian-roulette factor 1, everything else 0.
+static void slot_default(uint32_t *w) {
+    std::memset(w, 0, kSlotWords * 4);
+    wput<int32_t>(w, SO_LIGHT, -1);
+    wput<int32_t>(w, SO_LPRIM, -1);
+    for (int k = 0; k < kVMax; k++) {
+        wput<int32_t>(w, SO_CV + 3 * k, -1); wput<float>(w, SO_CV + 3 * k + 2, 1.0f);
+        wput<int32_t>(w, SO_LV + 3 * k, -1); wput<float>(w, SO_LV + 3 * k + 2, 1.0f);
+    }
+}
+// A record and its contrib8 row (valid, rgb, lsScore, ssScore, screen) into a slot.
+static void slot_pack(uint32_t *w, const drmlt_path_rec &R, const float *c8) {
+    slot_default(w);
+    wput<int32_t>(w, SO_C, R.c); wput<int32_t>(w, SO_L, R.l);
+    wput<int32_t>(w, SO_LIGHT, R.light); wput<int32_t>(w, SO_LPRIM, R.lprim);
+    wput<int32_t>(w, SO_VALID, c8[0] != 0.0f ? 1 : 0);
+    for (int k = 0; k < 3; k++) wput<float>(w, SO_RGB + k, c8[1 + k]);
+    wput<float>(w, SO_LS, c8[4]); wput<float>(w, SO_SS, c8[5]);
+    wput<float>(w, SO_SCREEN, c8[6]); wput<float>(w, SO_SCREEN + 1, c8[7]);
+    for (int k = 0; k < kDMax; k++) wput<float>(w, SO_U + k, R.u[k]);
+    for (int k = 0; k < kVMax; k++) {
+        wput<int32_t>(w, SO_CV + 3 * k, R.cprim[k]); wput<float>(w, SO_CV + 3 * k + 1, R.cdisc[k]); wput<float>(w, SO_CV + 3 * k + 2, R.crr[k]);
+        wput<int32_t>(w, SO_LV + 3 * k, R.lgprim[k]); wput<float>(w, SO_LV + 3 * k + 1, R.ldisc[k]); wput<float>(w, SO_LV + 3 * k + 2, R.lrr[k]);
+    }
+}
+// A slot as a record and its contrib8 row.
+static void slot_unpack(const uint32_t *w, drmlt_path_rec &R, float *o) {
+    R.c = wget<int32_t>(w, SO_C); R.l = wget<int32_t>(w, SO_L);
+    R.light = wget<int32_t>(w, SO_LIGHT); R.lprim = wget<int32_t>(w, SO_LPRIM);
+    R.screen[0] = wget<float>(w, SO_SCREEN); R.screen[1] = wget<float>(w, SO_SCREEN + 1);
+    for (int k = 0; k < kDMax; k++) R.u[k] = wget<float>(w, SO_U + k);
+    for (int k = 0; k < kVMax; k++) {
+        R.cprim[k] = wget<int32_t>(w, SO_CV + 3 * k); R.cdisc[k] = wget<float>(w, SO_CV + 3 * k + 1); R.crr[k] = wget<float>(w, SO_CV + 3 * k + 2);
+        R.lgprim[k] = wget<int32_t>(w, SO_LV + 3 * k); R.ldisc[k] = wget<float>(w, SO_LV + 3 * k + 1); R.lrr[k] = wget<float>(w, SO_LV + 3 * k + 2);
+    }
+    o[0] = wget<int32_t>(w, SO_VALID) ? 1.0f : 0.0f;
+    for (int k = 0; k < 3; k++) o[1 + k] = wget<float>(w, SO_RGB + k);
+    o[4] = wget<float>(w, SO_LS); o[5] = wget<float>(w, SO_SS);
+    o[6] = R.screen[0]; o[7] = R.screen[1];
+}
+
+// Host-side mirror of the seed records into chain slot 0; slots 1-3 get the
+// seed's vertex counts and are otherwise empty.
 extern "C" int drmlt_seed_chains(drmlt_ctx *c, const drmlt_path_rec *st, const float *c8, int64_t n, float b, const float *lf, int32_t nLen) {
     if (!c || !st || !c8 || !lf || n <= 0 || nLen <= 0 || nLen > kMaxBins) return fail(DRMLT_ERR_ARG, "bad argument");
     // records flagged invalid (contrib8[0] == 0) seed an empty invalid state,
@@ -541,37 +593,28 @@ extern "C" int drmlt_seed_chains(drmlt_ctx *c, const drmlt_path_rec *st, const f
     int r = alloc_chains(c, int(n));
     if (r) return r;
     Chains &ch = c->ch;
-    const size_t C = size_t(n), NS = ch.NS;
-    std::vector<int> ci(NS, 0), li(NS, 0), lig(NS, -1), lpr(NS, -1), valid(NS, 0), perm(C);
-    std::vector<float> scr(2 * NS, 0), u(32 * NS, 0), cd(16 * NS, 0), ld(16 * NS, 0), crr(16 * NS, 1), lrr(16 * NS, 1), rgb(3 * NS, 0),
-        ls(NS, 0), ss(NS, 0);
-    std::vector<int> cp(16 * NS, -1), lp(16 * NS, -1);
-    for (size_t i = 0; i < C; i++) {
-        perm[i] = 0 | (1 << 2) | (2 << 4);
-        if (c8[8 * i] == 0.0f) continue;  // empty (c = l = 0) and invalid
-        const drmlt_path_rec &R = st[i];
-        for (int k = 0; k < 4; k++) { ci[i + k * C] = R.c; li[i + k * C] = R.l; }
-        lig[i] = R.light; lpr[i] = R.lprim;
-        scr[i] = c8[8 * i + 6]; scr[i + NS] = c8[8 * i + 7];
-        for (int k = 0; k < 32; k++) u[k * NS + i] = R.u[k];
-        for (int k = 0; k < 16; k++) {
-            cp[k * NS + i] = R.cprim[k]; lp[k * NS + i] = R.lgprim[k];
-            cd[k * NS + i] = R.cdisc[k]; ld[k * NS + i] = R.ldisc[k];
-            crr[k * NS + i] = R.crr[k]; lrr[k * NS + i] = R.lrr[k];
-        }
-        valid[i] = 1;
-        rgb[i] = c8[8 * i + 1]; rgb[i + NS] = c8[8 * i + 2]; rgb[i + 2 * NS] = c8[8 * i + 3];
-        ls[i] = c8[8 * i + 4]; ss[i] = c8[8 * i + 5];
-        perm[i] = 0 | (1 << 2) | (2 << 4);
-    }
+    const size_t C = size_t(n), W = kSlotWords;
+    std::vector<uint32_t> recs(C * W);
+    std::vector<int> perm(C, 0 | (1 << 2) | (2 << 4));
     hipStream_t s = c->stream;
-    auto put = [&](auto *d, const auto &v) { return hipMemcpyAsync(d, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, s); };
-    HIPCHK(put(ch.c, ci)); HIPCHK(put(ch.l, li)); HIPCHK(put(ch.light, lig)); HIPCHK(put(ch.lprim, lpr));
-    HIPCHK(put(ch.screen, scr)); HIPCHK(put(ch.u, u)); HIPCHK(put(ch.cprim, cp)); HIPCHK(put(ch.lgprim, lp));
-    HIPCHK(put(ch.cdisc, cd)); HIPCHK(put(ch.ldisc, ld)); HIPCHK(put(ch.crr, crr)); HIPCHK(put(ch.lrr, lrr));
-    HIPCHK(put(ch.valid, valid)); HIPCHK(put(ch.rgb, rgb)); HIPCHK(put(ch.ls, ls)); HIPCHK(put(ch.ss, ss)); HIPCHK(put(ch.perm, perm));
-    HIPCHK(hipMemsetAsync(ch.hasSplat, 0, NS * 4, s));
-    HIPCHK(hipMemsetAsync(ch.gInit, 0, NS * 4, s));
+    // slot 0: the seeds (a record flagged invalid, contrib8[0] == 0, stays empty)
+    for (size_t i = 0; i < C; i++) {
+        if (c8[8 * i] == 0.0f) slot_default(&recs[i * W]);
+        else slot_pack(&recs[i * W], st[i], c8 + 8 * i);
+    }
+    HIPCHK(hipMemcpyAsync(ch.slots, recs.data(), C * W * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // slots 1-3: empty states with the seed's c and l
+    for (size_t i = 0; i < C; i++) {
+        uint32_t *w = &recs[i * W];
+        const bool seeded = c8[8 * i] != 0.0f;
+        slot_default(w);
+        wput<int32_t>(w, SO_C, seeded ? st[i].c : 0);
+        wput<int32_t>(w, SO_L, seeded ? st[i].l : 0);
+    }
+    for (size_t k = 1; k < 4; k++) HIPCHK(hipMemcpyAsync(ch.slots + k * C * W, recs.data(), C * W * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ch.perm, perm.data(), C * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
     c->b = b;
     r = set_length_dist(c, lf, nLen);
     if (r) return r;
@@ -1245,45 +1288,35 @@ extern "C" int drmlt_eval_logf(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st
     return DRMLT_OK;
 }
 
-// The chain state in host memory: every slot's fields and the per-chain
-// scratch of the last iteration.
+// The state of chains 0..n-1 in host memory: their four slots (slot k of
+// chain i at slots[(k * n + i) * kSlotWords]) and the per-chain scratch of
+// the last iteration.
 struct HostChains {
-    std::vector<int> perm, ci, li, lig, lpr, valid, cp, lp, flags;
-    std::vector<float> scr, u, cd, ld, crr, lrr, rgb, ls, ss, a1, off;
+    size_t n = 0;
+    std::vector<int> perm, flags;
+    std::vector<uint32_t> slots;
+    std::vector<float> a1, off;
     std::vector<double> a2;
+    const uint32_t *slot(size_t chain, int k) const { return &slots[(size_t(k) * n + chain) * kSlotWords]; }
 };
-static int fetch_chains(drmlt_ctx *c, HostChains &h, bool scratch) {
+static int fetch_chains(drmlt_ctx *c, HostChains &h, size_t n, bool scratch) {
     const Chains &ch = c->ch;
-    const size_t C = size_t(ch.C), NS = size_t(ch.NS);
+    const size_t C = size_t(ch.C), W = kSlotWords;
     hipStream_t s = c->stream;
-    auto get = [&](auto *dev, size_t count, auto &host) {
+    auto get = [&](const auto *dev, size_t count, auto &host) {
         host.resize(count);
         return hipMemcpyAsync(host.data(), dev, count * sizeof(host[0]), hipMemcpyDeviceToHost, s);
     };
-    HIPCHK(get(ch.perm, C, h.perm)); HIPCHK(get(ch.c, NS, h.ci)); HIPCHK(get(ch.l, NS, h.li)); HIPCHK(get(ch.light, NS, h.lig));
-    HIPCHK(get(ch.lprim, NS, h.lpr)); HIPCHK(get(ch.valid, NS, h.valid)); HIPCHK(get(ch.cprim, 16 * NS, h.cp));
-    HIPCHK(get(ch.lgprim, 16 * NS, h.lp)); HIPCHK(get(ch.screen, 2 * NS, h.scr)); HIPCHK(get(ch.u, 32 * NS, h.u));
-    HIPCHK(get(ch.cdisc, 16 * NS, h.cd)); HIPCHK(get(ch.ldisc, 16 * NS, h.ld)); HIPCHK(get(ch.crr, 16 * NS, h.crr));
-    HIPCHK(get(ch.lrr, 16 * NS, h.lrr)); HIPCHK(get(ch.rgb, 3 * NS, h.rgb)); HIPCHK(get(ch.ls, NS, h.ls)); HIPCHK(get(ch.ss, NS, h.ss));
+    h.n = n;
+    h.slots.resize(4 * n * W);
+    HIPCHK(get(ch.perm, n, h.perm));
+    for (size_t k = 0; k < 4; k++)
+        HIPCHK(hipMemcpyAsync(&h.slots[k * n * W], ch.slots + k * C * W, n * W * 4, hipMemcpyDeviceToHost, s));
     if (scratch) {
-        HIPCHK(get(ch.flags, C, h.flags)); HIPCHK(get(ch.a1, C, h.a1)); HIPCHK(get(ch.a2d, C, h.a2)); HIPCHK(get(ch.off, 32 * C, h.off));
+        HIPCHK(get(ch.flags, n, h.flags)); HIPCHK(get(ch.a1, n, h.a1)); HIPCHK(get(ch.a2d, n, h.a2)); HIPCHK(get(ch.off, 32 * n, h.off));
     }
     HIPCHK(hipStreamSynchronize(s));
     return DRMLT_OK;
-}
-// Physical slot p (p = chain + C * k) as a record and its contrib8 row.
-static void slot_record(const HostChains &h, size_t NS, size_t p, drmlt_path_rec &R, float *o) {
-    R.c = h.ci[p]; R.l = h.li[p]; R.light = h.lig[p]; R.lprim = h.lpr[p];
-    R.screen[0] = h.scr[p]; R.screen[1] = h.scr[p + NS];
-    for (int k = 0; k < 32; k++) R.u[k] = h.u[k * NS + p];
-    for (int k = 0; k < 16; k++) {
-        R.cprim[k] = h.cp[k * NS + p]; R.lgprim[k] = h.lp[k * NS + p];
-        R.cdisc[k] = h.cd[k * NS + p]; R.ldisc[k] = h.ld[k * NS + p];
-        R.crr[k] = h.crr[k * NS + p]; R.lrr[k] = h.lrr[k * NS + p];
-    }
-    o[0] = h.valid[p] ? 1.0f : 0.0f;
-    o[1] = h.rgb[p]; o[2] = h.rgb[p + NS]; o[3] = h.rgb[p + 2 * NS];
-    o[4] = h.ls[p]; o[5] = h.ss[p]; o[6] = h.scr[p]; o[7] = h.scr[p + NS];
 }
 
 extern "C" int drmlt_read_chains(drmlt_ctx *c, int64_t n, drmlt_path_rec *st, float *c8) {
@@ -1292,9 +1325,8 @@ extern "C" int drmlt_read_chains(drmlt_ctx *c, int64_t n, drmlt_path_rec *st, fl
     if (n > c->ch.C) return fail(DRMLT_ERR_ARG, "more states asked for than there are chains");
     if (n == 0) return DRMLT_OK;
     HostChains h;
-    if (int r = fetch_chains(c, h, false)) return r;
-    const size_t C = size_t(c->ch.C), NS = size_t(c->ch.NS);
-    for (int64_t i = 0; i < n; i++) slot_record(h, NS, size_t(i) + C * size_t(h.perm[i] & 3), st[i], c8 + 8 * size_t(i));
+    if (int r = fetch_chains(c, h, size_t(n), false)) return r;
+    for (int64_t i = 0; i < n; i++) slot_unpack(h.slot(size_t(i), h.perm[i] & 3), st[i], c8 + 8 * size_t(i));
     return DRMLT_OK;
 }
 
@@ -1305,15 +1337,14 @@ extern "C" int drmlt_read_step(drmlt_ctx *c, int64_t n, drmlt_path_rec *slots, f
     if (n > c->ch.C) return fail(DRMLT_ERR_ARG, "more chains asked for than there are");
     if (n == 0) return DRMLT_OK;
     HostChains h;
-    if (int r = fetch_chains(c, h, true)) return r;
-    const size_t C = size_t(c->ch.C), NS = size_t(c->ch.NS);
+    if (int r = fetch_chains(c, h, size_t(n), true)) return r;
     for (int64_t i = 0; i < n; i++) {
-        for (int k = 0; k < 4; k++) slot_record(h, NS, size_t(i) + C * size_t(k), slots[4 * i + k], c8 + 8 * (4 * size_t(i) + k));
+        for (int k = 0; k < 4; k++) slot_unpack(h.slot(size_t(i), k), slots[4 * i + k], c8 + 8 * (4 * size_t(i) + k));
         perm[i] = h.perm[i];
         flags[i] = h.flags[i];
         a1[i] = h.a1[i];
         alpha2[i] = (h.flags[i] & 2) ? h.a2[i] : 0.0;
-        for (int k = 0; k < 32; k++) off[32 * i + k] = h.off[size_t(k) * C + size_t(i)];
+        for (int k = 0; k < 32; k++) off[32 * i + k] = h.off[32 * size_t(i) + k];
     }
     return DRMLT_OK;
 }

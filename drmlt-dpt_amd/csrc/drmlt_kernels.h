@@ -15,35 +15,71 @@ constexpr int kGSmallD = 8;
 #define DRMLT_GAUSS_GROUP 8  // lanes per small Gaussian (measured: 8 beats 16 and 32)
 #endif
 
+// One path-state slot is a record of kSlotWords 32-bit words (nine 64-byte
+// lines): a header line (c, l, light, light triangle, valid, hasSplat,
+// gInit, contribution rgb, scores, screen position), two lines of primary
+// samples u[32], then the camera and the light vertices, each vertex's
+// (triangle, discrete sample, Russian-roulette factor) side by side. The
+// stage kernels take chains in length-bin order, so neighbouring lanes hold
+// unrelated slots; with whole records a lane's reads and writes of one state
+// stay within a few lines instead of one line per field.
+constexpr int kSlotWords = 144;
+enum : int {
+    SO_C = 0, SO_L = 1, SO_LIGHT = 2, SO_LPRIM = 3, SO_VALID = 4, SO_HASSPLAT = 5, SO_GINIT = 6,
+    SO_RGB = 8, SO_LS = 11, SO_SS = 12, SO_SCREEN = 13,
+    SO_U = 16,    // u[32]
+    SO_CV = 48,   // camera vertex k: words SO_CV + 3k .. + 2
+    SO_LV = 96,   // light vertex k: words SO_LV + 3k .. + 2
+};
+static_assert(SO_LV + 3 * kVMax == kSlotWords && SO_U + kDMax == SO_CV, "slot record layout");
+// A per-slot field: the field of slot p sits kSlotWords words after the
+// one of slot p - 1.
+template <class T> struct SlotField {
+    T *base;
+    DEV T &operator[](size_t p) const { return base[p * kSlotWords]; }
+};
+
 // Chain state: 4 physical slots per chain (current, first- and second-stage
-// proposal, Green's y*), all fields structure-of-arrays with stride NS = 4C.
-// perm[chain] packs which of the first three physical slots plays current /
-// first / second (2 bits each), so accepting a proposal is an index swap.
+// proposal, Green's y*); slot p = chain + C * k is the record at
+// slots + p * kSlotWords. perm[chain] packs which of the first three
+// physical slots plays current / first / second (2 bits each), so accepting
+// a proposal is an index swap.
 struct Chains {
     int C, NS, GS;  // chains, state slots (4C), Gaussian slots (3C)
-    int *c, *l, *light, *lprim;
-    float *screen, *u;
-    int *cprim, *lgprim;
-    float *cdisc, *ldisc, *crr, *lrr;
-    int *valid, *hasSplat, *gInit;
-    float *rgb, *ls, *ss;  // rgb [3][NS]
+    uint32_t *slots;       // [NS][kSlotWords] state records
+    SlotField<int> c, l, light, lprim, valid, hasSplat, gInit;
+    SlotField<float> rgb, ls, ss, screen;  // rgb and screen: the first of 3 and 2 words
     int *perm;
     double *gauss;         // [3C][kGSlot] Gaussian slots (dev_gauss.h), one per state slot 0..3C-1
     // per chain scratch of the DR step
-    float *off;            // [32][C] H2MC offset
+    float *off;            // [C][32] H2MC offset
     float *grad, *hess;    // [C][24], [C][24*24] derivative output (Hessian: upper triangle)
     float *a1, *a2;        // stage acceptances
     double *a2d;
     int *flags;            // bit0 acc1, bit1 doSecond, bit2 isAniso, bit3 large, bit4 z valid, bit5 y valid, bit6 y* valid
 };
+// Points the per-slot fields of `ch` into ch.slots (host side, after allocation).
+inline void bind_slot_fields(Chains &ch) {
+    int *i = reinterpret_cast<int *>(ch.slots);
+    float *f = reinterpret_cast<float *>(ch.slots);
+    ch.c.base = i + SO_C; ch.l.base = i + SO_L; ch.light.base = i + SO_LIGHT; ch.lprim.base = i + SO_LPRIM;
+    ch.valid.base = i + SO_VALID; ch.hasSplat.base = i + SO_HASSPLAT; ch.gInit.base = i + SO_GINIT;
+    ch.rgb.base = f + SO_RGB; ch.ls.base = f + SO_LS; ch.ss.base = f + SO_SS; ch.screen.base = f + SO_SCREEN;
+}
+DEV float *slot_u(const Chains &ch, int p) { return reinterpret_cast<float *>(ch.slots + size_t(p) * kSlotWords) + SO_U; }
+DEV float *chain_off(const Chains &ch, int chain) { return ch.off + size_t(chain) * 32; }
 
 DEV SRef slot_ref(const Chains &ch, int p) {
+    uint32_t *b = ch.slots + size_t(p) * kSlotWords;
+    int *ib = reinterpret_cast<int *>(b);
+    float *fb = reinterpret_cast<float *>(b);
     SRef r;
-    r.c = ch.c + p; r.l = ch.l + p; r.light = ch.light + p; r.lprim = ch.lprim + p;
-    r.screen = ch.screen + p; r.u = ch.u + p;
-    r.cprim = ch.cprim + p; r.lgprim = ch.lgprim + p;
-    r.cdisc = ch.cdisc + p; r.ldisc = ch.ldisc + p; r.crr = ch.crr + p; r.lrr = ch.lrr + p;
-    r.s = ch.NS;
+    r.c = ib + SO_C; r.l = ib + SO_L; r.light = ib + SO_LIGHT; r.lprim = ib + SO_LPRIM;
+    r.screen = fb + SO_SCREEN; r.u = fb + SO_U;
+    r.cprim = ib + SO_CV; r.cdisc = fb + SO_CV + 1; r.crr = fb + SO_CV + 2;
+    r.lgprim = ib + SO_LV; r.ldisc = fb + SO_LV + 1; r.lrr = fb + SO_LV + 2;
+    r.s = 1;
+    r.vs = 3;
     return r;
 }
 DEV GSlot g_ref(const Chains &ch, int p) { return GSlot{ch.gauss + size_t(p) * kGSlot}; }

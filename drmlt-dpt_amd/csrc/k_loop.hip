@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(64) k_prepass_seed(DevScene Sc, Opts o, Chains
     if (!found) {
         // replay did not reproduce the strategy: leave a well-formed, invalid state
         ch.c[p] = 2; ch.l[p] = 2; ch.light[p] = 0; ch.lprim[p] = Sc.lightTris[0];
-        for (int k = 0; k < kVMax; k++) { ch.cprim[k * ch.NS + p] = 0; ch.lgprim[k * ch.NS + p] = 0; }
+        for (int k = 0; k < kVMax; k++) { ref.cprim[k * ref.vs] = 0; ref.lgprim[k * ref.vs] = 0; }
         sink.kept.valid = false;
         sink.kept.contrib = V3<float>(0, 0, 0);
         sink.kept.lsScore = sink.kept.ssScore = 0.0f;
