@@ -379,15 +379,18 @@ def main():
     # 128 B per 4-wide BVH node visited (one cache line), 48 B per triangle tested
     alg_bytes = nodes * 128 + tris * 48 + state_bytes
     achieved = alg_bytes / (stage_ms[dom] / 1000.0) / 1e9
-    traffic, traffic_src, valu = None, None, None
+    traffic, traffic_src, valu, lanes = None, None, None, None
+    k = STAGE_KERNELS[dom]
+    if sched.get("stage_waves") == 3 and k in ("k_b_stage1", "k_b_stage2_propose", "k_b_stage2_accept"):
+        k += "_w3"  # the path-stage kernels bounded to three waves per SIMD
     prof = os.path.join(ROOT, "profiles", f"dr_step_config{args.config}_hbm_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
             pt = json.load(f)
-        k = STAGE_KERNELS[dom]
         if k in pt.get("kernels", {}):
             traffic = int(pt["kernels"][k]["hbm_bytes_per_launch"])
             valu = pt["kernels"][k].get("valu_util")
+            lanes = pt["kernels"][k].get("lanes_per_valu")
             traffic_src = (f"rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of {k} at commit {pt['commit']}, scaled by a "
                            f"256 MiB device copy (x{pt['fetch_scale']:.2f}, x{pt['write_scale']:.2f}); "
                            f"profiles/dr_step_config{args.config}_hbm_traffic.md")
@@ -399,8 +402,10 @@ def main():
         roof["traffic_source"] = traffic_src
     if valu is not None:
         roof["valu_util"] = valu
-        roof["valu_util_source"] = (f"4 x SQ_ACTIVE_INST_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8) of {STAGE_KERNELS[dom]}, "
+        roof["valu_util_source"] = (f"4 x SQ_ACTIVE_INST_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8) of {k}, "
                                     f"profiles/dr_step_config{args.config}_valu.md")
+    if lanes is not None:
+        roof["active_lanes_per_valu_instr"] = lanes  # SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU, of 64
 
     ctx.close()
 
@@ -439,7 +444,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         ncpu = min(args.cpu_threads, os.cpu_count() or 1)
-        cchains, citers = {1: (1024, 2048), 2: (16384, 512)}.get(args.config, (16384, 256))
+        # about 10-20 s of chain loop on 16 threads
+        cchains, citers = {1: (1024, 2048), 2: (16384, 512)}.get(args.config, (16384, 768))
         model, host_cpus = cpu_info()
         log(f"cpu baseline: the oracle loop, {cchains} chains x {citers} iterations on {ncpu} threads ({model}, {host_cpus} CPUs)")
         cval, csec, cacc = cpu_baseline(sc, ncpu, cchains, citers)
