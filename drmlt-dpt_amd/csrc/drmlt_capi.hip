@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "kernels_decl.h"
+#include "serialized.h"
 
 using namespace drmlt;
 
@@ -239,6 +240,11 @@ struct drmlt_ctx {
     Lists LX[kMaxParts - 1];
     // launch schedule of the batched step (drmlt_set_schedule / drmlt_autotune)
     drmlt_schedule sched = {0, 0, 0, 0, 0, 0, 0};
+    // the scene's triangles by serialized key, their material types and
+    // area lights (drmlt_path_func_derv; built on first use)
+    int nmat = 0;
+    TriKeyMap triMap;
+    std::vector<int> triMatType, triLight;
 };
 
 static DrParams make_params(const drmlt_ctx *c) {
@@ -361,6 +367,7 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
             return bail(fail(DRMLT_ERR_HIP, "stream"));
     const int nt = sc->num_triangles;
     c->nt = nt;
+    c->nmat = sc->num_materials;
     // BVH over the triangles (host build, leaf-order copy for traversal)
     // the binary depth cap is lowered until no ray can need more than the
     // device traversal stack
@@ -1412,6 +1419,158 @@ extern "C" int drmlt_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st
 // results go through this device's `scratch` array (at most kNoAllocStates
 // states per call), cleared by fill kernels. Overlapping calls on one GPU
 // share the array and race.
+// ------------------------------------------------ the reference's serialized form
+// Material types and area lights per triangle from host scene arrays.
+static void scene_facts_host(const drmlt_scene_desc *sd, std::vector<int> &matType, std::vector<int> &triLight) {
+    const int nt = sd->num_triangles;
+    matType.resize(nt);
+    triLight.resize(nt);
+    for (int i = 0; i < nt; i++) {
+        const int m = sd->tri_mat[i];
+        matType[i] = (m >= 0 && m < sd->num_materials) ? int(sd->materials[16 * size_t(m)]) : -1;
+        triLight[i] = sd->tri_light[i];
+    }
+}
+// (p0, p1 - p0, p2 - p0) per triangle in float, as the context's scene holds them.
+static std::vector<float> tri_keys_host(const drmlt_scene_desc *sd) {
+    const int nt = sd->num_triangles;
+    std::vector<float> k(9 * size_t(nt));
+    for (int i = 0; i < nt; i++) {
+        const float *p = sd->tri_p + 9 * size_t(i);
+        float *o = &k[9 * size_t(i)];
+        for (int a = 0; a < 3; a++) {
+            o[a] = p[a];
+            o[3 + a] = p[3 + a] - p[a];
+            o[6 + a] = p[6 + a] - p[a];
+        }
+    }
+    return k;
+}
+static int serialized_args_ok(int32_t cam, int32_t lgt, int64_t n, const float *lens, const float *primary, int64_t pstride,
+                              const float *vert, int64_t vstride) {
+    if (n < 0 || cam < 1 || lgt < 0 || cam + lgt < 3 || cam + lgt - 1 > 12 || (cam == 1 && lgt < 2))
+        return fail(DRMLT_ERR_ARG, "vertex counts out of range (c >= 1, l >= 0, 3 <= c + l, path length <= 12)");
+    if (n > 0 && (!lens || !primary || !vert)) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (pstride < ser_primary_size(cam, lgt)) return fail(DRMLT_ERR_ARG, "primary stride shorter than GetPrimaryParamSize(c, l)");
+    if (vstride <= 3) return fail(DRMLT_ERR_ARG, "vertParams stride too short");
+    return DRMLT_OK;
+}
+static int decode_all(const TriKeyMap &tm, const SceneFacts &sf, int32_t cam, int32_t lgt, int64_t n, const float *lens,
+                      const float *primary, int64_t pstride, const float *vert, int64_t vstride, drmlt_path_rec *out) {
+    std::string err;
+    for (int64_t i = 0; i < n; i++)
+        if (decode_subpath(tm, sf, cam, lgt, lens + 2 * i, primary + pstride * i, vert + vstride * i, vstride, out[i], err))
+            return fail(DRMLT_ERR_ARG, "serialized state " + std::to_string(i) + ": " + err);
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_material_from_serialized(const float *bsdf, int32_t two_sided, float *material) {
+    if (!bsdf || !material) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!material_from_serialized(bsdf, two_sided != 0, material)) return fail(DRMLT_ERR_ARG, "unknown serialized BSDF type");
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_serialized_to_rec(const drmlt_scene_desc *sd, int32_t cam_depth, int32_t lgt_depth, int64_t n, const float *lens,
+                                       const float *primary, int64_t primary_stride, const float *vert_params, int64_t vert_stride,
+                                       drmlt_path_rec *states) {
+    if (!sd || (n > 0 && !states) || sd->num_triangles <= 0 || !sd->tri_p || !sd->tri_mat || !sd->tri_light || !sd->materials)
+        return fail(DRMLT_ERR_ARG, "bad argument");
+    if (int r = serialized_args_ok(cam_depth, lgt_depth, n, lens, primary, primary_stride, vert_params, vert_stride)) return r;
+    if (n == 0) return DRMLT_OK;
+    TriKeyMap tm;
+    tm.build(sd->num_triangles, tri_keys_host(sd).data());
+    std::vector<int> matType, triLight;
+    scene_facts_host(sd, matType, triLight);
+    return decode_all(tm, SceneFacts{matType.data(), triLight.data()}, cam_depth, lgt_depth, n, lens, primary, primary_stride, vert_params,
+                      vert_stride, states);
+}
+
+// The context's triangle keys, material types and area lights, read back
+// from the device scene once.
+static int ensure_tri_map(drmlt_ctx *c) {
+    if (!c->triMap.empty()) return DRMLT_OK;
+    const size_t nt = size_t(c->nt);
+    std::vector<float4> s0(nt), s1(nt), s2(nt);
+    std::vector<int> tmat(nt);
+    std::vector<float> mats(16 * size_t(c->nmat));
+    c->triLight.resize(nt);
+    HIPCHK(hipMemcpy(s0.data(), c->S.sp0, nt * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(s1.data(), c->S.se1, nt * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(s2.data(), c->S.se2, nt * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tmat.data(), c->S.tmat, nt * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(c->triLight.data(), c->S.tlight, nt * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(mats.data(), c->S.mats, mats.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<float> keys(9 * nt);
+    for (size_t i = 0; i < nt; i++) {
+        const float4 *v[3] = {&s0[i], &s1[i], &s2[i]};
+        for (int a = 0; a < 3; a++) {
+            keys[9 * i + 3 * a] = v[a]->x;
+            keys[9 * i + 3 * a + 1] = v[a]->y;
+            keys[9 * i + 3 * a + 2] = v[a]->z;
+        }
+    }
+    c->triMatType.resize(nt);
+    for (size_t i = 0; i < nt; i++) c->triMatType[i] = int(mats[16 * size_t(tmat[i])]);
+    c->triMap.build(int(nt), keys.data());
+    return DRMLT_OK;
+}
+
+// The scene layout (scene.cpp:126-131) against the context's scene: the
+// light-coordinate option, sampleToCam (column-major), the camera position,
+// the pixel count and dist.
+static int scene_params_ok(const drmlt_ctx *c, const float *sp) {
+    const DevCamera &K = c->S.cam;
+    auto near = [](float a, float b) { return std::fabs(a - b) <= 1e-4f * (1.0f + std::fabs(b)); };
+    if ((sp[0] != 0.0f) != (c->opt.use_light_coordinate_sampling != 0))
+        return fail(DRMLT_ERR_ARG, "sceneParams: useLightCoordinateSampling differs from the context's option");
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            if (!near(sp[1 + 4 * i + j], K.s2c[4 * j + i])) return fail(DRMLT_ERR_ARG, "sceneParams: sampleToCam differs from the context's camera");
+    // camToWorld: isMoving, translate[0] (the camera position), ...
+    for (int a = 0; a < 3; a++)
+        if (!near(sp[18 + a], K.c2w[4 * a + 3])) return fail(DRMLT_ERR_ARG, "sceneParams: camera position differs from the context's camera");
+    if (sp[32] != float(K.W) * float(K.H)) return fail(DRMLT_ERR_ARG, "sceneParams: pixel count differs from the context's film");
+    if (!near(sp[33], K.dist)) return fail(DRMLT_ERR_ARG, "sceneParams: camera dist differs from the context's camera");
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_path_func_derv(drmlt_ctx *c, int32_t cam_depth, int32_t lgt_depth, int64_t n, const float *lens, const float *primary,
+                                    int64_t primary_stride, const float *scene_params, const float *vert_params, int64_t vert_stride,
+                                    float *grad, float *hess, int32_t *okOut) {
+    if (!c || !scene_params || (n > 0 && (!grad || !hess || !okOut))) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (int r = serialized_args_ok(cam_depth, lgt_depth, n, lens, primary, primary_stride, vert_params, vert_stride)) return r;
+    if (int r = scene_params_ok(c, scene_params)) return r;
+    if (n == 0) return DRMLT_OK;
+    if (int r = ensure_tri_map(c)) return r;
+    std::vector<drmlt_path_rec> recs(n);
+    if (int r = decode_all(c->triMap, SceneFacts{c->triMatType.data(), c->triLight.data()}, cam_depth, lgt_depth, n, lens, primary,
+                           primary_stride, vert_params, vert_stride, recs.data()))
+        return r;
+    std::vector<double> g(size_t(n) * 32), h(size_t(n) * 32 * 32);
+    std::vector<int32_t> ok(n);
+    if (int r = drmlt_grad_hess(c, n, recs.data(), 0, g.data(), h.data(), ok.data())) return r;
+    const int D = 2 * std::max(cam_depth + lgt_depth - 1, 2);
+    for (int64_t i = 0; i < n; i++) {
+        float *gi = grad + size_t(i) * D, *hi = hess + size_t(i) * D * D;
+        bool finite = ok[i] != 0;
+        for (int r = 0; r < D && finite; r++) {
+            gi[r] = float(g[size_t(i) * 32 + r]);
+            finite = std::isfinite(gi[r]);
+            for (int k = 0; k < D && finite; k++) {
+                hi[r * D + k] = float(h[(size_t(i) * 32 + r) * 32 + k]);
+                finite = std::isfinite(hi[r * D + k]);
+            }
+        }
+        if (!finite) {
+            // as the reference's caller does with a non-finite result (drsmallstep.cpp:149-155)
+            std::fill(gi, gi + D, 0.0f);
+            std::fill(hi, hi + size_t(D) * D, 0.0f);
+        }
+        okOut[i] = finite ? 1 : 0;
+    }
+    return DRMLT_OK;
+}
+
 extern "C" int drmlt_grad_hess_noalloc(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st, int32_t fp64, double *grad, double *hess,
                                        int32_t *okOut) {
     if (!c || n < 0 || !st || !grad || !hess || !okOut) return fail(DRMLT_ERR_ARG, "bad argument");

@@ -19,6 +19,7 @@ EXPORTS = [
     "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
     "drmlt_grad_hess", "drmlt_grad_hess_noalloc", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
     "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains", "drmlt_read_step", "drmlt_gaussian_eval_f64",
+    "drmlt_path_func_derv", "drmlt_serialized_to_rec", "drmlt_material_from_serialized",
     "drmlt_write_exr", "drmlt_stats_text", "drmlt_write_stats", "drmlt_render",
     "drmlt_scene_load", "drmlt_scene_get", "drmlt_scene_output", "drmlt_scene_free",
 ]
@@ -312,6 +313,45 @@ class Context:
         g = np.zeros((n, 32)); h = np.zeros((n, 32, 32)); ok = np.zeros(n, np.int32)
         check(lib().drmlt_grad_hess_noalloc(self.h, C.c_int64(n), P(states), int(fp64), P(g), P(h), P(ok)))
         return g, h, ok
+
+    def path_func_derv(self, c, l, lens, primary, scene_params, vert_params):
+        """The reference's PathFuncDerv call, batched over the rows of lens
+        (n x 2), primary (n x >= 2 max(c + l - 1, 2) + 1) and vert_params
+        (n x V), for one (c, l): grad n x D, hess n x D x D (float32) and ok."""
+        lens = np.ascontiguousarray(lens, np.float32).reshape(-1, 2)
+        primary = np.ascontiguousarray(primary, np.float32)
+        vert = np.ascontiguousarray(vert_params, np.float32)
+        sp = np.ascontiguousarray(scene_params, np.float32)
+        n = len(lens)
+        D = 2 * max(c + l - 1, 2)
+        g = np.zeros((n, D), np.float32); h = np.zeros((n, D, D), np.float32); ok = np.zeros(n, np.int32)
+        check(lib().drmlt_path_func_derv(self.h, int(c), int(l), C.c_int64(n), P(lens), P(primary), C.c_int64(primary.shape[1]), P(sp),
+                                         P(vert), C.c_int64(vert.shape[1]), P(g), P(h), P(ok)))
+        return g, h, ok
+
+
+def serialized_to_rec(scene, c, l, lens, primary, vert_params):
+    """Host only: the reference's serialized states (the drmlt_path_func_derv
+    layout) as path records of `scene` (a scene dict)."""
+    keep = [np.ascontiguousarray(scene[k]) for k in ("tri_p", "tri_n", "tri_mat", "tri_light", "mats", "light_tris", "light_emit", "cam")]
+    tp, tn, tm, tl, m, lt, le, cam = keep
+    d = SceneDesc(len(tp), P(tp), P(tn), P(tm), P(tl), len(m), P(m), len(lt), P(lt), P(le), P(cam))
+    lens = np.ascontiguousarray(lens, np.float32).reshape(-1, 2)
+    primary = np.ascontiguousarray(primary, np.float32)
+    vert = np.ascontiguousarray(vert_params, np.float32)
+    n = len(lens)
+    out = np.zeros(n, PATH_REC)
+    check(lib().drmlt_serialized_to_rec(C.byref(d), int(c), int(l), C.c_int64(n), P(lens), P(primary), C.c_int64(primary.shape[1]),
+                                        P(vert), C.c_int64(vert.shape[1]), P(out)))
+    return out
+
+
+def material_from_serialized(bsdf, two_sided=False):
+    """A serialized BSDF record (10 floats) as a 16-float material."""
+    b = np.ascontiguousarray(bsdf, np.float32)
+    m = np.zeros(16, np.float32)
+    check(lib().drmlt_material_from_serialized(P(b), int(two_sided), P(m)))
+    return m
 
 
 def compute_gaussian(D, sc, sigma, grad, hess):

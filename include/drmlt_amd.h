@@ -350,6 +350,51 @@ int drmlt_grad_hess(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int
  * overlap on one GPU share that array and race. */
 int drmlt_grad_hess_noalloc(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int32_t fp64, double *grad, double *hess, int32_t *ok);
 
+/* The reference's derivative call with its own argument set, batched:
+ * PathFuncDerv(lens, primary, sceneParams, vertParams, grad, hess)
+ * (src/path.h:145-149) as DRMLT's H2MC step calls it
+ * (src/integrators/delayed/drsmallstep.cpp:140-148), for n states of one
+ * (camera, light) vertex count pair (the key of the reference's
+ * staticFuncDervMap). State i reads lens + 2 i (the contribution's screen
+ * position), primary + primary_stride i (the time, then the primary sample in
+ * GetPathPos order; primary_stride >= GetPrimaryParamSize(c, l) =
+ * 2 max(c + l - 1, 2) + 1) and vert_params + vert_stride i, written by the
+ * reference's Serialize(scene, path, subPath) (src/path.cpp:2303-2391; the
+ * reference sizes it GetVertParamSize(maxDervDepth, maxDervDepth),
+ * path.cpp:2291-2301). scene_params is Serialize(scene)
+ * (src/scene.cpp:126-131, 38 floats), checked against the context's scene
+ * and options. The serialized triangles are matched to the context's by the
+ * exact bits of their (p0, p1 - p0, p2 - p0); the BSDF and light records
+ * are checked against the triangles' materials and area lights, and the
+ * values are the context's own (the scene the context was created from).
+ * Out: grad n x D, hess n x D x D (row-major), D = 2 max(c + l - 1, 2), the
+ * derivatives of log luminance with respect to primary[1..D] (the static
+ * function differentiates all primary entries but the time), computed in
+ * FP32 like the reference's Float; a non-finite result is written as zeros,
+ * as the reference's caller does (drsmallstep.cpp:149-155), and ok[i] = 0
+ * there. Needs c + l - 1 <= 12. */
+int drmlt_path_func_derv(drmlt_ctx *ctx, int32_t cam_depth, int32_t lgt_depth, int64_t n, const float *lens, const float *primary,
+                         int64_t primary_stride, const float *scene_params, const float *vert_params, int64_t vert_stride, float *grad,
+                         float *hess, int32_t *ok);
+
+/* Host only: the serialized states drmlt_path_func_derv reads, as path
+ * records of the scene `scene` (the arrays a context would be created from),
+ * with the same matching and checks. Fields the serialized layout does not
+ * carry take the record defaults: bsdfDiscrete of the last camera vertex 0,
+ * the Russian-roulette weight of each subpath's last vertex 1. */
+int drmlt_serialized_to_rec(const drmlt_scene_desc *scene, int32_t cam_depth, int32_t lgt_depth, int64_t n, const float *lens,
+                            const float *primary, int64_t primary_stride, const float *vert_params, int64_t vert_stride,
+                            drmlt_path_rec *states);
+
+/* Host only: a serialized BSDF record (BSDF::Serialize: type, then Lambertian
+ * Kd; Phong Kd, Ks, exponent, KsWeight; RoughDielectric Ks, Kt, eta, invEta,
+ * alpha -- src/lambertian.cpp:10-13, src/phong.cpp:13-19,
+ * src/roughdielectric.cpp:13-20) as a 16-float material of
+ * drmlt_scene_desc, with the given twoSided flag (the reference keeps it out
+ * of the record). For scene flattening on the reference side
+ * (include/drmlt_reference_flatten.hpp). */
+int drmlt_material_from_serialized(const float *bsdf, int32_t two_sided, float *material);
+
 /* PH2MCSmallStep::ComputeGaussian (drsmallstep.cpp:18-93, 219-266) in FP64
  * on the current device: D[i] dimensions, sc[i] the state's ssScore, sigma the
  * prior, grad n x 32, hess n x 32 x 32. Outputs mean n x 32, covL and invCov

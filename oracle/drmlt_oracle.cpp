@@ -485,3 +485,173 @@ extern "C" void orc_sample_param(void *h, int prim, const double *pos3, double *
     OrcScene *s = (OrcScene *)h;
     GetSampleParam(s->d, prim, V3<double>(pos3[0], pos3[1], pos3[2]), u2[0], u2[1]);
 }
+
+// ---------------------------------------------------------------- serialized form
+// The reference's serialized subpath (Serialize(scene, path, subPath),
+// path.cpp:2303-2391) and scene (Serialize(scene), scene.cpp:126-131 with
+// camera.cpp:30-36, animatedtransform.h:28-34, bounds.h:30-33 and 48-51),
+// written from path records of this scene: the inputs of the reference's
+// derivative functions. Record sizes: triangle-mesh shape 46
+// (trianglemesh.cpp:3-11), BSDF 10 (bsdf.cpp:7-11), light 56 (light.cpp:7-10).
+namespace {
+constexpr int kOShape = 46, kOBsdf = 10, kOLight = 56;
+struct OWriter {
+    float *b;
+    int pos = 0;
+    void put(float v) { b[pos++] = v; }
+    void put3(const V3<float> &v) { put(v.x); put(v.y); put(v.z); }
+    void skip(int n) { pos += n; }
+};
+// TriangleMesh::Serialize (trianglemesh.cpp:143-185), static mesh (the time-1
+// copies repeat time 0), no uvs; invArea: the inverse area of the emitter's
+// triangle range, else of the triangle
+void ser_shape(const SceneT<float> &s, int prim, OWriter &w) {
+    const int start = w.pos;
+    w.put(0.0f);  // ShapeType::TriangleMesh
+    w.put(0.0f);  // isMoving
+    const V3<float> p0(s.fp0[3 * prim], s.fp0[3 * prim + 1], s.fp0[3 * prim + 2]);
+    const V3<float> e1(s.fe1[3 * prim], s.fe1[3 * prim + 1], s.fe1[3 * prim + 2]);
+    const V3<float> e2(s.fe2[3 * prim], s.fe2[3 * prim + 1], s.fe2[3 * prim + 2]);
+    for (int t = 0; t < 2; t++) {
+        w.put3(p0); w.put3(e1); w.put3(e2);
+        w.put3(s.n0[prim]); w.put3(s.n1[prim]); w.put3(s.n2[prim]);
+    }
+    w.put(1.0f);  // BoolToFloat(st.size() == 0)
+    w.skip(6);    // uvs undefined
+    const int li = s.triLight[prim];
+    const float area = li >= 0 ? s.lights[li].totalArea : 0.5f * Length(Cross(e1, e2));
+    w.put(1.0f / area);
+    w.pos = start + kOShape;
+}
+// Lambertian / Phong / RoughDielectric ::Serialize (lambertian.cpp:10-13,
+// phong.cpp:13-19, roughdielectric.cpp:13-20)
+void ser_bsdf(const SceneT<float> &s, int prim, OWriter &w) {
+    const int start = w.pos;
+    const Material<float> &M = s.mats[s.triMat[prim]];
+    w.put(float(M.type));
+    if (M.type == 0) {
+        w.put3(M.a);
+    } else if (M.type == 1) {
+        w.put3(M.a); w.put3(M.b); w.put(M.expo); w.put(M.ksWeight);
+    } else {
+        w.put3(M.a); w.put3(M.b); w.put(M.eta); w.put(M.invEta); w.put(M.alpha);
+    }
+    w.pos = start + kOBsdf;
+}
+// AreaLight::Serialize (arealight.cpp:17-22)
+void ser_light(const SceneT<float> &s, int prim, OWriter &w) {
+    const int start = w.pos;
+    w.put(1.0f);  // LightType::AreaLight
+    ser_shape(s, prim, w);
+    w.put3(s.lights[s.triLight[prim]].emission);
+    w.pos = start + kOLight;
+}
+// MakeQuaternion (quaternion.cpp:4-36) of a rotation matrix
+void quat_of(const float m[4][4], float q[4]) {
+    const double tr = double(m[0][0]) + m[1][1] + m[2][2];
+    if (tr > 1e-7) {
+        double sq = std::sqrt(tr + 1.0);
+        q[3] = float(sq / 2.0);
+        sq = 0.5 / sq;
+        q[0] = float((m[2][1] - m[1][2]) * sq);
+        q[1] = float((m[0][2] - m[2][0]) * sq);
+        q[2] = float((m[1][0] - m[0][1]) * sq);
+    } else {
+        const int nxt[3] = {1, 2, 0};
+        int i = 0;
+        if (m[1][1] > m[0][0]) i = 1;
+        if (m[2][2] > m[i][i]) i = 2;
+        const int j = nxt[i], k = nxt[j];
+        double sq = std::sqrt((m[i][i] - (m[j][j] + m[k][k])) + 1.0);
+        double qq[3];
+        qq[i] = sq * 0.5;
+        if (sq != 0.0) sq = 0.5 / sq;
+        q[3] = float((m[k][j] - m[j][k]) * sq);
+        qq[j] = (m[j][i] + m[i][j]) * sq;
+        qq[k] = (m[k][i] + m[i][k]) * sq;
+        q[0] = float(qq[0]); q[1] = float(qq[1]); q[2] = float(qq[2]);
+    }
+}
+}  // namespace
+
+extern "C" void orc_serialize_scene(void *h, int useLightCoord, float *sp) {
+    const SceneT<float> &s = ((OrcScene *)h)->f;
+    OWriter w{sp};
+    w.put(useLightCoord ? 1.0f : 0.0f);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) w.put(s.cam.s2c[j][i]);  // column-major (utils.h:348-355)
+    float q[4];
+    quat_of(s.cam.c2w, q);
+    const V3<float> pos(s.cam.c2w[0][3], s.cam.c2w[1][3], s.cam.c2w[2][3]);
+    w.put(0.0f);  // isMoving
+    w.put3(pos); w.put3(pos);
+    for (int t = 0; t < 2; t++) for (int k = 0; k < 4; k++) w.put(q[k]);
+    w.put(float(s.cam.W) * float(s.cam.H));
+    w.put(s.cam.dist);
+    // BSphere of the scene bounds: center, radius (bounds.h:30-33)
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int t = 0; t < s.nt; t++)
+        for (int v = 0; v < 3; v++)
+            for (int a = 0; a < 3; a++) {
+                const float x = s.fp0[3 * t + a] + (v == 1 ? s.fe1[3 * t + a] : v == 2 ? s.fe2[3 * t + a] : 0.0f);
+                lo[a] = std::min(lo[a], x); hi[a] = std::max(hi[a], x);
+            }
+    const V3<float> c(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]));
+    w.put3(c);
+    w.put(0.5f * Length(V3<float>(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2])));
+}
+
+// primary: n x pstride (time 0, then u in GetPathPos order); vert: n x
+// vstride, zero-filled past what the path writes.
+extern "C" void orc_serialize(void *h, int n, const PathRec *recs, int pstride, float *primary, int vstride, float *vert) {
+    const SceneT<float> &s = ((OrcScene *)h)->f;
+    for (int i = 0; i < n; i++) {
+        const PathRec &R = recs[i];
+        float *pr = primary + size_t(pstride) * i;
+        std::fill(pr, pr + pstride, 0.0f);
+        const int P = R.c + R.l - 1, D = 2 * std::max(P, 2);
+        pr[0] = 0.0f;  // path.time
+        for (int k = 0; k < D; k++) pr[1 + k] = R.u[k];
+        float *v = vert + size_t(vstride) * i;
+        std::fill(v, v + vstride, 0.0f);
+        OWriter w{v};
+        w.put3(V3<float>(s.cam.c2w[0][3], s.cam.c2w[1][3], s.cam.c2w[2][3]));  // lensVertexPos
+        const int c = R.c, l = R.l;
+        if (l > 1) {
+            w.put(PickLightProb(s, R.light));
+            ser_light(s, R.lprim, w);
+            bool done = false;
+            for (int d = 0; d < l - 1; d++) {
+                ser_shape(s, R.lgprim[d], w);
+                w.put(R.ldisc[d]);
+                // useAbsoluteParam (path.cpp:273) at the default roughness threshold 0.03
+                w.put(Roughness(s.mats[s.triMat[R.lgprim[d]]], R.ldisc[d]) > 0.03f ? 1.0f : 0.0f);
+                ser_bsdf(s, R.lgprim[d], w);
+                if (d == l - 2 && c == 1) { done = true; break; }
+                if (d == l - 2) break;
+                w.put(R.lrr[d]);
+            }
+            if (done) continue;
+        }
+        for (int d = 0; d < c - 1; d++) {
+            ser_shape(s, R.cprim[d], w);
+            if (d == c - 2) {
+                if (l == 0) {
+                    ser_light(s, R.cprim[d], w);
+                    w.put(PickLightProb(s, s.triLight[R.cprim[d]]));
+                } else if (l == 1) {
+                    ser_light(s, R.lprim, w);
+                    ser_bsdf(s, R.cprim[d], w);
+                    w.put(PickLightProb(s, R.light));
+                } else {
+                    ser_bsdf(s, R.cprim[d], w);
+                }
+                break;
+            }
+            w.put(R.cdisc[d]);
+            w.put(Roughness(s.mats[s.triMat[R.cprim[d]]], R.cdisc[d]) > 0.03f ? 1.0f : 0.0f);
+            ser_bsdf(s, R.cprim[d], w);
+            w.put(R.crr[d]);
+        }
+    }
+}

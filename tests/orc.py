@@ -95,6 +95,24 @@ class Scene:
         lib().orc_generate_subpath(C.c_void_p(self.h), int(dbl), n, P(cl), P(uni), uni.shape[1], P(out), P(c8))
         return out, c8
 
+    def serialize(self, recs, max_depth=9):
+        """The reference's serialized form of path records (Serialize,
+        path.cpp:2303-2391): primary n x GetPrimaryParamSize(max, max) and
+        vertParams n x GetVertParamSize(max, max), as the reference sizes them."""
+        recs = np.ascontiguousarray(recs, PATH_REC)
+        n = len(recs)
+        ps = 2 * max(2 * max_depth - 1, 2) + 1
+        vs = 2 * max_depth * (46 + 10 + 3) + 56 + 5
+        primary = np.zeros((n, ps), np.float32); vert = np.zeros((n, vs), np.float32)
+        lib().orc_serialize(C.c_void_p(self.h), n, P(recs), ps, P(primary), vs, P(vert))
+        return primary, vert
+
+    def serialize_scene(self, use_light_coord=True):
+        """Serialize(scene) (scene.cpp:126-131): 38 floats."""
+        sp = np.zeros(38, np.float32)
+        lib().orc_serialize_scene(C.c_void_p(self.h), int(use_light_coord), P(sp))
+        return sp
+
     def generate_bidir(self, uni, min_depth, max_depth, dbl=True, maxc=64):
         uni = np.ascontiguousarray(uni, np.float32)
         n = len(uni)
