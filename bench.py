@@ -327,7 +327,15 @@ def main():
     chains = args.chains or int(dpt["numchains"])
     opt = capi.options_from_dpt(dpt, num_chains=chains, seed=1234)
     log(f"scene cfg{args.config}: {len(sc['tri_p'])} triangles, {chains} chains, built in {time.time() - t0:.1f}s")
+    # the BVH: the context's device build, and the host build of the same
+    # tree for comparison (a context that holds only the scene)
+    hctx = capi.Context(sc, capi.options_from_dpt(dpt, num_chains=1024, seed=1234, host_bvh=1))
+    bvh_host = hctx.bvh_info()
+    hctx.close()
     ctx = capi.Context(sc, opt)
+    bvh = ctx.bvh_info()
+    log(f"BVH {bvh['nodes']} 4-wide nodes, depth {bvh['depth4']}, stack {bvh['stack_need']}: device build {bvh['build_ms']:.1f} ms, "
+        f"host build {bvh_host['build_ms']:.1f} ms")
     t1 = time.time()
     b, lf = ctx.prepass()
     ctx.direct_lighting()
@@ -483,6 +491,10 @@ def main():
         "derivatives": deriv,
         "equal_time_rmse": study,
         "config2_maxdepth9": cfg2,
+        "bvh": {"builder": "device" if bvh["on_device"] else "host", "build_ms": round(bvh["build_ms"], 2),
+                "host_build_ms": round(bvh_host["build_ms"], 2), "nodes_4wide": bvh["nodes"], "depth4": bvh["depth4"],
+                "stack_need": bvh["stack_need"], "same_tree": (bvh["nodes"], bvh["depth4"], bvh["stack_need"]) ==
+                (bvh_host["nodes"], bvh_host["depth4"], bvh_host["stack_need"])},
     }
     print(json.dumps(out), flush=True)
 

@@ -77,7 +77,6 @@ struct Builder {
         float ext = -1;
         for (int k = 0; k < 3; k++)
             if (cb.hi[k] - cb.lo[k] > ext) { ext = cb.hi[k] - cb.lo[k]; axis = k; }
-        int mid = first + n / 2;
         // binned SAH over the centroid extent of every axis (the longest
         // axis alone with DRMLT_BVH_AXES=1)
         constexpr int NB = DRMLT_BVH_BINS;
@@ -116,19 +115,19 @@ struct Builder {
                 if (cost < best) { best = cost; bestAxis = k; bestSplit = b; }
             }
         }
-        if (bestSplit > 0) {
-            const int k = bestAxis;
-            const float scale = NB / (cb.hi[k] - cb.lo[k]);
-            auto it = std::partition(idx.begin() + first, idx.begin() + first + n, [&](int t) {
-                return std::min(std::max(int((cen[3 * t + k] - cb.lo[k]) * scale), 0), NB - 1) < bestSplit;
-            });
-            mid = int(it - idx.begin());
-            if (mid == first || mid == first + n) mid = first + n / 2;
+        if (bestSplit <= 0) {
+            // every centroid equal (no bin split leaves both sides
+            // non-empty): a leaf of all of them, which the device build
+            // (bvh_device.hip) makes too
+            outIndex = first;
+            outCount = n;
+            return;
         }
-        if (mid == first + n / 2) {
-            std::nth_element(idx.begin() + first, idx.begin() + mid, idx.begin() + first + n,
-                             [&](int a, int b) { return cen[3 * a + axis] < cen[3 * b + axis]; });
-        }
+        const float scale = NB / (cb.hi[bestAxis] - cb.lo[bestAxis]);
+        auto it = std::partition(idx.begin() + first, idx.begin() + first + n, [&](int t) {
+            return std::min(std::max(int((cen[3 * t + bestAxis] - cb.lo[bestAxis]) * scale), 0), NB - 1) < bestSplit;
+        });
+        const int mid = int(it - idx.begin());
         const int node = int(bnodes.size());
         bnodes.emplace_back();
         int li, lc, ri, rc;
@@ -206,20 +205,23 @@ int BuildBvh(int nt, const float *triP, int maxDepth, BvhOut &out) {
         for (int k = 0; k < 3; k++) b.cen[3 * size_t(i) + k] = 0.5f * (bx.lo[k] + bx.hi[k]);
     }
     b.bnodes.reserve(size_t(nt) / 2 + 2);
-    if (nt <= b.maxLeaf) {
-        // a root whose two children are the same leaf (testing a triangle twice
-        // gives the same hit)
+    int rc = nt;
+    if (nt > b.maxLeaf) {
+        int ri;
+        Box rb;
+        b.build(0, nt, 0, ri, rc, rb);
+    }
+    if (rc > 0) {
+        // the whole scene is one leaf (a few triangles, or all centroids
+        // equal): a root whose two children are that leaf (testing a
+        // triangle twice gives the same hit)
         BNode r;
         Box lb = b.rangeBox(0, nt);
         Builder::pad(lb);
         r.box[0] = r.box[1] = lb;
         r.index[0] = r.index[1] = 0;
         r.count[0] = r.count[1] = nt;
-        b.bnodes.push_back(r);
-    } else {
-        int ri, rc;
-        Box rb;
-        b.build(0, nt, 0, ri, rc, rb);
+        b.bnodes.assign(1, r);
     }
     b.nodes.reserve(b.bnodes.size() * 32 / 2 + 32);
     b.collapse(0, 0, 0);

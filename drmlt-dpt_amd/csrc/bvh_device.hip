@@ -1,0 +1,468 @@
+// Device BVH build replacing the reference's Embree build (scene.cpp:42-54):
+// the binned-SAH tree of the host builder (bvh_build.cpp: 32 bins over the
+// centroid extent of each axis, leaves of at most 4 triangles, a depth cap,
+// child boxes padded) built top-down one tree level per launch, one
+// 256-thread workgroup per node, then collapsed into the 4-wide layout the
+// traversal reads one 4-wide level per launch. The split decisions use the
+// host builder's arithmetic in the same order (no contraction), so the tree
+// has the same nodes; nodes are numbered in allocation order instead of depth
+// first, which leaves every query's answer unchanged (the traversal's closest
+// hit breaks ties by scene order).
+#include "bvh_device.h"
+
+#include <chrono>
+#include <vector>
+
+namespace drmlt {
+namespace {
+
+constexpr int kBins = 32;
+constexpr int kLeaf = 4;
+constexpr int kWG = 256;
+
+// one binary node: two child slots (padded box lo.xyz hi.xyz, index, count:
+// count > 0 leaf range [index, index + count), 0 internal node `index`)
+struct BNodeD {
+    float box[2][6];
+    int index[2], count[2];
+};
+// a node to build: the triangle range [first, first + count) of the level's
+// index array, its depth, and the parent slot it fills (2 node + side; -1: root)
+struct BTask {
+    int first, count, depth, parent;
+};
+// a 4-wide node to write from binary node b
+struct CTask {
+    int b, depth, pushed, out;
+};
+
+// order-preserving float <-> int for LDS atomic min / max
+__device__ inline int ford(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ inline float fdec(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+__device__ inline float wmin(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline float wmax(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Box::area of the host builder, without contraction
+__device__ inline float box_area(const float *lo, const float *hi) {
+#pragma clang fp contract(off)
+    float d[3];
+    for (int k = 0; k < 3; k++) d[k] = fmaxf(hi[k] - lo[k], 0.0f);
+    return 2.0f * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+}
+// the host builder's split cost: area x count of both sides
+__device__ inline float sah_cost(const float *llo, const float *lhi, int lc, const float *rlo, const float *rhi, int rc) {
+#pragma clang fp contract(off)
+    return box_area(llo, lhi) * float(lc) + box_area(rlo, rhi) * float(rc);
+}
+// Builder::pad of the host builder
+__device__ inline void box_pad(float *lo, float *hi) {
+#pragma clang fp contract(off)
+    for (int k = 0; k < 3; k++) {
+        const float p = 1e-5f * (fabsf(lo[k]) + fabsf(hi[k])) + 1e-7f;
+        lo[k] -= p;
+        hi[k] += p;
+    }
+}
+__device__ inline int bin_of(float c, float lo, float scale) {
+#pragma clang fp contract(off)
+    return min(max(int((c - lo) * scale), 0), kBins - 1);
+}
+
+// Per triangle: box of its corners and its centroid.
+__global__ void __launch_bounds__(256) k_bvh_prep(int nt, const float *triP, float4 *blo, float4 *bhi, float4 *cen, int *idx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const float *p = triP + 9 * size_t(i);
+    float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+    for (int v = 0; v < 3; v++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], p[3 * v + k]);
+            hi[k] = fmaxf(hi[k], p[3 * v + k]);
+        }
+    blo[i] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    bhi[i] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    cen[i] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.0f);
+    idx[i] = i;
+}
+
+// One tree level: workgroup t builds task t (Builder::build of the host
+// builder, one call).
+__global__ void __launch_bounds__(kWG) k_bvh_level(const BTask *tasks, const int *idxIn, int *idxOut, int *order, const float4 *blo,
+                                                  const float4 *bhi, const float4 *cen, BNodeD *bn, int *nodeCount, BTask *next,
+                                                  int *nextCount, int maxDepth, int *depthMax, int cap, int *overflow) {
+    const BTask T = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ float red[12][kWG / 64];
+    __shared__ float box[12];  // range box lo, hi; centroid box lo, hi
+    __shared__ int bcnt[3][kBins];
+    __shared__ int bbox[3][kBins][6];
+    __shared__ int dec[4];     // axis, split, mid, node
+    __shared__ int wsum[2][kWG / 64];
+    // 1. the range's box and its centroids' box
+    float v[12];
+    for (int k = 0; k < 3; k++) { v[k] = 3e38f; v[3 + k] = -3e38f; v[6 + k] = 3e38f; v[9 + k] = -3e38f; }
+    for (int i = tid; i < T.count; i += kWG) {
+        const int t = idxIn[T.first + i];
+        const float4 a = blo[t], b = bhi[t], c = cen[t];
+        const float al[3] = {a.x, a.y, a.z}, bh[3] = {b.x, b.y, b.z}, cc[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; k++) {
+            v[k] = fminf(v[k], al[k]); v[3 + k] = fmaxf(v[3 + k], bh[k]);
+            v[6 + k] = fminf(v[6 + k], cc[k]); v[9 + k] = fmaxf(v[9 + k], cc[k]);
+        }
+    }
+    for (int k = 0; k < 12; k++) {
+        const bool isMin = (k % 6) < 3;
+        const float r = isMin ? wmin(v[k]) : wmax(v[k]);
+        if (lane == 0) red[k][wave] = r;
+    }
+    __syncthreads();
+    if (tid < 12) {
+        const bool isMin = (tid % 6) < 3;
+        float r = red[tid][0];
+        for (int w = 1; w < kWG / 64; w++) r = isMin ? fminf(r, red[tid][w]) : fmaxf(r, red[tid][w]);
+        box[tid] = r;
+    }
+    __syncthreads();
+    bool leaf = T.count <= kLeaf || T.depth >= maxDepth;
+    if (tid == 0) {
+        atomicMax(depthMax, T.depth);
+        if (T.parent >= 0) {
+            float lo[3] = {box[0], box[1], box[2]}, hi[3] = {box[3], box[4], box[5]};
+            box_pad(lo, hi);
+            BNodeD &P = bn[T.parent >> 1];
+            const int side = T.parent & 1;
+            for (int k = 0; k < 3; k++) { P.box[side][k] = lo[k]; P.box[side][3 + k] = hi[k]; }
+        }
+    }
+    if (leaf) {
+        for (int i = tid; i < T.count; i += kWG) order[T.first + i] = idxIn[T.first + i];
+        if (tid == 0 && T.parent >= 0) {
+            bn[T.parent >> 1].index[T.parent & 1] = T.first;
+            bn[T.parent >> 1].count[T.parent & 1] = T.count;
+        }
+        if (tid == 0 && T.parent < 0) atomicAdd(overflow + 1, 1);  // the root is one leaf: the host builder's case
+        return;
+    }
+    // 2. bins over the centroid extent of each axis
+    const float clo[3] = {box[6], box[7], box[8]}, chi[3] = {box[9], box[10], box[11]};
+    float ext[3], scale[3];
+    for (int k = 0; k < 3; k++) {
+        ext[k] = chi[k] - clo[k];
+        scale[k] = ext[k] > 0 ? float(kBins) / ext[k] : 0.0f;
+    }
+    for (int i = tid; i < 3 * kBins; i += kWG) {
+        const int k = i / kBins, b = i % kBins;
+        bcnt[k][b] = 0;
+        for (int j = 0; j < 3; j++) { bbox[k][b][j] = ford(3e38f); bbox[k][b][3 + j] = ford(-3e38f); }
+    }
+    __syncthreads();
+    for (int i = tid; i < T.count; i += kWG) {
+        const int t = idxIn[T.first + i];
+        const float4 a = blo[t], bb = bhi[t], c = cen[t];
+        const float al[3] = {a.x, a.y, a.z}, bh[3] = {bb.x, bb.y, bb.z}, cc[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; k++) {
+            if (!(ext[k] > 0)) continue;
+            const int b = bin_of(cc[k], clo[k], scale[k]);
+            atomicAdd(&bcnt[k][b], 1);
+            for (int j = 0; j < 3; j++) {
+                atomicMin(&bbox[k][b][j], ford(al[j]));
+                atomicMax(&bbox[k][b][3 + j], ford(bh[j]));
+            }
+        }
+    }
+    __syncthreads();
+    // 3. the split: the host builder's loops in its order
+    if (tid == 0) {
+        float best = 3e38f;
+        int bestAxis = -1, bestSplit = -1;
+        for (int k = 0; k < 3; k++) {
+            if (!(ext[k] > 0)) continue;
+            float llo[kBins][3], lhi[kBins][3];
+            int lcnt[kBins];
+            float alo[3] = {3e38f, 3e38f, 3e38f}, ahi[3] = {-3e38f, -3e38f, -3e38f};
+            int cnt = 0;
+            for (int b = 0; b < kBins - 1; b++) {
+                for (int j = 0; j < 3; j++) {
+                    alo[j] = fminf(alo[j], fdec(bbox[k][b][j]));
+                    ahi[j] = fmaxf(ahi[j], fdec(bbox[k][b][3 + j]));
+                    llo[b][j] = alo[j];
+                    lhi[b][j] = ahi[j];
+                }
+                cnt += bcnt[k][b];
+                lcnt[b] = cnt;
+            }
+            for (int j = 0; j < 3; j++) { alo[j] = 3e38f; ahi[j] = -3e38f; }
+            cnt = 0;
+            for (int b = kBins - 1; b > 0; b--) {
+                for (int j = 0; j < 3; j++) {
+                    alo[j] = fminf(alo[j], fdec(bbox[k][b][j]));
+                    ahi[j] = fmaxf(ahi[j], fdec(bbox[k][b][3 + j]));
+                }
+                cnt += bcnt[k][b];
+                if (lcnt[b - 1] == 0 || cnt == 0) continue;
+                const float cost = sah_cost(llo[b - 1], lhi[b - 1], lcnt[b - 1], alo, ahi, cnt);
+                if (cost < best) { best = cost; bestAxis = k; bestSplit = b; }
+            }
+        }
+        dec[0] = bestAxis;
+        dec[1] = bestSplit;
+    }
+    __syncthreads();
+    const int axis = dec[0], split = dec[1];
+    // no bin split leaves both sides non-empty (every centroid equal): a leaf
+    // of all of them, as the host builder makes
+    leaf = split <= 0;
+    if (leaf) {
+        for (int i = tid; i < T.count; i += kWG) order[T.first + i] = idxIn[T.first + i];
+        if (tid == 0 && T.parent >= 0) {
+            bn[T.parent >> 1].index[T.parent & 1] = T.first;
+            bn[T.parent >> 1].count[T.parent & 1] = T.count;
+        }
+        if (tid == 0 && T.parent < 0) atomicAdd(overflow + 1, 1);  // the root is one leaf: the host builder's case
+        return;
+    }
+    // 4. stable partition into idxOut: left of the split, then right
+    int mid;
+    {
+        int nl = 0;
+        for (int i = tid; i < T.count; i += kWG) {
+            const float4 c = cen[idxIn[T.first + i]];
+            const float cc = axis == 0 ? c.x : axis == 1 ? c.y : c.z;
+            nl += bin_of(cc, clo[axis], scale[axis]) < split ? 1 : 0;
+        }
+        for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o);
+        if (lane == 0) wsum[0][wave] = nl;
+        __syncthreads();
+        int nLeft = 0;
+        for (int w = 0; w < kWG / 64; w++) nLeft += wsum[0][w];
+        mid = T.first + nLeft;
+        __syncthreads();
+        int lbase = 0, rbase = 0;
+        for (int c0 = 0; c0 < T.count; c0 += kWG) {
+            const int i = c0 + tid;
+            int t = -1;
+            bool left = false;
+            if (i < T.count) {
+                t = idxIn[T.first + i];
+                const float4 c = cen[t];
+                const float cc = axis == 0 ? c.x : axis == 1 ? c.y : c.z;
+                left = bin_of(cc, clo[axis], scale[axis]) < split;
+            }
+            const unsigned long long ml = __ballot(left), mr = __ballot(i < T.count && !left);
+            const unsigned long long below = (1ull << lane) - 1ull;
+            if (lane == 0) { wsum[0][wave] = __popcll(ml); wsum[1][wave] = __popcll(mr); }
+            __syncthreads();
+            int lo = 0, ro = 0, lt = 0, rt = 0;
+            for (int w = 0; w < kWG / 64; w++) {
+                if (w < wave) { lo += wsum[0][w]; ro += wsum[1][w]; }
+                lt += wsum[0][w]; rt += wsum[1][w];
+            }
+            if (i < T.count) {
+                if (left) idxOut[T.first + lbase + lo + __popcll(ml & below)] = t;
+                else idxOut[mid + rbase + ro + __popcll(mr & below)] = t;
+            }
+            lbase += lt;
+            rbase += rt;
+            __syncthreads();
+        }
+    }
+    // 5. this node and its two children's tasks
+    if (tid == 0) {
+        const int node = T.parent < 0 ? 0 : atomicAdd(nodeCount, 1);
+        const int q = atomicAdd(nextCount, 2);
+        if (node >= cap || q + 1 >= cap) {
+            atomicAdd(overflow, 1);  // not reached: a split leaves more than four triangles per internal node
+        } else {
+            if (T.parent >= 0) {
+                bn[T.parent >> 1].index[T.parent & 1] = node;
+                bn[T.parent >> 1].count[T.parent & 1] = 0;
+            }
+            next[q] = BTask{T.first, mid - T.first, T.depth + 1, 2 * node};
+            next[q + 1] = BTask{mid, T.first + T.count - mid, T.depth + 1, 2 * node + 1};
+        }
+    }
+}
+
+// One 4-wide level: thread t writes the 4-wide node of task t (the host
+// builder's collapse: open the internal child of largest area until four
+// slots are used).
+__global__ void __launch_bounds__(256) k_bvh_collapse(const CTask *tasks, int n, const BNodeD *bn, int nb, float4 *nodes, int *nodeCount,
+                                                     CTask *next, int *nextCount, int *depthMax, int *stackMax, int *overflow) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const CTask T = tasks[t];
+    float sb[4][6];
+    int si[4], sc[4];
+    int ns = 2;
+    for (int k = 0; k < 2; k++) {
+        for (int j = 0; j < 6; j++) sb[k][j] = bn[T.b].box[k][j];
+        si[k] = bn[T.b].index[k];
+        sc[k] = bn[T.b].count[k];
+    }
+    while (ns < 4) {
+        int open = -1;
+        float best = -1;
+        for (int k = 0; k < ns; k++) {
+            const float a = box_area(sb[k], sb[k] + 3);
+            if (sc[k] == 0 && a > best) { best = a; open = k; }
+        }
+        if (open < 0) break;
+        if (si[open] < 0 || si[open] >= nb) { atomicAdd(overflow, 1); return; }  // not reached: every slot names a built node
+        const BNodeD &c = bn[si[open]];
+        for (int j = 0; j < 6; j++) { sb[open][j] = c.box[0][j]; sb[ns][j] = c.box[1][j]; }
+        si[ns] = c.index[1]; sc[ns] = c.count[1];
+        si[open] = c.index[0]; sc[open] = c.count[0];
+        ns++;
+    }
+    atomicMax(depthMax, T.depth);
+    const int pushHere = T.pushed + (ns - 1);
+    atomicMax(stackMax, pushHere);
+    int internal = 0;
+    for (int k = 0; k < ns; k++) internal += sc[k] == 0 ? 1 : 0;
+    const int base = internal ? atomicAdd(nodeCount, internal) : 0;
+    const int qb = internal ? atomicAdd(nextCount, internal) : 0;
+    if (base + internal > nb || qb + internal > nb || T.out >= nb) { atomicAdd(overflow, 1); return; }  // not reached
+    int child[4], count[4];
+    int j = 0;
+    for (int k = 0; k < 4; k++) {
+        if (k >= ns) { child[k] = 0; count[k] = -1; continue; }
+        if (sc[k] > 0) { child[k] = si[k]; count[k] = sc[k]; continue; }
+        child[k] = base + j;
+        count[k] = 0;
+        next[qb + j] = CTask{si[k], T.depth + 1, pushHere, base + j};
+        j++;
+    }
+    float nd[32];
+    for (int k = 0; k < 4; k++) {
+        const bool used = k < ns;
+        nd[0 + k] = used ? sb[k][0] : 3e38f; nd[4 + k] = used ? sb[k][3] : -3e38f;
+        nd[8 + k] = used ? sb[k][1] : 3e38f; nd[12 + k] = used ? sb[k][4] : -3e38f;
+        nd[16 + k] = used ? sb[k][2] : 3e38f; nd[20 + k] = used ? sb[k][5] : -3e38f;
+        nd[24 + k] = __int_as_float(child[k]);
+        nd[28 + k] = __int_as_float(count[k]);
+    }
+    float4 *o = nodes + 8 * size_t(T.out);
+    for (int q = 0; q < 8; q++) o[q] = make_float4(nd[4 * q], nd[4 * q + 1], nd[4 * q + 2], nd[4 * q + 3]);
+}
+
+// Leaf-order triangle records: p0 with the scene index in .w, e1, e2.
+__global__ void __launch_bounds__(256) k_bvh_tris(int nt, const int *order, const float4 *s0, const float4 *s1, const float4 *s2, float4 *tri) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt) return;
+    const int i = min(max(order[k], 0), nt - 1);  // every position is written; the clamp keeps a defect from reading out of bounds
+    const float4 p = s0[i];
+    tri[3 * size_t(k)] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+    tri[3 * size_t(k) + 1] = s1[i];
+    tri[3 * size_t(k) + 2] = s2[i];
+}
+
+}  // namespace
+
+hipError_t BuildBvhDevice(int nt, const float *triP, const float4 *s0, const float4 *s1, const float4 *s2, int maxDepth, hipStream_t s,
+                          DeviceBvh &out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    out = DeviceBvh();
+    if (nt <= kLeaf) return hipErrorInvalidValue;  // the host builder's single-leaf root covers these
+    hipError_t e = hipSuccess;
+    float4 *blo = nullptr, *bhi = nullptr, *cen = nullptr;
+    int *idxA = nullptr, *idxB = nullptr, *order = nullptr, *counters = nullptr;
+    BTask *tA = nullptr, *tB = nullptr;
+    BNodeD *bn = nullptr;
+    CTask *cA = nullptr, *cB = nullptr;
+    // binary nodes: at most nt - 1 (every split leaves both sides non-empty)
+    const size_t maxB = size_t(nt), maxTasks = size_t(nt);
+#define BVH_CHK(x) do { e = (x); if (e != hipSuccess) goto done; } while (0)
+    BVH_CHK(hipMalloc(&blo, sizeof(float4) * nt));
+    BVH_CHK(hipMalloc(&bhi, sizeof(float4) * nt));
+    BVH_CHK(hipMalloc(&cen, sizeof(float4) * nt));
+    BVH_CHK(hipMalloc(&idxA, sizeof(int) * nt));
+    BVH_CHK(hipMalloc(&idxB, sizeof(int) * nt));
+    BVH_CHK(hipMalloc(&order, sizeof(int) * nt));
+    BVH_CHK(hipMalloc(&tA, sizeof(BTask) * maxTasks));
+    BVH_CHK(hipMalloc(&tB, sizeof(BTask) * maxTasks));
+    BVH_CHK(hipMalloc(&bn, sizeof(BNodeD) * maxB));
+    BVH_CHK(hipMalloc(&counters, sizeof(int) * 8));  // 0 binary nodes, 1 next tasks, 2 depth, 3 4-wide nodes, 4 depth4, 5 stack, 6 overflow
+    {
+        BVH_CHK(hipMemsetAsync(order, 0, sizeof(int) * nt, s));
+        BVH_CHK(hipMemsetAsync(bn, 0, sizeof(BNodeD) * maxB, s));
+        hipLaunchKernelGGL(k_bvh_prep, dim3((nt + 255) / 256), dim3(256), 0, s, nt, triP, blo, bhi, cen, idxA);
+        BVH_CHK(hipGetLastError());
+        const int init[8] = {1, 0, 0, 1, 0, 0, 0, 0};  // the root is binary node 0 and 4-wide node 0
+        BVH_CHK(hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, s));
+        const BTask root{0, nt, 0, -1};
+        BVH_CHK(hipMemcpyAsync(tA, &root, sizeof(root), hipMemcpyHostToDevice, s));
+        int ntasks = 1, overflow = 0;
+        for (int level = 0; ntasks > 0; level++) {
+            if (level > maxDepth + 1) { e = hipErrorUnknown; goto done; }  // not reached: tasks at the depth cap are leaves
+            BVH_CHK(hipMemsetAsync(counters + 1, 0, sizeof(int), s));
+            hipLaunchKernelGGL(k_bvh_level, dim3(ntasks), dim3(kWG), 0, s, tA, idxA, idxB, order, blo, bhi, cen, bn, counters, tB,
+                               counters + 1, maxDepth, counters + 2, int(maxTasks), counters + 6);
+            BVH_CHK(hipGetLastError());
+            BVH_CHK(hipMemcpyAsync(&ntasks, counters + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+            BVH_CHK(hipMemcpyAsync(&overflow, counters + 6, sizeof(int), hipMemcpyDeviceToHost, s));
+            BVH_CHK(hipStreamSynchronize(s));
+            if (overflow || size_t(ntasks) > maxTasks) { e = hipErrorUnknown; goto done; }
+            std::swap(tA, tB);
+            std::swap(idxA, idxB);
+        }
+        int rootLeaf = 0;
+        BVH_CHK(hipMemcpy(&rootLeaf, counters + 7, sizeof(int), hipMemcpyDeviceToHost));
+        if (rootLeaf) { e = hipErrorInvalidValue; goto done; }
+        int nb = 0;
+        BVH_CHK(hipMemcpy(&nb, counters, sizeof(int), hipMemcpyDeviceToHost));
+        // 4-wide collapse: at most nb 4-wide nodes
+        BVH_CHK(hipMalloc(&out.nodes, sizeof(float4) * 8 * size_t(nb)));
+        BVH_CHK(hipMalloc(&cA, sizeof(CTask) * size_t(nb)));
+        BVH_CHK(hipMalloc(&cB, sizeof(CTask) * size_t(nb)));
+        const CTask croot{0, 0, 0, 0};
+        BVH_CHK(hipMemcpyAsync(cA, &croot, sizeof(croot), hipMemcpyHostToDevice, s));
+        int nc = 1;
+        for (int level = 0; nc > 0; level++) {
+            if (level > maxDepth + 1) { e = hipErrorUnknown; goto done; }  // not reached: 4-wide levels never outnumber binary ones
+            BVH_CHK(hipMemsetAsync(counters + 1, 0, sizeof(int), s));
+            hipLaunchKernelGGL(k_bvh_collapse, dim3((nc + 255) / 256), dim3(256), 0, s, cA, nc, bn, nb, out.nodes, counters + 3, cB,
+                               counters + 1, counters + 4, counters + 5, counters + 6);
+            BVH_CHK(hipGetLastError());
+            BVH_CHK(hipMemcpyAsync(&nc, counters + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+            BVH_CHK(hipMemcpyAsync(&overflow, counters + 6, sizeof(int), hipMemcpyDeviceToHost, s));
+            BVH_CHK(hipStreamSynchronize(s));
+            if (overflow || nc > nb) { e = hipErrorUnknown; goto done; }
+            std::swap(cA, cB);
+        }
+        BVH_CHK(hipMalloc(&out.tri, sizeof(float4) * 3 * size_t(nt)));
+        hipLaunchKernelGGL(k_bvh_tris, dim3((nt + 255) / 256), dim3(256), 0, s, nt, order, s0, s1, s2, out.tri);
+        BVH_CHK(hipGetLastError());
+        int c[8];
+        BVH_CHK(hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, s));
+        BVH_CHK(hipStreamSynchronize(s));
+        out.numNodes = c[3];
+        out.depth = c[2];
+        out.depth4 = c[4];
+        out.stackNeed = c[5];
+    }
+done:
+#undef BVH_CHK
+    for (void *p : {(void *)blo, (void *)bhi, (void *)cen, (void *)idxA, (void *)idxB, (void *)order, (void *)tA, (void *)tB, (void *)bn,
+                    (void *)counters, (void *)cA, (void *)cB})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) {
+        if (out.nodes) (void)hipFree(out.nodes);
+        if (out.tri) (void)hipFree(out.tri);
+        out.nodes = nullptr;
+        out.tri = nullptr;
+    }
+    out.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return e;
+}
+
+}  // namespace drmlt

@@ -19,6 +19,7 @@
 
 #include "kernels_decl.h"
 #include "serialized.h"
+#include "bvh_device.h"
 
 using namespace drmlt;
 
@@ -243,6 +244,10 @@ struct drmlt_ctx {
     // the scene's triangles by serialized key, their material types and
     // area lights (drmlt_path_func_derv; built on first use)
     int nmat = 0;
+    // the BVH: built on the device (1) or the host (0), its build time
+    // (milliseconds, wall clock), 4-wide nodes and traversal stack bound
+    int bvhOnDevice = 0, bvhNodes = 0, bvhStack = 0;
+    double bvhMs = 0;
     TriKeyMap triMap;
     std::vector<int> triMatType, triLight;
 };
@@ -340,6 +345,16 @@ extern "C" int drmlt_ctx_destroy(drmlt_ctx *c) {
     return DRMLT_OK;
 }
 
+extern "C" int drmlt_bvh_info(drmlt_ctx *c, int32_t *onDevice, double *ms, int32_t *nodes, int32_t *depth4, int32_t *stack) {
+    if (!c || !onDevice || !ms || !nodes || !depth4 || !stack) return fail(DRMLT_ERR_ARG, "bad argument");
+    *onDevice = c->bvhOnDevice;
+    *ms = c->bvhMs;
+    *nodes = c->bvhNodes;
+    *depth4 = c->bvhDepth;
+    *stack = c->bvhStack;
+    return DRMLT_OK;
+}
+
 extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options *opt, drmlt_ctx **out) {
     if (!sc || !opt || !out) return fail(DRMLT_ERR_ARG, "null argument");
     if (sc->num_triangles <= 0 || sc->num_lights <= 0 || sc->num_materials <= 0) return fail(DRMLT_ERR_ARG, "empty scene");
@@ -368,33 +383,81 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     const int nt = sc->num_triangles;
     c->nt = nt;
     c->nmat = sc->num_materials;
-    // BVH over the triangles (host build, leaf-order copy for traversal)
-    // the binary depth cap is lowered until no ray can need more than the
-    // device traversal stack
-    BvhOut bvh;
-    for (int cap = 48;; cap -= 4) {
-        BuildBvh(nt, sc->tri_p, cap, bvh);
-        if (bvh.stackNeed <= kBvhStack) break;
-        if (cap <= 8) return bail(fail(DRMLT_ERR_ARG, "BVH does not fit the traversal stack"));
-    }
-    c->bvhDepth = bvh.depth4;
-    std::vector<float4> tri(3 * size_t(nt)), s0(nt), s1(nt), s2(nt);
+    std::vector<float4> s0(nt), s1(nt), s2(nt);
     for (int i = 0; i < nt; i++) {
         const float *p = sc->tri_p + 9 * size_t(i);
         s0[i] = make_float4(p[0], p[1], p[2], 0.0f);
         s1[i] = make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], 0.0f);
         s2[i] = make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], 0.0f);
     }
-    for (int k = 0; k < nt; k++) {
-        const int i = bvh.order[k];
-        int bits;
-        std::memcpy(&bits, &i, 4);
-        float w;
-        std::memcpy(&w, &bits, 4);
-        tri[3 * size_t(k)] = make_float4(s0[i].x, s0[i].y, s0[i].z, w);
-        tri[3 * size_t(k) + 1] = s1[i];
-        tri[3 * size_t(k) + 2] = s2[i];
+    hipStream_t s = c->stream;
+    auto up = [&](auto **p, const auto *src, size_t n) {
+        hipError_t e = upload(p, src, n, s);
+        c->sceneAllocs.push_back((void *)*p);
+        return e;
+    };
+    float4 *d_tri = nullptr, *d_nodes = nullptr, *d_s0, *d_s1, *d_s2;
+    if (up(&d_s0, s0.data(), nt) != hipSuccess || up(&d_s1, s1.data(), nt) != hipSuccess || up(&d_s2, s2.data(), nt) != hipSuccess)
+        return bail(fail(DRMLT_ERR_HIP, "scene upload"));
+    // BVH over the triangles and the leaf-order triangle copy for traversal,
+    // built on the device (bvh_device.hip) unless the options ask for the
+    // host builder (or the scene is a single leaf); the binary depth cap is
+    // lowered until no ray can need more than the traversal stack
+    const auto tb0 = std::chrono::steady_clock::now();
+    bool hostBuild = c->opt.host_bvh != 0 || nt <= 4;
+    DeviceBvh db;
+    if (!hostBuild) {
+        float *d_triP = nullptr;
+        if (upload(&d_triP, sc->tri_p, size_t(nt) * 9, s) != hipSuccess) return bail(fail(DRMLT_ERR_HIP, "scene upload"));
+        hipError_t e = hipSuccess;
+        for (int cap = 48;; cap -= 4) {
+            e = BuildBvhDevice(nt, d_triP, d_s0, d_s1, d_s2, cap, s, db);
+            if (e != hipSuccess || db.stackNeed <= kBvhStack || cap <= 8) break;
+            (void)hipFree(db.nodes);
+            (void)hipFree(db.tri);
+        }
+        (void)hipFree(d_triP);
+        if (e == hipErrorInvalidValue) hostBuild = true;  // a single-leaf scene
+        else if (e != hipSuccess) return bail(fail(DRMLT_ERR_HIP, std::string("device BVH build: ") + hipGetErrorString(e)));
     }
+    if (!hostBuild) {
+        c->sceneAllocs.push_back(db.nodes);
+        c->sceneAllocs.push_back(db.tri);
+        if (db.stackNeed > kBvhStack) return bail(fail(DRMLT_ERR_ARG, "BVH does not fit the traversal stack"));
+        d_nodes = db.nodes;
+        d_tri = db.tri;
+        c->bvhDepth = db.depth4;
+        c->bvhNodes = db.numNodes;
+        c->bvhStack = db.stackNeed;
+        c->bvhOnDevice = 1;
+    } else {
+        BvhOut bvh;
+        for (int cap = 48;; cap -= 4) {
+            BuildBvh(nt, sc->tri_p, cap, bvh);
+            if (bvh.stackNeed <= kBvhStack) break;
+            if (cap <= 8) return bail(fail(DRMLT_ERR_ARG, "BVH does not fit the traversal stack"));
+        }
+        std::vector<float4> tri(3 * size_t(nt));
+        for (int k = 0; k < nt; k++) {
+            const int i = bvh.order[k];
+            int bits;
+            std::memcpy(&bits, &i, 4);
+            float w;
+            std::memcpy(&w, &bits, 4);
+            tri[3 * size_t(k)] = make_float4(s0[i].x, s0[i].y, s0[i].z, w);
+            tri[3 * size_t(k) + 1] = s1[i];
+            tri[3 * size_t(k) + 2] = s2[i];
+        }
+        if (up(&d_tri, tri.data(), 3 * size_t(nt)) != hipSuccess ||
+            up(&d_nodes, (const float4 *)bvh.nodes.data(), bvh.nodes.size() / 4) != hipSuccess)
+            return bail(fail(DRMLT_ERR_HIP, "scene upload"));
+        if (hipStreamSynchronize(s) != hipSuccess) return bail(fail(DRMLT_ERR_HIP, "scene upload"));
+        c->bvhDepth = bvh.depth4;
+        c->bvhNodes = int(bvh.nodes.size() / 32);
+        c->bvhStack = bvh.stackNeed;
+        c->bvhOnDevice = 0;
+    }
+    c->bvhMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb0).count();
     // lights: area cdf per light (TriangleMesh::SetAreaLight, trianglemesh.cpp:274-290)
     const int nl = sc->num_lights;
     std::vector<float> area(nl), cdfAll, funcAll;
@@ -426,19 +489,9 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     float pfi;
     build_cdf(weights.data(), nl, pickCdf, pfi);
     DevScene &S = c->S;
-    hipStream_t s = c->stream;
-    auto up = [&](auto **p, const auto *src, size_t n) {
-        hipError_t e = upload(p, src, n, s);
-        c->sceneAllocs.push_back((void *)*p);
-        return e;
-    };
-    float4 *d_tri, *d_nodes, *d_s0, *d_s1, *d_s2;
     float *d_sn, *d_mats, *d_le, *d_area, *d_cdf, *d_func, *d_pick;
     int *d_mat, *d_light, *d_lt, *d_cdfOff;
-    bool ok = up(&d_tri, tri.data(), 3 * size_t(nt)) == hipSuccess &&
-              up(&d_nodes, (const float4 *)bvh.nodes.data(), bvh.nodes.size() / 4) == hipSuccess &&
-              up(&d_s0, s0.data(), nt) == hipSuccess && up(&d_s1, s1.data(), nt) == hipSuccess && up(&d_s2, s2.data(), nt) == hipSuccess &&
-              up(&d_sn, sc->tri_n, size_t(nt) * 9) == hipSuccess && up(&d_mat, sc->tri_mat, nt) == hipSuccess &&
+    bool ok = up(&d_sn, sc->tri_n, size_t(nt) * 9) == hipSuccess && up(&d_mat, sc->tri_mat, nt) == hipSuccess &&
               up(&d_light, sc->tri_light, nt) == hipSuccess && up(&d_mats, sc->materials, size_t(sc->num_materials) * 16) == hipSuccess &&
               up(&d_lt, sc->light_tris, size_t(nl) * 2) == hipSuccess && up(&d_le, sc->light_emit, size_t(nl) * 4) == hipSuccess &&
               up(&d_area, area.data(), nl) == hipSuccess && up(&d_cdf, cdfAll.data(), cdfAll.size()) == hipSuccess &&

@@ -16,7 +16,7 @@ ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result"]
 UNITS = ["k_plain.hip", "k_loop.hip", "k_count.hip", "k_deriv.hip", "k_deriv64.hip", "drmlt_capi.hip", "bvh_build.cpp", "host_io.cpp", "scene_load.cpp",
-         "serialized.cpp"]
+         "serialized.cpp", "bvh_device.hip"]
 
 
 def _compile(unit):

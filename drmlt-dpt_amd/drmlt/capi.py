@@ -19,7 +19,7 @@ EXPORTS = [
     "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
     "drmlt_grad_hess", "drmlt_grad_hess_noalloc", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
     "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains", "drmlt_read_step", "drmlt_gaussian_eval_f64",
-    "drmlt_path_func_derv", "drmlt_serialized_to_rec", "drmlt_material_from_serialized",
+    "drmlt_path_func_derv", "drmlt_serialized_to_rec", "drmlt_material_from_serialized", "drmlt_bvh_info",
     "drmlt_write_exr", "drmlt_stats_text", "drmlt_write_stats", "drmlt_render",
     "drmlt_scene_load", "drmlt_scene_get", "drmlt_scene_output", "drmlt_scene_free",
 ]
@@ -52,7 +52,7 @@ class Options(C.Structure):
                 ("aniso_perturb_std_dev", C.c_float), ("acceptance_map", C.c_int32),
                 ("max_derivatives_depth", C.c_int32), ("quirk_ignore_aniso_std_dev", C.c_int32),
                 ("quirk_chad_mis", C.c_int32), ("integrator", C.c_int32), ("uniform_mixing_prob", C.c_float),
-                ("lens_perturb_prob", C.c_float), ("seed", C.c_uint64)]
+                ("lens_perturb_prob", C.c_float), ("seed", C.c_uint64), ("host_bvh", C.c_int32)]
 
 
 class Schedule(C.Structure):
@@ -154,6 +154,13 @@ class Context:
         check(lib().drmlt_ctx_create(C.byref(d), C.byref(options), C.byref(h)))
         self.h = h
         self.W, self.H = int(cam[10]), int(cam[11])
+
+    def bvh_info(self):
+        """The context's BVH: built on the device, build ms, nodes, 4-wide depth, stack bound."""
+        dev, n, d4, st = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        ms = C.c_double()
+        check(lib().drmlt_bvh_info(self.h, C.byref(dev), C.byref(ms), C.byref(n), C.byref(d4), C.byref(st)))
+        return {"on_device": bool(dev.value), "build_ms": ms.value, "nodes": n.value, "depth4": d4.value, "stack_need": st.value}
 
     def close(self):
         if getattr(self, "h", None):

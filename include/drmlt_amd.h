@@ -102,6 +102,10 @@ typedef struct {
     float uniform_mixing_prob;  /* uniformmixingprobability (dptoptions.h:29, h2mc only) */
     float lens_perturb_prob;    /* lensperturbprob (dptoptions.h:20); h2mc needs 0 here */
     uint64_t seed;
+    /* 0 (default): the context builds its BVH on the device; 1: on the host.
+     * Both run the same binned-SAH build and answer every ray query alike
+     * (the reference builds with Embree, src/scene.cpp:42-54). */
+    int32_t host_bvh;
 } drmlt_options;
 
 /* One path state (the reference's Path + SubpathContrib, src/path.h:12-107):
@@ -164,6 +168,12 @@ int drmlt_options_default(drmlt_options *opt);
  * src/scene.cpp:10-54): uploads the scene and builds the BVH. */
 int drmlt_ctx_create(const drmlt_scene_desc *scene, const drmlt_options *opt, drmlt_ctx **out);
 int drmlt_ctx_destroy(drmlt_ctx *ctx);
+
+/* The context's BVH: whether it was built on the device (1) or the host (0),
+ * its build time in milliseconds (wall clock, triangle arrays already on the
+ * host), its 4-wide node count, 4-wide depth and the largest traversal stack
+ * a ray can need. No reference counterpart (Embree builds internally). */
+int drmlt_bvh_info(drmlt_ctx *ctx, int32_t *on_device, double *build_ms, int32_t *nodes, int32_t *depth4, int32_t *stack_need);
 
 /* DRIntegrator::PrePass (delayed.cpp:14-173) on the device: num_init_samples
  * bidirectional paths, the normalisation b, the path-length distribution and
