@@ -140,7 +140,8 @@ extern "C" void orc_bsdf_eval_batch(const float *mat, int adjoint, int n, const 
         q[3] = o.cosWo; q[4] = o.pdf; q[5] = o.revPdf;
     }
 }
-// Plants mistake `which` in the BSDF code (orc_core.h, g_plant); 0 removes it.
+// Plants mistake `which` (orc_core.h, g_plant: the BSDF code; 20 and 21 the
+// lens bookkeeping of orc_path.h); 0 removes it.
 extern "C" void orc_plant(int which) { g_plant = which; }
 
 template <class F>
@@ -185,6 +186,88 @@ extern "C" void orc_generate_subpath(void *h, int dbl, int n, const int *cl, con
     OrcScene *s = (OrcScene *)h;
     if (dbl) GenSubT(s->d, n, cl, uni, nuni, out, c8);
     else GenSubT(s->f, n, cl, uni, nuni, out, c8);
+}
+
+// The lens score and lensVertexPos of a state from its fixed-topology
+// re-evaluation (the state's own triangles, unperturbed, no occlusion).
+template <class F> static F LensPrepare(const SceneT<F> &s, const Opts<F> &o, PathS<F> &p) {
+    PathS<F> q = p;
+    F off[DMAX] = {0}, di[2 * VMAX] = {0};
+    std::vector<Contrib<F>> res;
+    PerturbPathBidir(s, o, off, di, q, res, false, true);
+    p.lensPos = q.lensPos;
+    return res.empty() ? F(0) : res[0].lensScore;
+}
+// Per state: lensScore, lensVertexPos (4 floats) of the fixed re-evaluation.
+extern "C" void orc_lens_score(void *h, int dbl, int n, const PathRec *in, float *out4) {
+    OrcScene *s = (OrcScene *)h;
+    for (int i = 0; i < n; i++) {
+        if (dbl) {
+            PathS<double> p = FromRec<double>(in[i]);
+            out4[4 * i] = float(LensPrepare(s->d, Opts<double>(), p));
+            out4[4 * i + 1] = float(p.lensPos.x); out4[4 * i + 2] = float(p.lensPos.y); out4[4 * i + 3] = float(p.lensPos.z);
+        } else {
+            PathS<float> p = FromRec<float>(in[i]);
+            out4[4 * i] = LensPrepare(s->f, Opts<float>(), p);
+            out4[4 * i + 1] = p.lensPos.x; out4[4 * i + 2] = p.lensPos.y; out4[4 * i + 3] = p.lensPos.z;
+        }
+    }
+}
+// GenerateSubpath's tracked lensScore per sample (0 without a contribution).
+extern "C" void orc_generate_subpath_lens(void *h, int dbl, int n, const int *cl, const float *uni, int nuni, float *lens) {
+    OrcScene *s = (OrcScene *)h;
+    for (int i = 0; i < n; i++) {
+        std::vector<Contrib<double>> rd;
+        std::vector<Contrib<float>> rf;
+        if (dbl) {
+            ArraySrc<double> g(uni + size_t(i) * nuni, nuni);
+            PathS<double> p;
+            GenerateSubpath(s->d, Opts<double>(), cl[2 * i], cl[2 * i + 1], p, rd, g);
+            lens[i] = rd.empty() ? 0.0f : float(rd[0].lensScore);
+        } else {
+            ArraySrc<float> g(uni + size_t(i) * nuni, nuni);
+            PathS<float> p;
+            GenerateSubpath(s->f, Opts<float>(), cl[2 * i], cl[2 * i + 1], p, rf, g);
+            lens[i] = rf.empty() ? 0.0f : rf[0].lensScore;
+        }
+    }
+}
+template <class F>
+static void LensT(const SceneT<F> &s, int n, const PathRec *in, const float *screen, int occl, int fixed, PathRec *out, float *c9) {
+    Opts<F> o;
+    for (int i = 0; i < n; i++) {
+        PathS<F> p = FromRec<F>(in[i]);
+        LensPrepare(s, o, p);
+        const F sc[2] = {F(screen[2 * i]), F(screen[2 * i + 1])};
+        std::vector<Contrib<F>> res;
+        PerturbLensBidir(s, o, sc, p, res, occl != 0, fixed != 0);
+        ToRec(p, out[i]);
+        ContribOut(res, c9 + 9 * i);
+        c9[9 * i + 8] = res.empty() ? 0.0f : float(res[0].lensScore);
+        if (!res.empty()) { out[i].screen[0] = c9[9 * i + 6]; out[i].screen[1] = c9[9 * i + 7]; }
+    }
+}
+// PerturbLensBidir (path.cpp:1914-2067) of each state to screen position
+// screen[2i..2i+1]: the perturbed state, contribution (8 floats) and lensScore.
+extern "C" void orc_perturb_lens(void *h, int dbl, int n, const PathRec *in, const float *screen, int occl, int fixed, PathRec *out, float *c9) {
+    OrcScene *s = (OrcScene *)h;
+    if (dbl) LensT(s->d, n, in, screen, occl, fixed, out, c9);
+    else LensT(s->f, n, in, screen, occl, fixed, out, c9);
+}
+// Central differences (step hstep) of the lens function (LogLensFixed) at each
+// state's own screen position (the contribution's): log lensScore, gradient 2,
+// Hessian 2 x 2.
+extern "C" void orc_lens_grad_hess_fd(void *h, int n, const PathRec *in, const float *screen, double hstep, double *logl, double *grad,
+                                      double *hess) {
+    OrcScene *s = (OrcScene *)h;
+    Opts<double> o;
+    for (int i = 0; i < n; i++) {
+        PathS<double> p = FromRec<double>(in[i]);
+        LensPrepare(s->d, o, p);
+        const double s0[2] = {screen[2 * i], screen[2 * i + 1]};
+        logl[i] = LogLensFixed(s->d, o, p, s0);
+        LensGradHessFD(s->d, o, p, s0, hstep, grad + 2 * i, hess + 4 * i);
+    }
 }
 
 // Per sample up to maxc contributions, each 10 floats: c, l, r, g, b, lsScore, ssScore, sx, sy, mis.
@@ -385,9 +468,11 @@ extern "C" int orc_prepass(void *h, int numInit, int maxDepth, int numChains, ui
 // iterations each, nthreads std::threads (chunk size 1, as ParallelFor). The
 // film returned is the indirect buffer (un-normalised). opts: type, largeStepProb,
 // perturbStdDev, anisoPerturbProb, anisoStdDev, discreteStdDev, roughThr,
-// maxDervDepth, acceptanceMap, chadMisQuirk.
+// maxDervDepth, acceptanceMap, chadMisQuirk, fdStep, secondStage, h2mc,
+// uniformMixing, lensPerturbProb, lensStdDev. stats: the eight DR counters,
+// then lens mutations and their acceptances.
 extern "C" void orc_render(void *h, const double *opts, const PathRec *seeds, const float *seedC8, int numChains, const float *lengthFunc, int nLen,
-                float b, int64_t itersPerChain, int nthreads, uint64_t seed, float *film, uint64_t *stats8, double *seconds) {
+                float b, int64_t itersPerChain, int nthreads, uint64_t seed, float *film, uint64_t *stats10, double *seconds) {
     OrcScene *s = (OrcScene *)h;
     DrOptions dro;
     dro.type = int(opts[0]);
@@ -404,6 +489,8 @@ extern "C" void orc_render(void *h, const double *opts, const PathRec *seeds, co
     dro.secondStage = opts[11] != 0;
     dro.h2mc = opts[12] != 0;
     dro.uniformMixing = opts[13];
+    dro.lensPerturbProb = opts[14];
+    dro.lensStdDev = opts[15];
     if (dro.h2mc) dro.secondStage = false;
     if (!dro.secondStage) dro.acceptanceMap = false;
     Dist1D<float> ld;
@@ -442,6 +529,9 @@ extern "C" void orc_render(void *h, const double *opts, const PathRec *seeds, co
         cur.sp.contrib = V3<float>(c8[1], c8[2], c8[3]);
         cur.sp.lsScore = c8[4]; cur.sp.ssScore = c8[5];
         cur.sp.screen[0] = c8[6]; cur.sp.screen[1] = c8[7];
+        // a seed's lens score and lensVertexPos: the fixed re-evaluation (the
+        // seed records carry neither)
+        if (dro.h2mc && dro.lensPerturbProb > 0) cur.sp.lensScore = LensPrepare(s->f, cx.o, cur.path);
         for (int64_t it = 0; it < itersPerChain; it++) DrIteration(cx, cur, first, second, star, rng, st[t]);
     });
     auto t1 = std::chrono::steady_clock::now();
@@ -455,9 +545,11 @@ extern "C" void orc_render(void *h, const double *opts, const PathRec *seeds, co
     for (auto &x : st) {
         tot.largeStep += x.largeStep; tot.largeStepAcc += x.largeStepAcc; tot.stage1 += x.stage1; tot.stage1Acc += x.stage1Acc;
         tot.iso += x.iso; tot.isoAcc += x.isoAcc; tot.h2mc += x.h2mc; tot.h2mcAcc += x.h2mcAcc;
+        tot.lens += x.lens; tot.lensAcc += x.lensAcc;
     }
-    stats8[0] = tot.largeStep; stats8[1] = tot.largeStepAcc; stats8[2] = tot.stage1; stats8[3] = tot.stage1Acc;
-    stats8[4] = tot.iso; stats8[5] = tot.isoAcc; stats8[6] = tot.h2mc; stats8[7] = tot.h2mcAcc;
+    stats10[0] = tot.largeStep; stats10[1] = tot.largeStepAcc; stats10[2] = tot.stage1; stats10[3] = tot.stage1Acc;
+    stats10[4] = tot.iso; stats10[5] = tot.isoAcc; stats10[6] = tot.h2mc; stats10[7] = tot.h2mcAcc;
+    stats10[8] = tot.lens; stats10[9] = tot.lensAcc;
 }
 
 

@@ -80,16 +80,19 @@ struct DrOptions {
     // 125-160: large or isotropic small step, splats (1 - a) and a, accept with
     // probability a); the acceptance map is a DR option and stays off there
     bool secondStage = true;
-    // integrator=h2mc with lensPerturbProb = 0 (src/integrators/h2mc.cpp:
-    // 196-250 with H2MCSmallStep::Mutate 448-600 minus its lens branch): a
-    // small step is isotropic with probability uniformMixing, else one H2MC
-    // move accepted with min(1, ratio); no second stage follows
+    // integrator=h2mc (src/integrators/h2mc.cpp:196-250 with
+    // H2MCSmallStep::Mutate 448-600): a small step is isotropic with
+    // probability uniformMixing; otherwise, when the current state has a
+    // lens score, a lens mutation with probability lensPerturbProb, else one
+    // H2MC move; each accepted with min(1, ratio); no second stage follows
     bool h2mc = false;
     double uniformMixing = 0.2;
+    double lensPerturbProb = 0.0, lensStdDev = 0.01;  // dptoptions.h:20-21
 };
 
 struct DrStats {
     uint64_t largeStep = 0, largeStepAcc = 0, stage1 = 0, stage1Acc = 0, iso = 0, isoAcc = 0, h2mc = 0, h2mcAcc = 0;
+    uint64_t lens = 0, lensAcc = 0;
 };
 
 template <class F> struct MState {
@@ -99,6 +102,8 @@ template <class F> struct MState {
     F scoreSum = 0;
     bool gInit = false;
     Gauss g;
+    bool lgInit = false;  // lensGaussianInitialized
+    Gauss lg;             // the lens Gaussian (2-D, in screen coordinates)
     bool hasSplat = false;
     F spl[2] = {0, 0};
     V3<F> splVal;
@@ -134,9 +139,35 @@ inline void GradHessFD(const SceneT<double> &s, const Opts<double> &o, const Pat
     }
 }
 
+// The function the reference's lens derivatives differentiate
+// (PathFuncMode::Lens, path.cpp:2987-3175): log lensScore of the state with
+// its screen position moved to `screen`, triangles fixed, no occlusion.
+inline double LogLensFixed(const SceneT<double> &s, const Opts<double> &o, const PathS<double> &p0, const double screen[2]) {
+    PathS<double> p = p0;
+    std::vector<Contrib<double>> out;
+    PerturbLensBidir(s, o, screen, p, out, false, true);
+    if (out.empty() || !(out[0].lensScore > 0)) return -std::numeric_limits<double>::infinity();
+    return std::log(out[0].lensScore);
+}
+// Central differences of LogLensFixed at the screen position s0 (the stencils of GradHessFD).
+inline void LensGradHessFD(const SceneT<double> &s, const Opts<double> &o, const PathS<double> &p, const double s0[2], double h, double *grad,
+                           double *hess) {
+    auto F = [&](int i, double di, int j, double dj) {
+        double sc[2] = {s0[0], s0[1]};
+        if (i >= 0) sc[i] += di;
+        if (j >= 0) sc[j] += dj;
+        return LogLensFixed(s, o, p, sc);
+    };
+    const double f0 = F(-1, 0, -1, 0);
+    for (int i = 0; i < 2; i++) grad[i] = (F(i, h, -1, 0) - F(i, -h, -1, 0)) / (2 * h);
+    for (int i = 0; i < 2; i++) hess[i * 2 + i] = (F(i, 2 * h, -1, 0) - 2 * f0 + F(i, -2 * h, -1, 0)) / (4 * h * h);
+    hess[1] = hess[2] = (F(0, h, 1, h) - F(0, h, 1, -h) - F(0, -h, 1, h) + F(0, -h, 1, -h)) / (4 * h * h);
+}
+
 template <class F> inline PathS<double> ToDouble(const PathS<F> &p) {
     PathS<double> q;
     q.c = p.c; q.l = p.l;
+    q.lensPos = V3<double>(p.lensPos.x, p.lensPos.y, p.lensPos.z);
     for (int i = 0; i < 2; i++) { q.screen[i] = p.screen[i]; q.lPos[i] = p.lPos[i]; q.lDir[i] = p.lDir[i]; }
     q.lLight = p.lLight; q.lPrim = p.lPrim;
     auto cv = [](const SurfV<F> &v) {
@@ -186,6 +217,31 @@ template <class F> inline void InitGaussian(DrChainCtx<F> &cx, MState<F> &st) {
     st.gInit = true;
 }
 
+// initLensGaussian (h2mc.cpp:466-499): lens derivatives exist for c + l > 3
+// within the derivative depth (path.cpp:3195-3197, 3268-3283); zero
+// derivatives when lensScore <= 1e-15 or non-finite; ComputeGaussian with the
+// lens sigma and the state's ssScore (h2mc.cpp:607-644).
+template <class F> inline void InitLensGaussian(DrChainCtx<F> &cx, MState<F> &st) {
+    const int c = st.sp.c, l = st.sp.l, P = c + l - 1;
+    H2Param pr(cx.dro.lensStdDev);
+    if (c >= 1 && l >= 0 && c + l > 3 && P <= cx.dro.maxDervDepth) {
+        double g[2] = {0, 0}, h[4] = {0, 0, 0, 0};
+        if (double(st.sp.lensScore) > 1e-15) {
+            const PathS<double> pd = ToDouble(st.path);
+            const double s0[2] = {double(st.sp.screen[0]), double(st.sp.screen[1])};
+            LensGradHessFD(*cx.sd, cx.od, pd, s0, cx.dro.fdStep, g, h);
+            bool fin = true;
+            for (double x : g) fin = fin && std::isfinite(x);
+            for (double x : h) fin = fin && std::isfinite(x);
+            if (!fin) { g[0] = g[1] = 0; h[0] = h[1] = h[2] = h[3] = 0; }
+        }
+        ComputeGaussian(pr, double(st.sp.ssScore), 2, g, h, st.lg);
+    } else {
+        IsotropicGaussian(2, pr.sigma, st.lg);
+    }
+    st.lgInit = true;
+}
+
 template <class F> inline void DiscOffsets(Philox &rng, double sd, F *d) {
     for (int i = 0; i < 2 * VMAX; i++) d[i] = F(sd * rng.normal());
 }
@@ -200,6 +256,7 @@ template <class F> inline F SmallStepMutate(DrChainCtx<F> &cx, MState<F> &cur, M
     std::vector<Contrib<F>> out;
     PerturbPathBidir(*cx.s, cx.o, off, disc, prop.path, out);
     prop.gInit = false;
+    prop.lgInit = false;  // the reference leaves the proposal object's flag as it was (see DESIGN)
     F a = 0;
     if (!out.empty()) {
         prop.sp = out[0];
@@ -232,6 +289,7 @@ template <class F> inline F LargeStepMutate(DrChainCtx<F> &cx, MState<F> &cur, M
     std::vector<Contrib<F>> out;
     GenerateSubpath(*cx.s, cx.o, cam, lgt, prop.path, out, g);
     prop.gInit = false;
+    prop.lgInit = false;
     F a = 0;
     if (!out.empty()) {
         (void)g.uni();  // contribution pick (one candidate when multiplexed)
@@ -263,6 +321,7 @@ template <class F> inline double H2MCMutate(DrChainCtx<F> &cx, MState<F> &cur, M
     prop.path = cur.path;
     std::vector<Contrib<F>> out;
     PerturbPathBidir(*cx.s, cx.o, off, disc, prop.path, out);
+    prop.lgInit = false;  // h2mc.cpp:583
     if (out.empty()) return 0.0;
     prop.sp = out[0];
     InitGaussian(cx, prop);
@@ -281,6 +340,35 @@ template <class F> inline double H2MCMutate(DrChainCtx<F> &cx, MState<F> &cur, M
     return a;
 }
 
+// h2mc.cpp:463-530 (the lens branch of H2MCSmallStep::Mutate): offset ~ the
+// current state's lens Gaussian, screen position moved by it (mod 1),
+// PerturbLensBidir, acceptance min(1, N_z(-off) / N_x(off) lensScore_z /
+// lensScore_x).
+template <class F> inline F LensMutate(DrChainCtx<F> &cx, MState<F> &cur, MState<F> &prop, Philox &rng) {
+    if (!cur.lgInit) InitLensGaussian(cx, cur);
+    double n[2], offd[2];
+    n[0] = rng.normal();
+    n[1] = rng.normal();
+    GenerateSample(cur.lg, n, offd);
+    const F off[2] = {F(offd[0]), F(offd[1])};
+    const F screen[2] = {Modulo(cur.sp.screen[0] + off[0], F(1)), Modulo(cur.sp.screen[1] + off[1], F(1))};
+    prop.path = cur.path;
+    std::vector<Contrib<F>> out;
+    PerturbLensBidir(*cx.s, cx.o, screen, prop.path, out);
+    prop.gInit = false;
+    prop.lgInit = false;
+    if (out.empty()) return F(0);
+    prop.sp = out[0];
+    InitLensGaussian(cx, prop);
+    const double x[2] = {double(off[0]), double(off[1])}, neg[2] = {-double(off[0]), -double(off[1])};
+    const double py = GaussianLogPdf(x, cur.lg), px = GaussianLogPdf(neg, prop.lg);
+    const F a = ClampRef(F(std::exp(px - py) * double(prop.sp.lensScore) / double(cur.sp.lensScore)), F(0), F(1));
+    prop.hasSplat = true;
+    prop.spl[0] = out[0].screen[0]; prop.spl[1] = out[0].screen[1];
+    prop.splVal = out[0].contrib * (cx.b / out[0].lsScore);
+    return a;
+}
+
 template <class F> inline void WeightedSplat(DrChainCtx<F> &cx, const MState<F> &st, F w) {
     if (cx.dro.acceptanceMap || !st.hasSplat) return;
     V3<F> v = st.splVal * w;
@@ -294,12 +382,25 @@ template <class F> inline void DrIteration(DrChainCtx<F> &cx, MState<F> &cur, MS
     double a2 = 0;
     bool acc1 = false, acc2 = false, isAniso = false;
     const bool isLarge = FlipCoin(F(cx.dro.largeStepProb), F(rng.uni()));
+    // h2mc small steps (h2mc.cpp:451-466): the uniform-mixing coin, then,
+    // when the current state has a lens score, the lens coin
+    bool h2mcMove = false, lensMove = false;
+    if (cur.valid && !isLarge && cx.dro.h2mc && !(double(rng.uni()) < cx.dro.uniformMixing)) {
+        lensMove = cur.sp.lensScore > F(0) && double(rng.uni()) < cx.dro.lensPerturbProb;
+        h2mcMove = !lensMove;
+    }
     if (!cur.valid || isLarge) {
         st.largeStep++;
         a1 = LargeStepMutate(cx, cur, first, rng);
         if (IsInvalid(a1)) { a1 = 0; acc1 = false; }
         else { acc1 = FlipCoin(a1, F(rng.uni())); if (acc1) st.largeStepAcc++; }
-    } else if (cx.dro.h2mc && !(double(rng.uni()) < cx.dro.uniformMixing)) {
+    } else if (lensMove) {
+        // the lens mutation: a first-stage move accepted with probability a
+        st.lens++;
+        a1 = LensMutate(cx, cur, first, rng);
+        if (IsInvalid(a1)) { a1 = 0; acc1 = false; }
+        else { acc1 = FlipCoin(a1, F(rng.uni())); if (acc1) st.lensAcc++; }
+    } else if (h2mcMove) {
         // the H2MC move as the only stage: a1 = 0 (so w1 = 0 below), a2 = min(1, ratio)
         isAniso = true;
         st.h2mc++;

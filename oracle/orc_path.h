@@ -2,9 +2,10 @@
 // (path.cpp) restated: subpath generation, fixed-topology re-evaluation
 // (PerturbPathBidir), MIS bookkeeping and the primary-sample layout.
 //
-// Simplifications relative to the reference, all outside the DR loop:
-// lens throughput / lensScore are not tracked (only lens mutations read them);
-// scenes are static, so `time` carries no information.
+// Lens bookkeeping (lensContrib, lensScore, lensVertexPos) is tracked as the
+// reference tracks it; the h2mc integrator's lens mutation reads it.
+// Simplification relative to the reference: scenes are static, so `time`
+// carries no information.
 #pragma once
 #include "orc_scene.h"
 
@@ -40,14 +41,15 @@ template <class F> struct PathS {
     F screen[2] = {0, 0};
     F lPos[2] = {0, 0}, lDir[2] = {0, 0};
     int lLight = 0, lPrim = -1;
+    V3<F> lensPos;  // lensVertexPos: the second camera vertex of the last evaluation (path.h:50-107)
     std::vector<SurfV<F>> cam, lgt;
 };
-// path.h:12-24 (lensScore not tracked)
+// path.h:12-24
 template <class F> struct Contrib {
     int c = 0, l = 0;
     F screen[2] = {0, 0};
     V3<F> contrib;
-    F lsScore = 0, ssScore = 0, mis = 0;
+    F lsScore = 0, ssScore = 0, lensScore = 0, mis = 0;
 };
 
 template <class F> struct Opts {
@@ -62,6 +64,7 @@ template <class F> struct BState {
     V3<F> wi;
     F prev = 0, thisw = 0;  // accMISWPrev, accMISWThis
     V3<F> thr;
+    V3<F> lensC;  // lensContrib
     F ssJ = 0, lcJ = 0, lastPdf = 0;
 };
 
@@ -91,6 +94,7 @@ template <class F> inline void EmitFromCamera(const SceneT<F> &s, const F screen
     st.prev = MIS(npix / pdfW);
     st.thisw = 0;
     st.ssJ = 1;
+    st.lensC = V3<F>(1, 1, 1);
 }
 // path.cpp:489-505
 template <class F> inline void EmitFromLight(const SceneT<F> &s, const Opts<F> &o, F pick, const PathS<F> &p, Ray<F> &r, BState<F> &st) {
@@ -103,6 +107,7 @@ template <class F> inline void EmitFromLight(const SceneT<F> &s, const Opts<F> &
     st.prev = MIS(dPdf / emPdf);
     st.thisw = o.chadMisQuirk ? F(0) : MIS(cosL / emPdf);
     st.ssJ = 1;
+    st.lensC = V3<F>(0, 0, 0);
 }
 // path.cpp:507-517
 template <class F> inline void ConvertMIS(const Ray<F> &r, BState<F> &st) {
@@ -113,9 +118,10 @@ template <class F> inline void ConvertMIS(const Ray<F> &r, BState<F> &st) {
 }
 
 // path.cpp:519-613. Returns false when the reference clears the contribution
-// list (a degenerate zero-length edge).
+// list (a degenerate zero-length edge). prevLensC: the lensContrib of the
+// light-subpath state before this vertex.
 template <class F> inline bool ConnectToCamera(const SceneT<F> &s, const Opts<F> &o, bool occl, int lgtDepth, const BState<F> &st,
-                                               const SurfV<F> &v, const V3<F> &prevPos, std::vector<Contrib<F>> &out) {
+                                               const SurfV<F> &v, const V3<F> &prevLensC, const V3<F> &prevPos, std::vector<Contrib<F>> &out) {
     Ray<F> cr;
     SamplePrimary(s.cam, F(0.5), F(0.5), cr);
     V3<F> toCam = cr.org - st.isect.pos;
@@ -132,8 +138,13 @@ template <class F> inline bool ConnectToCamera(const SceneT<F> &s, const Opts<F>
     F fac = SNC(true, st.wi, st.isect, toCam);
     if (fac <= F(0)) return true;
     b.contrib = b.contrib * fac;
+    V3<F> lensC(0, 0, 0);
     if (Roughness(m, v.disc) > o.roughThr && lgtDepth >= 1) {
-        if (DistanceSquared(st.isect.pos, prevPos) <= F(0)) { out.clear(); return false; }
+        BsdfOut<F> e;
+        BsdfEval(m, false, toCam, st.isect.sn, st.wi, e);
+        const F dd = DistanceSquared(st.isect.pos, prevPos);
+        if (dd <= F(0)) { out.clear(); return false; }
+        lensC = e.contrib.mul(prevLensC) * (F(1) / dd);
     }
     const F cosAtCam = -Dot(cr.dir, toCam);
     const F ipcd = s.cam.dist / cosAtCam;
@@ -148,14 +159,15 @@ template <class F> inline bool ConnectToCamera(const SceneT<F> &s, const Opts<F>
     if (score > F(0)) {
         Contrib<F> k;
         k.c = 1; k.l = 2 + lgtDepth; k.screen[0] = sx; k.screen[1] = sy;
-        k.contrib = c; k.lsScore = score; k.ssScore = score * st.ssJ; k.mis = mis;
+        k.contrib = c; k.lsScore = score; k.ssScore = score * st.ssJ; k.lensScore = Luminance(lensC); k.mis = mis;
         out.push_back(k);
     }
     return true;
 }
 
-// path.cpp:615-724 (BSDFSampling<adjoint, perturb>). `next` may alias `cur`.
-template <class F> inline bool BSDFSampling(const SceneT<F> &s, const Opts<F> &o, bool adjoint, bool perturb, const BState<F> &cur,
+// path.cpp:615-724 (BSDFSampling<adjoint, perturb>) at subpath vertex `depth`.
+// `next` may alias `cur`.
+template <class F> inline bool BSDFSampling(const SceneT<F> &s, const Opts<F> &o, bool adjoint, bool perturb, int depth, const BState<F> &cur,
                                             SurfV<F> &v, BState<F> &next, V3<F> &dir, V3<F> &bc) {
     const Material<F> &m = MatOf(s, v.prim);
     BsdfOut<F> b;
@@ -184,7 +196,29 @@ template <class F> inline bool BSDFSampling(const SceneT<F> &s, const Opts<F> &o
     }
     F fac = SNC(adjoint, cur.wi, cur.isect, dir);
     if (fac <= F(0)) return false;
+    // lensContrib (path.cpp:664-711); bc is still contrib / pdf here
+    const bool grazing = std::fabs(b.cosWo) < F(1e-2) || std::fabs(Dot(dir, cur.isect.gn)) < F(1e-2);
+    if (v.useAbs == F(1)) {
+        if (adjoint) {
+            next.lensC = grazing ? V3<F>(0, 0, 0) : bc * b.pdf;
+        } else if (depth == 0) {
+            next.lensC = grazing ? V3<F>(0, 0, 0) : cur.lensC.mul(bc * b.pdf);
+        } else if (depth == 1) {
+            BsdfOut<F> imp;
+            BsdfEval(m, true, dir, cur.isect.sn, cur.wi, imp);
+            if (std::fabs(imp.cosWo) < F(1e-2) || std::fabs(Dot(cur.wi, cur.isect.gn)) < F(1e-2)) {
+                next.lensC = V3<F>(0, 0, 0);
+            } else {
+                next.lensC = g_plant == 21 ? cur.lensC : cur.lensC.mul(imp.contrib * SNC(true, dir, cur.isect, cur.wi));
+            }
+        } else {
+            next.lensC = cur.lensC;
+        }
+    } else {
+        next.lensC = (adjoint || depth <= 1) ? V3<F>(0, 0, 0) : cur.lensC;
+    }
     bc = bc * fac;
+    if (adjoint) next.lensC = next.lensC * fac;
     next.lastPdf = b.pdf;
     const F newThis = MIS(b.cosWo / b.pdf) * (cur.thisw * MIS(b.revPdf) + cur.prev);
     next.thisw = newThis;
@@ -208,6 +242,16 @@ template <class F> inline void DirectLighting(const SceneT<F> &s, const Opts<F> 
     F fac = SNC(false, st.wi, st.isect, toL);
     if (fac <= F(0)) return;
     b.contrib = b.contrib * fac;
+    V3<F> lensC = st.lensC;
+    if (camDepth == 1) {
+        if (Roughness(m, v.disc) > o.roughThr) {
+            BsdfOut<F> imp;
+            BsdfEval(m, true, toL, st.isect.sn, st.wi, imp);
+            lensC = lensC.mul(imp.contrib * SNC(true, toL, st.isect, st.wi));
+        } else {
+            lensC = V3<F>(0, 0, 0);
+        }
+    }
     V3<F> c = st.thr.mul(b.contrib).mul(lc) * (F(1) / pick);
     const F wL = MIS(b.pdf / (pick * dPdf));
     const F wC = MIS(emPdf * b.cosWo / (dPdf * cosL)) * (st.prev + st.thisw * MIS(b.revPdf));
@@ -218,6 +262,7 @@ template <class F> inline void DirectLighting(const SceneT<F> &s, const Opts<F> 
         Contrib<F> k;
         k.c = 2 + camDepth; k.l = 1; k.screen[0] = screen[0]; k.screen[1] = screen[1];
         k.contrib = c; k.lsScore = score; k.ssScore = score * st.ssJ; k.mis = mis;
+        k.lensScore = camDepth >= 1 ? Luminance(lensC) : F(0);
         out.push_back(k);
     }
 }
@@ -245,13 +290,14 @@ template <class F> inline void HandleHitLight(const SceneT<F> &s, int camDepth, 
             Contrib<F> k;
             k.c = 2 + camDepth; k.l = 0; k.screen[0] = screen[0]; k.screen[1] = screen[1];
             k.contrib = c; k.lsScore = score; k.ssScore = score * st.ssJ; k.mis = mis;
+            k.lensScore = camDepth >= 2 ? Luminance(st.lensC) : F(0);
             out.push_back(k);
         }
     }
 }
 
 // path.cpp:862-976
-template <class F> inline void ConnectVertex(const SceneT<F> &s, bool occl, int camDepth, int lgtDepth, const BState<F> &ls,
+template <class F> inline void ConnectVertex(const SceneT<F> &s, const Opts<F> &o, bool occl, int camDepth, int lgtDepth, const BState<F> &ls,
                                              const SurfV<F> &lv, const BState<F> &cs, const SurfV<F> &cv, const F screen[2],
                                              std::vector<Contrib<F>> &out) {
     V3<F> toL = ls.isect.pos - cs.isect.pos;
@@ -273,6 +319,19 @@ template <class F> inline void ConnectVertex(const SceneT<F> &s, bool occl, int 
     if (lf <= F(0)) return;
     lb.contrib = lb.contrib * lf;
     const F G = F(1) / d2;
+    V3<F> lensC = cs.lensC;
+    if (camDepth == 0) {
+        const bool camAbs = Roughness(cm, cv.disc) > o.roughThr, lgtAbs = Roughness(lm, lv.disc) > o.roughThr;
+        lensC = (camAbs && lgtAbs) ? lb.contrib.mul(cb.contrib) * G : V3<F>(0, 0, 0);
+    } else if (camDepth == 1) {
+        if (Roughness(cm, cv.disc) > o.roughThr) {
+            BsdfOut<F> imp;
+            BsdfEval(cm, true, toL, cs.isect.sn, cs.wi, imp);
+            lensC = lensC.mul(imp.contrib * SNC(true, toL, cs.isect, cs.wi));
+        } else {
+            lensC = V3<F>(0, 0, 0);
+        }
+    }
     const F camPdfA = cb.pdf * lb.cosWo * G;
     const F lgtPdfA = lb.pdf * cb.cosWo * G;
     const F wL = MIS(camPdfA) * (ls.prev + ls.thisw * MIS(lb.revPdf));
@@ -285,13 +344,13 @@ template <class F> inline void ConnectVertex(const SceneT<F> &s, bool occl, int 
     if (score > F(0)) {
         Contrib<F> k;
         k.c = 2 + camDepth; k.l = 2 + lgtDepth; k.screen[0] = screen[0]; k.screen[1] = screen[1];
-        k.contrib = c; k.lsScore = score; k.ssScore = score * ssJ; k.mis = mis;
+        k.contrib = c; k.lsScore = score; k.ssScore = score * ssJ; k.lensScore = Luminance(lensC); k.mis = mis;
         out.push_back(k);
     }
 }
 
 // path.cpp:1430-1478 (bidirectional LightCoordinateSampling)
-template <class F> inline bool LightCoordinateSampling(const SceneT<F> &s, const Opts<F> &o, bool occl, const SurfV<F> &cur,
+template <class F> inline bool LightCoordinateSampling(const SceneT<F> &s, const Opts<F> &o, bool occl, int camDepth, const SurfV<F> &cur,
                                                        const SurfV<F> &nxt, BState<F> &st, V3<F> &dir, V3<F> &bc) {
     const Material<F> &m = MatOf(s, cur.prim);
     const int li = s.triLight[nxt.prim];
@@ -310,6 +369,15 @@ template <class F> inline bool LightCoordinateSampling(const SceneT<F> &s, const
     st.ssJ *= std::fabs(Dot(dir, nn) * (F(1) / d2)) * b.pdf;
     st.thisw = MIS(b.cosWo / b.pdf) * (st.thisw * MIS(b.revPdf) + st.prev);
     st.prev = MIS(F(1) / b.pdf);
+    if (camDepth == 1) {  // path.cpp:1462-1476
+        if (Roughness(m, cur.disc) > o.roughThr) {
+            BsdfOut<F> imp;
+            BsdfEval(m, true, dir, st.isect.sn, st.wi, imp);
+            st.lensC = st.lensC.mul(imp.contrib * SNC(true, dir, st.isect, st.wi));
+        } else {
+            st.lensC = V3<F>(0, 0, 0);
+        }
+    }
     return true;
 }
 
@@ -378,6 +446,7 @@ template <class F> inline void PerturbPathBidir(const SceneT<F> &s, const Opts<F
         EmitFromLight(s, o, pick, p, r, ls);
         r.minT = F(kIsectEps);
         r.maxT = std::numeric_limits<F>::infinity();
+        V3<F> prevLensC(0, 0, 0);
         for (int d = 0; d < (int)p.lgt.size(); d++) {
             SurfV<F> &v = p.lgt[d];
             if (!isectAt(v, r, ls.isect)) return;
@@ -385,14 +454,15 @@ template <class F> inline void PerturbPathBidir(const SceneT<F> &s, const Opts<F
             v.disc = Modulo(v.disc + discOff[VMAX + d], F(1));
             ConvertMIS(r, ls);
             if (d == (int)p.lgt.size() - 1 && p.c == 1) {
-                ConnectToCamera(s, o, occl, d, ls, v, r.org, out);
+                ConnectToCamera(s, o, occl, d, ls, v, prevLensC, r.org, out);
                 return;
             }
             if (d == (int)p.lgt.size() - 1) break;
             Perturb(v.rnd[0], off, k); Perturb(v.rnd[1], off, k);
             V3<F> bc;
-            if (!BSDFSampling(s, o, true, true, ls, v, ls, r.dir, bc)) return;
+            if (!BSDFSampling(s, o, true, true, d, ls, v, ls, r.dir, bc)) return;
             ls.thr = ls.thr * v.rr;
+            prevLensC = ls.lensC;
             r.org = ls.isect.pos;
         }
     }
@@ -416,7 +486,10 @@ template <class F> inline void PerturbPathBidir(const SceneT<F> &s, const Opts<F
         if (!hit) return;
         v.disc = Modulo(v.disc + discOff[d], F(1));
         if (d == 1) {
-            if (DistanceSquared(cs.isect.pos, r.org) <= F(0)) { out.clear(); return; }
+            p.lensPos = cs.isect.pos;
+            const F dd = DistanceSquared(cs.isect.pos, r.org);
+            if (dd <= F(0)) { out.clear(); return; }
+            cs.lensC = cs.lensC * (g_plant == 20 ? F(1) : F(1) / dd);  // 20: planted mistake (tests)
         }
         if (d == (int)p.cam.size() - 1) {
             if (p.l == 1) {
@@ -424,7 +497,7 @@ template <class F> inline void PerturbPathBidir(const SceneT<F> &s, const Opts<F
                 Perturb(v.dlRnd[0], off, k); Perturb(v.dlRnd[1], off, k);
                 DirectLighting(s, o, occl, d, cs, p.screen, pick, v, out);
             } else {
-                ConnectVertex(s, occl, d, (int)p.lgt.size() - 1, ls, p.lgt.back(), cs, v, p.screen, out);
+                ConnectVertex(s, o, occl, d, (int)p.lgt.size() - 1, ls, p.lgt.back(), cs, v, p.screen, out);
             }
             return;
         }
@@ -435,9 +508,167 @@ template <class F> inline void PerturbPathBidir(const SceneT<F> &s, const Opts<F
         }
         V3<F> bc;
         if (lcp) {
-            if (!LightCoordinateSampling(s, o, occl, v, p.cam.back(), cs, r.dir, bc)) return;
+            if (!LightCoordinateSampling(s, o, occl, d, v, p.cam.back(), cs, r.dir, bc)) return;
         } else {
-            if (!BSDFSampling(s, o, false, true, cs, v, cs, r.dir, bc)) return;
+            if (!BSDFSampling(s, o, false, true, d, cs, v, cs, r.dir, bc)) return;
+        }
+        cs.thr = cs.thr * v.rr;
+        r.org = cs.isect.pos;
+        r.minT = F(kIsectEps);
+        r.maxT = std::numeric_limits<F>::infinity();
+    }
+}
+
+// path.cpp:1835-1877 (PerturbLensLastVertex): the light subpath's next-to-last
+// vertex `cur` aims at the point the camera ray through `screen` meets on the
+// last vertex `nxt` (found by a scene query; on nxt's own triangle when
+// fixedPrims). Its bsdfRndParam becomes that direction's spherical coordinates.
+template <class F> inline bool PerturbLensLastVertex(const SceneT<F> &s, const Opts<F> &o, bool occl, bool fixedPrims, const F screen[2],
+                                                     BState<F> &st, SurfV<F> &cur, SurfV<F> &nxt, V3<F> &dir) {
+    const Material<F> &m = MatOf(s, cur.prim);
+    cur.useAbs = Roughness(m, cur.disc) > o.roughThr ? F(1) : F(0);
+    if (cur.useAbs == F(0)) return false;
+    Ray<F> r;
+    SamplePrimary(s.cam, screen[0], screen[1], r);
+    Isect<F> ni;
+    if (fixedPrims) {
+        if (!ReIntersect(s, nxt.prim, r, ni)) return false;
+    } else {
+        if (!Intersect(s, r, ni)) return false;
+        nxt.prim = ni.prim;
+    }
+    dir = ni.pos - st.isect.pos;
+    const F dist = Length(dir);
+    dir = dir * (F(1) / dist);
+    if (occl && Occluded(s, st.isect.pos, dir, dist)) return false;
+    F jac;
+    ToSphericalCoord(dir, cur.rnd[0], cur.rnd[1], jac);
+    BsdfOut<F> b;
+    BsdfEval(m, true, st.wi, st.isect.sn, dir, b);
+    if (b.contrib.isZero() || b.pdf <= F(0)) return false;
+    const F fac = SNC(true, dir, st.isect, st.wi);
+    if (fac <= F(0)) return false;
+    b.contrib = b.contrib * fac;
+    st.lensC = b.contrib;
+    const V3<F> bc = b.contrib * (F(1) / b.pdf);
+    st.ssJ *= b.pdf * std::fabs(std::sin(std::acos(dir.z)));
+    st.thisw = MIS(b.cosWo / b.pdf) * (st.thisw * MIS(b.revPdf) + st.prev);
+    st.prev = MIS(F(1) / b.pdf);
+    st.thr = st.thr.mul(bc);
+    return true;
+}
+// path.cpp:1879-1912 (PerturbLensFirstVertex): the first camera vertex aims at
+// the second one's position of the last evaluation (lensVertexPos).
+template <class F> inline bool PerturbLensFirstVertex(const SceneT<F> &s, const Opts<F> &o, bool occl, const V3<F> &lensPos, BState<F> &st,
+                                                      SurfV<F> &v, V3<F> &dir) {
+    const Material<F> &m = MatOf(s, v.prim);
+    v.useAbs = Roughness(m, v.disc) > o.roughThr ? F(1) : F(0);
+    if (v.useAbs == F(0)) return false;
+    dir = lensPos - st.isect.pos;
+    const F dist = Length(dir);
+    dir = dir * (F(1) / dist);
+    if (occl && Occluded(s, st.isect.pos, dir, dist)) return false;
+    F jac;
+    ToSphericalCoord(dir, v.rnd[0], v.rnd[1], jac);
+    BsdfOut<F> b;
+    BsdfEval(m, false, st.wi, st.isect.sn, dir, b);
+    if (b.contrib.isZero() || b.pdf <= F(0) || std::fabs(b.cosWo) < F(1e-2) || std::fabs(Dot(dir, st.isect.gn)) < F(1e-2)) return false;
+    st.lensC = st.lensC.mul(b.contrib);
+    const V3<F> bc = b.contrib * (F(1) / b.pdf);
+    st.ssJ *= b.pdf * std::fabs(std::sin(std::acos(dir.z)));
+    st.thisw = MIS(b.cosWo / b.pdf) * (st.thisw * MIS(b.revPdf) + st.prev);
+    st.prev = MIS(F(1) / b.pdf);
+    st.thr = st.thr.mul(bc);
+    return true;
+}
+// path.cpp:1914-2067 (PerturbLensBidir): the state re-evaluated with its
+// screen position moved to `screen`. Every vertex keeps its triangle except
+// the one the new camera ray meets (the first camera vertex; for c = 1 the
+// last light vertex), which a scene query finds; the first camera vertex (the
+// next-to-last light vertex for c = 1) then aims at the fixed next vertex.
+// occl = false and fixedPrims = true give the function whose derivatives in
+// `screen` the reference's lens Gaussian takes (PathFuncMode::Lens,
+// path.cpp:2987-3175).
+template <class F> inline void PerturbLensBidir(const SceneT<F> &s, const Opts<F> &o, const F screen[2], PathS<F> &p,
+                                                std::vector<Contrib<F>> &out, bool occl = true, bool fixedPrims = false) {
+    BState<F> ls;
+    if (p.l > 1) {
+        const F pick = PickLightProb(s, p.lLight);
+        Ray<F> r;
+        EmitFromLight(s, o, pick, p, r, ls);
+        r.minT = F(kIsectEps);
+        r.maxT = std::numeric_limits<F>::infinity();
+        V3<F> prevLensC(0, 0, 0);
+        const int nl = (int)p.lgt.size();
+        for (int d = 0; d < nl; d++) {
+            SurfV<F> &v = p.lgt[d];
+            if (!ReIntersect(s, v.prim, r, ls.isect)) return;
+            ls.wi = -r.dir;
+            ConvertMIS(r, ls);
+            if (d == nl - 1 && p.c == 1) {
+                ConnectToCamera(s, o, occl, d, ls, v, prevLensC, r.org, out);
+                return;
+            }
+            if (d == nl - 1) break;
+            V3<F> bc;
+            if (d == nl - 2 && p.c == 1) {
+                if (!PerturbLensLastVertex(s, o, occl, fixedPrims, screen, ls, v, p.lgt[nl - 1], r.dir)) return;
+            } else {
+                if (!BSDFSampling(s, o, true, true, d, ls, v, ls, r.dir, bc)) return;
+            }
+            ls.thr = ls.thr * v.rr;
+            prevLensC = ls.lensC;
+            r.org = ls.isect.pos;
+        }
+    }
+    p.screen[0] = screen[0];
+    p.screen[1] = screen[1];
+    Ray<F> r;
+    BState<F> cs;
+    EmitFromCamera(s, p.screen, r, cs);
+    bool lcp = false;
+    const int nc = (int)p.cam.size();
+    for (int d = 0; d < nc; d++) {
+        SurfV<F> &v = p.cam[d];
+        bool hit = true;
+        if (d == 0 && !fixedPrims) {
+            hit = Intersect(s, r, cs.isect);
+            v.prim = hit ? cs.isect.prim : -1;
+        } else if (v.prim < 0) {
+            hit = false;
+        } else if (!ReIntersect(s, v.prim, r, cs.isect)) {
+            return;
+        }
+        cs.wi = -r.dir;
+        if (hit) ConvertMIS(r, cs);
+        if (d == nc - 1 && p.l == 0) {
+            const int li = hit ? s.triLight[v.prim] : -1;
+            if (li >= 0) HandleHitLight(s, d, li, r, p.screen, cs, out);
+            return;
+        }
+        if (!hit) return;
+        if (d == 1) {
+            p.lensPos = cs.isect.pos;
+            const F dd = DistanceSquared(cs.isect.pos, r.org);
+            if (dd <= F(0)) { out.clear(); return; }
+            cs.lensC = cs.lensC * (g_plant == 20 ? F(1) : F(1) / dd);  // 20: planted mistake (tests)
+        }
+        if (d == nc - 1) {
+            if (p.l == 1) DirectLighting(s, o, false, d, cs, p.screen, PickLightProb(s, v.dlLight), v, out);
+            else ConnectVertex(s, o, occl && d == 0, d, (int)p.lgt.size() - 1, ls, p.lgt.back(), cs, v, p.screen, out);
+            return;
+        }
+        if (d == nc - 2 && p.l == 0) {
+            const int nprim = p.cam.back().prim;
+            if (nprim >= 0 && s.triLight[nprim] >= 0) lcp = true;
+        }
+        V3<F> bc;
+        if (d == 0) {
+            if (!PerturbLensFirstVertex(s, o, occl, p.lensPos, cs, v, r.dir)) return;
+        } else if (lcp) {
+            if (!LightCoordinateSampling(s, o, false, d, v, p.cam.back(), cs, r.dir, bc)) return;
+        } else {
+            if (!BSDFSampling(s, o, false, true, d, cs, v, cs, r.dir, bc)) return;
         }
         cs.thr = cs.thr * v.rr;
         r.org = cs.isect.pos;
@@ -478,6 +709,7 @@ template <class F> inline void GenerateSubpath(const SceneT<F> &s, const Opts<F>
         EmitFromLight(s, o, pick, p, r, ls);
         r.minT = F(kIsectEps);
         r.maxT = std::numeric_limits<F>::infinity();
+        V3<F> prevLensC(0, 0, 0);
         for (int d = 0;; d++) {
             p.lgt.push_back(SurfV<F>());
             SurfV<F> &v = p.lgt.back();
@@ -488,15 +720,16 @@ template <class F> inline void GenerateSubpath(const SceneT<F> &s, const Opts<F>
             ConvertMIS(r, ls);
             if (d + 2 == lgtLen) {
                 if (camLen == 1) {
-                    ConnectToCamera(s, o, true, d, ls, v, r.org, out);
+                    ConnectToCamera(s, o, true, d, ls, v, prevLensC, r.org, out);
                     return;
                 }
                 break;
             }
             v.rnd[0] = g.uni(); v.rnd[1] = g.uni();
             V3<F> bc;
-            if (!BSDFSampling(s, o, true, false, ls, v, ls, r.dir, bc)) return;
+            if (!BSDFSampling(s, o, true, false, d, ls, v, ls, r.dir, bc)) return;
             v.rr = 1;
+            prevLensC = ls.lensC;
             r.org = ls.isect.pos;
         }
     }
@@ -526,7 +759,12 @@ template <class F> inline void GenerateSubpath(const SceneT<F> &s, const Opts<F>
             return;
         }
         if (!hit) return;
-        if (d == 1 && DistanceSquared(cs.isect.pos, r.org) <= F(0)) return;
+        if (d == 1) {
+            p.lensPos = cs.isect.pos;
+            const F dd = DistanceSquared(cs.isect.pos, r.org);
+            if (dd <= F(0)) return;
+            cs.lensC = cs.lensC * (g_plant == 20 ? F(1) : F(1) / dd);  // 20: planted mistake (tests)
+        }
         v.disc = g.uni();
         if (d + 2 == camLen) {
             if (lgtLen == 1) {
@@ -534,13 +772,13 @@ template <class F> inline void GenerateSubpath(const SceneT<F> &s, const Opts<F>
                 DirectLightingInit(s, v, pick, g);
                 DirectLighting(s, o, true, d, cs, p.screen, pick, v, out);
             } else {
-                ConnectVertex(s, true, d, lgtLen - 2, ls, p.lgt.back(), cs, v, p.screen, out);
+                ConnectVertex(s, o, true, d, lgtLen - 2, ls, p.lgt.back(), cs, v, p.screen, out);
             }
             return;
         }
         v.rnd[0] = g.uni(); v.rnd[1] = g.uni();
         V3<F> bc;
-        if (!BSDFSampling(s, o, false, false, cs, v, cs, r.dir, bc)) return;
+        if (!BSDFSampling(s, o, false, false, d, cs, v, cs, r.dir, bc)) return;
         v.rr = 1;
         r.org = cs.isect.pos;
         r.minT = F(kIsectEps);
@@ -571,6 +809,7 @@ template <class F> inline void GeneratePathBidir(const SceneT<F> &s, const Opts<
     EmitFromLight(s, o, pick, p, r, lsv[0]);
     r.minT = F(kIsectEps);
     r.maxT = std::numeric_limits<F>::infinity();
+    V3<F> prevLensC(0, 0, 0);
     for (int d = 0;; d++) {
         p.lgt.push_back(SurfV<F>());
         SurfV<F> &v = p.lgt.back();
@@ -581,14 +820,15 @@ template <class F> inline void GeneratePathBidir(const SceneT<F> &s, const Opts<
         lsv[d].wi = -r.dir;
         ConvertMIS(r, lsv[d]);
         if (d + 2 >= minDepth) {
-            if (!ConnectToCamera(s, o, true, d, lsv[d], v, r.org, out)) {}
+            if (!ConnectToCamera(s, o, true, d, lsv[d], v, prevLensC, r.org, out)) {}
         }
         if (maxDepth != -1 && d + 2 >= maxDepth) break;
         lsv.push_back(BState<F>());
         v.rnd[0] = g.uni(); v.rnd[1] = g.uni();
         V3<F> bc;
-        if (!BSDFSampling(s, o, true, false, lsv[d], v, lsv[d + 1], r.dir, bc)) { lsv.pop_back(); break; }
+        if (!BSDFSampling(s, o, true, false, d, lsv[d], v, lsv[d + 1], r.dir, bc)) { lsv.pop_back(); break; }
         if (!RussianRoulette(d, bc, v.rr, lsv[d + 1].thr, g)) { lsv.pop_back(); break; }
+        prevLensC = lsv[d + 1].lensC;
         r.org = lsv[d].isect.pos;
     }
     BState<F> cs;
@@ -617,7 +857,12 @@ template <class F> inline void GeneratePathBidir(const SceneT<F> &s, const Opts<
             }
         }
         if (!hit || (maxDepth != -1 && d + 1 >= maxDepth)) break;
-        if (d == 1 && DistanceSquared(cs.isect.pos, r.org) <= F(0)) { out.clear(); return; }
+        if (d == 1) {
+            p.lensPos = cs.isect.pos;
+            const F dd = DistanceSquared(cs.isect.pos, r.org);
+            if (dd <= F(0)) { out.clear(); return; }
+            cs.lensC = cs.lensC * (g_plant == 20 ? F(1) : F(1) / dd);  // 20: planted mistake (tests)
+        }
         v.disc = g.uni();
         if (d + 2 >= minDepth) {
             F dpick = 1;
@@ -626,10 +871,10 @@ template <class F> inline void GeneratePathBidir(const SceneT<F> &s, const Opts<
         }
         int maxL = maxDepth == -1 ? (int)lsv.size() - 1 : std::min(maxDepth - d - 3, (int)lsv.size() - 1);
         for (int ld = 0; ld <= maxL; ld++)
-            if (d + ld + 3 >= minDepth) ConnectVertex(s, true, d, ld, lsv[ld], p.lgt[ld], cs, v, p.screen, out);
+            if (d + ld + 3 >= minDepth) ConnectVertex(s, o, true, d, ld, lsv[ld], p.lgt[ld], cs, v, p.screen, out);
         v.rnd[0] = g.uni(); v.rnd[1] = g.uni();
         V3<F> bc;
-        if (!BSDFSampling(s, o, false, false, cs, v, cs, r.dir, bc)) break;
+        if (!BSDFSampling(s, o, false, false, d, cs, v, cs, r.dir, bc)) break;
         if (!RussianRoulette(d, bc, v.rr, cs.thr, g)) break;
         r.org = cs.isect.pos;
         r.minT = F(kIsectEps);

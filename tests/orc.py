@@ -135,6 +135,35 @@ class Scene:
         lib().orc_logf_fixed(C.c_void_p(self.h), n, P(recs), P(off), int(chad_quirk), P(out))
         return out
 
+    def lens_score(self, recs, dbl=True):
+        """lensScore and lensVertexPos of each state's fixed re-evaluation: n x 4."""
+        recs = np.ascontiguousarray(recs, PATH_REC)
+        out = np.zeros((len(recs), 4), np.float32)
+        lib().orc_lens_score(C.c_void_p(self.h), int(dbl), len(recs), P(recs), P(out))
+        return out
+
+    def generate_subpath_lens(self, cl, uni, dbl=True):
+        """GenerateSubpath's own (tracked) lensScore per sample."""
+        cl = np.ascontiguousarray(cl, np.int32); uni = np.ascontiguousarray(uni, np.float32)
+        out = np.zeros(len(cl), np.float32)
+        lib().orc_generate_subpath_lens(C.c_void_p(self.h), int(dbl), len(cl), P(cl), P(uni), uni.shape[1], P(out))
+        return out
+
+    def perturb_lens(self, recs, screen, dbl=True, occl=True, fixed=False):
+        """PerturbLensBidir to screen positions (n x 2): states, c9 = contribution 8 + lensScore."""
+        recs = np.ascontiguousarray(recs, PATH_REC); n = len(recs)
+        screen = np.ascontiguousarray(screen, np.float32).reshape(n, 2)
+        out = np.zeros(n, PATH_REC); c9 = np.zeros((n, 9), np.float32)
+        lib().orc_perturb_lens(C.c_void_p(self.h), int(dbl), n, P(recs), P(screen), int(occl), int(fixed), P(out), P(c9))
+        return out, c9
+
+    def lens_grad_hess_fd(self, recs, screen, h=1e-4):
+        recs = np.ascontiguousarray(recs, PATH_REC); n = len(recs)
+        screen = np.ascontiguousarray(screen, np.float32).reshape(n, 2)
+        logl = np.zeros(n); g = np.zeros((n, 2)); H = np.zeros((n, 2, 2))
+        lib().orc_lens_grad_hess_fd(C.c_void_p(self.h), n, P(recs), P(screen), C.c_double(h), P(logl), P(g), P(H))
+        return logl, g, H
+
     def direct_pass(self, spp, seed=1, threads=8):
         film = np.zeros((self.H, self.W, 3), np.float32)
         lib().orc_direct_pass(C.c_void_p(self.h), spp, C.c_uint64(seed), threads, P(film))
@@ -151,15 +180,16 @@ class Scene:
     def render(self, seeds, c8, length_func, b, iters, opts=None, threads=8, seed=7):
         o = {"type": 0, "largestepprob": 0.3, "perturbstddev": 0.01, "anisoperturbprob": 0.5, "anisostddev": 0.0025,
              "discretestddev": 0.01, "roughthr": 0.03, "maxdervdepth": 9, "acceptancemap": 0, "chadquirk": 1, "fdstep": 2e-6,
-             "secondstage": 1, "h2mc": 0, "uniformmixing": 0.2}
+             "secondstage": 1, "h2mc": 0, "uniformmixing": 0.2, "lensperturbprob": 0.0, "lensperturbstddev": 0.01}
         o.update(opts or {})
         ov = np.array([o["type"], o["largestepprob"], o["perturbstddev"], o["anisoperturbprob"], o["anisostddev"],
                        o["discretestddev"], o["roughthr"], o["maxdervdepth"], o["acceptancemap"], o["chadquirk"],
-                       o["fdstep"], o["secondstage"], o["h2mc"], o["uniformmixing"]],
+                       o["fdstep"], o["secondstage"], o["h2mc"], o["uniformmixing"], o["lensperturbprob"],
+                       o["lensperturbstddev"]],
                       np.float64)
         seeds = np.ascontiguousarray(seeds, PATH_REC); c8 = np.ascontiguousarray(c8, np.float32)
         lf = np.ascontiguousarray(length_func, np.float32)
-        film = np.zeros((self.H, self.W, 3), np.float32); stats = np.zeros(8, np.uint64); sec = np.zeros(1)
+        film = np.zeros((self.H, self.W, 3), np.float32); stats = np.zeros(10, np.uint64); sec = np.zeros(1)
         lib().orc_render(C.c_void_p(self.h), P(ov), P(seeds), P(c8), len(seeds), P(lf), len(lf), C.c_float(b),
                          C.c_int64(iters), threads, C.c_uint64(seed), P(film), P(stats), P(sec))
         return film, stats, float(sec[0])

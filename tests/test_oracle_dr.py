@@ -121,7 +121,7 @@ def test_stats_structure(small):
     b, lf, seeds, c8, _ = s.prepass(20000, 5, 64, seed=4)
     iters = 2000
     film, st, sec = s.render(seeds, c8, lf, b, iters, threads=4, seed=9)
-    large, large_acc, s1, s1_acc, iso, iso_acc, h2, h2_acc = (int(x) for x in st)
+    large, large_acc, s1, s1_acc, iso, iso_acc, h2, h2_acc = (int(x) for x in st[:8])
     total = 64 * iters
     assert large + s1 == total
     assert large / total == pytest.approx(0.3, abs=0.02)  # largestepprob
