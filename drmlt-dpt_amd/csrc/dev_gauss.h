@@ -27,16 +27,17 @@
 namespace drmlt {
 
 constexpr int kGMax = 24;          // largest D with derivatives (path length 12)
-constexpr int kSlotP = kGMax * kGMax, kSlotM = kSlotP + kGMax, kSlotLD = kSlotM + kGMax;
 constexpr int kGSlot = 640;        // doubles per Gaussian slot (V, p, mean, logDet, padding to 64 B)
 
-struct GSlot {
+// A Gaussian of dimension up to DM stored at b: V (DM x DM), p, mean, logDet.
+template <int DM> struct GSlotT {
     double *b;
-    DEV double &V(int i, int k) const { return b[i * kGMax + k]; }
-    DEV double &p(int k) const { return b[kSlotP + k]; }
-    DEV double &m(int i) const { return b[kSlotM + i]; }
-    DEV double &ld() const { return b[kSlotLD]; }
+    DEV double &V(int i, int k) const { return b[i * DM + k]; }
+    DEV double &p(int k) const { return b[DM * DM + k]; }
+    DEV double &m(int i) const { return b[DM * DM + DM + i]; }
+    DEV double &ld() const { return b[DM * DM + 2 * DM]; }
 };
+using GSlot = GSlotT<kGMax>;
 
 // drsmallstep.h:26-42; posOffset is 0.5 * (e^L + e^-L - 1) as computed there
 struct H2Param {
@@ -103,7 +104,7 @@ DEV void rr_pair(int D, int r, int k, int &p, int &q) {
 }
 
 // gaussian.cpp:4-16 in factored form: V = I, p = 1/sigma^2, mean 0.
-template <class X> DEV void gauss_iso(const X &x, int D, double sigma, const GSlot &g) {
+template <class X, class GS> DEV void gauss_iso(const X &x, int D, double sigma, const GS &g) {
 #pragma clang fp contract(off)
     for (int t = x.lane; t < D * D; t += x.n) g.V(t / D, t % D) = (t / D == t % D) ? 1.0 : 0.0;
     for (int i = x.lane; i < D; i += x.n) {
@@ -184,8 +185,8 @@ template <class X, class W> DEV void gauss_eigen(const X &x, int D, W &w) {
 // drsmallstep.cpp:219-266 + 18-93. grad(i) and hess(i, j) for i <= j (the
 // upper triangle of the row-major Hessian; Eigen reads the lower triangle of
 // the column-major view of the same buffer). Writes the Gaussian to g.
-template <class X, class GP, class HP, class W>
-DEV void gauss_compute(const X &x, const H2Param &pr, double sc, int D, GP grad, HP hess, W &w, const GSlot &g) {
+template <class X, class GP, class HP, class W, class GS>
+DEV void gauss_compute(const X &x, const H2Param &pr, double sc, int D, GP grad, HP hess, W &w, const GS &g) {
 #pragma clang fp contract(off)
     constexpr int kGS = W::kS;
     for (int t = x.lane; t < D * D; t += x.n) {
@@ -251,8 +252,8 @@ DEV void gauss_compute(const X &x, const H2Param &pr, double sc, int D, GP grad,
 
 // GenerateSample (gaussian.cpp:36-47): x = mean + covL n with
 // covL = V diag(p^-1/2); nrm(j) supplies the j-th standard normal.
-template <class X, class NP, class W, class OW>
-DEV void gauss_sample(const X &x, int D, const GSlot &g, NP nrm, W &w, OW out) {
+template <class X, class GS, class NP, class W, class OW>
+DEV void gauss_sample(const X &x, int D, const GS &g, NP nrm, W &w, OW out) {
 #pragma clang fp contract(off)
     for (int j = x.lane; j < D; j += x.n) w.vec[j] = nrm(j) / sqrt(g.p(j));
     x.sync();
@@ -266,8 +267,8 @@ DEV void gauss_sample(const X &x, int D, const GSlot &g, NP nrm, W &w, OW out) {
 
 // GaussianLogPdf (gaussian.cpp:18-26) of sign * xv(i):
 // -D/2 log 2pi + logDet/2 - 1/2 sum_k p_k (V^T (x - mean))_k^2.
-template <class X, class XP, class W>
-DEV double gauss_logpdf(const X &x, int D, XP xv, double sign, const GSlot &g, W &w) {
+template <class X, class XP, class W, class GS>
+DEV double gauss_logpdf(const X &x, int D, XP xv, double sign, const GS &g, W &w) {
 #pragma clang fp contract(off)
     for (int i = x.lane; i < D; i += x.n) w.vec[i] = sign * xv(i) - g.m(i);
     x.sync();

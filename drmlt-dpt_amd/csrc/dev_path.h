@@ -13,6 +13,12 @@
 // the triangles of the state are re-intersected instead of traced and no
 // occlusion tests are made (the function the reference's AD twin
 // differentiates, path.cpp:2987-3175).
+//
+// With the template flag LENS the evaluation also keeps the reference's lens
+// bookkeeping (lensContrib per subpath state, the contribution's lensScore
+// and lensVertexPos, path.cpp:433-976, 1430-1478): the h2mc integrator's lens
+// mutation reads it. Without it (the DR loop's kernels) none of that code is
+// compiled in.
 #pragma once
 #include "dev_scene.h"
 #include "dev_bsdf.h"
@@ -45,6 +51,8 @@ template <class T> struct POut {
     float screen[2];
     float mis;
     int c, l;
+    T lensScore;         // LENS evaluations only
+    V3<float> lensPos;   // LENS evaluations only: the second camera vertex (lensVertexPos)
 };
 
 // path.cpp:433-444
@@ -55,6 +63,7 @@ template <class T> struct BState {
     V3<T> thr;
     T ssJ;
     float lcJ;
+    V3<T> lensC;  // lensContrib (LENS evaluations only)
 };
 
 template <class T> DEV T MIS(const T &p) { return p * p; }
@@ -100,6 +109,7 @@ template <class T> DEV void emit_from_camera(const DevScene &S, const T &sx, con
     st.prev = MIS(T(npix) / pdfW);
     st.thisw = T(0);
     st.ssJ = T(1);
+    st.lensC = vconst<T>(1, 1, 1);
 }
 // path.cpp:489-505 + arealight.cpp:77-104
 template <class T> DEV void emit_from_light(const DevScene &S, const Opts &o, int li, int prim, const T &p0, const T &p1, const T &d0,
@@ -124,6 +134,7 @@ template <class T> DEV void emit_from_light(const DevScene &S, const Opts &o, in
     st.prev = MIS(dPdf / emPdf);
     st.thisw = o.chadMisQuirk ? T(0) : MIS(cosL / emPdf);
     st.ssJ = T(1);
+    st.lensC = vconst<T>(0, 0, 0);
 }
 // path.cpp:507-517
 template <class T> DEV void convert_mis(const RayT<T> &r, BState<T> &st) {
@@ -143,9 +154,12 @@ template <class T> DEV void emit(POut<T> &out, int c, int l, const V3<T> &contri
     out.mis = mis;
 }
 
-// path.cpp:519-613. Returns false when the reference clears the contribution list.
-template <class T, bool FIXED> DEV bool connect_to_camera(const DevScene &S, const Opts &o, int lgtDepth, const BState<T> &st, int prim,
-                                                          float disc, const V3<T> &prevPos, POut<T> &out, ShadowQ *sh = nullptr) {
+// path.cpp:519-613. Returns false when the reference clears the contribution
+// list. LENS: prevLensC is the lensContrib of the light-subpath state before
+// this vertex.
+template <class T, bool FIXED, bool LENS = false>
+DEV bool connect_to_camera(const DevScene &S, const Opts &o, int lgtDepth, const BState<T> &st, int prim, float disc, const V3<T> &prevPos,
+                           POut<T> &out, ShadowQ *sh = nullptr, const V3<T> *prevLensC = nullptr) {
     RayT<T> cr;
     sample_primary(S.cam, T(0.5f), T(0.5f), cr);
     V3<T> toCam = cr.org - st.isect.pos;
@@ -162,8 +176,16 @@ template <class T, bool FIXED> DEV bool connect_to_camera(const DevScene &S, con
     T fac = snc(true, st.wi, st.isect, toCam);
     if (val(fac) <= 0) return true;
     b.contrib = b.contrib * fac;
+    V3<T> lensC;
+    if constexpr (LENS) lensC = vconst<T>(0, 0, 0);
     if (roughness(m, disc) > o.roughThr && lgtDepth >= 1) {
-        if (val(length2(st.isect.pos - prevPos)) <= 0) { out.valid = false; return false; }
+        const T dd = length2(st.isect.pos - prevPos);
+        if (val(dd) <= 0) { out.valid = false; return false; }
+        if constexpr (LENS) {
+            BOut<T> e;
+            bsdf_eval(m, false, toCam, st.isect.sn, st.wi, e);
+            lensC = mul(e.contrib, *prevLensC) * (T(1) / dd);
+        }
     }
     const T cosAtCam = -dot(cr.dir, toCam);
     const T ipcd = T(S.cam.dist) / cosAtCam;
@@ -175,15 +197,20 @@ template <class T, bool FIXED> DEV bool connect_to_camera(const DevScene &S, con
     const T surf2img = b.cosWo / i2surf;
     V3<T> c = mul((b.contrib * mis) * (T(1) / (T(npix) * surf2img)), st.thr);
     const T score = luminance(c);
-    if (val(score) > 0) emit(out, 1, 2 + lgtDepth, c, score, st.ssJ, float(val(sx)), float(val(sy)), float(val(mis)));
+    if (val(score) > 0) {
+        emit(out, 1, 2 + lgtDepth, c, score, st.ssJ, float(val(sx)), float(val(sy)), float(val(mis)));
+        if constexpr (LENS) out.lensScore = luminance(lensC);
+    }
     return true;
 }
 
 // path.cpp:615-724 (BSDFSampling<adjoint, perturb>); cur and next may alias.
 // rnd holds bsdfRndParam; for a generated absolute-parameterised vertex it is
-// replaced by the spherical coordinates of the sampled direction.
-template <class T> DEV bool bsdf_sampling(const DevScene &S, const Opts &o, bool adjoint, bool perturb, const BState<T> &cur, int prim,
-                                          float disc, T &rnd0, T &rnd1, BState<T> &next, V3<T> &dir, V3<T> &bcOut) {
+// replaced by the spherical coordinates of the sampled direction. LENS:
+// lensContrib at subpath vertex `depth` (path.cpp:664-711).
+template <class T, bool LENS = false>
+DEV bool bsdf_sampling(const DevScene &S, const Opts &o, bool adjoint, bool perturb, const BState<T> &cur, int prim, float disc, T &rnd0, T &rnd1,
+                       BState<T> &next, V3<T> &dir, V3<T> &bcOut, int depth = 0) {
     const Mat m = mat_of(S, prim);
     BOut<T> b;
     const bool useAbs = roughness(m, disc) > o.roughThr;
@@ -216,6 +243,30 @@ template <class T> DEV bool bsdf_sampling(const DevScene &S, const Opts &o, bool
     }
     T fac = snc(adjoint, cur.wi, cur.isect, dir);
     if (val(fac) <= 0) return false;
+    if constexpr (LENS) {
+        // bc is still contrib / pdf here
+        const bool grazing = fabsf(float(val(b.cosWo))) < 1e-2f || fabsf(float(val(dot(dir, cur.isect.gn)))) < 1e-2f;
+        V3<T> lc;
+        if (useAbs) {
+            if (adjoint) {
+                lc = grazing ? vconst<T>(0, 0, 0) : bc * b.pdf;
+            } else if (depth == 0) {
+                lc = grazing ? vconst<T>(0, 0, 0) : mul(cur.lensC, bc * b.pdf);
+            } else if (depth == 1) {
+                BOut<T> imp;
+                bsdf_eval(m, true, dir, cur.isect.sn, cur.wi, imp);
+                if (fabsf(float(val(imp.cosWo))) < 1e-2f || fabsf(float(val(dot(cur.wi, cur.isect.gn)))) < 1e-2f)
+                    lc = vconst<T>(0, 0, 0);
+                else
+                    lc = mul(cur.lensC, imp.contrib * snc(true, dir, cur.isect, cur.wi));
+            } else {
+                lc = cur.lensC;
+            }
+        } else {
+            lc = (adjoint || depth <= 1) ? vconst<T>(0, 0, 0) : cur.lensC;
+        }
+        next.lensC = adjoint ? lc * fac : lc;
+    }
     bc = bc * fac;
     bcOut = bc;
     const T newThis = MIS(b.cosWo / b.pdf) * (cur.thisw * MIS(b.revPdf) + cur.prev);
@@ -228,9 +279,10 @@ template <class T> DEV bool bsdf_sampling(const DevScene &S, const Opts &o, bool
 }
 
 // path.cpp:726-810 (bidirectional DirectLighting)
-template <class T, bool FIXED> DEV void direct_lighting(const DevScene &S, const Opts &o, int camDepth, const BState<T> &st, float sx,
-                                                        float sy, int li, int lprim, const T &r0, const T &r1, int prim, POut<T> &out,
-                                                        ShadowQ *sh = nullptr) {
+// LENS: disc is the vertex's bsdfDiscrete.
+template <class T, bool FIXED, bool LENS = false>
+DEV void direct_lighting(const DevScene &S, const Opts &o, int camDepth, const BState<T> &st, float sx, float sy, int li, int lprim, const T &r0,
+                         const T &r1, int prim, POut<T> &out, ShadowQ *sh = nullptr, float disc = 0.0f) {
     const float pick = pick_light_prob(S, li);
     const Mat m = mat_of(S, prim);
     V3<T> toL, lc;
@@ -249,11 +301,26 @@ template <class T, bool FIXED> DEV void direct_lighting(const DevScene &S, const
     const T mis = T(1) / (wL + T(1) + wC);
     c = c * mis;
     const T score = luminance(c);
-    if (val(score) > 0) emit(out, 2 + camDepth, 1, c, score, st.ssJ, sx, sy, float(val(mis)));
+    if (val(score) > 0) {
+        emit(out, 2 + camDepth, 1, c, score, st.ssJ, sx, sy, float(val(mis)));
+        if constexpr (LENS) {
+            V3<T> lensC = st.lensC;
+            if (camDepth == 1) {
+                if (roughness(m, disc) > o.roughThr) {
+                    BOut<T> imp;
+                    bsdf_eval(m, true, toL, st.isect.sn, st.wi, imp);
+                    lensC = mul(lensC, imp.contrib * snc(true, toL, st.isect, st.wi));
+                } else {
+                    lensC = vconst<T>(0, 0, 0);
+                }
+            }
+            out.lensScore = camDepth >= 1 ? luminance(lensC) : T(0);
+        }
+    }
 }
 // path.cpp:812-860 (bidirMIS = true)
-template <class T> DEV void handle_hit_light(const DevScene &S, int camDepth, int li, const RayT<T> &r, float sx, float sy, const BState<T> &st,
-                                             POut<T> &out) {
+template <class T, bool LENS = false>
+DEV void handle_hit_light(const DevScene &S, int camDepth, int li, const RayT<T> &r, float sx, float sy, const BState<T> &st, POut<T> &out) {
     V3<T> em;
     T dPdf, emPdf;
     light_emission(S, li, r.dir, st.isect.sn, em, dPdf, emPdf);
@@ -270,13 +337,18 @@ template <class T> DEV void handle_hit_light(const DevScene &S, int camDepth, in
             misf = float(val(mis));
         }
         T score = luminance(c);
-        if (val(score) > 0) emit(out, 2 + camDepth, 0, c, score, st.ssJ, sx, sy, misf);
+        if (val(score) > 0) {
+            emit(out, 2 + camDepth, 0, c, score, st.ssJ, sx, sy, misf);
+            if constexpr (LENS) out.lensScore = camDepth >= 2 ? luminance(st.lensC) : T(0);
+        }
     }
 }
 // path.cpp:862-976
-template <class T, bool FIXED> DEV void connect_vertex(const DevScene &S, int camDepth, int lgtDepth, const BState<T> &ls, int lprimV,
-                                                       const BState<T> &cs, int cprimV, float sx, float sy, POut<T> &out,
-                                                       ShadowQ *sh = nullptr) {
+// LENS: o, cdisc and ldisc (the two vertices' bsdfDiscrete) decide their
+// parameterisations.
+template <class T, bool FIXED, bool LENS = false>
+DEV void connect_vertex(const DevScene &S, int camDepth, int lgtDepth, const BState<T> &ls, int lprimV, const BState<T> &cs, int cprimV, float sx,
+                        float sy, POut<T> &out, ShadowQ *sh = nullptr, const Opts *o = nullptr, float cdisc = 0.0f, float ldisc = 0.0f) {
     V3<T> toL = ls.isect.pos - cs.isect.pos;
     const T d2 = length2(toL), d = s_sqrt(d2);
     toL = toL * (T(1) / d);
@@ -305,11 +377,32 @@ template <class T, bool FIXED> DEV void connect_vertex(const DevScene &S, int ca
     c = c * mis;
     const T ssJ = ls.ssJ * cs.ssJ;
     const T score = luminance(c);
-    if (val(score) > 0) emit(out, 2 + camDepth, 2 + lgtDepth, c, score, ssJ, sx, sy, float(val(mis)));
+    if (val(score) > 0) {
+        emit(out, 2 + camDepth, 2 + lgtDepth, c, score, ssJ, sx, sy, float(val(mis)));
+        if constexpr (LENS) {
+            V3<T> lensC = cs.lensC;
+            if (camDepth == 0) {
+                const bool camAbs = roughness(cm, cdisc) > o->roughThr, lgtAbs = roughness(lm, ldisc) > o->roughThr;
+                lensC = (camAbs && lgtAbs) ? mul(lb.contrib, cb.contrib) * G : vconst<T>(0, 0, 0);
+            } else if (camDepth == 1) {
+                if (roughness(cm, cdisc) > o->roughThr) {
+                    BOut<T> imp;
+                    bsdf_eval(cm, true, toL, cs.isect.sn, cs.wi, imp);
+                    lensC = mul(lensC, imp.contrib * snc(true, toL, cs.isect, cs.wi));
+                } else {
+                    lensC = vconst<T>(0, 0, 0);
+                }
+            }
+            out.lensScore = luminance(lensC);
+        }
+    }
 }
 // path.cpp:1430-1478 (bidirectional LightCoordinateSampling)
-template <class T, bool FIXED> DEV bool light_coordinate_sampling(const DevScene &S, int prim, const T &r0, const T &r1, int nprim,
-                                                                  BState<T> &st, V3<T> &dir, ShadowQ *sh = nullptr) {
+// LENS: lensContrib at camera vertex camDepth (path.cpp:1462-1476); o and
+// disc (the vertex's bsdfDiscrete) decide its parameterisation.
+template <class T, bool FIXED, bool LENS = false>
+DEV bool light_coordinate_sampling(const DevScene &S, int prim, const T &r0, const T &r1, int nprim, BState<T> &st, V3<T> &dir,
+                                   ShadowQ *sh = nullptr, int camDepth = -1, const Opts *o = nullptr, float disc = 0.0f) {
     const Mat m = mat_of(S, prim);
     const int li = S.tlight[nprim];
     V3<T> np, nn;
@@ -327,6 +420,17 @@ template <class T, bool FIXED> DEV bool light_coordinate_sampling(const DevScene
     st.ssJ = st.ssJ * (s_fabs(dot(dir, nn) * (T(1) / d2)) * b.pdf);
     st.thisw = MIS(b.cosWo / b.pdf) * (st.thisw * MIS(b.revPdf) + st.prev);
     st.prev = MIS(T(1) / b.pdf);
+    if constexpr (LENS) {
+        if (camDepth == 1) {
+            if (roughness(m, disc) > o->roughThr) {
+                BOut<T> imp;
+                bsdf_eval(m, true, dir, st.isect.sn, st.wi, imp);
+                st.lensC = mul(st.lensC, imp.contrib * snc(true, dir, st.isect, st.wi));
+            } else {
+                st.lensC = vconst<T>(0, 0, 0);
+            }
+        }
+    }
     return true;
 }
 
@@ -393,7 +497,8 @@ struct DiscNone {
 // U yields the perturbed primary samples in GetPathPos order; DISC(slot) gives
 // the discrete offset of camera vertex slot d (slot d) or light vertex d (slot
 // kVMax + d). FIXED: re-intersect the state's triangles, no occlusion, no writes.
-template <class T, bool FIXED, class U, class DISC>
+// LENS: also out.lensScore and out.lensPos.
+template <class T, bool FIXED, class U, class DISC, bool LENS = false>
 DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &uprov, const DISC &disc, POut<T> &out) {
     out.valid = false;
     const int c = *ref.c, l = *ref.l, s = ref.s, vs = ref.vs;
@@ -407,6 +512,8 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
         emit_from_light(S, o, li, lp, p0, p1, d0, d1, r, ls);
         r.minT = kIsectEps;
         r.maxT = __builtin_inff();
+        V3<T> prevLensC;
+        if constexpr (LENS) prevLensC = vconst<T>(0, 0, 0);
         for (int d = 0; d < nl; d++) {
             int prim;
             if (FIXED) {
@@ -423,15 +530,16 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
             convert_mis(r, ls);
             lastLightPrim = prim;
             if (d == nl - 1 && c == 1) {
-                connect_to_camera<T, FIXED>(S, o, d, ls, prim, dv, r.org, out);
+                connect_to_camera<T, FIXED, LENS>(S, o, d, ls, prim, dv, r.org, out, nullptr, &prevLensC);
                 if (out.valid && !FIXED) { ref.screen[0] = out.screen[0]; ref.screen[s] = out.screen[1]; }
                 return;
             }
             if (d == nl - 1) break;
             T r0 = uprov.next(), r1 = uprov.next();
             V3<T> dir, bc;
-            if (!bsdf_sampling(S, o, true, true, ls, prim, dv, r0, r1, ls, dir, bc)) return;
+            if (!bsdf_sampling<T, LENS>(S, o, true, true, ls, prim, dv, r0, r1, ls, dir, bc, d)) return;
             ls.thr = ls.thr * T(ref.lrr[d * vs]);
+            if constexpr (LENS) prevLensC = ls.lensC;
             r.org = ls.isect.pos;
             r.dir = dir;
         }
@@ -457,20 +565,28 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
         if (d == nc - 1 && l == 0) {
             if (hit) {
                 const int li = S.tlight[prim];
-                if (li >= 0) handle_hit_light(S, d, li, r, sxf, syf, cs, out);
+                if (li >= 0) handle_hit_light<T, LENS>(S, d, li, r, sxf, syf, cs, out);
             }
             return;
         }
         if (!hit) return;
         float dv = ref.cdisc[d * vs];
         if (!FIXED) { dv = modulo1(dv + disc(d)); ref.cdisc[d * vs] = dv; }
-        if (d == 1 && val(length2(cs.isect.pos - r.org)) <= 0) { out.valid = false; return; }
+        if (d == 1) {
+            const T dd = length2(cs.isect.pos - r.org);
+            if (val(dd) <= 0) { out.valid = false; return; }
+            if constexpr (LENS) {
+                out.lensPos = lower(cs.isect.pos);
+                cs.lensC = cs.lensC * (T(1) / dd);
+            }
+        }
         if (d == nc - 1) {
             if (l == 1) {
                 T r0 = uprov.next(), r1 = uprov.next();
-                direct_lighting<T, FIXED>(S, o, d, cs, sxf, syf, *ref.light, *ref.lprim, r0, r1, prim, out);
+                direct_lighting<T, FIXED, LENS>(S, o, d, cs, sxf, syf, *ref.light, *ref.lprim, r0, r1, prim, out, nullptr, dv);
             } else {
-                connect_vertex<T, FIXED>(S, d, nl - 1, ls, lastLightPrim, cs, prim, sxf, syf, out);
+                connect_vertex<T, FIXED, LENS>(S, d, nl - 1, ls, lastLightPrim, cs, prim, sxf, syf, out, nullptr, &o, dv,
+                                               nl > 0 ? ref.ldisc[(nl - 1) * vs] : 0.0f);
             }
             return;
         }
@@ -481,9 +597,9 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
         }
         V3<T> dir, bc;
         if (lcp) {
-            if (!light_coordinate_sampling<T, FIXED>(S, prim, r0, r1, ref.cprim[(nc - 1) * vs], cs, dir)) return;
+            if (!light_coordinate_sampling<T, FIXED, LENS>(S, prim, r0, r1, ref.cprim[(nc - 1) * vs], cs, dir, nullptr, d, &o, dv)) return;
         } else {
-            if (!bsdf_sampling(S, o, false, true, cs, prim, dv, r0, r1, cs, dir, bc)) return;
+            if (!bsdf_sampling<T, LENS>(S, o, false, true, cs, prim, dv, r0, r1, cs, dir, bc, d)) return;
         }
         cs.thr = cs.thr * T(ref.crr[d * vs]);
         r.org = cs.isect.pos;
@@ -491,6 +607,254 @@ DEV void perturb_eval(const DevScene &S, const Opts &o, const SRef &ref, U &upro
         r.minT = kIsectEps;
         r.maxT = __builtin_inff();
     }
+}
+
+// ------------------------------------------------------------------ lens mutation
+// |sin(acos(z))| of a unit direction's z, in a form jets carry.
+template <class T> DEV T sin_of_acos(const T &z) { return s_sqrt(s_max(T(1) - z * z, typename Base<T>::type(0))); }
+
+// path.cpp:1879-1912 (PerturbLensFirstVertex): the first camera vertex aims
+// at lensPos, the second vertex's position of the state's last evaluation.
+// Not FIXED: occlusion tested, the direction's spherical coordinates written
+// to u[k], u[k + 1] (the vertex's bsdfRndParam).
+template <class T, bool FIXED>
+DEV bool lens_first_vertex(const DevScene &S, const Opts &o, const V3<float> &lensPos, BState<T> &st, int prim, float disc, const SRef &ref,
+                           int k, V3<T> &dir) {
+    const Mat m = mat_of(S, prim);
+    if (!(roughness(m, disc) > o.roughThr)) return false;
+    dir = lift<T>(lensPos) - st.isect.pos;
+    const T dist = s_sqrt(length2(dir));
+    dir = dir * (T(1) / dist);
+    if constexpr (!FIXED) {
+        if (occluded(S, st.isect.pos, dir, dist)) return false;
+        float u0, u1, jac;
+        to_spherical(lower(dir), u0, u1, jac);
+        ref.u[k * ref.s] = u0;
+        ref.u[(k + 1) * ref.s] = u1;
+    }
+    BOut<T> b;
+    bsdf_eval(m, false, st.wi, st.isect.sn, dir, b);
+    if (is_zero(b.contrib) || val(b.pdf) <= 0 || fabsf(float(val(b.cosWo))) < 1e-2f || fabsf(float(val(dot(dir, st.isect.gn)))) < 1e-2f)
+        return false;
+    st.lensC = mul(st.lensC, b.contrib);
+    const V3<T> bc = b.contrib * (T(1) / b.pdf);
+    st.ssJ = st.ssJ * (b.pdf * sin_of_acos(dir.z));
+    st.thisw = MIS(b.cosWo / b.pdf) * (st.thisw * MIS(b.revPdf) + st.prev);
+    st.prev = MIS(T(1) / b.pdf);
+    st.thr = mul(st.thr, bc);
+    return true;
+}
+// path.cpp:1835-1877 (PerturbLensLastVertex): the light subpath's
+// next-to-last vertex aims at the point the camera ray through scr meets on
+// the last vertex (light vertex slot nxt): a ray query, which names that
+// vertex's new triangle (not FIXED), or its own triangle (FIXED).
+template <class T, bool FIXED>
+DEV bool lens_last_vertex(const DevScene &S, const Opts &o, const T scr[2], BState<T> &st, int prim, float disc, const SRef &ref, int nxt,
+                          int k, V3<T> &dir) {
+    const Mat m = mat_of(S, prim);
+    if (!(roughness(m, disc) > o.roughThr)) return false;
+    RayT<T> r;
+    sample_primary(S.cam, scr[0], scr[1], r);
+    Isect<T> ni;
+    if constexpr (FIXED) {
+        if (!re_intersect(S, ref.lgprim[nxt * ref.vs], r, ni)) return false;
+    } else {
+        if (!intersect(S, r, ni)) return false;
+        ref.lgprim[nxt * ref.vs] = ni.prim;
+    }
+    dir = ni.pos - st.isect.pos;
+    const T dist = s_sqrt(length2(dir));
+    dir = dir * (T(1) / dist);
+    if constexpr (!FIXED) {
+        if (occluded(S, st.isect.pos, dir, dist)) return false;
+        float u0, u1, jac;
+        to_spherical(lower(dir), u0, u1, jac);
+        ref.u[k * ref.s] = u0;
+        ref.u[(k + 1) * ref.s] = u1;
+    }
+    BOut<T> b;
+    bsdf_eval(m, true, st.wi, st.isect.sn, dir, b);
+    if (is_zero(b.contrib) || val(b.pdf) <= 0) return false;
+    const T fac = snc(true, dir, st.isect, st.wi);
+    if (val(fac) <= 0) return false;
+    b.contrib = b.contrib * fac;
+    st.lensC = b.contrib;
+    const V3<T> bc = b.contrib * (T(1) / b.pdf);
+    st.ssJ = st.ssJ * (b.pdf * sin_of_acos(dir.z));
+    st.thisw = MIS(b.cosWo / b.pdf) * (st.thisw * MIS(b.revPdf) + st.prev);
+    st.prev = MIS(T(1) / b.pdf);
+    st.thr = mul(st.thr, bc);
+    return true;
+}
+
+// PerturbLensBidir (path.cpp:1914-2067) of the state behind `ref` with its
+// screen position moved to scr. Every vertex keeps its triangle (the state's
+// own primary samples, no offsets, no discrete perturbation) except the one
+// the new camera ray meets: the first camera vertex, or for c = 1 the last
+// light vertex, found by a ray query. Not FIXED: the lens mutation's
+// proposal; writes the new triangle, the re-aimed vertex's primary samples,
+// the screen position and the contribution's screen position into the state.
+// FIXED: re-intersection only, no occlusion tests, no writes -- the lens
+// function whose derivatives in scr give the lens Gaussian (PathFuncMode::
+// Lens, path.cpp:2987-3175). Tracks the lens bookkeeping (LENS).
+template <class T, bool FIXED>
+DEV void perturb_lens_eval(const DevScene &S, const Opts &o, const SRef &ref, const T scr[2], const V3<float> &lensPos, POut<T> &out) {
+    out.valid = false;
+    const int c = *ref.c, l = *ref.l, s = ref.s, vs = ref.vs;
+    const int nl = l - 1, nc = c - 1;
+    BState<T> ls;
+    int lastLightPrim = -1;
+    int k = 0;  // GetPathPos index of the next primary sample
+    if (l > 1) {
+        const int li = *ref.light, lp = *ref.lprim;
+        RayT<T> r;
+        emit_from_light(S, o, li, lp, T(ref.u[0]), T(ref.u[s]), T(ref.u[2 * s]), T(ref.u[3 * s]), r, ls);
+        k = 4;
+        r.minT = kIsectEps;
+        r.maxT = __builtin_inff();
+        V3<T> prevLensC = vconst<T>(0, 0, 0);
+        for (int d = 0; d < nl; d++) {
+            const int prim = ref.lgprim[d * vs];
+            if (!re_intersect(S, prim, r, ls.isect)) return;
+            ls.wi = -r.dir;
+            const float dv = ref.ldisc[d * vs];
+            convert_mis(r, ls);
+            lastLightPrim = prim;
+            if (d == nl - 1 && c == 1) {
+                connect_to_camera<T, FIXED, true>(S, o, d, ls, prim, dv, r.org, out, nullptr, &prevLensC);
+                if (out.valid && !FIXED) { ref.screen[0] = out.screen[0]; ref.screen[s] = out.screen[1]; }
+                return;
+            }
+            if (d == nl - 1) break;
+            V3<T> dir, bc;
+            if (d == nl - 2 && c == 1) {
+                if (!lens_last_vertex<T, FIXED>(S, o, scr, ls, prim, dv, ref, nl - 1, k, dir)) return;
+            } else {
+                T r0 = T(ref.u[k * s]), r1 = T(ref.u[(k + 1) * s]);
+                if (!bsdf_sampling<T, true>(S, o, true, true, ls, prim, dv, r0, r1, ls, dir, bc, d)) return;
+            }
+            k += 2;
+            ls.thr = ls.thr * T(ref.lrr[d * vs]);
+            prevLensC = ls.lensC;
+            r.org = ls.isect.pos;
+            r.dir = dir;
+        }
+    }
+    if (c < 2) return;
+    const float sxf = float(val(scr[0])), syf = float(val(scr[1]));
+    if constexpr (!FIXED) {
+        ref.u[k * s] = sxf;
+        ref.u[(k + 1) * s] = syf;
+        ref.screen[0] = sxf;
+        ref.screen[s] = syf;
+    }
+    k += 2;
+    RayT<T> r;
+    BState<T> cs;
+    emit_from_camera(S, scr[0], scr[1], r, cs);
+    bool lcp = false;
+    for (int d = 0; d < nc; d++) {
+        int prim = ref.cprim[d * vs];
+        bool hit = true;
+        if (d == 0 && !FIXED) {
+            hit = intersect(S, r, cs.isect);
+            prim = hit ? cs.isect.prim : -1;
+            ref.cprim[0] = prim;
+        } else if (prim < 0) {
+            hit = false;
+        } else if (!re_intersect(S, prim, r, cs.isect)) {
+            return;
+        }
+        cs.wi = -r.dir;
+        if (hit) convert_mis(r, cs);
+        if (d == nc - 1 && l == 0) {
+            if (hit) {
+                const int li = S.tlight[prim];
+                if (li >= 0) handle_hit_light<T, true>(S, d, li, r, sxf, syf, cs, out);
+            }
+            return;
+        }
+        if (!hit) return;
+        const float dv = ref.cdisc[d * vs];
+        if (d == 1) {
+            const T dd = length2(cs.isect.pos - r.org);
+            if (val(dd) <= 0) { out.valid = false; return; }
+            out.lensPos = lower(cs.isect.pos);
+            cs.lensC = cs.lensC * (T(1) / dd);
+        }
+        if (d == nc - 1) {
+            if (l == 1) {
+                const T r0 = T(ref.u[k * s]), r1 = T(ref.u[(k + 1) * s]);
+                direct_lighting<T, true, true>(S, o, d, cs, sxf, syf, *ref.light, *ref.lprim, r0, r1, prim, out, nullptr, dv);
+            } else if (!FIXED && d == 0) {
+                connect_vertex<T, false, true>(S, d, nl - 1, ls, lastLightPrim, cs, prim, sxf, syf, out, nullptr, &o, dv,
+                                               ref.ldisc[(nl - 1) * vs]);
+            } else {
+                connect_vertex<T, true, true>(S, d, nl - 1, ls, lastLightPrim, cs, prim, sxf, syf, out, nullptr, &o, dv,
+                                              ref.ldisc[(nl - 1) * vs]);
+            }
+            return;
+        }
+        if (d == nc - 2 && l == 0) {
+            const int np = ref.cprim[(nc - 1) * vs];
+            if (np >= 0 && S.tlight[np] >= 0) lcp = true;
+        }
+        V3<T> dir, bc;
+        if (d == 0) {
+            if (!lens_first_vertex<T, FIXED>(S, o, lensPos, cs, prim, dv, ref, k, dir)) return;
+        } else {
+            T r0 = T(ref.u[k * s]), r1 = T(ref.u[(k + 1) * s]);
+            if (lcp) {
+                if (!light_coordinate_sampling<T, true, true>(S, prim, r0, r1, ref.cprim[(nc - 1) * vs], cs, dir, nullptr, d, &o, dv)) return;
+            } else {
+                if (!bsdf_sampling<T, true>(S, o, false, true, cs, prim, dv, r0, r1, cs, dir, bc, d)) return;
+            }
+        }
+        k += 2;
+        cs.thr = cs.thr * T(ref.crr[d * vs]);
+        r.org = cs.isect.pos;
+        r.dir = dir;
+        r.minT = kIsectEps;
+        r.maxT = __builtin_inff();
+    }
+}
+
+// Gradient and Hessian of log lensScore in the screen position at scr (the
+// lens function, FIXED perturb_lens_eval) by two forward-over-forward passes:
+// g = (d/ds0, d/ds1), h = (H00, H01, H11). False without a contribution.
+DEV bool lens_jets(const DevScene &S, const Opts &o, const SRef &ref, const float scr[2], const V3<float> &lensPos, float g[2], float h[3]) {
+    for (int row = 0; row < 2; row++) {
+        Jet<float, 2> sj[2];
+        for (int k = 0; k < 2; k++) {
+            sj[k] = Jet<float, 2>(scr[k]);
+            sj[k].t = k == row ? 1.0f : 0.0f;
+            sj[k].g[0] = k == 0 ? 1.0f : 0.0f;
+            sj[k].g[1] = k == 1 ? 1.0f : 0.0f;
+        }
+        POut<Jet<float, 2>> out;
+        perturb_lens_eval<Jet<float, 2>, true>(S, o, ref, sj, lensPos, out);
+        if (!out.valid || !(out.lensScore.v > 0.0f)) return false;
+        const Jet<float, 2> lf = s_log(out.lensScore);
+        if (row == 0) {
+            g[0] = lf.g[0]; g[1] = lf.g[1];
+            h[0] = lf.h[0]; h[1] = lf.h[1];
+        } else {
+            h[2] = lf.h[1];
+        }
+    }
+    return true;
+}
+
+// The lens score and lensVertexPos of the state behind ref: its
+// fixed-topology re-evaluation with the lens bookkeeping (the value the
+// state's own evaluation tracked in the reference). 0 without a contribution.
+DEV float lens_state_eval(const DevScene &S, const Opts &o, const SRef &ref, V3<float> &lensPos) {
+    PlainU<float> up{ref.u, ref.s, 0};
+    POut<float> out;
+    out.lensPos = V3<float>(0.0f, 0.0f, 0.0f);
+    perturb_eval<float, true, PlainU<float>, DiscNone, true>(S, o, ref, up, DiscNone{}, out);
+    lensPos = out.lensPos;
+    return out.valid ? out.lensScore : 0.0f;
 }
 
 // Copy a state between two references (header + D dims + vertex fields).

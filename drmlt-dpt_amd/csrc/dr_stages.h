@@ -15,7 +15,7 @@
 
 namespace drmlt {
 
-enum : uint32_t { TAG_S1 = 1, TAG_S2 = 2, TAG_S3 = 3, TAG_PRE = 16, TAG_DIRECT = 17 };
+enum : uint32_t { TAG_S1 = 1, TAG_S2 = 2, TAG_S3 = 3, TAG_LENS = 4, TAG_PRE = 16, TAG_DIRECT = 17 };
 
 // ------------------------------------------------------------------ helpers
 struct GenArray {  // uniform stream from an array (parity tests)
@@ -96,7 +96,8 @@ DEV void count_if(unsigned long long *ctr, bool pred) {
     const int leader = __ffsll((long long)__ballot(1)) - 1;
     if (int(threadIdx.x & 63) == leader && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
 }
-enum { ST_LARGE = 0, ST_LARGE_ACC, ST_S1, ST_S1_ACC, ST_ISO, ST_ISO_ACC, ST_H2MC, ST_H2MC_ACC };
+enum { ST_LARGE = 0, ST_LARGE_ACC, ST_S1, ST_S1_ACC, ST_ISO, ST_ISO_ACC, ST_H2MC, ST_H2MC_ACC, ST_LENS, ST_LENS_ACC, ST_COUNT };
+static_assert(ST_COUNT == kStatCounters, "loop counters");
 
 DEV int perm_cur(int perm) { return perm & 3; }
 DEV int perm_first(int perm) { return (perm >> 2) & 3; }
@@ -222,6 +223,100 @@ template <class X, class W> DEV void h2mc_ratio(const X &x, const Chains &ch, in
     if (x.lane == 0) ch.a2d[chain] = exp(px - py) * double(ch.ss[ps]) / double(ch.ss[pc]);
 }
 
+// ------------------------------------------------------------ lens mutation
+// The h2mc integrator's lens mutation (h2mc.cpp:463-530). A state's lens
+// score and lensVertexPos come from its fixed-topology re-evaluation with the
+// lens bookkeeping (lens_state_eval), computed once per state and kept in the
+// slot header (lens flags bit 0) and its lens slot.
+DEV void lens_score_ensure(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain) {
+    const int pc = chain + ch.C * perm_cur(ch.perm[chain]);
+    if (ch.lflags[pc] & 1) return;
+    float ls = 0.0f;
+    V3<float> lp(0.0f, 0.0f, 0.0f);
+    const int c = ch.c[pc], l = ch.l[pc];
+    const bool wellFormed = c >= 1 && l >= 0 && c <= kVMax + 1 && l <= kVMax + 1 && c + l - 1 >= 2 && 2 * (c + l - 1) <= kDMax;
+    if (ch.valid[pc] && wellFormed) ls = lens_state_eval(Sc, P.o, slot_ref(ch, pc), lp);
+    ch.lscore[pc] = isfinite(ls) ? ls : 0.0f;
+    set_lens_pos(ch, pc, lp);
+    ch.lflags[pc] = (ch.lflags[pc] & ~1) | 1;
+}
+// initLensGaussian (h2mc.cpp:466-499) of state slot p: lens derivatives exist
+// for c + l > 3 within the derivative depth (path.cpp:3195-3197); they are
+// zero when lensScore <= 1e-15 or not finite; ComputeGaussian with the lens
+// sigma and the state's ssScore (h2mc.cpp:607-644), else isotropic.
+DEV void lens_gaussian(const DevScene &Sc, const Chains &ch, const DrParams &P, int p, GWorkT<2> &w) {
+    const SeqExec x;
+    const int c = ch.c[p], l = ch.l[p];
+    const LSlot G = lens_ref(ch, p);
+    if (c >= 1 && l >= 0 && c + l > 3 && c + l - 1 <= P.maxDervDepth) {
+        float g[2] = {0.0f, 0.0f}, h[3] = {0.0f, 0.0f, 0.0f};
+        if (double(ch.lscore[p]) > 1e-15) {
+            const float scr[2] = {(&ch.screen[p])[0], (&ch.screen[p])[1]};
+            const bool ok = lens_jets(Sc, P.o, slot_ref(ch, p), scr, lens_pos(ch, p), g, h);
+            if (!ok || !(isfinite(g[0]) && isfinite(g[1]) && isfinite(h[0]) && isfinite(h[1]) && isfinite(h[2]))) {
+                g[0] = g[1] = 0.0f;
+                h[0] = h[1] = h[2] = 0.0f;
+            }
+        }
+        gauss_compute(x, P.h2lens, double(ch.ss[p]), 2, [&](int i) { return double(g[i]); },
+                      [&](int i, int j) { return double(i == 0 ? (j == 0 ? h[0] : h[1]) : h[2]); }, w, G);
+    } else {
+        gauss_iso(x, 2, P.h2lens.sigma, G);
+    }
+    ch.lflags[p] = ch.lflags[p] | 2;
+}
+// One lens mutation of `chain` into its first-proposal slot, accepted with
+// min(1, N_z(-off) / N_x(off) lensScore_z / lensScore_x); splats (1 - a, a),
+// state swap and counts as h2mc.cpp:206-243 do them. Draws from the chain's
+// lens stream. The plain form calls it from dr_stage1, the batched form from
+// k_b_lens over list 5.
+DEV void lens_move(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter, float *film, unsigned long long *stats) {
+    Philox g(P.seed, iter, chain, TAG_LENS);
+    const SeqExec x;
+    GWorkT<2> w;
+    const int perm = ch.perm[chain], C = ch.C;
+    const int pc = chain + C * perm_cur(perm), pf = chain + C * perm_first(perm);
+    if (!(ch.lflags[pc] & 2)) lens_gaussian(Sc, ch, P, pc, w);
+    float off[2];
+    gauss_sample(x, 2, lens_ref(ch, pc), [&](int) { return g.normal(); }, w, [&](int i, double v) { off[i] = float(v); });
+    const float scr[2] = {modulo1((&ch.screen[pc])[0] + off[0]), modulo1((&ch.screen[pc])[1] + off[1])};
+    const V3<float> lp = lens_pos(ch, pc);
+    copy_state(slot_ref(ch, pc), slot_ref(ch, pf));
+    POut<float> out;
+    out.lensPos = lp;
+    perturb_lens_eval<float, false>(Sc, P.o, slot_ref(ch, pf), scr, lp, out);
+    ch.gInit[pf] = 0;
+    ch.lflags[pf] = 0;
+    count_state(Sc, ch.c[pc], ch.l[pc]);
+    float a = 0.0f;
+    if (out.valid) {
+        store_contrib(ch, pf, out);
+        ch.hasSplat[pf] = 1;
+        ch.lscore[pf] = out.lensScore;
+        set_lens_pos(ch, pf, out.lensPos);
+        ch.lflags[pf] = 1;
+        lens_gaussian(Sc, ch, P, pf, w);
+        const double py = gauss_logpdf(x, 2, [&](int i) { return double(off[i]); }, 1.0, lens_ref(ch, pc), w);
+        const double px = gauss_logpdf(x, 2, [&](int i) { return double(off[i]); }, -1.0, lens_ref(ch, pf), w);
+        a = clamp_ref(float(exp(px - py) * double(out.lensScore) / double(ch.lscore[pc])));
+    }
+    bool acc = false;
+    if (is_invalid(a)) a = 0.0f;
+    else acc = flip_coin(a, g.uni());
+    count_if(stats + ST_LENS, true);
+    count_if(stats + ST_LENS_ACC, acc);
+    ch.a1[chain] = a;
+    ch.flags[chain] = 128 | (acc ? 1 : 0) | (out.valid ? 32 : 0);
+    const int W = Sc.cam.W, H = Sc.cam.H;
+    const float wx = 1.0f - a;
+    if (wx > 0.0f) splat(ch, P, W, H, film, pc, wx);
+    if (a > 0.0f) splat(ch, P, W, H, film, pf, a);
+    if (acc) {
+        ch.perm[chain] = make_perm(perm_first(perm), perm_cur(perm), perm_second(perm));
+        ch.valid[pf] = 1;
+    }
+}
+
 // ------------------------------------------------------------ DR stages
 // Each stage is split into a beginning (random draws that choose the move,
 // the proposal slot), the path evaluation, and an end (acceptance, splats,
@@ -236,6 +331,7 @@ struct Stage1 {
     int perm, pc, pf;
     bool isLarge, curValid;
     bool h2mcMove;  // h2mc integrator: this small step is an H2MC move, made by the second-stage kernels
+    bool lensMove;  // h2mc integrator: this small step is a lens mutation (lens_move)
     int cam, lgt;  // large step: the sampled strategy
 };
 // copyState: copy the current state into the proposal slot for a small step
@@ -253,15 +349,24 @@ DEV Stage1 stage1_begin(const Chains &ch, const DrParams &P, int chain, Philox &
     a.curValid = ch.valid[a.pc] != 0 && wellFormed;
     a.cam = a.lgt = 0;
     a.h2mcMove = false;
+    a.lensMove = false;
     if (!a.curValid || a.isLarge) {
         const int len = sample_discrete(P.lenCdf, P.nLen, g.uni());
         a.lgt = min(max(int(g.uni() * float(len + 1)), 0), len);
         a.cam = len - a.lgt + 1;
     } else {
-        // h2mc (H2MCSmallStep::Mutate, h2mc.cpp:448-460): isotropic with
-        // probability uniformMixingProbability, else an H2MC move
-        if (P.integrator == 2) a.h2mcMove = !(g.uni() < P.uniformMixing);
-        if (copyState && !a.h2mcMove) copy_state(slot_ref(ch, a.pc), slot_ref(ch, a.pf));
+        // h2mc (H2MCSmallStep::Mutate, h2mc.cpp:448-466): isotropic with
+        // probability uniformMixingProbability; else, when the current state
+        // has a lens score (known: lens_score_ensure ran), a lens mutation with
+        // probability lensPerturbProb, else an H2MC move
+        if (P.integrator == 2) {
+            a.h2mcMove = !(g.uni() < P.uniformMixing);
+            if (a.h2mcMove && P.lensProb > 0.0f && ch.lscore[a.pc] > 0.0f && g.uni() < P.lensProb) {
+                a.lensMove = true;
+                a.h2mcMove = false;
+            }
+        }
+        if (copyState && !a.h2mcMove && !a.lensMove) copy_state(slot_ref(ch, a.pc), slot_ref(ch, a.pf));
     }
     return a;
 }
@@ -295,10 +400,17 @@ DEV int stage1_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int 
         ch.flags[chain] = 2 | 4;
         return 2 | 4;
     }
+    if (a.lensMove) {
+        // made by lens_move (flag bit 7)
+        ch.a1[chain] = 0.0f;
+        ch.flags[chain] = 128;
+        return 128;
+    }
     const int perm = a.perm, pc = a.pc, pf = a.pf;
     const bool isLarge = a.isLarge, curValid = a.curValid;
     float a1 = 0.0f;
     bool acc1 = false;
+    ch.lflags[pf] = 0;  // a new state: its lens score and lens Gaussian are not known
     if (!curValid || isLarge) {
         count_state(Sc, a.cam, a.lgt);
         ch.gInit[pf] = 0;
@@ -353,7 +465,12 @@ DEV int stage1_end(const DevScene &Sc, const Chains &ch, const DrParams &P, int 
 DEV int dr_stage1(const DevScene &Sc, const Chains &ch, const DrParams &P, int chain, uint32_t iter, float *film,
                   unsigned long long *stats) {
     Philox g(P.seed, iter, chain, TAG_S1);
+    if (P.lensProb > 0.0f) lens_score_ensure(Sc, ch, P, chain);
     const Stage1 a = stage1_begin(ch, P, chain, g);
+    if (a.lensMove) {
+        lens_move(Sc, ch, P, chain, iter, film, stats);
+        return ch.flags[chain];
+    }
     const SRef rf = slot_ref(ch, a.pf);
     POut<float> out;
     out.valid = false;
@@ -406,6 +523,7 @@ DEV void stage2_walk(const Chains &ch, const DrParams &P, int chain, const Stage
 DEV void stage2_end(const DevScene &Sc, const Chains &ch, int chain, const Stage2 &a, const POut<float> &out) {
     const int pc = a.pc, ps = a.ps;
     if (!a.isAniso) ch.gInit[ps] = 0;
+    ch.lflags[ps] = 0;  // h2mc.cpp:583: the proposal's lens Gaussian is not initialised
     count_state(Sc, ch.c[pc], ch.l[pc]);
     double a2 = 0.0;
     if (out.valid) {

@@ -264,6 +264,8 @@ static DrParams make_params(const drmlt_ctx *c) {
     P.maxDervDepth = o.max_derivatives_depth;
     P.integrator = o.integrator;
     P.uniformMixing = o.uniform_mixing_prob;
+    P.lensProb = o.integrator == 2 ? o.lens_perturb_prob : 0.0f;
+    P.h2lens = make_h2param(o.lens_perturb_std_dev);
     P.acceptanceMap = o.integrator == 0 ? o.acceptance_map : 0;  // a DR option (delayed.cpp:255-275)
     P.b = c->b;
     P.nLen = c->nLen;
@@ -312,6 +314,7 @@ extern "C" int drmlt_options_default(drmlt_options *o) {
     o->integrator = 0;
     o->uniform_mixing_prob = 0.2f;  // dptoptions.h:29
     o->lens_perturb_prob = 0.3f;    // dptoptions.h:20
+    o->lens_perturb_std_dev = 0.01f;  // dptoptions.h:21
     o->seed = 1;
     return DRMLT_OK;
 }
@@ -362,8 +365,10 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     if (opt->max_derivatives_depth > 12) return fail(DRMLT_ERR_ARG, "max_derivatives_depth must be <= 12");
     if (!opt->large_step_multiplexed) return fail(DRMLT_ERR_ARG, "only multiplexed large steps are implemented");
     if (opt->integrator < 0 || opt->integrator > 2) return fail(DRMLT_ERR_ARG, "integrator must be 0 (drmlt), 1 (mcmc) or 2 (h2mc)");
-    if (opt->integrator == 2 && opt->lens_perturb_prob != 0.0f)
-        return fail(DRMLT_ERR_ARG, "the h2mc integrator runs without lens perturbations here: set lensperturbprob to 0");
+    if (opt->integrator == 2 && !(opt->lens_perturb_prob >= 0.0f && opt->lens_perturb_prob <= 1.0f))
+        return fail(DRMLT_ERR_ARG, "lensperturbprob must be in [0, 1]");
+    if (opt->integrator == 2 && opt->lens_perturb_prob > 0.0f && !(opt->lens_perturb_std_dev > 0.0f))
+        return fail(DRMLT_ERR_ARG, "lensperturbstddev must be positive");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DRMLT_ERR_HIP, "no HIP device");
     drmlt_ctx *c = new drmlt_ctx();
@@ -512,12 +517,12 @@ extern "C" int drmlt_ctx_create(const drmlt_scene_desc *sc, const drmlt_options 
     c->W = S.cam.W;
     c->H = S.cam.H;
     const size_t filmN = size_t(c->W) * c->H * 3;
-    if (dalloc(&c->filmInd, filmN) != hipSuccess || dalloc(&c->filmDir, filmN) != hipSuccess || dalloc(&c->stats, 8) != hipSuccess ||
+    if (dalloc(&c->filmInd, filmN) != hipSuccess || dalloc(&c->filmDir, filmN) != hipSuccess || dalloc(&c->stats, kStatCounters) != hipSuccess ||
         dalloc(&c->iterDev, 1) != hipSuccess)
         return bail(fail(DRMLT_ERR_HIP, "film allocation"));
     (void)hipMemsetAsync(c->filmInd, 0, filmN * 4, s);
     (void)hipMemsetAsync(c->filmDir, 0, filmN * 4, s);
-    (void)hipMemsetAsync(c->stats, 0, 64, s);
+    (void)hipMemsetAsync(c->stats, 0, kStatCounters * 8, s);
     (void)hipMemsetAsync(c->iterDev, 0, 4, s);
     for (auto &e : c->ev) (void)hipEventCreate(&e);
     c->evInit = true;
@@ -548,6 +553,9 @@ static int alloc_chains(drmlt_ctx *c, int C) {
               A(&ch.off, 32 * size_t(C)) && A(&ch.grad, size_t(kGMax) * C) && A(&ch.hess, size_t(kGMax * kGMax) * C) &&
               A(&ch.a1, size_t(C)) && A(&ch.a2, size_t(C)) && A(&ch.a2d, size_t(C)) && A(&ch.flags, size_t(C)) &&
               A(&c->L.count, kListKinds * kMaxBins) && A(&c->L.items, size_t(kListKinds) * kMaxBins * C);
+    // lens Gaussians and lensVertexPos: the h2mc integrator with lens mutations only
+    if (c->opt.integrator == 2 && c->opt.lens_perturb_prob > 0.0f) ok = ok && A(&ch.lensG, size_t(kLSlot) * GS);
+    else ch.lensG = nullptr;
     for (Lists &lx : c->LX) ok = ok && A(&lx.count, kListKinds * kMaxBins) && A(&lx.items, size_t(kListKinds) * kMaxBins * C);
     if (!ok) return fail(DRMLT_ERR_HIP, "chain state allocation");
     bind_slot_fields(ch);
@@ -679,7 +687,7 @@ extern "C" int drmlt_seed_chains(drmlt_ctx *c, const drmlt_path_rec *st, const f
     r = set_length_dist(c, lf, nLen);
     if (r) return r;
     HIPCHK(hipMemsetAsync(c->filmInd, 0, size_t(c->W) * c->H * 3 * 4, s));
-    HIPCHK(hipMemsetAsync(c->stats, 0, 64, s));
+    HIPCHK(hipMemsetAsync(c->stats, 0, kStatCounters * 8, s));
     c->itersDone = 0;
     HIPCHK(hipStreamSynchronize(s));
     return DRMLT_OK;
@@ -755,7 +763,7 @@ extern "C" int drmlt_prepass(drmlt_ctx *c, double *bOut, float *lfOut, int32_t n
     r = set_length_dist(c, lf.data(), nLen);
     if (r) return r;
     HIPCHK(hipMemsetAsync(c->filmInd, 0, size_t(c->W) * c->H * 3 * 4, s));
-    HIPCHK(hipMemsetAsync(c->stats, 0, 64, s));
+    HIPCHK(hipMemsetAsync(c->stats, 0, kStatCounters * 8, s));
     HIPCHK(hipStreamSynchronize(s));
     c->itersDone = 0;
     if (bOut) *bOut = htot / double(N);
@@ -817,10 +825,24 @@ static PathStageKernels path_stage_kernels(const drmlt_ctx *c) {
     return {k_b_stage1, k_b_stage2_propose, k_b_stage2_accept};
 }
 
+// h2mc with lens mutations: the lens scores before stage 1 (its lens coin
+// reads them) and the lens mutations of list 5 after it.
+static void launch_lens_scores(drmlt_ctx *c, const DrParams &P, hipStream_t s) {
+    if (P.lensProb > 0.0f)
+        hipLaunchKernelGGL(k_b_lens_score, dim3(std::max(1, std::min((P.chainCount + 63) / 64, persistent_blocks(c)))), dim3(64), 0, s, c->S, c->ch,
+                           P);
+}
+static void launch_lens_moves(drmlt_ctx *c, const DrParams &P, hipStream_t s, const Lists &L, const StepGrid &G) {
+    if (P.lensProb > 0.0f)
+        hipLaunchKernelGGL(k_b_lens, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
+}
+
 static void launch_step_kernels(drmlt_ctx *c, const DrParams &P, hipStream_t s, const Lists &L, const StepGrid &G) {
     const PathStageKernels K = path_stage_kernels(c);
+    launch_lens_scores(c, P, s);
     if (P.binStages == 1) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, L);
     hipLaunchKernelGGL(K.stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
+    launch_lens_moves(c, P, s, L, G);
     if (P.integrator == 1) return;
     hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, L, 4, 0);
     hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, L, 0, 0, 0, c->iterDev);
@@ -839,8 +861,10 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
             static_cast<DevSceneData &>(Sk[k]) = c->S;
             Sk[k].counters = counters + 4 * k;
         }
+        launch_lens_scores(c, P, s);  // the lens kernels have no counting form: their work is not counted
         if (P.binStages == 1) hipLaunchKernelGGL(k_b_stage1_bin_counted, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
         hipLaunchKernelGGL(k_b_stage1_counted, dim3(G.stage), dim3(64), 0, s, Sk[0], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+        launch_lens_moves(c, P, s, c->L, G);
         if (P.integrator != 1) {
             hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[1], c->ch, P, c->L, 4, 0);
             hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[2], c->ch, P, c->L, 0, 0, 0, c->iterDev);
@@ -878,8 +902,10 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
     }
     const PathStageKernels K = path_stage_kernels(c);
     if (timed) (void)hipEventRecord(c->ev[0], s);
+    launch_lens_scores(c, P, s);  // timed with stage 1
     if (P.binStages == 1) hipLaunchKernelGGL(k_b_stage1_bin, dim3(G.stage), dim3(64), 0, s, c->ch, P, c->iterDev, c->L);
     hipLaunchKernelGGL(K.stage1, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+    launch_lens_moves(c, P, s, c->L, G);
     if (timed) (void)hipEventRecord(c->ev[1], s);
     if (P.integrator != 1) hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, c->L, 4, 0);
     if (timed) (void)hipEventRecord(c->ev[2], s);
@@ -1104,13 +1130,14 @@ extern "C" int drmlt_read_film(drmlt_ctx *c, float *rgb, int32_t w, int32_t h, i
 
 extern "C" int drmlt_read_stats(drmlt_ctx *c, drmlt_stats *st) {
     if (!c || !st) return fail(DRMLT_ERR_ARG, "bad argument");
-    unsigned long long v[8];
-    HIPCHK(hipMemcpyAsync(v, c->stats, 64, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long v[kStatCounters];
+    HIPCHK(hipMemcpyAsync(v, c->stats, sizeof(v), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     st->large_step = v[0]; st->large_step_accepted = v[1];
     st->stage1 = v[2]; st->stage1_accepted = v[3];
     st->stage2_iso = v[4]; st->stage2_iso_accepted = v[5];
     st->stage2_h2mc = v[6]; st->stage2_h2mc_accepted = v[7];
+    st->lens_step = v[8]; st->lens_step_accepted = v[9];
     return DRMLT_OK;
 }
 
@@ -1124,8 +1151,9 @@ extern "C" int drmlt_write_exr(const char *path, const float *rgb, int32_t w, in
 
 extern "C" int drmlt_stats_text(const drmlt_stats *st, int32_t format, char *buf, int32_t cap) {
     if (!st || cap < 0 || (cap > 0 && !buf) || format < 0 || format > 2) return fail(DRMLT_ERR_ARG, "bad argument");
-    const uint64_t v[8] = {st->large_step, st->large_step_accepted, st->stage1, st->stage1_accepted,
-                           st->stage2_iso, st->stage2_iso_accepted, st->stage2_h2mc, st->stage2_h2mc_accepted};
+    const uint64_t v[10] = {st->large_step, st->large_step_accepted, st->stage1, st->stage1_accepted,
+                            st->stage2_iso, st->stage2_iso_accepted, st->stage2_h2mc, st->stage2_h2mc_accepted,
+                            st->lens_step, st->lens_step_accepted};
     const std::string t = stats_text(v, format);
     if (cap > 0) {
         const size_t n = std::min(t.size(), size_t(cap - 1));
@@ -1148,8 +1176,9 @@ extern "C" int drmlt_write_stats(drmlt_ctx *c, const char *path, int32_t referen
     drmlt_stats st;
     const int rc = drmlt_read_stats(c, &st);
     if (rc != DRMLT_OK) return rc;
-    const uint64_t v[8] = {st.large_step, st.large_step_accepted, st.stage1, st.stage1_accepted,
-                           st.stage2_iso, st.stage2_iso_accepted, st.stage2_h2mc, st.stage2_h2mc_accepted};
+    const uint64_t v[10] = {st.large_step, st.large_step_accepted, st.stage1, st.stage1_accepted,
+                            st.stage2_iso, st.stage2_iso_accepted, st.stage2_h2mc, st.stage2_h2mc_accepted,
+                            st.lens_step, st.lens_step_accepted};
     return write_text(path, stats_text(v, c->opt.integrator != 0 ? 2 : (reference_labels != 0 ? 1 : 0)), false);
 }
 
@@ -1345,6 +1374,26 @@ extern "C" int drmlt_eval_logf(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st
     DGET(okOut, ok, n);
     HIPCHK(hipStreamSynchronize(s));
     kernel_time(c);
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_lens_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st, const float *screen, float *out9, int32_t *okOut) {
+    if (!c || n < 0 || !st || !screen || !out9 || !okOut) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!recs_ok(c, st, n) || !recs_fixed_ok(st, n)) return fail(DRMLT_ERR_ARG, "path record out of range");
+    if (n == 0) return DRMLT_OK;
+    hipStream_t s = c->stream;
+    DBuf<char> r;
+    DBuf<float> sc, o9;
+    DBuf<int> ok;
+    DUP(r, (const char *)st, n * sizeof(drmlt_path_rec));
+    DUP(sc, screen, 2 * n);
+    DNEW(o9, 9 * n); DNEW(ok, n);
+    const DrParams P = make_params(c);
+    hipLaunchKernelGGL(k_lens_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.o, int(n), r.p, sc.p, o9.p, ok.p);
+    HIPCHK(hipGetLastError());
+    DGET(out9, o9, 9 * n);
+    DGET(okOut, ok, n);
+    HIPCHK(hipStreamSynchronize(s));
     return DRMLT_OK;
 }
 

@@ -161,22 +161,23 @@ int write_exr(const char *path, const float *rgb, int w, int h, int compression,
 // prints the H2MC ratio and count and vice versa); format 0 names each
 // line's own component. Format 2 is the Stats::output of the mcmc and h2mc
 // integrators (mcmc.cpp:76-86, h2mc.cpp:140-150): large steps, small steps
-// (isotropic first-stage moves plus H2MC moves), and lens steps, which
-// neither makes here (mcmc's SmallStep never does, mutation.cpp:126-165; h2mc
-// runs with lensperturbprob 0), at 0 of 0.
-std::string stats_text(const uint64_t s[8], int format) {
+// (isotropic first-stage moves plus H2MC moves), and lens steps (mcmc's
+// SmallStep never makes one, mutation.cpp:126-165: 0 of 0 there, printed as
+// the reference's 0 / 0).
+std::string stats_text(const uint64_t s[10], int format) {
     const float rLarge = float(s[1]) / float(s[0]), r1 = float(s[3]) / float(s[2]);
     const float rIso = float(s[5]) / float(s[4]), rH2 = float(s[7]) / float(s[6]);
     std::ostringstream o;
     o << "\n";
     o << "Rendering Statistics \n";
     if (format == 2) {
-        volatile float zero = 0.0f;  // divided at run time, as the reference's 0 / 0 is (x86-64: the negative NaN)
-        const float rLens = zero / zero;
+        // divided at run time, as the reference's is (x86-64: 0 / 0 is the negative NaN)
+        volatile float lensAcc = float(s[9]), lensAll = float(s[8]);
+        const float rLens = lensAcc / lensAll;
         o << "    Large step mutations acceptance ratio       : " << rLarge << " (" << s[0] << ")\n";
         const float rSmall = float(s[3] + s[7]) / float(s[2] + s[6]);
         o << "    Small step mutations acceptance ratio       : " << rSmall << " (" << (s[2] + s[6]) << ")\n";
-        o << "    Lens mutations acceptance ratio    :                    " << rLens << " (" << 0 << ")\n";
+        o << "    Lens mutations acceptance ratio    :                    " << rLens << " (" << s[8] << ")\n";
         o << "\n";
         return o.str();
     }

@@ -43,6 +43,26 @@ __global__ void __launch_bounds__(64) k_logf_rec_f64(DevScene Sc, Opts o, int n,
     logf_rec<double>(blockIdx.x * blockDim.x + threadIdx.x, Sc, o, n, recs, logf, okOut);
 }
 
+// The lens bookkeeping of each record -- lens score and lensVertexPos of its
+// fixed re-evaluation (lens_state_eval) -- and the gradient and Hessian of
+// the lens function at screen position scr[2i], scr[2i + 1] (lens_jets):
+// out9[9i..] = lensScore, lensPos (3), d/ds0, d/ds1, H00, H01, H11;
+// ok[i] = 0 where the lens function has no contribution there.
+__global__ void __launch_bounds__(64) k_lens_rec(DevScene Sc, Opts o, int n, char *recs, const float *scr, float *out9, int *okOut) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SRef ref = rec_ref(recs + size_t(i) * kRecBytes);
+    V3<float> lp;
+    const float ls = lens_state_eval(Sc, o, ref, lp);
+    float g[2] = {0.0f, 0.0f}, h[3] = {0.0f, 0.0f, 0.0f};
+    const float s2[2] = {scr[2 * i], scr[2 * i + 1]};
+    const bool ok = lens_jets(Sc, o, ref, s2, lp, g, h);
+    float *q = out9 + 9 * size_t(i);
+    q[0] = ls; q[1] = lp.x; q[2] = lp.y; q[3] = lp.z;
+    q[4] = g[0]; q[5] = g[1]; q[6] = h[0]; q[7] = h[1]; q[8] = h[2];
+    okOut[i] = ok ? 1 : 0;
+}
+
 __global__ void __launch_bounds__(64) k_gradhess_rec_f32(DevScene Sc, Opts o, int n, char *recs, const int *passOff, int totalPasses, float *grad, float *hess,
                                    int *okOut) {
     gradhess_rec<float>(blockIdx.x * blockDim.x + threadIdx.x, Sc, o, n, recs, passOff, totalPasses, grad, hess, okOut);

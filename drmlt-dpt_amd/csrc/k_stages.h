@@ -101,18 +101,37 @@ DEV void stage1_kernel(const DevScene &Sc, const Chains &ch, const DrParams &P, 
         stage1_walk(ch, P, a, g, w);
         POut<float> out;
         out.valid = false;
-        walk_run(Sc, P.o, ch, P.discreteStdDev, w, !a.h2mcMove, out);
+        walk_run(Sc, P.o, ch, P.discreteStdDev, w, !a.h2mcMove && !a.lensMove, out);
         const int flags = stage1_end(Sc, ch, P, chain, a, out, g, film, stats);
         // second stage: list 1 (every chain with one), list 0 (H2MC), list 4
-        // (H2MC needing the current state's derivatives)
+        // (H2MC needing the current state's derivatives); list 5: lens mutations
         const bool second = (flags & 2) != 0, h2mc = second && (flags & 4);
         const int pc = a.pc, bin = length_bin(ch.c[pc] + ch.l[pc] - 1);
         list_push_wave(L, 1, ch.C, bin, chain, second);
         list_push_wave(L, 0, ch.C, bin, chain, h2mc);
         list_push_wave(L, 4, ch.C, bin, chain,
                        h2mc && !ch.gInit[pc] && has_derivative(ch.c[pc], ch.l[pc], P.maxDervDepth) && double(ch.ss[pc]) > 1e-15);
+        list_push_wave(L, 5, ch.C, bin, chain, (flags & 128) != 0);
     }
 }
+
+#ifndef DRMLT_COUNT_WORK
+// h2mc with lens mutations: the lens score (and lensVertexPos) of every
+// current state that does not carry one yet, before stage 1 draws the lens
+// coin.
+__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_lens_score)(DevScene Sc, Chains ch, DrParams P) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.chainCount; i += gridDim.x * blockDim.x)
+        lens_score_ensure(Sc, ch, P, P.chainBase + i);
+}
+// The lens mutations of list 5, one per lane (each with its own ray queries).
+__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_lens)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
+                                                             unsigned long long *stats, Lists L) {
+    const uint32_t iter = *iterp;
+    const int n = list_length(L, 5);
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x)
+        lens_move(Sc, ch, P, list_item(L, 5, ch.C, it), iter, film, stats);
+}
+#endif
 
 // Gaussian work of one list item (a chain). kind 0 (which 0): the current
 // state's Gaussian unless it is cached, then the proposal offset. kind 2

@@ -16,6 +16,9 @@ __global__ void k_b_stage1_w3(DevScene, Chains, DrParams, const uint32_t *, floa
 __global__ void k_b_stage2_propose_w3(DevScene, Chains, DrParams, const uint32_t *, Lists);
 __global__ void k_b_stage2_accept_w3(DevScene, Chains, DrParams, uint32_t *, float *, unsigned long long *, Lists);
 __global__ void k_b_reset(Lists, uint32_t *, int);
+// h2mc lens mutations: lens scores of the current states, then the mutations of list 5
+__global__ void k_b_lens_score(DevScene, Chains, DrParams);
+__global__ void k_b_lens(DevScene, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
 // counting forms of the stage kernels (k_count.hip)
 __global__ void k_b_stage1_bin_counted(Chains, DrParams, const uint32_t *, Lists);
 __global__ void k_b_stage1_counted(DevSceneCounted, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
@@ -28,6 +31,7 @@ __global__ void k_prepass_seed(DevScene, Opts, Chains, int, uint64_t, const int 
 __global__ void k_direct(DevScene, Opts, int, uint64_t, float *);
 __global__ void k_trace(DevScene, int, const float *, const float *, const float *, const float *, int, int *, float *);
 __global__ void k_genbidir_rec(DevScene, Opts, int, int, int, const float *, int, int, float *, int *);
+__global__ void k_lens_rec(DevScene, Opts, int, char *, const float *, float *, int *);
 __global__ void k_logf_rec_f32(DevScene, Opts, int, const char *, double *, int *);
 __global__ void k_logf_rec_f64(DevScene, Opts, int, const char *, double *, int *);
 __global__ void k_gradhess_rec_f32(DevScene, Opts, int, char *, const int *, int, float *, float *, int *);

@@ -768,12 +768,13 @@ struct Loader {
             else if (nm == "uselightcoordinatesampling") o.use_light_coordinate_sampling = t;
             else if (nm == "largestepmultiplexed") o.large_step_multiplexed = t;
             else if (nm == "type") o.type = v == "mira" ? 1 : 0;
-            else if (nm == "anisoperturbstddev") o.aniso_perturb_std_dev = to_float(v, "anisoperturbstddev");
+            else if (nm == "anisoperturbstddev") o.aniso_perturb_std_dev = o.lens_perturb_std_dev = to_float(v, "anisoperturbstddev");
             else if (nm == "anisoperturbprob") o.aniso_perturb_prob = to_float(v, "anisoperturbprob");
             else if (nm == "acceptanceMap") o.acceptance_map = t;
             else if (nm == "uniformmixingprobability") o.uniform_mixing_prob = to_float(v, "uniformmixingprobability");
             else if (nm == "lensperturbprob") o.lens_perturb_prob = to_float(v, "lensperturbprob");
-            else if (nm == "bidirectional" || nm == "lensperturbstddev" || nm == "reportintervalspp") {
+            else if (nm == "lensperturbstddev") o.lens_perturb_std_dev = to_float(v, "lensperturbstddev");
+            else if (nm == "bidirectional" || nm == "reportintervalspp") {
                 // parsed by the reference, used by its other integrators only
             } else std::fprintf(stderr, "Unknown dpt option:%s\n", nm.c_str());
         }

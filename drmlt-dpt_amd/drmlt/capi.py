@@ -17,7 +17,7 @@ EXPORTS = [
     "drmlt_last_error", "drmlt_options_default", "drmlt_ctx_create", "drmlt_ctx_destroy", "drmlt_prepass",
     "drmlt_seed_chains", "drmlt_direct_lighting", "drmlt_run", "drmlt_read_film", "drmlt_read_stats", "drmlt_read_timing",
     "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
-    "drmlt_grad_hess", "drmlt_grad_hess_noalloc", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
+    "drmlt_grad_hess", "drmlt_grad_hess_noalloc", "drmlt_lens_grad_hess", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
     "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains", "drmlt_read_step", "drmlt_gaussian_eval_f64",
     "drmlt_path_func_derv", "drmlt_serialized_to_rec", "drmlt_material_from_serialized", "drmlt_bvh_info",
     "drmlt_write_exr", "drmlt_stats_text", "drmlt_write_stats", "drmlt_render",
@@ -52,7 +52,8 @@ class Options(C.Structure):
                 ("aniso_perturb_std_dev", C.c_float), ("acceptance_map", C.c_int32),
                 ("max_derivatives_depth", C.c_int32), ("quirk_ignore_aniso_std_dev", C.c_int32),
                 ("quirk_chad_mis", C.c_int32), ("integrator", C.c_int32), ("uniform_mixing_prob", C.c_float),
-                ("lens_perturb_prob", C.c_float), ("seed", C.c_uint64), ("host_bvh", C.c_int32)]
+                ("lens_perturb_prob", C.c_float), ("seed", C.c_uint64), ("host_bvh", C.c_int32),
+                ("lens_perturb_std_dev", C.c_float)]
 
 
 class Schedule(C.Structure):
@@ -66,7 +67,8 @@ class Schedule(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("large_step", "large_step_accepted", "stage1", "stage1_accepted", "stage2_iso",
-                                          "stage2_iso_accepted", "stage2_h2mc", "stage2_h2mc_accepted")]
+                                          "stage2_iso_accepted", "stage2_h2mc", "stage2_h2mc_accepted", "lens_step",
+                                          "lens_step_accepted")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -120,6 +122,7 @@ _KEYMAP = {
     "largestepmultiplexed": ("large_step_multiplexed", int), "anisoperturbprob": ("aniso_perturb_prob", float),
     "anisoperturbstddev": ("aniso_perturb_std_dev", float), "acceptanceMap": ("acceptance_map", int),
     "uniformmixingprobability": ("uniform_mixing_prob", float), "lensperturbprob": ("lens_perturb_prob", float),
+    "lensperturbstddev": ("lens_perturb_std_dev", float),
 }
 
 
@@ -135,6 +138,8 @@ def options_from_dpt(dpt: dict, **overrides):
         elif k in _KEYMAP:
             f, cast = _KEYMAP[k]
             setattr(o, f, cast(bool(v) if isinstance(v, bool) else v))
+            if k == "anisoperturbstddev":  # the reference's parser writes lensPerturbStdDev (parsescene.cpp:553-554)
+                o.lens_perturb_std_dev = float(v)
     for k, v in overrides.items():
         setattr(o, k, v)
     return o
@@ -305,6 +310,18 @@ class Context:
         check(lib().drmlt_eval_logf(self.h, C.c_int64(n), P(states), int(fp64), P(lf), P(ok)))
         return lf, ok
 
+    def lens_grad_hess(self, states, screen):
+        """Lens score and lensVertexPos of each state, and the gradient and
+        Hessian of log lensScore in the screen position at `screen` (n x 2):
+        returns (n x 9 floats, ok)."""
+        states = np.ascontiguousarray(states, PATH_REC)
+        n = len(states)
+        screen = np.ascontiguousarray(screen, np.float32).reshape(n, 2)
+        out = np.zeros((n, 9), np.float32)
+        ok = np.zeros(n, np.int32)
+        check(lib().drmlt_lens_grad_hess(self.h, C.c_int64(n), P(states), P(screen), P(out), P(ok)))
+        return out, ok
+
     def grad_hess(self, states, fp64=False):
         states = np.ascontiguousarray(states, PATH_REC)
         n = len(states)
@@ -417,7 +434,7 @@ def stats_text(stats: dict, reference_labels=True, integrator="drmlt"):
     """Stats::output of a stats dict (Context.stats()) as the reference prints
     it: the DR integrator's text (with its swapped labels, or corrected ones),
     or the mcmc / h2mc integrators' (small steps: first-stage plus H2MC moves)."""
-    s = Stats(*[int(stats[n]) for n, _ in Stats._fields_])
+    s = Stats(*[int(stats.get(n, 0)) for n, _ in Stats._fields_])
     fmt = 2 if integrator in ("mcmc", "h2mc") else int(bool(reference_labels))
     n = lib().drmlt_stats_text(C.byref(s), fmt, None, 0)
     if n < 0:

@@ -94,18 +94,23 @@ typedef struct {
      * (MCMCIntegrator, src/integrators/mcmc.cpp:11-244: the loop's first
      * stage alone -- a large or isotropic small step, splats weighted 1 - a
      * and a, acceptance with probability a; no acceptance map), 2 "h2mc"
-     * (H2MCIntegrator, src/integrators/h2mc.cpp:196-600, without lens
-     * perturbations: lens_perturb_prob must be 0 -- a small step is
-     * isotropic with probability uniform_mixing_prob, else one H2MC move
-     * accepted with min(1, ratio)). */
+     * (H2MCIntegrator, src/integrators/h2mc.cpp:196-600: a small step is
+     * isotropic with probability uniform_mixing_prob; otherwise, when the
+     * current state has a lens score, a lens mutation (PerturbLensBidir,
+     * path.cpp:1914-2067) with probability lens_perturb_prob, else one H2MC
+     * move; each accepted with min(1, ratio)). */
     int32_t integrator;
     float uniform_mixing_prob;  /* uniformmixingprobability (dptoptions.h:29, h2mc only) */
-    float lens_perturb_prob;    /* lensperturbprob (dptoptions.h:20); h2mc needs 0 here */
+    float lens_perturb_prob;    /* lensperturbprob (dptoptions.h:20, h2mc only) */
     uint64_t seed;
     /* 0 (default): the context builds its BVH on the device; 1: on the host.
      * Both run the same binned-SAH build and answer every ray query alike
      * (the reference builds with Embree, src/scene.cpp:42-54). */
     int32_t host_bvh;
+    /* lensperturbstddev (dptoptions.h:21, h2mc only): sigma of the lens
+     * Gaussian. The reference's XML key anisoperturbstddev also writes it
+     * (parsescene.cpp:553-554); the scene loader does the same. */
+    float lens_perturb_std_dev;
 } drmlt_options;
 
 /* One path state (the reference's Path + SubpathContrib, src/path.h:12-107):
@@ -129,6 +134,8 @@ typedef struct {
     uint64_t stage1, stage1_accepted;
     uint64_t stage2_iso, stage2_iso_accepted;
     uint64_t stage2_h2mc, stage2_h2mc_accepted;
+    /* h2mc integrator: lens mutations and their acceptances (h2mc.cpp:109-115) */
+    uint64_t lens_step, lens_step_accepted;
 } drmlt_stats;
 
 /* Launch schedule of the batched form. The chains advance identically under
@@ -240,7 +247,7 @@ int drmlt_write_exr(const char *path, const float *rgb, int32_t w, int32_t h, in
  * reference's swapped second-stage labels (delayed.cpp:320-321), 0 names
  * each line's own component, 2 is the mcmc and h2mc integrators' text
  * (mcmc.cpp:76-86, h2mc.cpp:140-150: large steps, small steps -- first-stage
- * plus H2MC moves -- and lens steps, which neither makes here, at 0). */
+ * plus H2MC moves -- and lens steps). */
 int drmlt_stats_text(const drmlt_stats *stats, int32_t format, char *buf, int32_t cap);
 /* The same text of the context's counters into a file (truncated); a context
  * of the mcmc or h2mc integrator writes format 2 whatever reference_labels says. */
@@ -359,6 +366,16 @@ int drmlt_grad_hess(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int
  * GPU, held from the library's load until the process ends. Calls that
  * overlap on one GPU share that array and race. */
 int drmlt_grad_hess_noalloc(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, int32_t fp64, double *grad, double *hess, int32_t *ok);
+
+/* The h2mc integrator's lens derivative functions (chad's
+ * evaluate_path_bidir_{c}_{l}_lens _derv, PathFuncMode::Lens, path.cpp:
+ * 2987-3175; call site h2mc.cpp:473-481), batched: for each record its lens
+ * score and lensVertexPos (the fixed-topology re-evaluation that the lens
+ * mutation reads) and the gradient and Hessian of log lensScore in the
+ * screen position at screen[2i], screen[2i + 1], FP32. out9[9i..] =
+ * lensScore, lensVertexPos (3), d/ds0, d/ds1, H00, H01, H11; ok[i] = 0 where
+ * the lens function has no contribution. Every vertex must name a triangle. */
+int drmlt_lens_grad_hess(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, const float *screen, float *out9, int32_t *ok);
 
 /* The reference's derivative call with its own argument set, batched:
  * PathFuncDerv(lens, primary, sceneParams, vertParams, grad, hess)

@@ -279,8 +279,6 @@ def test_h2mc_integrator(which, request):
 
     sc = request.getfixturevalue(which)
     dpt = dict(sc["options"], integrator="h2mc", lensperturbprob=0.0)
-    with pytest.raises(capi.DrmltError, match="lensperturbprob"):
-        capi.Context(sc, capi.options_from_dpt(dict(dpt, lensperturbprob=0.3), num_chains=1024))
     res = {}
     for mode in (capi.PLAIN, capi.BATCHED, capi.BATCHED_GRAPH, "split"):
         ctx = capi.Context(sc, capi.options_from_dpt(dpt, num_init_samples=8192, num_chains=4096, seed=5))

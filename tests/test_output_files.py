@@ -150,3 +150,7 @@ def test_h2mc_stats_text():
              stage2_h2mc=80, stage2_h2mc_accepted=30)
     t = capi.stats_text(s, integrator="h2mc").splitlines()
     assert t[3] == "    Small step mutations acceptance ratio       : 0.4 (100)"
+    # lens mutations on their own line
+    t = capi.stats_text(dict(s, lens_step=40, lens_step_accepted=30), integrator="h2mc").splitlines()
+    assert t[3] == "    Small step mutations acceptance ratio       : 0.4 (100)"
+    assert t[4] == "    Lens mutations acceptance ratio    :                    0.75 (40)"
