@@ -265,7 +265,8 @@ def config2_depth9(steps, warmup, tune_steps):
 def accepted(st):
     """Accepted mutations in a stats dict: accept.first or accept.second of an
     iteration (delayed.cpp:565), counted per stage."""
-    return st["large_step_accepted"] + st["stage1_accepted"] + st["stage2_iso_accepted"] + st["stage2_h2mc_accepted"]
+    return (st["large_step_accepted"] + st["stage1_accepted"] + st["stage2_iso_accepted"] + st["stage2_h2mc_accepted"] +
+            st.get("lens_step_accepted", 0))
 
 
 def copy_calibration(mb):
@@ -418,10 +419,11 @@ def main():
     ctx.close()
 
     # the mcmc integrator (the first stage alone) and the h2mc integrator
-    # (lens moves off) on the same scene and chains
+    # (lens mutations at the reference's lensperturbprob 0.3) on the same
+    # scene and chains
     others = {}
     if not args.no_mcmc:
-        for name, extra in (("mcmc", {}), ("h2mc", {"lensperturbprob": 0.0})):
+        for name, extra in (("mcmc", {}), ("h2mc", {"lensperturbprob": 0.3})):
             mctx = capi.Context(sc, capi.options_from_dpt(dict(dpt, integrator=name, **extra), num_chains=chains, seed=1234))
             mctx.set_schedule(**sched)  # the path-stage scheduling tuned for these chains
             mctx.prepass()
