@@ -307,6 +307,8 @@ DEV void lens_move(const DevScene &Sc, const Chains &ch, const DrParams &P, int 
     count_if(stats + ST_LENS_ACC, acc);
     ch.a1[chain] = a;
     ch.flags[chain] = 128 | (acc ? 1 : 0) | (out.valid ? 32 : 0);
+    chain_off(ch, chain)[0] = off[0];  // the screen offset, for drmlt_read_step
+    chain_off(ch, chain)[1] = off[1];
     const int W = Sc.cam.W, H = Sc.cam.H;
     const float wx = 1.0f - a;
     if (wx > 0.0f) splat(ch, P, W, H, film, pc, wx);

@@ -1377,19 +1377,22 @@ extern "C" int drmlt_eval_logf(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st
     return DRMLT_OK;
 }
 
-extern "C" int drmlt_lens_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st, const float *screen, float *out9, int32_t *okOut) {
+extern "C" int drmlt_lens_grad_hess(drmlt_ctx *c, int64_t n, const drmlt_path_rec *st, const float *screen, const float *lensPos, float *out9,
+                                    int32_t *okOut) {
     if (!c || n < 0 || !st || !screen || !out9 || !okOut) return fail(DRMLT_ERR_ARG, "bad argument");
     if (!recs_ok(c, st, n) || !recs_fixed_ok(st, n)) return fail(DRMLT_ERR_ARG, "path record out of range");
     if (n == 0) return DRMLT_OK;
     hipStream_t s = c->stream;
     DBuf<char> r;
-    DBuf<float> sc, o9;
+    DBuf<float> sc, lp, o9;
     DBuf<int> ok;
     DUP(r, (const char *)st, n * sizeof(drmlt_path_rec));
     DUP(sc, screen, 2 * n);
+    if (lensPos) DUP(lp, lensPos, 3 * n);
     DNEW(o9, 9 * n); DNEW(ok, n);
     const DrParams P = make_params(c);
-    hipLaunchKernelGGL(k_lens_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.o, int(n), r.p, sc.p, o9.p, ok.p);
+    hipLaunchKernelGGL(k_lens_rec, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, c->S, P.o, int(n), r.p, sc.p, lensPos ? lp.p : nullptr, o9.p,
+                       ok.p);
     HIPCHK(hipGetLastError());
     DGET(out9, o9, 9 * n);
     DGET(okOut, ok, n);
@@ -1455,6 +1458,31 @@ extern "C" int drmlt_read_step(drmlt_ctx *c, int64_t n, drmlt_path_rec *slots, f
         alpha2[i] = (h.flags[i] & 2) ? h.a2[i] : 0.0;
         for (int k = 0; k < 32; k++) off[32 * i + k] = h.off[32 * size_t(i) + k];
     }
+    return DRMLT_OK;
+}
+
+extern "C" int drmlt_read_lens(drmlt_ctx *c, int64_t n, float *lscore, int32_t *lflags, float *lensPos) {
+    if (!c || n < 0 || !lscore || !lflags || !lensPos) return fail(DRMLT_ERR_ARG, "bad argument");
+    if (!c->haveChains) return fail(DRMLT_ERR_STATE, "chains are not seeded (call drmlt_prepass or drmlt_seed_chains)");
+    if (n > c->ch.C) return fail(DRMLT_ERR_ARG, "more chains asked for than there are");
+    if (n == 0) return DRMLT_OK;
+    HostChains h;
+    if (int r = fetch_chains(c, h, size_t(n), false)) return r;
+    const size_t C = size_t(c->ch.C);
+    std::vector<double> lg;
+    if (c->ch.lensG) {
+        // lens slots of physical slots 0..2: slot k of chain i at k * C + i
+        lg.resize(3 * C * kLSlot);
+        HIPCHK(hipMemcpy(lg.data(), c->ch.lensG, lg.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int64_t i = 0; i < n; i++)
+        for (int k = 0; k < 4; k++) {
+            const uint32_t *w = h.slot(size_t(i), k);
+            lscore[4 * i + k] = wget<float>(w, SO_LENS);
+            lflags[4 * i + k] = wget<int32_t>(w, SO_LFLAGS);
+            for (int d = 0; d < 3; d++)
+                lensPos[12 * i + 3 * k + d] = (k < 3 && !lg.empty()) ? float(lg[(size_t(k) * C + size_t(i)) * kLSlot + 9 + d]) : 0.0f;
+        }
     return DRMLT_OK;
 }
 

@@ -48,7 +48,8 @@ __global__ void __launch_bounds__(64) k_logf_rec_f64(DevScene Sc, Opts o, int n,
 // the lens function at screen position scr[2i], scr[2i + 1] (lens_jets):
 // out9[9i..] = lensScore, lensPos (3), d/ds0, d/ds1, H00, H01, H11;
 // ok[i] = 0 where the lens function has no contribution there.
-__global__ void __launch_bounds__(64) k_lens_rec(DevScene Sc, Opts o, int n, char *recs, const float *scr, float *out9, int *okOut) {
+__global__ void __launch_bounds__(64) k_lens_rec(DevScene Sc, Opts o, int n, char *recs, const float *scr, const float *lpIn, float *out9,
+                                                 int *okOut) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const SRef ref = rec_ref(recs + size_t(i) * kRecBytes);
@@ -56,7 +57,8 @@ __global__ void __launch_bounds__(64) k_lens_rec(DevScene Sc, Opts o, int n, cha
     const float ls = lens_state_eval(Sc, o, ref, lp);
     float g[2] = {0.0f, 0.0f}, h[3] = {0.0f, 0.0f, 0.0f};
     const float s2[2] = {scr[2 * i], scr[2 * i + 1]};
-    const bool ok = lens_jets(Sc, o, ref, s2, lp, g, h);
+    const V3<float> lpJ = lpIn ? V3<float>(lpIn[3 * i], lpIn[3 * i + 1], lpIn[3 * i + 2]) : lp;
+    const bool ok = lens_jets(Sc, o, ref, s2, lpJ, g, h);
     float *q = out9 + 9 * size_t(i);
     q[0] = ls; q[1] = lp.x; q[2] = lp.y; q[3] = lp.z;
     q[4] = g[0]; q[5] = g[1]; q[6] = h[0]; q[7] = h[1]; q[8] = h[2];

@@ -31,7 +31,7 @@ __global__ void k_prepass_seed(DevScene, Opts, Chains, int, uint64_t, const int 
 __global__ void k_direct(DevScene, Opts, int, uint64_t, float *);
 __global__ void k_trace(DevScene, int, const float *, const float *, const float *, const float *, int, int *, float *);
 __global__ void k_genbidir_rec(DevScene, Opts, int, int, int, const float *, int, int, float *, int *);
-__global__ void k_lens_rec(DevScene, Opts, int, char *, const float *, float *, int *);
+__global__ void k_lens_rec(DevScene, Opts, int, char *, const float *, const float *, float *, int *);
 __global__ void k_logf_rec_f32(DevScene, Opts, int, const char *, double *, int *);
 __global__ void k_logf_rec_f64(DevScene, Opts, int, const char *, double *, int *);
 __global__ void k_gradhess_rec_f32(DevScene, Opts, int, char *, const int *, int, float *, float *, int *);

@@ -18,7 +18,7 @@ EXPORTS = [
     "drmlt_seed_chains", "drmlt_direct_lighting", "drmlt_run", "drmlt_read_film", "drmlt_read_stats", "drmlt_read_timing",
     "drmlt_count_work", "drmlt_trace", "drmlt_perturb", "drmlt_generate_subpath", "drmlt_generate_bidir",
     "drmlt_grad_hess", "drmlt_grad_hess_noalloc", "drmlt_lens_grad_hess", "drmlt_compute_gaussian_f64", "drmlt_gaussian_logpdf_f64", "drmlt_dr_alpha2_f64",
-    "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains", "drmlt_read_step", "drmlt_gaussian_eval_f64",
+    "drmlt_get_schedule", "drmlt_set_schedule", "drmlt_autotune", "drmlt_eval_logf", "drmlt_read_chains", "drmlt_read_step", "drmlt_read_lens", "drmlt_gaussian_eval_f64",
     "drmlt_path_func_derv", "drmlt_serialized_to_rec", "drmlt_material_from_serialized", "drmlt_bvh_info",
     "drmlt_write_exr", "drmlt_stats_text", "drmlt_write_stats", "drmlt_render",
     "drmlt_scene_load", "drmlt_scene_get", "drmlt_scene_output", "drmlt_scene_free",
@@ -291,7 +291,8 @@ class Context:
     def read_step(self, n=None):
         """What the last iteration left in chains 0..n-1: a dict with the four
         physical slots (n x 4 records and n x 4 x 8 contrib8 rows), perm,
-        flags, a1, alpha2 and the H2MC offset (n x 32)."""
+        flags, a1, alpha2 and the H2MC offset (n x 32; after a lens
+        mutation, flag bit 7, its screen offset in the first two entries)."""
         n = int(self.opt.num_chains if n is None else n)
         slots = np.zeros((n, 4), PATH_REC)
         c8 = np.zeros((n, 4, 8), np.float32)
@@ -299,6 +300,18 @@ class Context:
         a2 = np.zeros(n); off = np.zeros((n, 32), np.float32)
         check(lib().drmlt_read_step(self.h, C.c_int64(n), P(slots), P(c8), P(perm), P(flags), P(a1), P(a2), P(off)))
         return {"slots": slots, "contrib8": c8, "perm": perm, "flags": flags, "a1": a1, "alpha2": a2, "off": off}
+
+    def read_lens(self, n=None):
+        """The lens score the loop holds for each of the four physical slots
+        of chains 0..n-1, its lens flags (bit 0 score known, bit 1 lens
+        Gaussian initialised) and the lensVertexPos held with it:
+        (n x 4 floats, n x 4 ints, n x 4 x 3 floats)."""
+        n = int(self.opt.num_chains if n is None else n)
+        ls = np.zeros((n, 4), np.float32)
+        lf = np.zeros((n, 4), np.int32)
+        lp = np.zeros((n, 4, 3), np.float32)
+        check(lib().drmlt_read_lens(self.h, C.c_int64(n), P(ls), P(lf), P(lp)))
+        return ls, lf, lp
 
     def eval_logf(self, states, fp64=False):
         """log f at each state with its triangles fixed (the function
@@ -310,16 +323,18 @@ class Context:
         check(lib().drmlt_eval_logf(self.h, C.c_int64(n), P(states), int(fp64), P(lf), P(ok)))
         return lf, ok
 
-    def lens_grad_hess(self, states, screen):
+    def lens_grad_hess(self, states, screen, lens_pos=None):
         """Lens score and lensVertexPos of each state, and the gradient and
-        Hessian of log lensScore in the screen position at `screen` (n x 2):
-        returns (n x 9 floats, ok)."""
+        Hessian of log lensScore in the screen position at `screen` (n x 2),
+        about the lensVertexPos `lens_pos` (n x 3; None: the re-evaluated
+        one): returns (n x 9 floats, ok)."""
         states = np.ascontiguousarray(states, PATH_REC)
         n = len(states)
         screen = np.ascontiguousarray(screen, np.float32).reshape(n, 2)
+        lp = None if lens_pos is None else np.ascontiguousarray(lens_pos, np.float32).reshape(n, 3)
         out = np.zeros((n, 9), np.float32)
         ok = np.zeros(n, np.int32)
-        check(lib().drmlt_lens_grad_hess(self.h, C.c_int64(n), P(states), P(screen), P(out), P(ok)))
+        check(lib().drmlt_lens_grad_hess(self.h, C.c_int64(n), P(states), P(screen), None if lp is None else P(lp), P(out), P(ok)))
         return out, ok
 
     def grad_hess(self, states, fp64=False):

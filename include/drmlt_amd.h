@@ -307,11 +307,23 @@ int drmlt_read_chains(drmlt_ctx *ctx, int64_t n, drmlt_path_rec *states, float *
  * current state, of the first- and of the second-stage proposal); flags
  * (bit 0 first stage accepted, 1 second stage run, 2 H2MC second stage,
  * 3 large step, 4 second-stage proposal valid, 5 first-stage proposal
- * valid, 6 y* traced and valid); a1; alpha2, the second-stage
- * acceptance (0 where no second stage ran); off, the H2MC offset (n x 32).
+ * valid, 6 y* traced and valid, 7 lens mutation, h2mc.cpp:463-530); a1;
+ * alpha2, the second-stage acceptance (0 where no second stage ran); off, the
+ * H2MC offset (n x 32), or after a lens mutation its screen offset in
+ * off[0..1].
  * Read before an iteration, perm names the roles the slots take in it. */
 int drmlt_read_step(drmlt_ctx *ctx, int64_t n, drmlt_path_rec *slots, float *contrib8, int32_t *perm, int32_t *flags, float *a1,
                     double *alpha2, float *off);
+
+/* The h2mc integrator's per-state lens bookkeeping of the four physical slots
+ * of chains 0..n-1 (the lensScore of the reference's Contrib and the
+ * MarkovState's lens Gaussian flag, h2mc.cpp:463-530): lscore (n x 4), the
+ * lens score the loop holds for the slot's state, and lflags (n x 4), bit 0
+ * set where that score is known (computed or carried over from a lens
+ * mutation), bit 1 where the state's lens Gaussian is initialised; and
+ * lens_pos (n x 4 x 3), the lensVertexPos held with that score (zeros for
+ * slot 3, which holds none, and where no lens mutation is enabled). */
+int drmlt_read_lens(drmlt_ctx *ctx, int64_t n, float *lscore, int32_t *lflags, float *lens_pos);
 
 /* ---------------- tier 2: batched functions ---------------- */
 
@@ -372,10 +384,13 @@ int drmlt_grad_hess_noalloc(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *sta
  * 2987-3175; call site h2mc.cpp:473-481), batched: for each record its lens
  * score and lensVertexPos (the fixed-topology re-evaluation that the lens
  * mutation reads) and the gradient and Hessian of log lensScore in the
- * screen position at screen[2i], screen[2i + 1], FP32. out9[9i..] =
- * lensScore, lensVertexPos (3), d/ds0, d/ds1, H00, H01, H11; ok[i] = 0 where
- * the lens function has no contribution. Every vertex must name a triangle. */
-int drmlt_lens_grad_hess(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, const float *screen, float *out9, int32_t *ok);
+ * screen position at screen[2i], screen[2i + 1], FP32, about the
+ * lensVertexPos lens_pos[3i..3i+2] (n x 3; NULL: the re-evaluated one).
+ * out9[9i..] = lensScore, lensVertexPos (3, re-evaluated), d/ds0, d/ds1,
+ * H00, H01, H11; ok[i] = 0 where the lens function has no contribution.
+ * Every vertex must name a triangle. */
+int drmlt_lens_grad_hess(drmlt_ctx *ctx, int64_t n, const drmlt_path_rec *states, const float *screen, const float *lens_pos, float *out9,
+                         int32_t *ok);
 
 /* The reference's derivative call with its own argument set, batched:
  * PathFuncDerv(lens, primary, sceneParams, vertParams, grad, hess)

@@ -267,3 +267,116 @@ def worst(r, k=5):
             out.append({"what": key, "chain": int(i), "err": float(d[i]), "flags": int(r["flags"][i]), "gpu": float(r[key + "_gpu"][i]),
                         "B": float(r[key + "_B"][i]), "a2pre": float(r["alpha2_pre_B"][i]), "astar": float(r["astar_B"][i])})
     return out
+
+
+def _lens_gaussian(c, l, ls, ss, g, h, ok, sigma, max_derv):
+    """initLensGaussian (h2mc.cpp:466-499) from given lens derivatives: the
+    lens Gaussian exists for c + l > 3 within the derivative depth, with the
+    derivatives zeroed where lensScore <= 1e-15 or they are not finite;
+    isotropic otherwise (gaussian.cpp:4-16)."""
+    if not (c >= 1 and l >= 0 and c + l > 3 and c + l - 1 <= max_derv):
+        return _iso_gaussian(2, sigma)
+    g32, h32 = np.zeros(32), np.zeros((32, 32))
+    if ls > 1e-15 and ok and np.isfinite(g).all() and np.isfinite(h).all():
+        g32[:2] = g
+        h32[:2, :2] = h
+    m, _, inv, ld = orc.compute_gaussian(np.array([2]), np.array([ss]), sigma, g32[None], h32[None])
+    return m[0], inv[0], float(ld[0])
+
+
+def _lens_accept(off, gx, gy, ls_x, ls_y):
+    # h2mc.cpp:516-523: min(1, N_y(-off) / N_x(off) * lensScore_y / lensScore_x)
+    py = orc.gaussian_logpdf(2, off, gx[0][:2], gx[1][:32, :32], gx[2])
+    px = orc.gaussian_logpdf(2, -off, gy[0][:2], gy[1][:32, :32], gy[2])
+    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+        a = float(np.exp(px - py) * ls_y / ls_x)
+    a = a if np.isnan(a) else min(max(a, 0.0), 1.0)
+    return 0.0 if _invalid(np.array([a]))[0] else a
+
+
+def analyse_lens(O, ctx, before, after, lens_after, sigma, max_derv=9, fd_step=2e-6):
+    """Per-state parity of the h2mc integrator's lens mutations
+    (h2mc.cpp:463-530) in one iteration. For each chain that made one (flag
+    bit 7), the lens offset the loop drew (read back in off[0..1]) moves the
+    current state x to the screen position the loop perturbed to; then
+    (A) the oracle's PerturbLensBidir in FP64 from x to that screen position
+        against the loop's proposal y: validity, triangles, f, lens score;
+    (B) the loop's acceptance a1 against the reference's formula computed
+        here with the oracle's FP64 ComputeGaussian / GaussianLogPdf over the
+        lens scores and lensVertexPos the loop held for x and y
+        (drmlt_read_lens after the iteration, lens_after) and the device's
+        lens derivatives about those positions (drmlt_lens_grad_hess);
+    (C) the same acceptance with y's lens score from the oracle's FP64
+        PerturbLensBidir and both lens Gaussians from central differences
+        (step fd_step) of the oracle's FP64 lens function, as the oracle's
+        own loop computes them; x's lens score is the one the loop held (the
+        score of the move that made x, which for a lens mutation's result is
+        that mutation's tracked score, h2mc.cpp:520-530)."""
+    L = np.nonzero((after["flags"] & 128) != 0)[0]
+    pb = before["perm"][L]
+    pc, pf = pb & 3, (pb >> 2) & 3
+    X, x8 = before["slots"][L, pc], before["contrib8"][L, pc]
+    Y, y8 = after["slots"][L, pf], after["contrib8"][L, pf]
+    off = after["off"][L, :2].astype(np.float32)
+    a1 = after["a1"][L].astype(np.float64)
+    y_dev = (after["flags"][L] & 32) != 0
+    scr = x8[:, 6:8] + off
+    scr = (scr - np.floor(scr)).astype(np.float32)
+    Yo, c9 = O.perturb_lens(X, scr, dbl=True)
+    y_o = c9[:, 0] > 0
+    both = y_dev & y_o
+    fb = both & (c9[:, 5] > 0)
+    f_rel = np.abs(y8[fb, 5].astype(np.float64) - c9[fb, 5]) / c9[fb, 5]
+    ls_x, ls_y = lens_after[0][L, pc].astype(np.float64), lens_after[0][L, pf].astype(np.float64)
+    known = ((lens_after[1][L, pc] & 1) != 0) & (~y_dev | ((lens_after[1][L, pf] & 1) != 0))
+    lp_x, lp_y = lens_after[2][L, pc], lens_after[2][L, pf]
+    ox, okx = ctx.lens_grad_hess(X, x8[:, 6:8], lp_x)
+    oy, oky = np.zeros((len(L), 9), np.float32), np.zeros(len(L), np.int32)
+    if y_dev.any():
+        oy[y_dev], oky[y_dev] = ctx.lens_grad_hess(Y[y_dev], y8[y_dev, 6:8], lp_y[y_dev])
+    lb = both & (c9[:, 8] > 0)
+    ls_rel = np.abs(ls_y[lb] - c9[lb, 8]) / c9[lb, 8]
+    gyf, hyf = np.zeros((len(L), 2)), np.zeros((len(L), 2, 2))
+    _, gxf, hxf = O.lens_grad_hess_fd(X, x8[:, 6:8], h=fd_step)
+    if y_o.any():
+        _, gyf[y_o], hyf[y_o] = O.lens_grad_hess_fd(Yo[y_o], c9[y_o, 6:8], h=fd_step)
+    aB, aC = np.zeros(len(L)), np.zeros(len(L))
+    for k in range(len(L)):
+        cx, lx = int(X["c"][k]), int(X["l"][k])
+        h2 = lambda o: np.array([[o[6], o[7]], [o[7], o[8]]], np.float64)
+        if y_dev[k]:
+            gx = _lens_gaussian(cx, lx, ls_x[k], float(x8[k, 5]), ox[k, 4:6].astype(np.float64), h2(ox[k]), okx[k], sigma, max_derv)
+            gy = _lens_gaussian(cx, lx, ls_y[k], float(y8[k, 5]), oy[k, 4:6].astype(np.float64), h2(oy[k]), oky[k], sigma, max_derv)
+            aB[k] = _lens_accept(off[k].astype(np.float64), gx, gy, ls_x[k], ls_y[k])
+        if y_o[k]:
+            gx = _lens_gaussian(cx, lx, ls_x[k], float(x8[k, 5]), gxf[k], hxf[k], True, sigma, max_derv)
+            gy = _lens_gaussian(cx, lx, float(c9[k, 8]), float(c9[k, 5]), gyf[k], hyf[k], True, sigma, max_derv)
+            aC[k] = _lens_accept(off[k].astype(np.float64), gx, gy, ls_x[k], float(c9[k, 8]))
+    return {"n": len(after["perm"]), "lens": len(L), "y_valid_agree": y_dev == y_o, "score_known": known,
+            "tri_equal": _same_triangles(Y[both], Yo[both]), "f_rel": f_rel, "ls_rel": ls_rel,
+            "a1_gpu": a1, "a1_B": aB, "a1_C": aC, "chains": L, "X": X, "ls_x": ls_x, "ls_y": ls_y,
+            "ls_x_fixed": ox[:, 0].astype(np.float64), "ls_y_fixed": oy[:, 0].astype(np.float64), "ls_y_oracle": c9[:, 8].astype(np.float64)}
+
+
+def summary_lens(r):
+    dB, dC = np.abs(r["a1_gpu"] - r["a1_B"]), np.abs(r["a1_gpu"] - r["a1_C"])
+    pct = lambda a, q: float(np.percentile(a, q)) if len(a) else 0.0
+    return {"chains": r["n"], "lens": r["lens"], "valid_agree": float(np.mean(r["y_valid_agree"])), "score_known": float(np.mean(r["score_known"])),
+            "tri_equal": float(np.mean(r["tri_equal"])) if len(r["tri_equal"]) else 1.0,
+            "f_compared": int(len(r["f_rel"])), "f_rel_median": pct(r["f_rel"], 50), "f_rel_p99": pct(r["f_rel"], 99),
+            "ls_rel_median": pct(r["ls_rel"], 50), "ls_rel_p99": pct(r["ls_rel"], 99),
+            "a1_B_within_1e-5": float(np.mean(dB <= 1e-5)), "a1_B_max": float(dB.max()) if len(dB) else 0.0,
+            "a1_C_within_1e-3": float(np.mean(dC <= 1e-3)), "a1_C_median": pct(dC, 50), "a1_C_p99": pct(dC, 99),
+            "accept_mean_gpu": float(r["a1_gpu"].mean()) if len(dB) else 0.0}
+
+
+def worst_lens(r, k=8):
+    """The lens chains with the largest (B) disagreement, for the log."""
+    d = np.abs(r["a1_gpu"] - r["a1_B"])
+    out = []
+    for i in np.argsort(-d)[:k]:
+        out.append({"chain": int(r["chains"][i]), "err": float(d[i]), "c": int(r["X"]["c"][i]), "l": int(r["X"]["l"][i]),
+                    "gpu": float(r["a1_gpu"][i]), "B": float(r["a1_B"][i]), "C": float(r["a1_C"][i]),
+                    "ls_x": float(r["ls_x"][i]), "ls_x_fixed": float(r["ls_x_fixed"][i]), "ls_y": float(r["ls_y"][i]),
+                    "ls_y_fixed": float(r["ls_y_fixed"][i]), "ls_y_oracle": float(r["ls_y_oracle"][i])})
+    return out

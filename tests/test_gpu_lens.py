@@ -43,7 +43,10 @@ def test_lens_score_and_derivatives_vs_oracle(which, request):
     ctx = capi.Context(sc, capi.options_from_dpt(dict(sc["options"], integrator="h2mc"), num_chains=1024, seed=3))
     say(f"lens_grad_hess {which} n={len(st)}")
     out, ok = ctx.lens_grad_hess(st, c8[:, 6:8])
+    # the same lensVertexPos passed in explicitly gives the same derivatives
+    out2, ok2 = ctx.lens_grad_hess(st, c8[:, 6:8], out[:, 1:4])
     ctx.close()
+    assert np.array_equal(ok, ok2) and np.array_equal(out, out2)
     ref = O.lens_score(st, dbl=True)
     rel = np.abs(out[:, 0] - ref[:, 0]) / ref[:, 0]
     say("lens score rel median / p99 / max", np.median(rel), np.percentile(rel, 99), rel.max())

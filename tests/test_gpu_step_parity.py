@@ -94,3 +94,39 @@ def test_step_acceptances_per_state_plain_form(request):
 
     r, s = _run(request, "cfg2_small", capi.PLAIN)
     _check(r, s)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cfg2_small"])
+def test_lens_step_acceptances_per_state(name, request):
+    """The h2mc integrator's lens mutations (lensperturbprob 0.5), one
+    iteration from identical states: each lens proposal against the oracle's
+    FP64 PerturbLensBidir from the same state to the same screen position
+    (validity, triangles, f, lens score), and the loop's acceptance against
+    h2mc.cpp:516-523 computed from the device's lens scores and lens
+    derivatives with the oracle's FP64 Gaussian, to 1e-5 absolute
+    (tests/step_parity.py, analyse_lens). The acceptance with the oracle's
+    own lens scores and central-difference derivatives is reported beside
+    it."""
+    from drmlt import capi
+
+    sc = request.getfixturevalue(name)
+    dpt = dict(sc["options"], integrator="h2mc", lensperturbprob=0.5)
+    opt = capi.options_from_dpt(dpt, num_init_samples=50000, num_chains=8192, seed=78)
+    ctx = capi.Context(sc, opt)
+    say(f"lens step parity {name}: pre-pass, 3 + 1 iterations")
+    ctx.prepass()
+    before, after = sp.run_one_iteration(ctx, 3, capi.BATCHED_GRAPH)
+    lens_after = ctx.read_lens()
+    O = orc.Scene(sc)
+    say(f"lens step parity {name}: oracle lens perturbation, scores and derivatives")
+    r = sp.analyse_lens(O, ctx, before, after, lens_after, float(opt.lens_perturb_std_dev), max_derv=int(opt.max_derivatives_depth))
+    ctx.close()
+    s = sp.summary_lens(r)
+    say(f"lens step parity {name}: {s}")
+    say(f"lens step parity {name}: worst {sp.worst_lens(r)}")
+    assert s["lens"] > 0.1 * s["chains"]
+    assert s["score_known"] == 1.0, s
+    assert s["valid_agree"] >= 0.995, s
+    assert s["tri_equal"] >= 0.995, s
+    assert s["f_rel_median"] <= 1e-5 and s["ls_rel_median"] <= 1e-5, s
+    assert s["a1_B_within_1e-5"] >= 0.999, s
