@@ -223,27 +223,46 @@ DEV void stage2_propose_kernel(const DevScene &Sc, const Chains &ch, const DrPar
     }
 }
 
-// Second-stage acceptance over the same chains; Green's y* walks where they
-// are needed.
+// Second-stage acceptance, in two kernels so that the y* walks run with
+// every lane busy: the first takes every chain with a second stage (list 1,
+// or chain order) and finishes those that trace no y* (one lane each, no
+// walk); it hands the others to list 6, binned by the second proposal's
+// length, which the second takes with the path-stage kernels' lane layout.
+// Both draw the chain's stage-3 stream from its start, so a chain's draws
+// do not depend on which kernel finishes it.
+__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_finish)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
+                                                                     unsigned long long *stats, Lists L) {
+    const uint32_t iter = *iterp;
+    const int n = P.binStages ? list_length(L, 1) : P.chainCount;
+    for (int it = int(blockIdx.x * blockDim.x + threadIdx.x); it < n; it += int(gridDim.x * blockDim.x)) {
+        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : P.chainBase + it;
+        const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
+        Stage3 a;
+        a.traceStar = false;
+        if (act) a = stage3_begin(ch, P, chain, false);
+        if (act && !a.traceStar) {
+            Philox g(P.seed, iter, chain, TAG_S3);
+            POut<float> out;
+            out.valid = false;
+            stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
+        }
+        list_push_wave(L, 6, ch.C, a.traceStar ? length_bin(ch.c[a.ps] + ch.l[a.ps] - 1) : 0, chain, a.traceStar);
+    }
+}
 DEV void stage2_accept_kernel(const DevScene &Sc, const Chains &ch, const DrParams &P, const uint32_t *iterp, float *film,
                               unsigned long long *stats, const Lists &L) {
     const uint32_t iter = *iterp;
-    const int n = P.binStages ? list_length(L, 1) : P.chainCount;
+    const int n = list_length(L, 6);
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
-        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : P.chainBase + it;
-        const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
+        const int chain = list_item(L, 6, ch.C, it);
         Philox g(P.seed, iter, chain, TAG_S3);
-        Stage3 a;
+        const Stage3 a = stage3_begin(ch, P, chain);
         Walk w;
-        a.traceStar = false;
-        if (act) {
-            a = stage3_begin(ch, P, chain);
-            if (a.traceStar) stage3_walk(ch, P, a, g, w);
-        }
+        stage3_walk(ch, P, a, g, w);
         POut<float> out;
         out.valid = false;
-        walk_run(Sc, P.o, ch, P.discreteStdDev, w, a.traceStar, out);
-        if (act) stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
+        walk_run(Sc, P.o, ch, P.discreteStdDev, w, true, out);
+        stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
     }
 }
 
