@@ -155,8 +155,9 @@ DEV void gauss_item(const X &x, const DevScene &Sc, const Chains &ch, const DrPa
 }
 
 // Bins 0..kGSmallBins-1 of a list hold paths of length <= 4 (D = 2 max(b, 2)
-// <= kGSmallD).
+// <= kGSmallD), bins below kGMidBins those of length <= 8 (D <= kGMidD).
 constexpr int kGSmallBins = kGSmallD / 2 + 1;
+constexpr int kGMidBins = kGMidD / 2 + 1;
 DEV int list_prefix(const Lists &L, int kind, int nbins) {
     int n = 0;
     for (int b = 0; b < nbins; b++) n += L.count[kind * kMaxBins + b];
@@ -178,21 +179,25 @@ DEV void gauss_groups(const DevScene &Sc, const Chains &ch, const DrParams &P, c
 // One-wave workgroups over a Gaussian list: first the small Gaussians
 // (D <= 8), 64 / DRMLT_GAUSS_GROUP per wave, a group of lanes and an LDS work
 // area each (a whole wave would leave most lanes idle on such matrices);
-// then the larger ones, one wave each. Every executor gives the same results.
+// then those up to D = 16, 64 / DRMLT_GAUSS_MID_GROUP per wave; then the
+// larger ones, one wave each. Every executor gives the same results.
 __global__ void __launch_bounds__(64, DRMLT_GAUSS_WAVES) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which,
                                                              int finishRatio, const uint32_t *iterp) {
-    constexpr int G = DRMLT_GAUSS_GROUP, NG = 64 / G;
+    constexpr int G = DRMLT_GAUSS_GROUP, NG = 64 / G, GM = DRMLT_GAUSS_MID_GROUP, NGM = 64 / GM;
     union Work {
         GWork big;
         GWorkT<kGSmallD> small[NG];
+        GWorkT<kGMidD> mid[NGM];
     };
     __shared__ Work w;
     const uint32_t iter = *iterp;
-    const int n = list_length(L, kind), nSmall = list_prefix(L, kind, kGSmallBins);
+    const int n = list_length(L, kind), nSmall = list_prefix(L, kind, kGSmallBins), nMid = list_prefix(L, kind, kGMidBins);
     gauss_groups<G>(Sc, ch, P, L, kind, which, finishRatio, iter, 0, nSmall, w.small);
     __syncthreads();
+    gauss_groups<GM>(Sc, ch, P, L, kind, which, finishRatio, iter, nSmall, nMid, w.mid);
+    __syncthreads();
     const WaveExec x{int(threadIdx.x)};
-    for (int it = nSmall + blockIdx.x; it < n; it += gridDim.x)
+    for (int it = nMid + blockIdx.x; it < n; it += gridDim.x)
         gauss_item(x, Sc, ch, P, list_item(L, kind, ch.C, it), which, finishRatio, iter, w.big);
 }
 // Second-stage proposals: list 1 (binned), or every chain in chain order with
