@@ -837,11 +837,12 @@ static void launch_lens_moves(drmlt_ctx *c, const DrParams &P, hipStream_t s, co
         hipLaunchKernelGGL(k_b_lens, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
 }
 
-// The second-stage acceptance: chains without a y* walk finished one lane
-// each, then the y* walks of list 6.
+// The second-stage acceptance: over binned lists, the chains without a y*
+// walk finished one lane each, then the y* walks of list 6; in chain order,
+// one kernel.
 static void launch_accept(drmlt_ctx *c, const DrParams &P, hipStream_t s, const Lists &L, const StepGrid &G, decltype(&k_b_stage2_accept) accept) {
     const dim3 fin(unsigned(std::max(1, std::min((P.chainCount + 63) / 64, persistent_blocks(c)))));
-    hipLaunchKernelGGL(k_b_stage2_finish, fin, dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
+    if (P.binStages) hipLaunchKernelGGL(k_b_stage2_finish, fin, dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
     hipLaunchKernelGGL(accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
 }
 
@@ -880,7 +881,8 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
             hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
             hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1, c->iterDev);
             const dim3 fin(unsigned(std::max(1, std::min((P.chainCount + 63) / 64, persistent_blocks(c)))));
-            hipLaunchKernelGGL(k_b_stage2_finish_counted, fin, dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
+            if (P.binStages)
+                hipLaunchKernelGGL(k_b_stage2_finish_counted, fin, dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
             hipLaunchKernelGGL(k_b_stage2_accept_counted, dim3(G.stage), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
         }
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
@@ -1043,23 +1045,24 @@ extern "C" int drmlt_autotune(drmlt_ctx *c, int32_t steps, drmlt_schedule *best,
     const int fill = std::min(64, std::max(1, (C + 2 * 4 * cus - 1) / (2 * 4 * cus)));
     std::vector<int> laneOpts = {64, 32};
     if (fill != 64 && fill != 32) laneOpts.push_back(fill);
+    // and the path-stage kernels' register bound (three resident waves per
+    // SIMD against two), searched together: the best workgroup count depends
+    // on it (with 1M chains the three-wave kernels do best at one workgroup
+    // per 64 chains, the two-wave ones at a quarter of that)
     for (int lanes : laneOpts)
         for (int bin = 0; bin <= 2; bin++) {
             const int perChain = (C + lanes - 1) / lanes;
-            for (int sb : {cus * 4, cus * 8, cus * 16, perChain}) {
-                cand = cur;
-                cand.stage_lanes = lanes;
-                cand.bin_stages = bin;
-                cand.stage_blocks = std::min(sb, perChain);
-                if ((r = measure(cand, 1, 7, ms))) return r;
-                if (ms < curMs) { cur = cand; curMs = ms; }
-            }
+            for (int sb : {cus * 4, cus * 8, cus * 16, perChain})
+                for (int waves : {2, 3}) {
+                    cand = cur;
+                    cand.stage_lanes = lanes;
+                    cand.bin_stages = bin;
+                    cand.stage_blocks = std::min(sb, perChain);
+                    cand.stage_waves = waves;
+                    if ((r = measure(cand, 1, 7, ms))) return r;
+                    if (ms < curMs) { cur = cand; curMs = ms; }
+                }
         }
-    // the path-stage kernels' register bound: three waves per SIMD against two
-    cand = cur;
-    cand.stage_waves = cur.stage_waves == 3 ? 2 : 3;
-    if ((r = measure(cand, 1, 7, ms))) return r;
-    if (ms < curMs) { cur = cand; curMs = ms; }
     const int kernelSlots[2][2] = {{2, 5}, {3, 6}};  // derivative and Gaussian kernels (current + proposal)
     for (int which = 0; which < 2; which++) {
         double kMs = 0;

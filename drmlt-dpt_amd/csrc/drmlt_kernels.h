@@ -9,15 +9,10 @@ namespace drmlt {
 constexpr int kJetK = 2;     // Hessian columns per derivative pass
 constexpr int kMaxBins = 16; // path-length bins (path length 0..15)
 // Gaussians of dimension up to kGSmallD are computed by groups of
-// DRMLT_GAUSS_GROUP lanes, several per wave; those up to kGMidD by groups of
-// DRMLT_GAUSS_MID_GROUP lanes; larger ones by a whole wave.
+// DRMLT_GAUSS_GROUP lanes, several per wave; larger ones by a whole wave.
 constexpr int kGSmallD = 8;
 #ifndef DRMLT_GAUSS_GROUP
 #define DRMLT_GAUSS_GROUP 8  // lanes per small Gaussian (measured: 8 beats 16 and 32)
-#endif
-constexpr int kGMidD = 16;
-#ifndef DRMLT_GAUSS_MID_GROUP
-#define DRMLT_GAUSS_MID_GROUP 32
 #endif
 
 // One path-state slot is a record of kSlotWords 32-bit words (nine 64-byte

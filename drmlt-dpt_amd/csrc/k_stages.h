@@ -155,9 +155,8 @@ DEV void gauss_item(const X &x, const DevScene &Sc, const Chains &ch, const DrPa
 }
 
 // Bins 0..kGSmallBins-1 of a list hold paths of length <= 4 (D = 2 max(b, 2)
-// <= kGSmallD), bins below kGMidBins those of length <= 8 (D <= kGMidD).
+// <= kGSmallD).
 constexpr int kGSmallBins = kGSmallD / 2 + 1;
-constexpr int kGMidBins = kGMidD / 2 + 1;
 DEV int list_prefix(const Lists &L, int kind, int nbins) {
     int n = 0;
     for (int b = 0; b < nbins; b++) n += L.count[kind * kMaxBins + b];
@@ -179,25 +178,21 @@ DEV void gauss_groups(const DevScene &Sc, const Chains &ch, const DrParams &P, c
 // One-wave workgroups over a Gaussian list: first the small Gaussians
 // (D <= 8), 64 / DRMLT_GAUSS_GROUP per wave, a group of lanes and an LDS work
 // area each (a whole wave would leave most lanes idle on such matrices);
-// then those up to D = 16, 64 / DRMLT_GAUSS_MID_GROUP per wave; then the
-// larger ones, one wave each. Every executor gives the same results.
+// then the larger ones, one wave each. Every executor gives the same results.
 __global__ void __launch_bounds__(64, DRMLT_GAUSS_WAVES) DRMLT_STAGE(k_b_gauss)(DevScene Sc, Chains ch, DrParams P, Lists L, int kind, int which,
                                                              int finishRatio, const uint32_t *iterp) {
-    constexpr int G = DRMLT_GAUSS_GROUP, NG = 64 / G, GM = DRMLT_GAUSS_MID_GROUP, NGM = 64 / GM;
+    constexpr int G = DRMLT_GAUSS_GROUP, NG = 64 / G;
     union Work {
         GWork big;
         GWorkT<kGSmallD> small[NG];
-        GWorkT<kGMidD> mid[NGM];
     };
     __shared__ Work w;
     const uint32_t iter = *iterp;
-    const int n = list_length(L, kind), nSmall = list_prefix(L, kind, kGSmallBins), nMid = list_prefix(L, kind, kGMidBins);
+    const int n = list_length(L, kind), nSmall = list_prefix(L, kind, kGSmallBins);
     gauss_groups<G>(Sc, ch, P, L, kind, which, finishRatio, iter, 0, nSmall, w.small);
     __syncthreads();
-    gauss_groups<GM>(Sc, ch, P, L, kind, which, finishRatio, iter, nSmall, nMid, w.mid);
-    __syncthreads();
     const WaveExec x{int(threadIdx.x)};
-    for (int it = nMid + blockIdx.x; it < n; it += gridDim.x)
+    for (int it = nSmall + blockIdx.x; it < n; it += gridDim.x)
         gauss_item(x, Sc, ch, P, list_item(L, kind, ch.C, it), which, finishRatio, iter, w.big);
 }
 // Second-stage proposals: list 1 (binned), or every chain in chain order with
@@ -228,24 +223,23 @@ DEV void stage2_propose_kernel(const DevScene &Sc, const Chains &ch, const DrPar
     }
 }
 
-// Second-stage acceptance, in two kernels so that the y* walks run with
-// every lane busy: the first takes every chain with a second stage (list 1,
-// or chain order) and finishes those that trace no y* (one lane each, no
+// Second-stage acceptance. Over binned lists (binStages set) it runs as two
+// kernels so that the y* walks run with every lane busy: the first takes the
+// chains of list 1 and finishes those that trace no y* (one lane each, no
 // walk); it hands the others to list 6, binned by the second proposal's
 // length, which the second takes with the path-stage kernels' lane layout.
-// Both draw the chain's stage-3 stream from its start, so a chain's draws
-// do not depend on which kernel finishes it.
+// In chain order (the schedule for few chains, where every resident wave
+// counts) one kernel walks the y* of the chains that need one beside the
+// others. Every form draws the chain's stage-3 stream from its start, so a
+// chain's draws do not depend on the kernel that finishes it.
 __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_finish)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
                                                                      unsigned long long *stats, Lists L) {
     const uint32_t iter = *iterp;
-    const int n = P.binStages ? list_length(L, 1) : P.chainCount;
+    const int n = list_length(L, 1);
     for (int it = int(blockIdx.x * blockDim.x + threadIdx.x); it < n; it += int(gridDim.x * blockDim.x)) {
-        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : P.chainBase + it;
-        const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
-        Stage3 a;
-        a.traceStar = false;
-        if (act) a = stage3_begin(ch, P, chain, false);
-        if (act && !a.traceStar) {
+        const int chain = list_item(L, 1, ch.C, it);
+        const Stage3 a = stage3_begin(ch, P, chain, false);
+        if (!a.traceStar) {
             Philox g(P.seed, iter, chain, TAG_S3);
             POut<float> out;
             out.valid = false;
@@ -257,17 +251,22 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_finish)(DevScene Sc
 DEV void stage2_accept_kernel(const DevScene &Sc, const Chains &ch, const DrParams &P, const uint32_t *iterp, float *film,
                               unsigned long long *stats, const Lists &L) {
     const uint32_t iter = *iterp;
-    const int n = list_length(L, 6);
+    const int n = P.binStages ? list_length(L, 6) : P.chainCount;
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
-        const int chain = list_item(L, 6, ch.C, it);
+        const int chain = P.binStages ? list_item(L, 6, ch.C, it) : P.chainBase + it;
+        const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
         Philox g(P.seed, iter, chain, TAG_S3);
-        const Stage3 a = stage3_begin(ch, P, chain);
+        Stage3 a;
         Walk w;
-        stage3_walk(ch, P, a, g, w);
+        a.traceStar = false;
+        if (act) {
+            a = stage3_begin(ch, P, chain);
+            if (a.traceStar) stage3_walk(ch, P, a, g, w);
+        }
         POut<float> out;
         out.valid = false;
-        walk_run(Sc, P.o, ch, P.discreteStdDev, w, true, out);
-        stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
+        walk_run(Sc, P.o, ch, P.discreteStdDev, w, a.traceStar, out);
+        if (act) stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
     }
 }
 
