@@ -565,7 +565,7 @@ struct Stage3 {
     float a1;
     double a2;
 };
-DEV Stage3 stage3_begin(const Chains &ch, const DrParams &P, int chain, bool copyStar = true) {
+DEV Stage3 stage3_begin(const Chains &ch, const DrParams &P, int chain) {
     Stage3 a;
     const int C = ch.C;
     a.perm = ch.perm[chain];
@@ -579,7 +579,7 @@ DEV Stage3 stage3_begin(const Chains &ch, const DrParams &P, int chain, bool cop
     a.a1 = ch.a1[chain];
     a.a2 = zValid ? ch.a2d[chain] : 0.0;
     a.traceStar = !is_invalid(a.a2) && P.type == 0 && P.integrator == 0 && a.a2 != 0.0 && ch.valid[a.ps];
-    if (a.traceStar && copyStar) copy_state(slot_ref(ch, a.ps), slot_ref(ch, a.pstar));
+    if (a.traceStar) copy_state(slot_ref(ch, a.ps), slot_ref(ch, a.pstar));
     return a;
 }
 DEV void stage3_walk(const Chains &ch, const DrParams &P, const Stage3 &a, Philox &g, Walk &w) {

@@ -837,15 +837,6 @@ static void launch_lens_moves(drmlt_ctx *c, const DrParams &P, hipStream_t s, co
         hipLaunchKernelGGL(k_b_lens, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
 }
 
-// The second-stage acceptance: over binned lists, the chains without a y*
-// walk finished one lane each, then the y* walks of list 6; in chain order,
-// one kernel.
-static void launch_accept(drmlt_ctx *c, const DrParams &P, hipStream_t s, const Lists &L, const StepGrid &G, decltype(&k_b_stage2_accept) accept) {
-    const dim3 fin(unsigned(std::max(1, std::min((P.chainCount + 63) / 64, persistent_blocks(c)))));
-    if (P.binStages) hipLaunchKernelGGL(k_b_stage2_finish, fin, dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
-    hipLaunchKernelGGL(accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
-}
-
 static void launch_step_kernels(drmlt_ctx *c, const DrParams &P, hipStream_t s, const Lists &L, const StepGrid &G) {
     const PathStageKernels K = path_stage_kernels(c);
     launch_lens_scores(c, P, s);
@@ -858,7 +849,7 @@ static void launch_step_kernels(drmlt_ctx *c, const DrParams &P, hipStream_t s, 
     hipLaunchKernelGGL(K.propose, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, L);
     hipLaunchKernelGGL(k_b_deriv, dim3(G.deriv), dim3(64), 0, s, c->S, c->ch, P, L, 2, 1);
     hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, L, 2, 1, 1, c->iterDev);
-    launch_accept(c, P, s, L, G, K.accept);
+    hipLaunchKernelGGL(K.accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, L);
 }
 
 static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, unsigned long long *counters = nullptr) {
@@ -880,9 +871,6 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
             hipLaunchKernelGGL(k_b_stage2_propose_counted, dim3(G.stage), dim3(64), 0, s, Sk[3], c->ch, P, c->iterDev, c->L);
             hipLaunchKernelGGL(k_b_deriv_counted, dim3(G.deriv), dim3(64), 0, s, Sk[4], c->ch, P, c->L, 2, 1);
             hipLaunchKernelGGL(k_b_gauss_counted, dim3(G.gauss), dim3(64), 0, s, Sk[5], c->ch, P, c->L, 2, 1, 1, c->iterDev);
-            const dim3 fin(unsigned(std::max(1, std::min((P.chainCount + 63) / 64, persistent_blocks(c)))));
-            if (P.binStages)
-                hipLaunchKernelGGL(k_b_stage2_finish_counted, fin, dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
             hipLaunchKernelGGL(k_b_stage2_accept_counted, dim3(G.stage), dim3(64), 0, s, Sk[6], c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
         }
         hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
@@ -929,7 +917,7 @@ static void launch_batched_step(drmlt_ctx *c, const DrParams &P, bool timed, uns
     if (timed) (void)hipEventRecord(c->ev[5], s);
     if (P.integrator != 1) hipLaunchKernelGGL(k_b_gauss, dim3(G.gauss), dim3(64), 0, s, c->S, c->ch, P, c->L, 2, 1, 1, c->iterDev);
     if (timed) (void)hipEventRecord(c->ev[6], s);
-    if (P.integrator != 1) launch_accept(c, P, s, c->L, G, K.accept);
+    if (P.integrator != 1) hipLaunchKernelGGL(K.accept, dim3(G.stage), dim3(64), 0, s, c->S, c->ch, P, c->iterDev, c->filmInd, c->stats, c->L);
     if (timed) (void)hipEventRecord(c->ev[7], s);
     hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, c->L, c->iterDev, 1);
     if (timed) (void)hipEventRecord(c->ev[8], s);

@@ -136,10 +136,8 @@ struct DrParams {
 // kind 4: the H2MC chains of kind 0 whose current state needs derivatives
 //         (no cached Gaussian, a path within the derivative depth, a
 //         positive score);
-// kind 5: h2mc chains making a lens mutation;
-// kind 6: Green & Mira chains that trace y* (binned by the second
-//         proposal's path length).
-constexpr int kListKinds = 7;
+// kind 5: h2mc chains making a lens mutation.
+constexpr int kListKinds = 6;
 // loop counters (the eight of drmlt_stats' DR part, then lens mutations and their acceptances)
 constexpr int kStatCounters = 10;
 // most chain ranges of the graph-replayed step on concurrent streams

@@ -223,37 +223,14 @@ DEV void stage2_propose_kernel(const DevScene &Sc, const Chains &ch, const DrPar
     }
 }
 
-// Second-stage acceptance. Over binned lists (binStages set) it runs as two
-// kernels so that the y* walks run with every lane busy: the first takes the
-// chains of list 1 and finishes those that trace no y* (one lane each, no
-// walk); it hands the others to list 6, binned by the second proposal's
-// length, which the second takes with the path-stage kernels' lane layout.
-// In chain order (the schedule for few chains, where every resident wave
-// counts) one kernel walks the y* of the chains that need one beside the
-// others. Every form draws the chain's stage-3 stream from its start, so a
-// chain's draws do not depend on the kernel that finishes it.
-__global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage2_finish)(DevScene Sc, Chains ch, DrParams P, const uint32_t *iterp, float *film,
-                                                                     unsigned long long *stats, Lists L) {
-    const uint32_t iter = *iterp;
-    const int n = list_length(L, 1);
-    for (int it = int(blockIdx.x * blockDim.x + threadIdx.x); it < n; it += int(gridDim.x * blockDim.x)) {
-        const int chain = list_item(L, 1, ch.C, it);
-        const Stage3 a = stage3_begin(ch, P, chain, false);
-        if (!a.traceStar) {
-            Philox g(P.seed, iter, chain, TAG_S3);
-            POut<float> out;
-            out.valid = false;
-            stage3_end(Sc, ch, P, chain, a, out, g, film, stats);
-        }
-        list_push_wave(L, 6, ch.C, a.traceStar ? length_bin(ch.c[a.ps] + ch.l[a.ps] - 1) : 0, chain, a.traceStar);
-    }
-}
+// Second-stage acceptance over the same chains; Green's y* walks where they
+// are needed.
 DEV void stage2_accept_kernel(const DevScene &Sc, const Chains &ch, const DrParams &P, const uint32_t *iterp, float *film,
                               unsigned long long *stats, const Lists &L) {
     const uint32_t iter = *iterp;
-    const int n = P.binStages ? list_length(L, 6) : P.chainCount;
+    const int n = P.binStages ? list_length(L, 1) : P.chainCount;
     for (int it = stage_first(P, n); it < n; it += gridDim.x * P.stageLanes) {
-        const int chain = P.binStages ? list_item(L, 6, ch.C, it) : P.chainBase + it;
+        const int chain = P.binStages ? list_item(L, 1, ch.C, it) : P.chainBase + it;
         const bool act = P.binStages || (ch.flags[chain] & 2) != 0;
         Philox g(P.seed, iter, chain, TAG_S3);
         Stage3 a;

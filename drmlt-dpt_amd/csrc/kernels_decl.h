@@ -11,7 +11,6 @@ __global__ void k_b_deriv(DevScene, Chains, DrParams, Lists, int, int);
 __global__ void k_b_gauss(DevScene, Chains, DrParams, Lists, int, int, int, const uint32_t *);
 __global__ void k_b_stage2_propose(DevScene, Chains, DrParams, const uint32_t *, Lists);
 __global__ void k_b_stage2_accept(DevScene, Chains, DrParams, uint32_t *, float *, unsigned long long *, Lists);
-__global__ void k_b_stage2_finish(DevScene, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
 // the same path-stage kernels bounded to three waves per SIMD
 __global__ void k_b_stage1_w3(DevScene, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
 __global__ void k_b_stage2_propose_w3(DevScene, Chains, DrParams, const uint32_t *, Lists);
@@ -27,7 +26,6 @@ __global__ void k_b_deriv_counted(DevSceneCounted, Chains, DrParams, Lists, int,
 __global__ void k_b_gauss_counted(DevSceneCounted, Chains, DrParams, Lists, int, int, int, const uint32_t *);
 __global__ void k_b_stage2_propose_counted(DevSceneCounted, Chains, DrParams, const uint32_t *, Lists);
 __global__ void k_b_stage2_accept_counted(DevSceneCounted, Chains, DrParams, uint32_t *, float *, unsigned long long *, Lists);
-__global__ void k_b_stage2_finish_counted(DevSceneCounted, Chains, DrParams, const uint32_t *, float *, unsigned long long *, Lists);
 __global__ void k_prepass_gen(DevScene, Opts, int, int, uint64_t, int *, int *, float *, double *, double *);
 __global__ void k_prepass_seed(DevScene, Opts, Chains, int, uint64_t, const int *, const int *);
 __global__ void k_direct(DevScene, Opts, int, uint64_t, float *);
