@@ -112,9 +112,13 @@ def _alpha2(dr_type, a2pre, astar, tr, a1):
     return np.where(_invalid(a2pre), 0.0, a)
 
 
-def analyse(O, ctx, lf, before, after, dr_type=0, sigma=0.01, max_derv=9):
+def analyse(O, ctx, lf, before, after, dr_type=0, sigma=0.01, max_derv=9, integrator=0):
     """Per-chain comparisons (A), (B), (C); returns a dict of arrays and the
-    masks of the chains each comparison covers."""
+    masks of the chains each comparison covers. integrator 1 (mcmc): the first
+    stage alone; 2 (h2mc, lens mutations off): an iteration is a large step,
+    an isotropic small step accepted with a1, or an H2MC move made by the
+    second-stage kernels and accepted with min(1, ratio) (h2mc.cpp:584-587),
+    which proposes no y."""
     n = len(after["perm"])
     idx = np.arange(n)
     pb = before["perm"]
@@ -130,6 +134,7 @@ def analyse(O, ctx, lf, before, after, dr_type=0, sigma=0.01, max_derv=9):
     second = (flags & 2) != 0
     aniso = second & ((flags & 4) != 0)
     y_gpu, z_gpu, s_gpu = (flags & 32) != 0, (flags & 16) != 0, (flags & 64) != 0
+    y_prop = ~second if integrator == 2 else np.ones(n, bool)  # chains whose iteration proposed a y
 
     nl, nt = len(O.sc["light_tris"]), len(O.sc["tri_p"])
     Yo, y8o = oracle_eval(O, Y, nl, nt)
@@ -142,6 +147,7 @@ def analyse(O, ctx, lf, before, after, dr_type=0, sigma=0.01, max_derv=9):
         both = both & (o8[:, 5] > 0)
         return np.abs(g8[both, 5].astype(np.float64) - o8[both, 5]) / o8[both, 5]
 
+    y_o = y_o & y_prop  # the slot of an H2MC move's chain holds no new y
     y_both, z_both = y_gpu & y_o, second & z_gpu & z_o
     lx, ly = X["c"] + X["l"] - 1, Y["c"] + Y["l"] - 1
     ss_x, ls_x = x8[:, 5].astype(np.float64), x8[:, 4].astype(np.float64)
@@ -180,8 +186,8 @@ def analyse(O, ctx, lf, before, after, dr_type=0, sigma=0.01, max_derv=9):
     a2preC = ratio(z8o[:, 5].astype(np.float64), z_o, log_gauss)
     # y* is traced where the reference traces it: Green & Mira, a valid
     # ratio that is not zero, a second proposal object marked valid
-    traceB = second & ~_invalid(a2preB) & (a2preB != 0) & ps_valid_before & (dr_type == 0)
-    traceC = second & ~_invalid(a2preC) & (a2preC != 0) & ps_valid_before & (dr_type == 0)
+    traceB = second & ~_invalid(a2preB) & (a2preB != 0) & ps_valid_before & (dr_type == 0) & (integrator == 0)
+    traceC = second & ~_invalid(a2preC) & (a2preC != 0) & ps_valid_before & (dr_type == 0) & (integrator == 0)
     with np.errstate(divide="ignore", invalid="ignore"):
         if dr_type == 0:
             astarB = np.where(traceB & s_gpu, _clamp(s8[:, 5].astype(np.float64) / z8[:, 5]), 0.0)
@@ -199,8 +205,14 @@ def analyse(O, ctx, lf, before, after, dr_type=0, sigma=0.01, max_derv=9):
             s2 = np.float64(np.float32(sigma)) ** 2
             with np.errstate(over="ignore"):
                 tr = np.exp(-0.5 * (dzy ** 2 * mask).sum(1) / s2 + 0.5 * (dxy ** 2 * mask).sum(1) / s2)
-    a2B = np.where(second, _alpha2(dr_type, a2preB, astarB, tr, after["a1"]), 0.0)
-    a2C = np.where(second, _alpha2(dr_type, a2preC, astarC, tr, after["a1"]), 0.0)
+    if integrator == 2:
+        # the h2mc integrator's H2MC move: min(1, ratio), 0 for an invalid ratio
+        with np.errstate(invalid="ignore"):
+            a2B = np.where(second & ~_invalid(a2preB), np.minimum(1.0, a2preB), 0.0)
+            a2C = np.where(second & ~_invalid(a2preC), np.minimum(1.0, a2preC), 0.0)
+    else:
+        a2B = np.where(second, _alpha2(dr_type, a2preB, astarB, tr, after["a1"]), 0.0)
+        a2C = np.where(second, _alpha2(dr_type, a2preC, astarC, tr, after["a1"]), 0.0)
     s_both = traceB & s_gpu & s_o
     return {"n": n, "second": second, "aniso": aniso, "large": gen, "flags": flags,
             "y_valid_agree": y_gpu == y_o, "z_valid_agree": ~second | (z_gpu == z_o), "s_valid_agree": ~traceB | (s_gpu == s_o),

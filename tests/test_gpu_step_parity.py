@@ -26,6 +26,11 @@ CASES = {
     "cfg4_small": dict(dpt={}, chains=8192, warm=3),
     "cfg5_small": dict(dpt={"acceptanceMap": False}, chains=8192, warm=3),
     "cfg1_mira": dict(scene="cfg1", dpt={"type": "mira"}, chains=4096, warm=3),
+    # the mcmc integrator (the first stage alone) and the h2mc integrator
+    # (isotropic moves with probability uniformmixing, else H2MC moves
+    # accepted with min(1, ratio); lens mutations are covered below)
+    "cfg1_mcmc": dict(scene="cfg1", dpt={"integrator": "mcmc"}, chains=4096, warm=3),
+    "cfg2_small_h2mc": dict(scene="cfg2_small", dpt={"integrator": "h2mc", "lensperturbprob": 0.0}, chains=8192, warm=3),
 }
 
 
@@ -43,7 +48,7 @@ def _run(request, name, mode):
     O = orc.Scene(sc)
     say(f"step parity {name}: oracle re-evaluation and derivatives of the H2MC states")
     r = sp.analyse(O, ctx, lf, before, after, dr_type=1 if dpt.get("type") == "mira" else 0, sigma=float(opt.perturb_std_dev),
-                   max_derv=int(opt.max_derivatives_depth))
+                   max_derv=int(opt.max_derivatives_depth), integrator=int(opt.integrator))
     ctx.close()
     s = sp.summary(r)
     ctx64 = capi.Context(sc, capi.options_from_dpt(dpt, num_chains=case["chains"], quirk_chad_mis=0))
@@ -59,8 +64,8 @@ def _run(request, name, mode):
     return r, s
 
 
-def _check(r, s):
-    assert s["second"] > 0.05 * s["chains"]
+def _check(r, s, min_second=0.05):
+    assert s["second"] >= min_second * s["chains"]
     # the loop and the oracle agree on which proposals are valid and on
     # the triangles of the valid ones
     assert s["valid_agree"] >= 0.995, s
@@ -86,6 +91,19 @@ def test_step_acceptances_per_state(name, request):
     _check(r, s)
     if name in ("cfg4_small", "cfg5_small"):
         assert s["aniso"] > 0.05 * s["chains"]  # H2MC moves were compared
+
+
+def test_step_acceptances_per_state_integrators(request):
+    """The mcmc integrator's first stage and the h2mc integrator's isotropic
+    and H2MC moves, per state, to the same bars."""
+    from drmlt import capi
+
+    r, s = _run(request, "cfg1_mcmc", capi.BATCHED_GRAPH)
+    assert s["second"] == 0
+    _check(r, s, min_second=0.0)
+    r, s = _run(request, "cfg2_small_h2mc", capi.BATCHED_GRAPH)
+    assert s["aniso"] > 0.3 * s["chains"]  # H2MC moves (1 - uniformmixing of the small steps)
+    _check(r, s)
 
 
 def test_step_acceptances_per_state_plain_form(request):
