@@ -1,10 +1,12 @@
 // Device BVH build replacing the reference's Embree build (scene.cpp:42-54):
 // the binned-SAH tree of the host builder (bvh_build.cpp: 32 bins over the
 // centroid extent of each axis, leaves of at most 4 triangles, a depth cap,
-// child boxes padded) built top-down one tree level per launch, one
-// 256-thread workgroup per node, then collapsed into the 4-wide layout the
-// traversal reads one 4-wide level per launch. The split decisions use the
-// host builder's arithmetic in the same order (no contraction), so the tree
+// child boxes padded) built top-down one tree level per launch -- one
+// 256-thread workgroup per node, or, while a level has fewer than 256 nodes,
+// one workgroup per 8192-triangle chunk of a node -- then collapsed into the
+// 4-wide layout the traversal reads one 4-wide level per launch. The split
+// decisions use the host builder's arithmetic in the same order (no
+// contraction), so the tree
 // has the same nodes; nodes are numbered in allocation order instead of depth
 // first, which leaves every query's answer unchanged (the traversal's closest
 // hit breaks ties by scene order).
@@ -76,6 +78,47 @@ __device__ inline void box_pad(float *lo, float *hi) {
 __device__ inline int bin_of(float c, float lo, float scale) {
 #pragma clang fp contract(off)
     return min(max(int((c - lo) * scale), 0), kBins - 1);
+}
+
+// The split of a node from its bins (counts cnt[k * kBins + b], boxes
+// bb[(k * kBins + b) * 6 + j] as ford values): the host builder's loops in
+// its order, the lowest-cost split (axis, first bin of the right side), or
+// split -1 when none leaves both sides non-empty.
+__device__ void sah_split(const float *ext, const int *cnt3, const int *bb3, int &axisOut, int &splitOut) {
+    float best = 3e38f;
+    int bestAxis = -1, bestSplit = -1;
+    for (int k = 0; k < 3; k++) {
+        if (!(ext[k] > 0)) continue;
+        const int *bcnt = cnt3 + k * kBins, *bbox = bb3 + k * kBins * 6;
+        float llo[kBins][3], lhi[kBins][3];
+        int lcnt[kBins];
+        float alo[3] = {3e38f, 3e38f, 3e38f}, ahi[3] = {-3e38f, -3e38f, -3e38f};
+        int cnt = 0;
+        for (int b = 0; b < kBins - 1; b++) {
+            for (int j = 0; j < 3; j++) {
+                alo[j] = fminf(alo[j], fdec(bbox[b * 6 + j]));
+                ahi[j] = fmaxf(ahi[j], fdec(bbox[b * 6 + 3 + j]));
+                llo[b][j] = alo[j];
+                lhi[b][j] = ahi[j];
+            }
+            cnt += bcnt[b];
+            lcnt[b] = cnt;
+        }
+        for (int j = 0; j < 3; j++) { alo[j] = 3e38f; ahi[j] = -3e38f; }
+        cnt = 0;
+        for (int b = kBins - 1; b > 0; b--) {
+            for (int j = 0; j < 3; j++) {
+                alo[j] = fminf(alo[j], fdec(bbox[b * 6 + j]));
+                ahi[j] = fmaxf(ahi[j], fdec(bbox[b * 6 + 3 + j]));
+            }
+            cnt += bcnt[b];
+            if (lcnt[b - 1] == 0 || cnt == 0) continue;
+            const float cost = sah_cost(llo[b - 1], lhi[b - 1], lcnt[b - 1], alo, ahi, cnt);
+            if (cost < best) { best = cost; bestAxis = k; bestSplit = b; }
+        }
+    }
+    axisOut = bestAxis;
+    splitOut = bestSplit;
 }
 
 // Per triangle: box of its corners and its centroid.
@@ -182,41 +225,7 @@ __global__ void __launch_bounds__(kWG) k_bvh_level(const BTask *tasks, const int
     }
     __syncthreads();
     // 3. the split: the host builder's loops in its order
-    if (tid == 0) {
-        float best = 3e38f;
-        int bestAxis = -1, bestSplit = -1;
-        for (int k = 0; k < 3; k++) {
-            if (!(ext[k] > 0)) continue;
-            float llo[kBins][3], lhi[kBins][3];
-            int lcnt[kBins];
-            float alo[3] = {3e38f, 3e38f, 3e38f}, ahi[3] = {-3e38f, -3e38f, -3e38f};
-            int cnt = 0;
-            for (int b = 0; b < kBins - 1; b++) {
-                for (int j = 0; j < 3; j++) {
-                    alo[j] = fminf(alo[j], fdec(bbox[k][b][j]));
-                    ahi[j] = fmaxf(ahi[j], fdec(bbox[k][b][3 + j]));
-                    llo[b][j] = alo[j];
-                    lhi[b][j] = ahi[j];
-                }
-                cnt += bcnt[k][b];
-                lcnt[b] = cnt;
-            }
-            for (int j = 0; j < 3; j++) { alo[j] = 3e38f; ahi[j] = -3e38f; }
-            cnt = 0;
-            for (int b = kBins - 1; b > 0; b--) {
-                for (int j = 0; j < 3; j++) {
-                    alo[j] = fminf(alo[j], fdec(bbox[k][b][j]));
-                    ahi[j] = fmaxf(ahi[j], fdec(bbox[k][b][3 + j]));
-                }
-                cnt += bcnt[k][b];
-                if (lcnt[b - 1] == 0 || cnt == 0) continue;
-                const float cost = sah_cost(llo[b - 1], lhi[b - 1], lcnt[b - 1], alo, ahi, cnt);
-                if (cost < best) { best = cost; bestAxis = k; bestSplit = b; }
-            }
-        }
-        dec[0] = bestAxis;
-        dec[1] = bestSplit;
-    }
+    if (tid == 0) sah_split(ext, &bcnt[0][0], &bbox[0][0][0], dec[0], dec[1]);
     __syncthreads();
     const int axis = dec[0], split = dec[1];
     // no bin split leaves both sides non-empty (every centroid equal): a leaf
@@ -290,6 +299,255 @@ __global__ void __launch_bounds__(kWG) k_bvh_level(const BTask *tasks, const int
             next[q] = BTask{T.first, mid - T.first, T.depth + 1, 2 * node};
             next[q + 1] = BTask{mid, T.first + T.count - mid, T.depth + 1, 2 * node + 1};
         }
+    }
+}
+
+// ---- levels with few nodes (the top of the tree): several workgroups per
+// node, each over a chunk of its triangle range; per-node boxes and bins meet
+// in global memory through atomic min / max / add, which give the values
+// the one-workgroup kernel computes, and the partition keeps the chunks'
+// order, so the tree is the same.
+constexpr int kChunk = 8192;    // triangles per workgroup
+constexpr int kWideTasks = 256; // levels with fewer nodes are built this way
+// a chunk [first, first + count) of task `task`, whose first chunk is chunk `firstChunk`
+struct WChunk {
+    int task, first, count, firstChunk;
+};
+// per task: range and centroid boxes (12 ford values), bins (counts and ford boxes), decision
+struct WTask {
+    int box[12];
+    int cnt[3][kBins];
+    int bb[3][kBins][6];
+    int axis, split, mid, leaf;
+};
+
+__global__ void __launch_bounds__(256) k_wide_init(int ntasks, WTask *w) {
+    const int t = blockIdx.x;
+    if (t >= ntasks) return;
+    WTask &W = w[t];
+    for (int i = threadIdx.x; i < 12; i += blockDim.x) W.box[i] = ford((i % 6) < 3 ? 3e38f : -3e38f);
+    for (int i = threadIdx.x; i < 3 * kBins; i += blockDim.x) {
+        const int k = i / kBins, b = i % kBins;
+        W.cnt[k][b] = 0;
+        for (int j = 0; j < 3; j++) { W.bb[k][b][j] = ford(3e38f); W.bb[k][b][3 + j] = ford(-3e38f); }
+    }
+}
+
+// 1. the range's box and its centroids' box, chunk by chunk
+__global__ void __launch_bounds__(kWG) k_wide_box(const WChunk *chunks, const int *idxIn, const float4 *blo, const float4 *bhi, const float4 *cen,
+                                                 WTask *w) {
+    const WChunk C = chunks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ float red[12][kWG / 64];
+    float v[12];
+    for (int k = 0; k < 3; k++) { v[k] = 3e38f; v[3 + k] = -3e38f; v[6 + k] = 3e38f; v[9 + k] = -3e38f; }
+    for (int i = tid; i < C.count; i += kWG) {
+        const int t = idxIn[C.first + i];
+        const float4 a = blo[t], b = bhi[t], c = cen[t];
+        const float al[3] = {a.x, a.y, a.z}, bh[3] = {b.x, b.y, b.z}, cc[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; k++) {
+            v[k] = fminf(v[k], al[k]); v[3 + k] = fmaxf(v[3 + k], bh[k]);
+            v[6 + k] = fminf(v[6 + k], cc[k]); v[9 + k] = fmaxf(v[9 + k], cc[k]);
+        }
+    }
+    for (int k = 0; k < 12; k++) {
+        const bool isMin = (k % 6) < 3;
+        const float r = isMin ? wmin(v[k]) : wmax(v[k]);
+        if (lane == 0) red[k][wave] = r;
+    }
+    __syncthreads();
+    if (tid < 12) {
+        const bool isMin = (tid % 6) < 3;
+        float r = red[tid][0];
+        for (int q = 1; q < kWG / 64; q++) r = isMin ? fminf(r, red[tid][q]) : fmaxf(r, red[tid][q]);
+        if (isMin) atomicMin(&w[C.task].box[tid], ford(r));
+        else atomicMax(&w[C.task].box[tid], ford(r));
+    }
+}
+
+// the bin parameters of a task from its centroid box (the one-workgroup
+// kernel's arithmetic)
+__device__ inline void wide_bin_params(const WTask &W, float *clo, float *ext, float *scale) {
+    for (int k = 0; k < 3; k++) {
+        clo[k] = fdec(W.box[6 + k]);
+        const float chi = fdec(W.box[9 + k]);
+        ext[k] = chi - clo[k];
+        scale[k] = ext[k] > 0 ? float(kBins) / ext[k] : 0.0f;
+    }
+}
+
+// 2. bins over the centroid extent of each axis, chunk by chunk
+__global__ void __launch_bounds__(kWG) k_wide_bins(const WChunk *chunks, const int *idxIn, const float4 *blo, const float4 *bhi, const float4 *cen,
+                                                  WTask *w) {
+    const WChunk C = chunks[blockIdx.x];
+    const int tid = threadIdx.x;
+    __shared__ int bcnt[3][kBins];
+    __shared__ int bbox[3][kBins][6];
+    WTask &W = w[C.task];
+    float clo[3], ext[3], scale[3];
+    wide_bin_params(W, clo, ext, scale);
+    for (int i = tid; i < 3 * kBins; i += kWG) {
+        const int k = i / kBins, b = i % kBins;
+        bcnt[k][b] = 0;
+        for (int j = 0; j < 3; j++) { bbox[k][b][j] = ford(3e38f); bbox[k][b][3 + j] = ford(-3e38f); }
+    }
+    __syncthreads();
+    for (int i = tid; i < C.count; i += kWG) {
+        const int t = idxIn[C.first + i];
+        const float4 a = blo[t], bb = bhi[t], c = cen[t];
+        const float al[3] = {a.x, a.y, a.z}, bh[3] = {bb.x, bb.y, bb.z}, cc[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; k++) {
+            if (!(ext[k] > 0)) continue;
+            const int b = bin_of(cc[k], clo[k], scale[k]);
+            atomicAdd(&bcnt[k][b], 1);
+            for (int j = 0; j < 3; j++) {
+                atomicMin(&bbox[k][b][j], ford(al[j]));
+                atomicMax(&bbox[k][b][3 + j], ford(bh[j]));
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < 3 * kBins; i += kWG) {
+        const int k = i / kBins, b = i % kBins;
+        if (bcnt[k][b] == 0) continue;
+        atomicAdd(&W.cnt[k][b], bcnt[k][b]);
+        for (int j = 0; j < 3; j++) {
+            atomicMin(&W.bb[k][b][j], bbox[k][b][j]);
+            atomicMax(&W.bb[k][b][3 + j], bbox[k][b][3 + j]);
+        }
+    }
+}
+
+// 3. per task (one wave): the parent's box slot, a leaf, or the split, this
+// node and its two children's tasks (steps 3 and 5 of the one-workgroup
+// kernel); the left count comes from the bins the partition uses
+__global__ void __launch_bounds__(64) k_wide_split(const BTask *tasks, int ntasks, const int *idxIn, int *order, WTask *w, BNodeD *bn,
+                                                   int *nodeCount, BTask *next, int *nextCount, int maxDepth, int *depthMax, int cap,
+                                                   int *overflow) {
+    const int t = blockIdx.x, tid = threadIdx.x;
+    if (t >= ntasks) return;
+    const BTask T = tasks[t];
+    WTask &W = w[t];
+    __shared__ int dec[2];
+    float clo[3], ext[3], scale[3];
+    wide_bin_params(W, clo, ext, scale);
+    bool leaf = T.count <= kLeaf || T.depth >= maxDepth;
+    if (tid == 0) {
+        atomicMax(depthMax, T.depth);
+        if (T.parent >= 0) {
+            float lo[3] = {fdec(W.box[0]), fdec(W.box[1]), fdec(W.box[2])}, hi[3] = {fdec(W.box[3]), fdec(W.box[4]), fdec(W.box[5])};
+            box_pad(lo, hi);
+            BNodeD &P = bn[T.parent >> 1];
+            const int side = T.parent & 1;
+            for (int k = 0; k < 3; k++) { P.box[side][k] = lo[k]; P.box[side][3 + k] = hi[k]; }
+        }
+        dec[0] = -1;
+        dec[1] = -1;
+        if (!leaf) sah_split(ext, &W.cnt[0][0], &W.bb[0][0][0], dec[0], dec[1]);
+    }
+    __syncthreads();
+    const int axis = dec[0], split = dec[1];
+    leaf = leaf || split <= 0;
+    if (leaf) {
+        for (int i = tid; i < T.count; i += 64) order[T.first + i] = idxIn[T.first + i];
+        if (tid == 0) {
+            if (T.parent >= 0) {
+                bn[T.parent >> 1].index[T.parent & 1] = T.first;
+                bn[T.parent >> 1].count[T.parent & 1] = T.count;
+            } else {
+                atomicAdd(overflow + 1, 1);  // the root is one leaf: the host builder's case
+            }
+            W.leaf = 1;
+        }
+        return;
+    }
+    if (tid == 0) {
+        int nLeft = 0;
+        for (int b = 0; b < split; b++) nLeft += W.cnt[axis][b];
+        const int mid = T.first + nLeft;
+        W.axis = axis;
+        W.split = split;
+        W.mid = mid;
+        W.leaf = 0;
+        const int node = T.parent < 0 ? 0 : atomicAdd(nodeCount, 1);
+        const int q = atomicAdd(nextCount, 2);
+        if (node >= cap || q + 1 >= cap) {
+            atomicAdd(overflow, 1);  // not reached: a split leaves more than four triangles per internal node
+        } else {
+            if (T.parent >= 0) {
+                bn[T.parent >> 1].index[T.parent & 1] = node;
+                bn[T.parent >> 1].count[T.parent & 1] = 0;
+            }
+            next[q] = BTask{T.first, mid - T.first, T.depth + 1, 2 * node};
+            next[q + 1] = BTask{mid, T.first + T.count - mid, T.depth + 1, 2 * node + 1};
+        }
+    }
+}
+
+__device__ inline bool wide_left(const WTask &W, const float *clo, const float *scale, const float4 &c) {
+    const float cc = W.axis == 0 ? c.x : W.axis == 1 ? c.y : c.z;
+    return bin_of(cc, clo[W.axis], scale[W.axis]) < W.split;
+}
+
+// 4a. left count of each chunk
+__global__ void __launch_bounds__(kWG) k_wide_count(const WChunk *chunks, const int *idxIn, const float4 *cen, const WTask *w, int *leftCnt) {
+    const WChunk C = chunks[blockIdx.x];
+    const WTask &W = w[C.task];
+    if (W.leaf) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ int ws[kWG / 64];
+    float clo[3], ext[3], scale[3];
+    wide_bin_params(W, clo, ext, scale);
+    int nl = 0;
+    for (int i = tid; i < C.count; i += kWG) nl += wide_left(W, clo, scale, cen[idxIn[C.first + i]]) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o);
+    if (lane == 0) ws[wave] = nl;
+    __syncthreads();
+    if (tid == 0) {
+        int s = 0;
+        for (int q = 0; q < kWG / 64; q++) s += ws[q];
+        leftCnt[blockIdx.x] = s;
+    }
+}
+
+// 4b. stable partition into idxOut, each chunk after the task's earlier chunks
+__global__ void __launch_bounds__(kWG) k_wide_scatter(const WChunk *chunks, const BTask *tasks, const int *idxIn, int *idxOut, const float4 *cen,
+                                                     const WTask *w, const int *leftCnt) {
+    const WChunk C = chunks[blockIdx.x];
+    const WTask &W = w[C.task];
+    if (W.leaf) return;
+    const BTask T = tasks[C.task];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ int wsum[2][kWG / 64];
+    float clo[3], ext[3], scale[3];
+    wide_bin_params(W, clo, ext, scale);
+    int lbase = 0;
+    for (int j = C.firstChunk; j < int(blockIdx.x); j++) lbase += leftCnt[j];
+    int rbase = (C.first - T.first) - lbase;
+    for (int c0 = 0; c0 < C.count; c0 += kWG) {
+        const int i = c0 + tid;
+        int t = -1;
+        bool left = false;
+        if (i < C.count) {
+            t = idxIn[C.first + i];
+            left = wide_left(W, clo, scale, cen[t]);
+        }
+        const unsigned long long ml = __ballot(left), mr = __ballot(i < C.count && !left);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (lane == 0) { wsum[0][wave] = __popcll(ml); wsum[1][wave] = __popcll(mr); }
+        __syncthreads();
+        int lo = 0, ro = 0, lt = 0, rt = 0;
+        for (int q = 0; q < kWG / 64; q++) {
+            if (q < wave) { lo += wsum[0][q]; ro += wsum[1][q]; }
+            lt += wsum[0][q]; rt += wsum[1][q];
+        }
+        if (i < C.count) {
+            if (left) idxOut[T.first + lbase + lo + __popcll(ml & below)] = t;
+            else idxOut[W.mid + rbase + ro + __popcll(mr & below)] = t;
+        }
+        lbase += lt;
+        rbase += rt;
+        __syncthreads();
     }
 }
 
@@ -379,8 +637,13 @@ hipError_t BuildBvhDevice(int nt, const float *triP, const float4 *s0, const flo
     BTask *tA = nullptr, *tB = nullptr;
     BNodeD *bn = nullptr;
     CTask *cA = nullptr, *cB = nullptr;
+    WTask *wt = nullptr;
+    WChunk *wc = nullptr;
+    int *leftCnt = nullptr;
+    std::vector<BTask> hostTasks;
+    std::vector<WChunk> hostChunks;
     // binary nodes: at most nt - 1 (every split leaves both sides non-empty)
-    const size_t maxB = size_t(nt), maxTasks = size_t(nt);
+    const size_t maxB = size_t(nt), maxTasks = size_t(nt), maxChunks = size_t(nt) / kChunk + kWideTasks + 1;
 #define BVH_CHK(x) do { e = (x); if (e != hipSuccess) goto done; } while (0)
     BVH_CHK(hipMalloc(&blo, sizeof(float4) * nt));
     BVH_CHK(hipMalloc(&bhi, sizeof(float4) * nt));
@@ -392,6 +655,9 @@ hipError_t BuildBvhDevice(int nt, const float *triP, const float4 *s0, const flo
     BVH_CHK(hipMalloc(&tB, sizeof(BTask) * maxTasks));
     BVH_CHK(hipMalloc(&bn, sizeof(BNodeD) * maxB));
     BVH_CHK(hipMalloc(&counters, sizeof(int) * 8));  // 0 binary nodes, 1 next tasks, 2 depth, 3 4-wide nodes, 4 depth4, 5 stack, 6 overflow
+    BVH_CHK(hipMalloc(&wt, sizeof(WTask) * kWideTasks));
+    BVH_CHK(hipMalloc(&wc, sizeof(WChunk) * maxChunks));
+    BVH_CHK(hipMalloc(&leftCnt, sizeof(int) * maxChunks));
     {
         BVH_CHK(hipMemsetAsync(order, 0, sizeof(int) * nt, s));
         BVH_CHK(hipMemsetAsync(bn, 0, sizeof(BNodeD) * maxB, s));
@@ -405,8 +671,31 @@ hipError_t BuildBvhDevice(int nt, const float *triP, const float4 *s0, const flo
         for (int level = 0; ntasks > 0; level++) {
             if (level > maxDepth + 1) { e = hipErrorUnknown; goto done; }  // not reached: tasks at the depth cap are leaves
             BVH_CHK(hipMemsetAsync(counters + 1, 0, sizeof(int), s));
-            hipLaunchKernelGGL(k_bvh_level, dim3(ntasks), dim3(kWG), 0, s, tA, idxA, idxB, order, blo, bhi, cen, bn, counters, tB,
-                               counters + 1, maxDepth, counters + 2, int(maxTasks), counters + 6);
+            if (ntasks < kWideTasks) {
+                // few nodes: each over chunks of kChunk triangles, one workgroup per chunk
+                hostTasks.resize(size_t(ntasks));
+                BVH_CHK(hipMemcpyAsync(hostTasks.data(), tA, sizeof(BTask) * size_t(ntasks), hipMemcpyDeviceToHost, s));
+                BVH_CHK(hipStreamSynchronize(s));
+                hostChunks.clear();
+                for (int t = 0; t < ntasks; t++) {
+                    const BTask &T = hostTasks[size_t(t)];
+                    const int first = int(hostChunks.size());
+                    for (int c0 = 0; c0 < T.count; c0 += kChunk) hostChunks.push_back(WChunk{t, T.first + c0, std::min(kChunk, T.count - c0), first});
+                }
+                const int nch = int(hostChunks.size());
+                if (size_t(nch) > maxChunks || nch == 0) { e = hipErrorUnknown; goto done; }  // not reached: the counts sum to at most nt
+                BVH_CHK(hipMemcpyAsync(wc, hostChunks.data(), sizeof(WChunk) * size_t(nch), hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL(k_wide_init, dim3(ntasks), dim3(256), 0, s, ntasks, wt);
+                hipLaunchKernelGGL(k_wide_box, dim3(nch), dim3(kWG), 0, s, wc, idxA, blo, bhi, cen, wt);
+                hipLaunchKernelGGL(k_wide_bins, dim3(nch), dim3(kWG), 0, s, wc, idxA, blo, bhi, cen, wt);
+                hipLaunchKernelGGL(k_wide_split, dim3(ntasks), dim3(64), 0, s, tA, ntasks, idxA, order, wt, bn, counters, tB, counters + 1,
+                                   maxDepth, counters + 2, int(maxTasks), counters + 6);
+                hipLaunchKernelGGL(k_wide_count, dim3(nch), dim3(kWG), 0, s, wc, idxA, cen, wt, leftCnt);
+                hipLaunchKernelGGL(k_wide_scatter, dim3(nch), dim3(kWG), 0, s, wc, tA, idxA, idxB, cen, wt, leftCnt);
+            } else {
+                hipLaunchKernelGGL(k_bvh_level, dim3(ntasks), dim3(kWG), 0, s, tA, idxA, idxB, order, blo, bhi, cen, bn, counters, tB,
+                                   counters + 1, maxDepth, counters + 2, int(maxTasks), counters + 6);
+            }
             BVH_CHK(hipGetLastError());
             BVH_CHK(hipMemcpyAsync(&ntasks, counters + 1, sizeof(int), hipMemcpyDeviceToHost, s));
             BVH_CHK(hipMemcpyAsync(&overflow, counters + 6, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -453,7 +742,7 @@ hipError_t BuildBvhDevice(int nt, const float *triP, const float4 *s0, const flo
 done:
 #undef BVH_CHK
     for (void *p : {(void *)blo, (void *)bhi, (void *)cen, (void *)idxA, (void *)idxB, (void *)order, (void *)tA, (void *)tB, (void *)bn,
-                    (void *)counters, (void *)cA, (void *)cB})
+                    (void *)counters, (void *)cA, (void *)cB, (void *)wt, (void *)wc, (void *)leftCnt})
         if (p) (void)hipFree(p);
     if (e != hipSuccess) {
         if (out.nodes) (void)hipFree(out.nodes);
