@@ -81,44 +81,52 @@ __device__ inline int bin_of(float c, float lo, float scale) {
 }
 
 // The split of a node from its bins (counts cnt[k * kBins + b], boxes
-// bb[(k * kBins + b) * 6 + j] as ford values): the host builder's loops in
-// its order, the lowest-cost split (axis, first bin of the right side), or
-// split -1 when none leaves both sides non-empty.
-__device__ void sah_split(const float *ext, const int *cnt3, const int *bb3, int &axisOut, int &splitOut) {
-    float best = 3e38f;
-    int bestAxis = -1, bestSplit = -1;
-    for (int k = 0; k < 3; k++) {
-        if (!(ext[k] > 0)) continue;
+// bb[(k * kBins + b) * 6 + j] as ford values), by one wave: the host
+// builder's choice -- over the axes with extent and the splits b = 31 .. 1
+// (left bins below b), the first in that order of the lowest cost
+// area x count of both sides -- with the 93 candidates spread over the lanes
+// (each candidate's boxes and counts are the host builder's running minima,
+// maxima and sums, which do not depend on the order they are taken in) and
+// the first lowest cost found by a wave reduction over (cost, position in
+// that order). axis / split -1: no split leaves both sides non-empty.
+__device__ void sah_split_wave(int lane, const float *ext, const int *cnt3, const int *bb3, int &axisOut, int &splitOut) {
+    constexpr int kCand = 3 * (kBins - 1);
+    float bestC = 3e38f;
+    int bestI = kCand;
+    for (int c = lane; c < kCand; c += 64) {
+        const int k = c / (kBins - 1), b = kBins - 1 - c % (kBins - 1);
+        const float e = k == 0 ? ext[0] : k == 1 ? ext[1] : ext[2];
+        if (!(e > 0)) continue;
         const int *bcnt = cnt3 + k * kBins, *bbox = bb3 + k * kBins * 6;
-        float llo[kBins][3], lhi[kBins][3];
-        int lcnt[kBins];
-        float alo[3] = {3e38f, 3e38f, 3e38f}, ahi[3] = {-3e38f, -3e38f, -3e38f};
-        int cnt = 0;
-        for (int b = 0; b < kBins - 1; b++) {
+        float llo[3] = {3e38f, 3e38f, 3e38f}, lhi[3] = {-3e38f, -3e38f, -3e38f};
+        float rlo[3] = {3e38f, 3e38f, 3e38f}, rhi[3] = {-3e38f, -3e38f, -3e38f};
+        int lc = 0, rc = 0;
+        for (int q = 0; q < b; q++) {
             for (int j = 0; j < 3; j++) {
-                alo[j] = fminf(alo[j], fdec(bbox[b * 6 + j]));
-                ahi[j] = fmaxf(ahi[j], fdec(bbox[b * 6 + 3 + j]));
-                llo[b][j] = alo[j];
-                lhi[b][j] = ahi[j];
+                llo[j] = fminf(llo[j], fdec(bbox[q * 6 + j]));
+                lhi[j] = fmaxf(lhi[j], fdec(bbox[q * 6 + 3 + j]));
             }
-            cnt += bcnt[b];
-            lcnt[b] = cnt;
+            lc += bcnt[q];
         }
-        for (int j = 0; j < 3; j++) { alo[j] = 3e38f; ahi[j] = -3e38f; }
-        cnt = 0;
-        for (int b = kBins - 1; b > 0; b--) {
+        for (int q = b; q < kBins; q++) {
             for (int j = 0; j < 3; j++) {
-                alo[j] = fminf(alo[j], fdec(bbox[b * 6 + j]));
-                ahi[j] = fmaxf(ahi[j], fdec(bbox[b * 6 + 3 + j]));
+                rlo[j] = fminf(rlo[j], fdec(bbox[q * 6 + j]));
+                rhi[j] = fmaxf(rhi[j], fdec(bbox[q * 6 + 3 + j]));
             }
-            cnt += bcnt[b];
-            if (lcnt[b - 1] == 0 || cnt == 0) continue;
-            const float cost = sah_cost(llo[b - 1], lhi[b - 1], lcnt[b - 1], alo, ahi, cnt);
-            if (cost < best) { best = cost; bestAxis = k; bestSplit = b; }
+            rc += bcnt[q];
         }
+        if (lc == 0 || rc == 0) continue;
+        const float cost = sah_cost(llo, lhi, lc, rlo, rhi, rc);
+        if (!(cost < 3e38f)) continue;
+        if (cost < bestC || (cost == bestC && c < bestI)) { bestC = cost; bestI = c; }
     }
-    axisOut = bestAxis;
-    splitOut = bestSplit;
+    for (int o = 32; o > 0; o >>= 1) {
+        const float oc = __shfl_xor(bestC, o);
+        const int oi = __shfl_xor(bestI, o);
+        if (oc < bestC || (oc == bestC && oi < bestI)) { bestC = oc; bestI = oi; }
+    }
+    axisOut = bestI < kCand ? bestI / (kBins - 1) : -1;
+    splitOut = bestI < kCand ? kBins - 1 - bestI % (kBins - 1) : -1;
 }
 
 // Per triangle: box of its corners and its centroid.
@@ -224,8 +232,12 @@ __global__ void __launch_bounds__(kWG) k_bvh_level(const BTask *tasks, const int
         }
     }
     __syncthreads();
-    // 3. the split: the host builder's loops in its order
-    if (tid == 0) sah_split(ext, &bcnt[0][0], &bbox[0][0][0], dec[0], dec[1]);
+    // 3. the split, by the first wave
+    if (wave == 0) {
+        int ax, sp;
+        sah_split_wave(lane, ext, &bcnt[0][0], &bbox[0][0][0], ax, sp);
+        if (tid == 0) { dec[0] = ax; dec[1] = sp; }
+    }
     __syncthreads();
     const int axis = dec[0], split = dec[1];
     // no bin split leaves both sides non-empty (every centroid equal): a leaf
@@ -443,7 +455,11 @@ __global__ void __launch_bounds__(64) k_wide_split(const BTask *tasks, int ntask
         }
         dec[0] = -1;
         dec[1] = -1;
-        if (!leaf) sah_split(ext, &W.cnt[0][0], &W.bb[0][0][0], dec[0], dec[1]);
+    }
+    if (!leaf) {
+        int ax, sp;
+        sah_split_wave(tid, ext, &W.cnt[0][0], &W.bb[0][0][0], ax, sp);
+        if (tid == 0) { dec[0] = ax; dec[1] = sp; }
     }
     __syncthreads();
     const int axis = dec[0], split = dec[1];
