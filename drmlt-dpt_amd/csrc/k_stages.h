@@ -33,14 +33,17 @@ namespace drmlt {
 // Wave-aggregated list_push for every active lane with `pred`: one atomic per
 // bin present in the wave, and the wave's items of one bin are stored next to
 // each other in lane order (neighbouring chains stay neighbours in the list).
-// Called by all active lanes of a wave together.
-DEV void list_push_wave(const Lists &L, int kind, int C, int bin, int item, bool pred) {
+// Called by all active lanes of a wave together. With a key, the wave's
+// items of one bin are also grouped by key (the derivative lists group by
+// camera-subpath length, so a wave of derivative passes mostly evaluates one
+// strategy).
+DEV void list_push_wave(const Lists &L, int kind, int C, int bin, int item, bool pred, int key = 0) {
     const int lane = int(threadIdx.x & 63);
     unsigned long long todo = __ballot(pred);
     while (todo) {
         const int leader = __ffsll((long long)todo) - 1;
-        const int b = __shfl(bin, leader);
-        const bool mine = pred && bin == b;
+        const int b = __shfl(bin, leader), k = __shfl(key, leader);
+        const bool mine = pred && bin == b && key == k;
         const unsigned long long m = __ballot(mine);
         int base = 0;
         if (lane == leader) base = atomicAdd(&L.count[kind * kMaxBins + b], __popcll(m));
@@ -110,7 +113,7 @@ DEV void stage1_kernel(const DevScene &Sc, const Chains &ch, const DrParams &P, 
         list_push_wave(L, 1, ch.C, bin, chain, second);
         list_push_wave(L, 0, ch.C, bin, chain, h2mc);
         list_push_wave(L, 4, ch.C, bin, chain,
-                       h2mc && !ch.gInit[pc] && has_derivative(ch.c[pc], ch.l[pc], P.maxDervDepth) && double(ch.ss[pc]) > 1e-15);
+                       h2mc && !ch.gInit[pc] && has_derivative(ch.c[pc], ch.l[pc], P.maxDervDepth) && double(ch.ss[pc]) > 1e-15, ch.c[pc]);
         list_push_wave(L, 5, ch.C, bin, chain, (flags & 128) != 0);
     }
 }
@@ -213,13 +216,14 @@ DEV void stage2_propose_kernel(const DevScene &Sc, const Chains &ch, const DrPar
         POut<float> out;
         out.valid = false;
         walk_run(Sc, P.o, ch, P.discreteStdDev, w, act, out);
-        int bin = 0, flags = 0;
+        int bin = 0, flags = 0, cam = 0;
         if (act) {
             stage2_end(Sc, ch, chain, a, out);
             flags = ch.flags[chain];
             bin = length_bin(ch.c[a.pc] + ch.l[a.pc] - 1);
+            cam = ch.c[a.pc];
         }
-        list_push_wave(L, 2, ch.C, bin, chain, (flags & 4) && (flags & 16));
+        list_push_wave(L, 2, ch.C, bin, chain, (flags & 4) && (flags & 16), cam);
     }
 }
 
