@@ -78,7 +78,8 @@ __global__ void __launch_bounds__(64) DRMLT_STAGE(k_b_stage1_bin)(Chains ch, DrP
         const int chain = P.chainBase + i;
         Philox g(P.seed, iter, chain, TAG_S1);
         const Stage1 a = stage1_begin(ch, P, chain, g);
-        list_push_wave(L, 3, ch.C, length_bin(stage1_length(ch, a)), chain, true);
+        const int cam = (!a.curValid || a.isLarge) ? a.cam : ch.c[a.pc];  // camera vertices of the path stage 1 evaluates
+        list_push_wave(L, 3, ch.C, length_bin(stage1_length(ch, a)), chain, true, cam);
     }
 }
 
@@ -110,7 +111,7 @@ DEV void stage1_kernel(const DevScene &Sc, const Chains &ch, const DrParams &P, 
         // (H2MC needing the current state's derivatives); list 5: lens mutations
         const bool second = (flags & 2) != 0, h2mc = second && (flags & 4);
         const int pc = a.pc, bin = length_bin(ch.c[pc] + ch.l[pc] - 1);
-        list_push_wave(L, 1, ch.C, bin, chain, second);
+        list_push_wave(L, 1, ch.C, bin, chain, second, ch.c[pc]);
         list_push_wave(L, 0, ch.C, bin, chain, h2mc);
         list_push_wave(L, 4, ch.C, bin, chain,
                        h2mc && !ch.gInit[pc] && has_derivative(ch.c[pc], ch.l[pc], P.maxDervDepth) && double(ch.ss[pc]) > 1e-15, ch.c[pc]);
