@@ -293,7 +293,7 @@ def main():
     ap.add_argument("--config", type=int, default=4)
     ap.add_argument("--chains", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the CPU baseline (the CPU share of one GPU on the box)")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the CPU baseline (the host CPU share of one GPU)")
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--tune-steps", type=int, default=8, help="timed steps per candidate schedule of the autotuner")
     ap.add_argument("--no-deriv", action="store_true", help="skip the derivative-kernel workload (config 3)")
@@ -464,7 +464,7 @@ def main():
                "sample": f"{cchains} chains x {citers} iterations of the same scene after a 20000-sample pre-pass "
                          f"({csec:.1f} s of chain loop)",
                "host_cpu_model": model, "host_cpu_count": host_cpus,
-               "threads_note": "one std::thread per core over chains; the GPU box grants one GPU's share of 16 CPU threads"}
+               "threads_note": "one std::thread per core over chains; the host grants one GPU's share of 16 CPU threads"}
         log(f"cpu baseline {cval:.4g} mutations/s on {ncpu} threads")
 
     out = {
