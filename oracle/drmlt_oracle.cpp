@@ -270,6 +270,40 @@ extern "C" void orc_lens_grad_hess_fd(void *h, int n, const PathRec *in, const f
     }
 }
 
+// The h2mc lens mutation's acceptance (LensMutateAt, h2mc.cpp:463-530) in
+// double for given screen offsets off[2i..2i+1]: the current state is the
+// record with its contribution row c8 (ssScore, screen position) and the lens
+// score of its fixed re-evaluation; lens Gaussians with sigma lensSigma from
+// central differences of step fdStep, derivatives up to path length
+// maxDerv. a[i] = 0 where the perturbed path has no contribution.
+extern "C" void orc_lens_accept(void *h, int n, const PathRec *in, const float *c8, const float *off, double lensSigma, double fdStep,
+                                int maxDerv, double *a) {
+    OrcScene *s = (OrcScene *)h;
+    DrChainCtx<double> cx;
+    cx.s = &s->d;
+    cx.sd = &s->d;
+    cx.dro.lensStdDev = lensSigma;
+    cx.dro.fdStep = fdStep;
+    cx.dro.maxDervDepth = maxDerv;
+    cx.b = 1.0;
+    for (int i = 0; i < n; i++) {
+        MState<double> cur, prop;
+        cur.valid = true;
+        cur.path = FromRec<double>(in[i]);
+        const float *r = c8 + 8 * size_t(i);
+        cur.sp.c = in[i].c;
+        cur.sp.l = in[i].l;
+        cur.sp.contrib = V3<double>(r[1], r[2], r[3]);
+        cur.sp.lsScore = r[4];
+        cur.sp.ssScore = r[5];
+        cur.sp.screen[0] = r[6];
+        cur.sp.screen[1] = r[7];
+        cur.sp.lensScore = LensPrepare(s->d, cx.o, cur.path);
+        const double o2[2] = {double(off[2 * i]), double(off[2 * i + 1])};
+        a[i] = LensMutateAt(cx, cur, prop, o2);
+    }
+}
+
 // Per sample up to maxc contributions, each 10 floats: c, l, r, g, b, lsScore, ssScore, sx, sy, mis.
 template <class F>
 static void GenBidirT(const SceneT<F> &s, int n, int minDepth, int maxDepth, const float *uni, int nuni, int maxc, float *cs, int *nc) {

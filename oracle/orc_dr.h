@@ -344,6 +344,7 @@ template <class F> inline double H2MCMutate(DrChainCtx<F> &cx, MState<F> &cur, M
 // current state's lens Gaussian, screen position moved by it (mod 1),
 // PerturbLensBidir, acceptance min(1, N_z(-off) / N_x(off) lensScore_z /
 // lensScore_x).
+template <class F> inline F LensMutateAt(DrChainCtx<F> &cx, MState<F> &cur, MState<F> &prop, const F *off);
 template <class F> inline F LensMutate(DrChainCtx<F> &cx, MState<F> &cur, MState<F> &prop, Philox &rng) {
     if (!cur.lgInit) InitLensGaussian(cx, cur);
     double n[2], offd[2];
@@ -351,6 +352,12 @@ template <class F> inline F LensMutate(DrChainCtx<F> &cx, MState<F> &cur, MState
     n[1] = rng.normal();
     GenerateSample(cur.lg, n, offd);
     const F off[2] = {F(offd[0]), F(offd[1])};
+    return LensMutateAt(cx, cur, prop, off);
+}
+// The lens mutation by a given screen offset (the current state's lens
+// Gaussian is initialised when missing).
+template <class F> inline F LensMutateAt(DrChainCtx<F> &cx, MState<F> &cur, MState<F> &prop, const F *off) {
+    if (!cur.lgInit) InitLensGaussian(cx, cur);
     const F screen[2] = {Modulo(cur.sp.screen[0] + off[0], F(1)), Modulo(cur.sp.screen[1] + off[1], F(1))};
     prop.path = cur.path;
     std::vector<Contrib<F>> out;

@@ -164,6 +164,16 @@ class Scene:
         lib().orc_lens_grad_hess_fd(C.c_void_p(self.h), n, P(recs), P(screen), C.c_double(h), P(logl), P(g), P(H))
         return logl, g, H
 
+    def lens_accept(self, recs, c8, off, sigma, fd_step=2e-6, max_derv=9):
+        """The oracle loop's lens-mutation acceptance (LensMutateAt) of each
+        state for the screen offset off (n x 2), in double."""
+        recs = np.ascontiguousarray(recs, PATH_REC); n = len(recs)
+        c8 = np.ascontiguousarray(c8, np.float32).reshape(n, 8)
+        off = np.ascontiguousarray(off, np.float32).reshape(n, 2)
+        a = np.zeros(n)
+        lib().orc_lens_accept(C.c_void_p(self.h), n, P(recs), P(c8), P(off), C.c_double(sigma), C.c_double(fd_step), int(max_derv), P(a))
+        return a
+
     def direct_pass(self, spp, seed=1, threads=8):
         film = np.zeros((self.H, self.W, 3), np.float32)
         lib().orc_direct_pass(C.c_void_p(self.h), spp, C.c_uint64(seed), threads, P(film))

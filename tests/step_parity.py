@@ -306,6 +306,30 @@ def _lens_accept(off, gx, gy, ls_x, ls_y):
     return 0.0 if _invalid(np.array([a]))[0] else a
 
 
+def lens_accept_fd(O, X, x8, off, ls_x, sigma, max_derv=9, fd_step=2e-6):
+    """The lens mutation's acceptance (h2mc.cpp:516-523) of states X (their
+    contribution rows x8, lens scores ls_x) moved by screen offsets off, from
+    the oracle: FP64 PerturbLensBidir to x's screen position plus off (mod 1),
+    lens Gaussians from central differences (step fd_step) of the FP64 lens
+    function at x's and the proposal's screen positions, the FP64 Gaussian
+    log densities. Returns (a, proposals, their c9 rows)."""
+    scr = x8[:, 6:8] + off
+    scr = (scr - np.floor(scr)).astype(np.float32)
+    Yo, c9 = O.perturb_lens(X, scr, dbl=True)
+    y_o = c9[:, 0] > 0
+    gyf, hyf = np.zeros((len(X), 2)), np.zeros((len(X), 2, 2))
+    _, gxf, hxf = O.lens_grad_hess_fd(X, x8[:, 6:8], h=fd_step)
+    if y_o.any():
+        _, gyf[y_o], hyf[y_o] = O.lens_grad_hess_fd(Yo[y_o], c9[y_o, 6:8], h=fd_step)
+    a = np.zeros(len(X))
+    for k in np.nonzero(y_o)[0]:
+        cx, lx = int(X["c"][k]), int(X["l"][k])
+        gx = _lens_gaussian(cx, lx, float(ls_x[k]), float(x8[k, 5]), gxf[k], hxf[k], True, sigma, max_derv)
+        gy = _lens_gaussian(cx, lx, float(c9[k, 8]), float(c9[k, 5]), gyf[k], hyf[k], True, sigma, max_derv)
+        a[k] = _lens_accept(off[k].astype(np.float64), gx, gy, float(ls_x[k]), float(c9[k, 8]))
+    return a, Yo, c9
+
+
 def analyse_lens(O, ctx, before, after, lens_after, sigma, max_derv=9, fd_step=2e-6):
     """Per-state parity of the h2mc integrator's lens mutations
     (h2mc.cpp:463-530) in one iteration. For each chain that made one (flag
@@ -332,14 +356,12 @@ def analyse_lens(O, ctx, before, after, lens_after, sigma, max_derv=9, fd_step=2
     off = after["off"][L, :2].astype(np.float32)
     a1 = after["a1"][L].astype(np.float64)
     y_dev = (after["flags"][L] & 32) != 0
-    scr = x8[:, 6:8] + off
-    scr = (scr - np.floor(scr)).astype(np.float32)
-    Yo, c9 = O.perturb_lens(X, scr, dbl=True)
+    ls_x, ls_y = lens_after[0][L, pc].astype(np.float64), lens_after[0][L, pf].astype(np.float64)
+    aC, Yo, c9 = lens_accept_fd(O, X, x8, off, ls_x, sigma, max_derv, fd_step)
     y_o = c9[:, 0] > 0
     both = y_dev & y_o
     fb = both & (c9[:, 5] > 0)
     f_rel = np.abs(y8[fb, 5].astype(np.float64) - c9[fb, 5]) / c9[fb, 5]
-    ls_x, ls_y = lens_after[0][L, pc].astype(np.float64), lens_after[0][L, pf].astype(np.float64)
     known = ((lens_after[1][L, pc] & 1) != 0) & (~y_dev | ((lens_after[1][L, pf] & 1) != 0))
     lp_x, lp_y = lens_after[2][L, pc], lens_after[2][L, pf]
     ox, okx = ctx.lens_grad_hess(X, x8[:, 6:8], lp_x)
@@ -348,11 +370,7 @@ def analyse_lens(O, ctx, before, after, lens_after, sigma, max_derv=9, fd_step=2
         oy[y_dev], oky[y_dev] = ctx.lens_grad_hess(Y[y_dev], y8[y_dev, 6:8], lp_y[y_dev])
     lb = both & (c9[:, 8] > 0)
     ls_rel = np.abs(ls_y[lb] - c9[lb, 8]) / c9[lb, 8]
-    gyf, hyf = np.zeros((len(L), 2)), np.zeros((len(L), 2, 2))
-    _, gxf, hxf = O.lens_grad_hess_fd(X, x8[:, 6:8], h=fd_step)
-    if y_o.any():
-        _, gyf[y_o], hyf[y_o] = O.lens_grad_hess_fd(Yo[y_o], c9[y_o, 6:8], h=fd_step)
-    aB, aC = np.zeros(len(L)), np.zeros(len(L))
+    aB = np.zeros(len(L))
     for k in range(len(L)):
         cx, lx = int(X["c"][k]), int(X["l"][k])
         h2 = lambda o: np.array([[o[6], o[7]], [o[7], o[8]]], np.float64)
@@ -360,10 +378,6 @@ def analyse_lens(O, ctx, before, after, lens_after, sigma, max_derv=9, fd_step=2
             gx = _lens_gaussian(cx, lx, ls_x[k], float(x8[k, 5]), ox[k, 4:6].astype(np.float64), h2(ox[k]), okx[k], sigma, max_derv)
             gy = _lens_gaussian(cx, lx, ls_y[k], float(y8[k, 5]), oy[k, 4:6].astype(np.float64), h2(oy[k]), oky[k], sigma, max_derv)
             aB[k] = _lens_accept(off[k].astype(np.float64), gx, gy, ls_x[k], ls_y[k])
-        if y_o[k]:
-            gx = _lens_gaussian(cx, lx, ls_x[k], float(x8[k, 5]), gxf[k], hxf[k], True, sigma, max_derv)
-            gy = _lens_gaussian(cx, lx, float(c9[k, 8]), float(c9[k, 5]), gyf[k], hyf[k], True, sigma, max_derv)
-            aC[k] = _lens_accept(off[k].astype(np.float64), gx, gy, ls_x[k], float(c9[k, 8]))
     return {"n": len(after["perm"]), "lens": len(L), "y_valid_agree": y_dev == y_o, "score_known": known,
             "tri_equal": _same_triangles(Y[both], Yo[both]), "f_rel": f_rel, "ls_rel": ls_rel,
             "a1_gpu": a1, "a1_B": aB, "a1_C": aC, "chains": L, "X": X, "ls_x": ls_x, "ls_y": ls_y,

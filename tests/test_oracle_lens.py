@@ -241,3 +241,35 @@ def test_h2mc_with_lens_converges_to_bdpt():
         rel = np.abs(ib[mask] - rb[mask]) / rb[mask]
         assert np.median(rel) < 0.08, (lp, np.median(rel))
         assert ib.sum() == pytest.approx(rb.sum(), rel=0.05)
+
+
+@pytest.mark.parametrize("which", ["cfg1", "cfg2_small"])
+def test_lens_acceptance_helper_matches_the_oracle_loop(which, request):
+    """The lens-mutation acceptance that the GPU per-state parity test
+    computes from the oracle's functions (step_parity.lens_accept_fd:
+    PerturbLensBidir, central-difference lens Gaussians, Gaussian log
+    densities) equals the oracle loop's own LensMutateAt for the same states
+    and offsets (an independent restatement checked against the oracle's
+    C++), and disagrees once the offset's sign is flipped."""
+    import step_parity as sp
+
+    sc = request.getfixturevalue(which)
+    O = orc.Scene(sc)
+    cl, uni, st, c8 = _states(O, 2000 if which == "cfg1" else 8000, 11)
+    ls = O.lens_score(st, dbl=True)[:, 0]
+    sel = np.where((c8[:, 0] > 0) & (ls > 0))[0][:200]
+    assert len(sel) > 40
+    X, x8, lsx = st[sel], c8[sel], ls[sel].astype(np.float64)
+    rng = np.random.default_rng(5)
+    sigma = 0.01
+    off = (rng.normal(size=(len(sel), 2)) * sigma).astype(np.float32)
+    a_helper, _, c9 = sp.lens_accept_fd(O, X, x8, off, lsx, sigma)
+    a_oracle = O.lens_accept(X, x8, off, sigma)
+    valid = c9[:, 0] > 0
+    assert valid.mean() > 0.8 and (a_oracle[valid] > 0).mean() > 0.5
+    err = np.abs(a_helper - a_oracle)
+    print(which, "helper vs oracle loop: max", err.max(), "median", np.median(err))
+    assert err.max() < 1e-4 and np.median(err) < 1e-6, (err.max(), np.median(err))
+    # the offset's sign flipped
+    a_planted, _, _ = sp.lens_accept_fd(O, X, x8, -off, lsx, sigma)
+    assert np.abs(a_planted - a_oracle)[valid].max() > 1e-3
