@@ -84,41 +84,52 @@ __device__ inline int bin_of(float c, float lo, float scale) {
 // bb[(k * kBins + b) * 6 + j] as ford values), by one wave: the host
 // builder's choice -- over the axes with extent and the splits b = 31 .. 1
 // (left bins below b), the first in that order of the lowest cost
-// area x count of both sides -- with the 93 candidates spread over the lanes
-// (each candidate's boxes and counts are the host builder's running minima,
-// maxima and sums, which do not depend on the order they are taken in) and
+// area x count of both sides -- with each candidate's boxes and counts (the
+// host builder's running minima, maxima and sums, which do not depend on the
+// order they are taken in) from scans over the bins, one bin per lane, and
 // the first lowest cost found by a wave reduction over (cost, position in
 // that order). axis / split -1: no split leaves both sides non-empty.
 __device__ void sah_split_wave(int lane, const float *ext, const int *cnt3, const int *bb3, int &axisOut, int &splitOut) {
     constexpr int kCand = 3 * (kBins - 1);
+    static_assert(kBins == 32, "one bin per lane of a half wave");
     float bestC = 3e38f;
     int bestI = kCand;
-    for (int c = lane; c < kCand; c += 64) {
-        const int k = c / (kBins - 1), b = kBins - 1 - c % (kBins - 1);
-        const float e = k == 0 ? ext[0] : k == 1 ? ext[1] : ext[2];
-        if (!(e > 0)) continue;
-        const int *bcnt = cnt3 + k * kBins, *bbox = bb3 + k * kBins * 6;
-        float llo[3] = {3e38f, 3e38f, 3e38f}, lhi[3] = {-3e38f, -3e38f, -3e38f};
-        float rlo[3] = {3e38f, 3e38f, 3e38f}, rhi[3] = {-3e38f, -3e38f, -3e38f};
-        int lc = 0, rc = 0;
-        for (int q = 0; q < b; q++) {
-            for (int j = 0; j < 3; j++) {
-                llo[j] = fminf(llo[j], fdec(bbox[q * 6 + j]));
-                lhi[j] = fmaxf(lhi[j], fdec(bbox[q * 6 + 3 + j]));
-            }
-            lc += bcnt[q];
+    // pass 0: axes 0 and 1 on the two half waves; pass 1: axis 2. Lane b of a
+    // half wave holds bin b; running minima, maxima and sums over the bins
+    // below and from b come from scans within the half wave.
+    for (int pass = 0; pass < 2; pass++) {
+        const int k = 2 * pass + (lane >> 5), b = lane & 31;
+        const bool act = k < 3;
+        const float e = k == 0 ? ext[0] : k == 1 ? ext[1] : k == 2 ? ext[2] : 0.0f;
+        float plo[3], phi[3], slo[3], shi[3];
+        int pc = act ? cnt3[k * kBins + b] : 0, sc = pc;
+        for (int j = 0; j < 3; j++) {
+            plo[j] = act ? fdec(bb3[(k * kBins + b) * 6 + j]) : 3e38f;
+            phi[j] = act ? fdec(bb3[(k * kBins + b) * 6 + 3 + j]) : -3e38f;
+            slo[j] = plo[j];
+            shi[j] = phi[j];
         }
-        for (int q = b; q < kBins; q++) {
+        for (int d = 1; d < 32; d <<= 1) {
+            const bool up = b >= d, down = b + d < 32;
+            const int tpc = __shfl_up(pc, d, 32), tsc = __shfl_down(sc, d, 32);
+            if (up) pc += tpc;
+            if (down) sc += tsc;
             for (int j = 0; j < 3; j++) {
-                rlo[j] = fminf(rlo[j], fdec(bbox[q * 6 + j]));
-                rhi[j] = fmaxf(rhi[j], fdec(bbox[q * 6 + 3 + j]));
+                const float a0 = __shfl_up(plo[j], d, 32), a1 = __shfl_up(phi[j], d, 32);
+                const float c0 = __shfl_down(slo[j], d, 32), c1 = __shfl_down(shi[j], d, 32);
+                if (up) { plo[j] = fminf(plo[j], a0); phi[j] = fmaxf(phi[j], a1); }
+                if (down) { slo[j] = fminf(slo[j], c0); shi[j] = fmaxf(shi[j], c1); }
             }
-            rc += bcnt[q];
         }
-        if (lc == 0 || rc == 0) continue;
-        const float cost = sah_cost(llo, lhi, lc, rlo, rhi, rc);
-        if (!(cost < 3e38f)) continue;
-        if (cost < bestC || (cost == bestC && c < bestI)) { bestC = cost; bestI = c; }
+        // the split before bin b: left = bins 0 .. b-1 (lane b-1's prefix), right = bins b .. 31
+        float llo[3], lhi[3];
+        const int lc = __shfl_up(pc, 1, 32);
+        for (int j = 0; j < 3; j++) { llo[j] = __shfl_up(plo[j], 1, 32); lhi[j] = __shfl_up(phi[j], 1, 32); }
+        if (act && e > 0 && b >= 1 && lc != 0 && sc != 0) {
+            const float cost = sah_cost(llo, lhi, lc, slo, shi, sc);
+            const int c = k * (kBins - 1) + (kBins - 1 - b);
+            if (cost < 3e38f && (cost < bestC || (cost == bestC && c < bestI))) { bestC = cost; bestI = c; }
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         const float oc = __shfl_xor(bestC, o);
@@ -312,6 +323,191 @@ __global__ void __launch_bounds__(kWG) k_bvh_level(const BTask *tasks, const int
             next[q + 1] = BTask{mid, T.first + T.count - mid, T.depth + 1, 2 * node + 1};
         }
     }
+}
+
+// ---- levels with many small nodes (the bottom of the tree): one wave per
+// node, four per workgroup, the steps of k_bvh_level with wave reductions and
+// a wave-local partition (the same values and the same order)
+constexpr int kWaveTasks = 4096;  // levels with at least this many nodes are built this way
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+// Internal nodes are not numbered here: flag[t] = 1 and mid[t] record them,
+// and k_bvh_emit numbers them in task order after a scan of the flags (one
+// atomic counter per node would serialise hundreds of thousands of nodes).
+__global__ void __launch_bounds__(kWG) k_bvh_level_wave(const BTask *tasks, int ntasks, const int *idxIn, int *idxOut, int *order,
+                                                       const float4 *blo, const float4 *bhi, const float4 *cen, BNodeD *bn, int *flag, int *midOut,
+                                                       int maxDepth, int *depthMax, int *overflow) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = int(blockIdx.x) * (kWG / 64) + wave;
+    __shared__ int bcntS[kWG / 64][3][kBins];
+    __shared__ int bboxS[kWG / 64][3][kBins][6];
+    if (t >= ntasks) return;  // the whole wave; nothing below waits on other waves
+    const BTask T = tasks[t];
+    int (*bcnt)[kBins] = bcntS[wave];
+    int (*bbox)[kBins][6] = bboxS[wave];
+    // 1. the range's box and its centroids' box
+    float v[12];
+    for (int k = 0; k < 3; k++) { v[k] = 3e38f; v[3 + k] = -3e38f; v[6 + k] = 3e38f; v[9 + k] = -3e38f; }
+    for (int i = lane; i < T.count; i += 64) {
+        const int q = idxIn[T.first + i];
+        const float4 a = blo[q], b = bhi[q], c = cen[q];
+        const float al[3] = {a.x, a.y, a.z}, bh[3] = {b.x, b.y, b.z}, cc[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; k++) {
+            v[k] = fminf(v[k], al[k]); v[3 + k] = fmaxf(v[3 + k], bh[k]);
+            v[6 + k] = fminf(v[6 + k], cc[k]); v[9 + k] = fmaxf(v[9 + k], cc[k]);
+        }
+    }
+    float box[12];
+    for (int k = 0; k < 12; k++) box[k] = (k % 6) < 3 ? wmin(v[k]) : wmax(v[k]);
+    bool leaf = T.count <= kLeaf || T.depth >= maxDepth;
+    if (lane == 0) {
+        if (t == 0) atomicMax(depthMax, T.depth);  // every node of a level has the same depth
+        if (T.parent >= 0) {
+            float lo[3] = {box[0], box[1], box[2]}, hi[3] = {box[3], box[4], box[5]};
+            box_pad(lo, hi);
+            BNodeD &P = bn[T.parent >> 1];
+            const int side = T.parent & 1;
+            for (int k = 0; k < 3; k++) { P.box[side][k] = lo[k]; P.box[side][3 + k] = hi[k]; }
+        }
+    }
+    int axis = -1, split = -1;
+    float clo[3], ext[3], scale[3];
+    if (!leaf) {
+        // 2. bins over the centroid extent of each axis
+        for (int k = 0; k < 3; k++) {
+            clo[k] = box[6 + k];
+            ext[k] = box[9 + k] - clo[k];
+            scale[k] = ext[k] > 0 ? float(kBins) / ext[k] : 0.0f;
+        }
+        for (int i = lane; i < 3 * kBins; i += 64) {
+            const int k = i / kBins, b = i % kBins;
+            bcnt[k][b] = 0;
+            for (int j = 0; j < 3; j++) { bbox[k][b][j] = ford(3e38f); bbox[k][b][3 + j] = ford(-3e38f); }
+        }
+        wave_sync();
+        for (int i = lane; i < T.count; i += 64) {
+            const int q = idxIn[T.first + i];
+            const float4 a = blo[q], bb = bhi[q], c = cen[q];
+            const float al[3] = {a.x, a.y, a.z}, bh[3] = {bb.x, bb.y, bb.z}, cc[3] = {c.x, c.y, c.z};
+            for (int k = 0; k < 3; k++) {
+                if (!(ext[k] > 0)) continue;
+                const int b = bin_of(cc[k], clo[k], scale[k]);
+                atomicAdd(&bcnt[k][b], 1);
+                for (int j = 0; j < 3; j++) {
+                    atomicMin(&bbox[k][b][j], ford(al[j]));
+                    atomicMax(&bbox[k][b][3 + j], ford(bh[j]));
+                }
+            }
+        }
+        wave_sync();
+        // 3. the split
+        sah_split_wave(lane, ext, &bcnt[0][0], &bbox[0][0][0], axis, split);
+        leaf = split <= 0;
+    }
+    if (leaf) {
+        for (int i = lane; i < T.count; i += 64) order[T.first + i] = idxIn[T.first + i];
+        if (lane == 0) {
+            flag[t] = 0;
+            if (T.parent >= 0) {
+                bn[T.parent >> 1].index[T.parent & 1] = T.first;
+                bn[T.parent >> 1].count[T.parent & 1] = T.count;
+            } else {
+                atomicAdd(overflow + 1, 1);  // the root is one leaf: the host builder's case
+            }
+        }
+        return;
+    }
+    // 4. stable partition into idxOut: left of the split, then right
+    auto isLeft = [&](int q) {
+        const float4 c = cen[q];
+        const float cc = axis == 0 ? c.x : axis == 1 ? c.y : c.z;
+        return bin_of(cc, clo[axis], scale[axis]) < split;
+    };
+    int nl = 0;
+    for (int i = lane; i < T.count; i += 64) nl += isLeft(idxIn[T.first + i]) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) nl += __shfl_xor(nl, o);
+    const int mid = T.first + nl;
+    int lbase = 0, rbase = 0;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int c0 = 0; c0 < T.count; c0 += 64) {
+        const int i = c0 + lane;
+        int q = -1;
+        bool left = false;
+        if (i < T.count) {
+            q = idxIn[T.first + i];
+            left = isLeft(q);
+        }
+        const unsigned long long ml = __ballot(left), mr = __ballot(i < T.count && !left);
+        if (i < T.count) {
+            if (left) idxOut[T.first + lbase + __popcll(ml & below)] = q;
+            else idxOut[mid + rbase + __popcll(mr & below)] = q;
+        }
+        lbase += __popcll(ml);
+        rbase += __popcll(mr);
+    }
+    // 5. an internal node: numbered by k_bvh_emit
+    if (lane == 0) {
+        flag[t] = 1;
+        midOut[t] = mid;
+    }
+}
+
+// Exclusive scan of the 0/1 flags of n tasks in blocks of kScanBlock:
+// position within the block (local) and each block's count (blockSum).
+constexpr int kScanBlock = 1024;
+__global__ void __launch_bounds__(kScanBlock) k_scan_local(int n, const int *flag, int *local, int *blockSum) {
+    __shared__ int ws[kScanBlock / 64];
+    const int i = int(blockIdx.x) * kScanBlock + int(threadIdx.x), lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool f = i < n && flag[i] != 0;
+    const unsigned long long m = __ballot(f);
+    if (lane == 0) ws[wave] = __popcll(m);
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wave; w++) off += ws[w];
+    if (i < n) local[i] = off + __popcll(m & ((1ull << lane) - 1ull));
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int w = 0; w < kScanBlock / 64; w++) s += ws[w];
+        blockSum[blockIdx.x] = s;
+    }
+}
+// The blocks' exclusive offsets in place and the total (one thread: at most
+// maxTasks / kScanBlock blocks).
+__global__ void k_scan_sums(int nb, int *blockSum, int *total) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int s = 0;
+    for (int b = 0; b < nb; b++) {
+        const int v = blockSum[b];
+        blockSum[b] = s;
+        s += v;
+    }
+    *total = s;
+}
+// The internal nodes of a wave-built level in task order: node
+// nodeCount + rank, children tasks 2 rank and 2 rank + 1.
+__global__ void __launch_bounds__(256) k_bvh_emit(const BTask *tasks, int ntasks, const int *flag, const int *midIn, const int *local,
+                                                 const int *blockOff, const int *nodeCount, BNodeD *bn, BTask *next, int cap, int *overflow) {
+    const int t = int(blockIdx.x) * 256 + int(threadIdx.x);
+    if (t >= ntasks || !flag[t]) return;
+    const BTask T = tasks[t];
+    const int rank = blockOff[t / kScanBlock] + local[t], node = *nodeCount + rank, mid = midIn[t];
+    if (node >= cap || 2 * rank + 1 >= cap || T.parent < 0) {
+        atomicAdd(overflow, 1);  // not reached: the root is built by the chunked levels
+        return;
+    }
+    bn[T.parent >> 1].index[T.parent & 1] = node;
+    bn[T.parent >> 1].count[T.parent & 1] = 0;
+    next[2 * rank] = BTask{T.first, mid - T.first, T.depth + 1, 2 * node};
+    next[2 * rank + 1] = BTask{mid, T.first + T.count - mid, T.depth + 1, 2 * node + 1};
+}
+// Counters after an emitted level: nodes += total, next tasks = 2 total.
+__global__ void k_bvh_emit_counts(const int *total, int *nodeCount, int *nextCount) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    *nodeCount += *total;
+    *nextCount = 2 * *total;
 }
 
 // ---- levels with few nodes (the top of the tree): several workgroups per
@@ -656,6 +852,7 @@ hipError_t BuildBvhDevice(int nt, const float *triP, const float4 *s0, const flo
     WTask *wt = nullptr;
     WChunk *wc = nullptr;
     int *leftCnt = nullptr;
+    int *wflag = nullptr, *wmid = nullptr, *wlocal = nullptr, *wblock = nullptr;
     std::vector<BTask> hostTasks;
     std::vector<WChunk> hostChunks;
     // binary nodes: at most nt - 1 (every split leaves both sides non-empty)
@@ -674,6 +871,10 @@ hipError_t BuildBvhDevice(int nt, const float *triP, const float4 *s0, const flo
     BVH_CHK(hipMalloc(&wt, sizeof(WTask) * kWideTasks));
     BVH_CHK(hipMalloc(&wc, sizeof(WChunk) * maxChunks));
     BVH_CHK(hipMalloc(&leftCnt, sizeof(int) * maxChunks));
+    BVH_CHK(hipMalloc(&wflag, sizeof(int) * maxTasks));
+    BVH_CHK(hipMalloc(&wmid, sizeof(int) * maxTasks));
+    BVH_CHK(hipMalloc(&wlocal, sizeof(int) * maxTasks));
+    BVH_CHK(hipMalloc(&wblock, sizeof(int) * (maxTasks / kScanBlock + 2)));
     {
         BVH_CHK(hipMemsetAsync(order, 0, sizeof(int) * nt, s));
         BVH_CHK(hipMemsetAsync(bn, 0, sizeof(BNodeD) * maxB, s));
@@ -708,6 +909,16 @@ hipError_t BuildBvhDevice(int nt, const float *triP, const float4 *s0, const flo
                                    maxDepth, counters + 2, int(maxTasks), counters + 6);
                 hipLaunchKernelGGL(k_wide_count, dim3(nch), dim3(kWG), 0, s, wc, idxA, cen, wt, leftCnt);
                 hipLaunchKernelGGL(k_wide_scatter, dim3(nch), dim3(kWG), 0, s, wc, tA, idxA, idxB, cen, wt, leftCnt);
+            } else if (ntasks >= kWaveTasks) {
+                // many small nodes: one wave each, internal nodes numbered in task order by a scan
+                const int nb = (ntasks + kScanBlock - 1) / kScanBlock;
+                hipLaunchKernelGGL(k_bvh_level_wave, dim3((ntasks + kWG / 64 - 1) / (kWG / 64)), dim3(kWG), 0, s, tA, ntasks, idxA, idxB, order,
+                                   blo, bhi, cen, bn, wflag, wmid, maxDepth, counters + 2, counters + 6);
+                hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanBlock), 0, s, ntasks, wflag, wlocal, wblock);
+                hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, nb, wblock, wblock + nb);
+                hipLaunchKernelGGL(k_bvh_emit, dim3((ntasks + 255) / 256), dim3(256), 0, s, tA, ntasks, wflag, wmid, wlocal, wblock, counters, bn, tB,
+                                   int(maxTasks), counters + 6);
+                hipLaunchKernelGGL(k_bvh_emit_counts, dim3(1), dim3(64), 0, s, wblock + nb, counters, counters + 1);
             } else {
                 hipLaunchKernelGGL(k_bvh_level, dim3(ntasks), dim3(kWG), 0, s, tA, idxA, idxB, order, blo, bhi, cen, bn, counters, tB,
                                    counters + 1, maxDepth, counters + 2, int(maxTasks), counters + 6);
@@ -758,7 +969,8 @@ hipError_t BuildBvhDevice(int nt, const float *triP, const float4 *s0, const flo
 done:
 #undef BVH_CHK
     for (void *p : {(void *)blo, (void *)bhi, (void *)cen, (void *)idxA, (void *)idxB, (void *)order, (void *)tA, (void *)tB, (void *)bn,
-                    (void *)counters, (void *)cA, (void *)cB, (void *)wt, (void *)wc, (void *)leftCnt})
+                    (void *)counters, (void *)cA, (void *)cB, (void *)wt, (void *)wc, (void *)leftCnt, (void *)wflag, (void *)wmid,
+                    (void *)wlocal, (void *)wblock})
         if (p) (void)hipFree(p);
     if (e != hipSuccess) {
         if (out.nodes) (void)hipFree(out.nodes);
