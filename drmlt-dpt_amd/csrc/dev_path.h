@@ -823,11 +823,12 @@ DEV void perturb_lens_eval(const DevScene &S, const Opts &o, const SRef &ref, co
 // lens function, FIXED perturb_lens_eval) by two forward-over-forward passes:
 // g = (d/ds0, d/ds1), h = (H00, H01, H11). False without a contribution.
 DEV bool lens_jets(const DevScene &S, const Opts &o, const SRef &ref, const float scr[2], const V3<float> &lensPos, float g[2], float h[3]) {
-    for (int row = 0; row < 2; row++) {
+    {
+        // row x: the gradient and the Hessian's first row
         Jet<float, 2> sj[2];
         for (int k = 0; k < 2; k++) {
             sj[k] = Jet<float, 2>(scr[k]);
-            sj[k].t = k == row ? 1.0f : 0.0f;
+            sj[k].t = k == 0 ? 1.0f : 0.0f;
             sj[k].g[0] = k == 0 ? 1.0f : 0.0f;
             sj[k].g[1] = k == 1 ? 1.0f : 0.0f;
         }
@@ -835,12 +836,22 @@ DEV bool lens_jets(const DevScene &S, const Opts &o, const SRef &ref, const floa
         perturb_lens_eval<Jet<float, 2>, true>(S, o, ref, sj, lensPos, out);
         if (!out.valid || !(out.lensScore.v > 0.0f)) return false;
         const Jet<float, 2> lf = s_log(out.lensScore);
-        if (row == 0) {
-            g[0] = lf.g[0]; g[1] = lf.g[1];
-            h[0] = lf.h[0]; h[1] = lf.h[1];
-        } else {
-            h[2] = lf.h[1];
+        g[0] = lf.g[0]; g[1] = lf.g[1];
+        h[0] = lf.h[0]; h[1] = lf.h[1];
+    }
+    {
+        // row y, column y: the second diagonal entry (one column carries it;
+        // the jet's columns are computed independently of each other)
+        Jet<float, 1> sj[2];
+        for (int k = 0; k < 2; k++) {
+            sj[k] = Jet<float, 1>(scr[k]);
+            sj[k].t = k == 1 ? 1.0f : 0.0f;
+            sj[k].g[0] = k == 1 ? 1.0f : 0.0f;
         }
+        POut<Jet<float, 1>> out;
+        perturb_lens_eval<Jet<float, 1>, true>(S, o, ref, sj, lensPos, out);
+        if (!out.valid || !(out.lensScore.v > 0.0f)) return false;
+        h[2] = s_log(out.lensScore).h[0];
     }
     return true;
 }
